@@ -1,0 +1,260 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/toast_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and only as the checker.  The product package
+toastpp_amd never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+P = C.c_void_p
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        sig = {
+            "om_mesh_load": (P, [C.c_char_p]),
+            "om_mesh_make": (P, [C.c_int, C.c_int, D, C.c_int, I]),
+            "om_mesh_free": (None, [P]),
+            "om_setup": (None, [P]),
+            "om_dim": (C.c_int, [P]), "om_nlen": (C.c_int, [P]), "om_elen": (C.c_int, [P]),
+            "om_get_nodes": (None, [P, D]), "om_get_elements": (None, [P, I]),
+            "om_get_bnd": (None, [P, I]), "om_get_regions": (None, [P, I]),
+            "om_get_size": (None, [P, D]), "om_get_bndside": (None, [P, U8]),
+            "om_get_intbff": (None, [P, D]), "om_get_intdd": (None, [P, D]),
+            "om_csr_nnz": (C.c_int, [P]), "om_csr": (None, [P, I, I]),
+            "om_assemble": (None, [P, I, I, D, D, D, C.c_double, D]),
+            "om_assemble_ff": (None, [P, I, I, D]),
+            "om_el_intfff": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "om_el_intfdd": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "om_el_intpff": (C.c_double, [P, C.c_int, C.c_int, C.c_int, D]),
+            "om_el_bndintpff": (C.c_double, [P, C.c_int, C.c_int, C.c_int, D]),
+            "om_a_keijzer": (C.c_double, [C.c_double]),
+            "om_coeffs": (None, [C.c_int, D, D, D, D, D, D]),
+            "om_qvec_gaussian": (None, [P, D, C.c_double, C.c_int, D]),
+            "om_qm_load": (P, [P, C.c_char_p]),
+            "om_qm_nq": (C.c_int, [P]), "om_qm_nm": (C.c_int, [P]), "om_qm_nqm": (C.c_int, [P]),
+            "om_qm_get": (None, [P, D, D, I, I]), "om_qm_free": (None, [P]),
+            "om_pull_to_boundary": (C.c_int, [P, D, D]),
+            "om_pcg": (C.c_int, [C.c_int, I, I, D, D, D, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "om_bicgstab_c": (C.c_int, [C.c_int, I, I, D, D, D, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "om_relres_c": (C.c_double, [C.c_int, I, I, D, D, D]),
+            "om_project": (None, [C.c_int, C.c_int, C.c_int, D, D, I, I, C.c_int, D]),
+            "om_jacobian_nodal": (None, [P, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _cplx(a):
+    """complex128 array -> interleaved float64 view (contiguous)"""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64)
+
+
+class Mesh:
+    """Oracle mesh: Mesh::operator>> + Mesh::Setup (mesh.cc:96-158, 1787-1825)."""
+
+    def __init__(self, path=None, nodes=None, elems=None):
+        L = lib()
+        if path is not None:
+            self.h = L.om_mesh_load(path.encode())
+            if not self.h:
+                raise IOError("oracle: cannot read mesh " + path)
+        else:
+            nodes = _c(nodes, np.float64)
+            elems = _c(elems, np.int32)
+            self.h = L.om_mesh_make(nodes.shape[1], nodes.shape[0], nodes, elems.shape[0], elems)
+        L.om_setup(self.h)
+        self.dim = L.om_dim(self.h)
+        self.n = L.om_nlen(self.h)
+        self.elen = L.om_elen(self.h)
+        self.nn = self.dim + 1
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.om_mesh_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        x = np.zeros((self.n, self.dim)); lib().om_get_nodes(self.h, x); return x
+
+    def elements(self):
+        e = np.zeros((self.elen, self.nn), np.int32); lib().om_get_elements(self.h, e); return e
+
+    def bnd(self):
+        b = np.zeros(self.n, np.int32); lib().om_get_bnd(self.h, b); return b
+
+    def regions(self):
+        r = np.zeros(self.n, np.int32); lib().om_get_regions(self.h, r); return r
+
+    def size(self):
+        s = np.zeros(self.elen); lib().om_get_size(self.h, s); return s
+
+    def bndside(self):
+        b = np.zeros((self.elen, self.nn), np.uint8); lib().om_get_bndside(self.h, b); return b
+
+    def intbff(self):
+        b = np.zeros((self.elen, self.nn, self.nn)); lib().om_get_intbff(self.h, b); return b
+
+    def intdd(self):
+        b = np.zeros((self.elen, self.nn, self.nn)); lib().om_get_intdd(self.h, b); return b
+
+    def csr(self):
+        L = lib()
+        nnz = L.om_csr_nnz(self.h)
+        rp = np.zeros(self.n + 1, np.int32); ci = np.zeros(nnz, np.int32)
+        L.om_csr(self.h, rp, ci)
+        return rp, ci
+
+    def sysmat(self, cmua, ckap, c2a, omega):
+        rp, ci = self.csr()
+        val = np.zeros(2 * len(ci))
+        lib().om_assemble(self.h, rp, ci, _c(cmua, np.float64), _c(ckap, np.float64), _c(c2a, np.float64),
+                          float(omega), val)
+        return rp, ci, val.view(np.complex128)
+
+    def massmat(self):
+        rp, ci = self.csr()
+        val = np.zeros(len(ci))
+        lib().om_assemble_ff(self.h, rp, ci, val)
+        return rp, ci, val
+
+    def qvec_gaussian(self, centres, width, neumann=True):
+        centres = np.atleast_2d(_c(centres, np.float64))
+        out = np.zeros((centres.shape[0], self.n))
+        for i, c in enumerate(centres):
+            q = np.zeros(self.n)
+            lib().om_qvec_gaussian(self.h, _c(c, np.float64), float(width), 1 if neumann else 0, q)
+            out[i] = q
+        return out
+
+    def read_qm(self, path):
+        L = lib()
+        h = L.om_qm_load(self.h, path.encode())
+        if not h:
+            raise IOError("oracle: cannot read QM " + path)
+        nq, nm, nqm = L.om_qm_nq(h), L.om_qm_nm(h), L.om_qm_nqm(h)
+        Q = np.zeros((nq, self.dim)); M = np.zeros((nm, self.dim))
+        lp = np.zeros(nq + 1, np.int32); li = np.zeros(nqm, np.int32)
+        L.om_qm_get(h, Q, M, lp, li)
+        L.om_qm_free(h)
+        return Q, M, lp, li
+
+
+def a_keijzer(n):
+    return lib().om_a_keijzer(float(n))
+
+
+def coeffs(mua, mus, ref):
+    mua, mus, ref = (_c(a, np.float64) for a in (mua, mus, ref))
+    n = len(mua)
+    cm, ck, c2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().om_coeffs(n, mua, mus, ref, cm, ck, c2)
+    return cm, ck, c2
+
+
+def pcg(rp, ci, val, b, tol, maxit=0, diag=False, x0=None):
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else _c(x0, np.float64).copy()
+    err = C.c_double(0)
+    it = lib().om_pcg(n, _c(rp, np.int32), _c(ci, np.int32), _c(val, np.float64), _c(b, np.float64), x,
+                      float(tol), int(maxit), 1 if diag else 0, C.byref(err))
+    return x, it, err.value
+
+
+def bicgstab(rp, ci, val, b, tol, maxit=0, diag=False, x0=None):
+    n = len(rp) - 1
+    x = np.zeros(n, np.complex128) if x0 is None else np.array(x0, np.complex128)
+    xv = x.view(np.float64)
+    err = C.c_double(0)
+    it = lib().om_bicgstab_c(n, _c(rp, np.int32), _c(ci, np.int32), _cplx(val), _cplx(b), xv,
+                             float(tol), int(maxit), 1 if diag else 0, C.byref(err))
+    return x, it, err.value
+
+
+def relres(rp, ci, val, b, x):
+    return lib().om_relres_c(len(rp) - 1, _c(rp, np.int32), _c(ci, np.int32), _cplx(val), _cplx(b), _cplx(x))
+
+
+def project(phi, mvec, lptr, lidx, logscale=True):
+    """phi: (nq, n) complex, mvec: (nm, n) complex dense."""
+    phi = np.ascontiguousarray(phi, np.complex128); mvec = np.ascontiguousarray(mvec, np.complex128)
+    nq, n = phi.shape
+    out = np.zeros(2 * int(lptr[-1]))
+    lib().om_project(n, nq, mvec.shape[0], phi.view(np.float64), mvec.view(np.float64), _c(lptr, np.int32),
+                     _c(lidx, np.int32), 1 if logscale else 0, out)
+    return out.view(np.complex128)
+
+
+def jacobian_nodal(mesh, dphi, aphi, lptr, lidx, proj, logscale=True):
+    dphi = np.ascontiguousarray(dphi, np.complex128); aphi = np.ascontiguousarray(aphi, np.complex128)
+    nq, nm = dphi.shape[0], aphi.shape[0]
+    nqm = int(lptr[-1])
+    J = np.zeros((2 * nqm, 2 * mesh.n))
+    lib().om_jacobian_nodal(mesh.h, nq, nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
+                            _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, J)
+    return J
+
+
+def forward(mesh, qm, mua, mus, ref, freq, qwidth=2.0, mwidth=2.0, solver="bicgstab", tol=1e-15,
+            maxit=10000, diag=False, logscale=True):
+    """fwdfem restated (src/supertoast/fwdfem.cc:56-260): returns (proj, phi, sysmat)."""
+    Q, M, lp, li = qm
+    cm, ck, c2 = coeffs(mua, mus, ref)
+    omega = freq * 2.0 * np.pi * 1e-6
+    rp, ci, val = mesh.sysmat(cm, ck, c2, omega)
+    qv = mesh.qvec_gaussian(Q, qwidth)
+    mv = mesh.qvec_gaussian(M, mwidth) * c2[None, :]
+    nq = Q.shape[0]
+    phi = np.zeros((nq, mesh.n), np.complex128)
+    for i in range(nq):
+        if solver == "cg":
+            x, _, _ = pcg(rp, ci, val.real.copy(), qv[i], tol, maxit, diag)
+            phi[i] = x
+        else:
+            x, _, _ = bicgstab(rp, ci, val, qv[i].astype(np.complex128), tol, maxit, diag)
+            phi[i] = x
+    proj = project(phi, mv.astype(np.complex128), lp, li, logscale)
+    return proj, phi, (rp, ci, val), qv, mv
+
+
+def read_fem(path):
+    """fwdfem output vector ('[v0 v1 ...]', fwdfem.cc:457-462)."""
+    with open(path) as f:
+        txt = f.read().replace("[", " ").replace("]", " ")
+    return np.array(txt.split(), dtype=np.float64)
+
+
+def cmpvec(a, b):
+    """Largest per-element relative difference |a-b|/max(|a|,|b|) (bintools/cmpvec.cc)."""
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    den = np.maximum(np.abs(a), np.abs(b))
+    den[den == 0] = 1.0
+    return float(np.max(np.abs(a - b) / den))

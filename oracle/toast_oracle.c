@@ -1,0 +1,903 @@
+/*
+ * toast_oracle.c -- FP64 CPU restatement of the Toast++ diffusion forward
+ * solve, projection and Jacobian path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in the product (toastpp_amd/) links or calls it.
+ *
+ * Every function restates the reference algorithm it names (file:line in the
+ * toastpp/toastpp tree).  Element integrals, assembly order and solver steps
+ * follow the reference operation by operation so that FP64 results agree to
+ * the last bit where the reference is deterministic.
+ *
+ * Pinning (how each part is checked; see tests/test_oracle.py):
+ *   - mesh reader / Setup / MarkBoundary / CSR / assembly / QVec_Gaussian /
+ *     QM loading with PullToBoundary / PCG / BiCGSTAB / ProjectAll + log:
+ *     end-to-end against the reference's own golden vectors
+ *     test/2D/fwdfem/testdata.zip and test/3D/fwdfem/testdata.zip, plus
+ *     closed forms (element integrals vs barycentric monomial integrals,
+ *     row sums of the mass matrix, zero row sums of the stiffness matrix).
+ *   - nodal Jacobian: finite differences of the projections with respect to
+ *     nodal cmua / ckappa (the check of test/2D/matlab/jacobian_test.m).
+ *   - raster (pixel) basis: closed-form checks only (linear-field
+ *     reproduction by the grid map, exact gradients of linear fields,
+ *     raster Jacobian vs nodal Jacobian contracted with the grid map).
+ *     The reference's EPS and edge rules are restated but not pinned by a
+ *     reference output: parity unpinned beyond those invariants.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OM_PI 3.1415926535897932384626433832795
+#define OM_C0 0.3
+
+typedef struct {
+    int dim, n, elen, nn, nsd, nsn;   /* nn nodes/element, nsd sides, nsn nodes/side */
+    double *x;                        /* n*dim */
+    int *el;                          /* elen*nn, 0-based */
+    int *nreg;                        /* node region (-1 if none) */
+    int *bnd;                         /* node boundary flag after MarkBoundary */
+    double *ga, *gb, *gc, *gd;        /* elen*nn geometry coefficients */
+    double *size;                     /* elen signed size */
+    double *sside;                    /* elen*4 side areas (Tet4) */
+    unsigned char *bndside;           /* elen*nsd */
+    unsigned char *bndel;             /* elen */
+    double *intdd;                    /* elen*nn*nn (full) */
+    double *intbff;                   /* elen*nn*nn (full) */
+} om_mesh;
+
+static const int TET_SN[4][3] = {{0,1,2},{0,3,1},{0,2,3},{1,3,2}};   /* tet4.cc:177 */
+static const int TRI_SN[3][2] = {{0,1},{1,2},{2,0}};                 /* tri3.cc:121-127 */
+
+static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!p) { fprintf(stderr, "oracle: out of memory\n"); exit(2); } return p; }
+
+/* ------------------------------------------------------------------------
+ * Mesh reader: "MeshData 5.0" (mesh.cc:1787-1825, ndlist.cc:146-159,
+ * node.cc:130-162, ellist.cc:154-190, element.cc:429-437).
+ * Only Tri3 ('o', ELID 15) and Tet4 ('c', ELID 3) are accepted.
+ * ------------------------------------------------------------------------ */
+static om_mesh *mesh_alloc(int dim, int n, int elen)
+{
+    om_mesh *m = (om_mesh*)xcalloc(1, sizeof(om_mesh));
+    m->dim = dim; m->n = n; m->elen = elen;
+    m->nn = dim + 1; m->nsd = dim + 1; m->nsn = dim;
+    m->x = (double*)xcalloc((size_t)n*dim, sizeof(double));
+    m->el = (int*)xcalloc((size_t)elen*m->nn, sizeof(int));
+    m->nreg = (int*)xcalloc(n, sizeof(int));
+    m->bnd = (int*)xcalloc(n, sizeof(int));
+    return m;
+}
+
+om_mesh *om_mesh_load(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return NULL;
+    char line[1024];
+    int found = 0;
+    while (fgets(line, sizeof line, f)) if (!strncmp(line, "MeshData 5.0", 12)) { found = 1; break; }
+    if (!found) { fclose(f); return NULL; }
+    int n = 0, dof = 0;
+    while (fgets(line, sizeof line, f)) if (sscanf(line, " NodeList %d %d", &n, &dof) >= 1) break;
+    double *xs = (double*)xcalloc((size_t)n*3, sizeof(double));
+    int *reg = (int*)xcalloc(n, sizeof(int));
+    int dim = 0;
+    for (int i = 0; i < n; i++) {
+        if (!fgets(line, sizeof line, f)) { fclose(f); return NULL; }
+        char *p = strchr(line, '[');
+        if (!p) { fclose(f); return NULL; }
+        p++;
+        int d = 0; char *end;
+        while (d < 3) {
+            double v = strtod(p, &end);
+            if (end == p) break;
+            xs[i*3+d] = v; d++; p = end;
+        }
+        if (i == 0) dim = d;
+        char *q = strchr(p, ']');
+        reg[i] = -1;
+        if (q && q[1] == 'R') reg[i] = atoi(q+2);
+    }
+    int elen = 0;
+    while (fgets(line, sizeof line, f)) if (!strncasecmp(line, "ElementList", 11)) { elen = atoi(line+11); break; }
+    om_mesh *m = mesh_alloc(dim, n, elen);
+    for (int i = 0; i < n; i++) { for (int d = 0; d < dim; d++) m->x[i*dim+d] = xs[i*3+d]; m->nreg[i] = reg[i]; }
+    free(xs); free(reg);
+    for (int e = 0; e < elen; e++) {
+        char c;
+        do { if (fscanf(f, " %c", &c) != 1) { fclose(f); return NULL; } } while (c == ' ');
+        int id = (c >= 'a' && c <= 'z') ? 1 + (c - 'a') : 1 + (c - '1');
+        if ((dim == 2 && id != 15) || (dim == 3 && id != 3)) { fclose(f); fprintf(stderr, "oracle: unsupported element type %d\n", id); return NULL; }
+        for (int k = 0; k < m->nn; k++) { int v; if (fscanf(f, "%d", &v) != 1) { fclose(f); return NULL; } m->el[e*m->nn+k] = v - 1; }
+        if (!fgets(line, sizeof line, f)) break;   /* rest of line */
+    }
+    fclose(f);
+    return m;
+}
+
+om_mesh *om_mesh_make(int dim, int n, const double *x, int elen, const int *el)
+{
+    om_mesh *m = mesh_alloc(dim, n, elen);
+    memcpy(m->x, x, sizeof(double)*(size_t)n*dim);
+    memcpy(m->el, el, sizeof(int)*(size_t)elen*m->nn);
+    for (int i = 0; i < n; i++) m->nreg[i] = -1;
+    return m;
+}
+
+void om_mesh_free(om_mesh *m)
+{
+    if (!m) return;
+    free(m->x); free(m->el); free(m->nreg); free(m->bnd);
+    free(m->ga); free(m->gb); free(m->gc); free(m->gd); free(m->size); free(m->sside);
+    free(m->bndside); free(m->bndel); free(m->intdd); free(m->intbff);
+    free(m);
+}
+
+int om_dim(const om_mesh *m) { return m->dim; }
+int om_nlen(const om_mesh *m) { return m->n; }
+int om_elen(const om_mesh *m) { return m->elen; }
+void om_get_nodes(const om_mesh *m, double *x) { memcpy(x, m->x, sizeof(double)*(size_t)m->n*m->dim); }
+void om_get_elements(const om_mesh *m, int *el) { memcpy(el, m->el, sizeof(int)*(size_t)m->elen*m->nn); }
+void om_get_bnd(const om_mesh *m, int *b) { memcpy(b, m->bnd, sizeof(int)*(size_t)m->n); }
+void om_get_regions(const om_mesh *m, int *r) { memcpy(r, m->nreg, sizeof(int)*(size_t)m->n); }
+void om_get_size(const om_mesh *m, double *s) { memcpy(s, m->size, sizeof(double)*(size_t)m->elen); }
+void om_get_bndside(const om_mesh *m, unsigned char *b) { memcpy(b, m->bndside, (size_t)m->elen*m->nsd); }
+void om_get_intbff(const om_mesh *m, double *b) { memcpy(b, m->intbff, sizeof(double)*(size_t)m->elen*m->nn*m->nn); }
+void om_get_intdd(const om_mesh *m, double *b) { memcpy(b, m->intdd, sizeof(double)*(size_t)m->elen*m->nn*m->nn); }
+
+/* ------------------------------------------------------------------------
+ * Element geometry.  Tet4: tet4.cc:113-171 (coefficients), :1011-1014 (size),
+ * tet4.cc:19-29 (side areas by Heron); Tri3: tri3.cc:79-90, :184-189.
+ * ------------------------------------------------------------------------ */
+static double pdist(const double *p, const double *q, int dim)   /* point.cc:33-41 */
+{
+    double d2 = 0.0;
+    for (int i = 0; i < dim; i++) d2 += (p[i]-q[i]) * (p[i]-q[i]);
+    return sqrt(d2);
+}
+
+static double tri_area3(const double *p1, const double *p2, const double *p3)  /* tet4.cc:19-29 */
+{
+    double a = pdist(p1, p2, 3), b = pdist(p2, p3, 3), c = pdist(p3, p1, 3);
+    double s = 0.5 * (a+b+c);
+    return sqrt(s * (s-a) * (s-b) * (s-c));
+}
+
+static void geometry(om_mesh *m)
+{
+    int E = m->elen, nn = m->nn;
+    m->ga = (double*)xcalloc((size_t)E*nn, sizeof(double));
+    m->gb = (double*)xcalloc((size_t)E*nn, sizeof(double));
+    m->gc = (double*)xcalloc((size_t)E*nn, sizeof(double));
+    m->gd = (double*)xcalloc((size_t)E*nn, sizeof(double));
+    m->size = (double*)xcalloc(E, sizeof(double));
+    m->sside = (double*)xcalloc((size_t)E*4, sizeof(double));
+    for (int e = 0; e < E; e++) {
+        const int *nd = m->el + e*nn;
+        double *a = m->ga + e*nn, *b = m->gb + e*nn, *c = m->gc + e*nn, *d = m->gd + e*nn;
+        if (m->dim == 3) {
+            const double *P0 = m->x + 3*nd[0], *P1 = m->x + 3*nd[1], *P2 = m->x + 3*nd[2], *P3 = m->x + 3*nd[3];
+            double x0 = P0[0], y0 = P0[1], z0 = P0[2];
+            double x1 = P1[0], y1 = P1[1], z1 = P1[2];
+            double x2 = P2[0], y2 = P2[1], z2 = P2[2];
+            double x3 = P3[0], y3 = P3[1], z3 = P3[2];
+            /* cofactor expansion, same operand order as tet4.cc:125-140 */
+            a[0] = x1*(y2*z3-y3*z2) - x2*(y1*z3-y3*z1) + x3*(y1*z2-y2*z1);
+            a[1] = x0*(y3*z2-y2*z3) - x2*(y3*z0-y0*z3) + x3*(y2*z0-y0*z2);
+            a[2] = x0*(y1*z3-y3*z1) - x1*(y0*z3-y3*z0) + x3*(y0*z1-y1*z0);
+            a[3] = x0*(y2*z1-y1*z2) - x1*(y2*z0-y0*z2) + x2*(y1*z0-y0*z1);
+            b[0] = y1*(z3-z2) + y2*(z1-z3) + y3*(z2-z1);
+            b[1] = y0*(z2-z3) + y2*(z3-z0) + y3*(z0-z2);
+            b[2] = y0*(z3-z1) + y1*(z0-z3) + y3*(z1-z0);
+            b[3] = y0*(z1-z2) + y1*(z2-z0) + y2*(z0-z1);
+            c[0] = x1*(z2-z3) + x2*(z3-z1) + x3*(z1-z2);
+            c[1] = x0*(z3-z2) + x2*(z0-z3) + x3*(z2-z0);
+            c[2] = x0*(z1-z3) + x1*(z3-z0) + x3*(z0-z1);
+            c[3] = x0*(z2-z1) + x1*(z0-z2) + x2*(z1-z0);
+            d[0] = x1*(y3-y2) + x2*(y1-y3) + x3*(y2-y1);
+            d[1] = x0*(y2-y3) + x2*(y3-y0) + x3*(y0-y2);
+            d[2] = x0*(y3-y1) + x1*(y0-y3) + x3*(y1-y0);
+            d[3] = x0*(y1-y2) + x1*(y2-y0) + x2*(y0-y1);
+            for (int sd = 0; sd < 4; sd++)
+                m->sside[e*4+sd] = tri_area3(m->x + 3*nd[TET_SN[sd][0]], m->x + 3*nd[TET_SN[sd][1]], m->x + 3*nd[TET_SN[sd][2]]);
+            m->size[e] = (1.0/6.0) * (a[0]+a[1]+a[2]+a[3]);
+        } else {
+            const double *P0 = m->x + 2*nd[0], *P1 = m->x + 2*nd[1], *P2 = m->x + 2*nd[2];
+            double x0 = P0[0], y0 = P0[1], x1 = P1[0], y1 = P1[1], x2 = P2[0], y2 = P2[1];
+            a[0] = x1*y2 - x2*y1;  b[0] = y1-y2;  c[0] = x2-x1;
+            a[1] = x2*y0 - x0*y2;  b[1] = y2-y0;  c[1] = x0-x2;
+            a[2] = x0*y1 - x1*y0;  b[2] = y0-y1;  c[2] = x1-x0;
+            m->size[e] = 0.5 * (a[0]+a[1]+a[2]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * MarkBoundary (mesh.cc:1614-1703): sides sorted by node tuple, a side that
+ * equals neither sorted neighbour is a boundary side; its nodes become
+ * boundary nodes.
+ * ------------------------------------------------------------------------ */
+typedef struct { int el, sd, nd[3]; } side_rec;
+static int g_nsn;
+static int side_cmp(const void *A, const void *B)
+{
+    const side_rec *a = (const side_rec*)A, *b = (const side_rec*)B;
+    for (int k = 0; k < g_nsn; k++) { if (a->nd[k] < b->nd[k]) return -1; if (a->nd[k] > b->nd[k]) return 1; }
+    /* ties are broken by element/side so the order is deterministic */
+    if (a->el != b->el) return a->el < b->el ? -1 : 1;
+    return a->sd < b->sd ? -1 : (a->sd > b->sd);
+}
+
+static void mark_boundary(om_mesh *m)
+{
+    int E = m->elen, nsd = m->nsd, nsn = m->nsn;
+    size_t S = (size_t)E*nsd;
+    side_rec *sd = (side_rec*)xcalloc(S, sizeof(side_rec));
+    for (int e = 0; e < E; e++)
+        for (int s = 0; s < nsd; s++) {
+            side_rec *r = sd + (size_t)e*nsd + s;
+            r->el = e; r->sd = s;
+            for (int k = 0; k < nsn; k++) r->nd[k] = m->el[e*m->nn + (m->dim == 3 ? TET_SN[s][k] : TRI_SN[s][k])];
+            for (int j = 0; j < nsn-1; j++) for (int k = j+1; k < nsn; k++)
+                if (r->nd[j] > r->nd[k]) { int t = r->nd[j]; r->nd[j] = r->nd[k]; r->nd[k] = t; }
+        }
+    g_nsn = nsn;
+    qsort(sd, S, sizeof(side_rec), side_cmp);
+    m->bndside = (unsigned char*)xcalloc(S, 1);
+    m->bndel = (unsigned char*)xcalloc(E, 1);
+    for (int i = 0; i < m->n; i++) m->bnd[i] = 0;
+    for (size_t i = 0; i < S; i++) {
+        int isbnd = 1;
+        if (i && !memcmp(sd[i].nd, sd[i-1].nd, sizeof(int)*nsn)) isbnd = 0;
+        if (i+1 < S && isbnd && !memcmp(sd[i].nd, sd[i+1].nd, sizeof(int)*nsn)) isbnd = 0;
+        if (isbnd) {
+            m->bndside[(size_t)sd[i].el*nsd + sd[i].sd] = 1;
+            m->bndel[sd[i].el] = 1;
+            for (int k = 0; k < nsn; k++) m->bnd[sd[i].nd[k]] = 1;
+        }
+    }
+    free(sd);
+}
+
+/* ------------------------------------------------------------------------
+ * PostInitialisation: intdd (tet4.cc:1016-1031, tri3.cc:490-506) and
+ * intbff (tet4.cc:1109-1122 with sym_bndintff :1039-1068; tri3.cc:561-582)
+ * ------------------------------------------------------------------------ */
+static int in_side_tet(int sd, int i) { return i == TET_SN[sd][0] || i == TET_SN[sd][1] || i == TET_SN[sd][2]; }
+
+static void post_init(om_mesh *m)
+{
+    int E = m->elen, nn = m->nn;
+    m->intdd = (double*)xcalloc((size_t)E*nn*nn, sizeof(double));
+    m->intbff = (double*)xcalloc((size_t)E*nn*nn, sizeof(double));
+    for (int e = 0; e < E; e++) {
+        const double *b = m->gb + e*nn, *c = m->gc + e*nn, *d = m->gd + e*nn;
+        double *dd = m->intdd + (size_t)e*nn*nn, *bf = m->intbff + (size_t)e*nn*nn;
+        if (m->dim == 3) {
+            double scale = 1.0/(36.0*m->size[e]);
+            for (int i = 0; i < 4; i++) for (int j = 0; j <= i; j++)
+                dd[i*4+j] = dd[j*4+i] = scale * (b[j]*b[i]+c[j]*c[i]+d[j]*d[i]);
+            if (m->bndel[e]) {
+                const double f = 2.0/24.0;
+                for (int sd = 0; sd < 4; sd++) {
+                    if (!m->bndside[e*4+sd]) continue;
+                    for (int i = 0; i < 4; i++) for (int j = 0; j <= i; j++) {
+                        double v = (in_side_tet(sd,i) && in_side_tet(sd,j)) ? (i == j ? 2.0 : 1.0) * f : 0.0;
+                        double t = v * m->sside[e*4+sd];
+                        bf[i*4+j] += t;
+                        if (i != j) bf[j*4+i] = bf[i*4+j];
+                    }
+                }
+            }
+        } else {
+            double scale = 0.25/m->size[e];
+            for (int i = 0; i < 3; i++) for (int j = 0; j <= i; j++)
+                dd[i*3+j] = dd[j*3+i] = scale * (b[i]*b[j] + c[i]*c[j]);
+            if (m->bndel[e]) {
+                for (int s = 0; s < 3; s++) {
+                    if (!m->bndside[e*3+s]) continue;
+                    int n0 = TRI_SN[s][0], n1 = TRI_SN[s][1];
+                    const double *p0 = m->x + 2*m->el[e*3+n0], *p1 = m->x + 2*m->el[e*3+n1];
+                    double dx = p0[0]-p1[0], dy = p0[1]-p1[1];
+                    double dl = hypot(dx, dy);
+                    for (int i = 0; i < 3; i++) {
+                        if (i != n0 && i != n1) continue;
+                        bf[i*3+i] += dl/3.0;
+                        for (int j = 0; j < i; j++)
+                            if (j == n0 || j == n1) { bf[i*3+j] += dl/6.0; bf[j*3+i] = bf[i*3+j]; }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Mesh::Setup (mesh.cc:96-158): Initialise, MarkBoundary, PostInitialisation */
+void om_setup(om_mesh *m)
+{
+    geometry(m);
+    mark_boundary(m);
+    post_init(m);
+}
+
+/* ------------------------------------------------------------------------
+ * Element integrals (reference tables restated as integer numerators)
+ * ------------------------------------------------------------------------ */
+static int mult3(int i, int j, int k)      /* 6 / 2 / 1 numerators of int l_i l_j l_k */
+{
+    if (i == j && j == k) return 6;
+    if (i == j || j == k || i == k) return 2;
+    return 1;
+}
+
+static double el_intff(const om_mesh *m, int e, int i, int j)   /* tet4.cc:348-355, tri3.cc:285-292 */
+{
+    if (m->dim == 3) return ((i == j ? 2.0 : 1.0) * (1.0/20.0)) * m->size[e];
+    return ((i == j ? 2.0 : 1.0) * (1.0/12.0)) * m->size[e];
+}
+
+static double el_intfff(const om_mesh *m, int e, int i, int j, int k)  /* tet4.cc:357-361, tri3.cc:294-297 */
+{
+    if (m->dim == 3) return ((double)mult3(i,j,k) / 120.0) * m->size[e];
+    static const double sc[3] = {1.0/60.0, 1.0/30.0, 1.0/10.0};
+    int idx = (i == j && j == k) ? 2 : ((i == j || j == k || i == k) ? 1 : 0);
+    return m->size[e] * sc[idx];
+}
+
+static double el_intfdd(const om_mesh *m, int e, int i, int j, int k)   /* tet4.cc:380-412, tri3.h:196-197 */
+{
+    (void)i;
+    int nn = m->nn;
+    if (m->dim == 3) {
+        if (j > k) { int t = j; j = k; k = t; }
+        const double *b = m->gb + e*nn, *c = m->gc + e*nn, *d = m->gd + e*nn;
+        double scale = 1.0/(144.0*m->size[e]);
+        return (b[j]*b[k]+c[j]*c[k]+d[j]*d[k]) * scale;
+    }
+    return m->intdd[(size_t)e*nn*nn + j*nn + k] * 0.3333333333;
+}
+
+static double el_intpff(const om_mesh *m, int e, int i, int j, const double *P)  /* tet4.cc:371-378, tri3.cc:325-353 */
+{
+    const int *nd = m->el + e*m->nn;
+    if (m->dim == 3) {
+        return ((double)mult3(0,i,j)/120.0 * P[nd[0]] +
+                (double)mult3(1,i,j)/120.0 * P[nd[1]] +
+                (double)mult3(2,i,j)/120.0 * P[nd[2]] +
+                (double)mult3(3,i,j)/120.0 * P[nd[3]]) * m->size[e];
+    }
+    if (i < j) { int t = i; i = j; j = t; }
+    const double i30 = 1.0/30.0;
+    double p0 = P[nd[0]], p1 = P[nd[1]], p2 = P[nd[2]];
+    double s;
+    if (i == 0) s = 3.0*p0 + p1 + p2;
+    else if (i == 1) s = (j == 0) ? p0 + p1 + 0.5*p2 : p0 + 3.0*p1 + p2;
+    else s = (j == 0) ? p0 + 0.5*p1 + p2 : (j == 1) ? 0.5*p0 + p1 + p2 : p0 + p1 + 3.0*p2;
+    return m->size[e]*i30 * s;
+}
+
+static double el_intpdd(const om_mesh *m, int e, int i, int j, const double *P)  /* tet4.cc:414-419, tri3.h:208-209 */
+{
+    const int *nd = m->el + e*m->nn;
+    if (m->dim == 3) {
+        double res = 0.0;
+        for (int k = 0; k < 4; k++) res += P[nd[k]];
+        return res * el_intfdd(m, e, 0, i, j);
+    }
+    return m->intdd[(size_t)e*9 + i*3 + j] * ((P[nd[0]]+P[nd[1]]+P[nd[2]])/3.0);
+}
+
+static double el_bndintpff(const om_mesh *m, int e, int i, int j, const double *P)  /* tet4.cc:819-832, tri3.h:217-218 */
+{
+    const int *nd = m->el + e*m->nn;
+    if (m->dim == 3) {
+        if (!m->bndel[e]) return 0.0;
+        double res = 0.0;
+        for (int sd = 0; sd < 4; sd++) {
+            if (!m->bndside[e*4+sd]) continue;
+            double sres = 0.0;
+            for (int k = 0; k < 4; k++) {
+                double t = (in_side_tet(sd,i) && in_side_tet(sd,j) && in_side_tet(sd,k)) ? (double)mult3(k,i,j) * (2.0/120.0) : 0.0;
+                sres += P[nd[k]] * t;
+            }
+            res += sres * m->sside[e*4+sd];
+        }
+        return res;
+    }
+    return m->intbff[(size_t)e*9 + i*3 + j] * ((P[nd[0]]+P[nd[1]]+P[nd[2]])/3.0);
+}
+
+/* ------------------------------------------------------------------------
+ * CSR structure (mesh.cc:422-549): sorted unique (row, col) element pairs,
+ * diagonal included.
+ * ------------------------------------------------------------------------ */
+static int pair_cmp(const void *A, const void *B)
+{
+    const int *a = (const int*)A, *b = (const int*)B;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    return a[1] < b[1] ? -1 : (a[1] > b[1]);
+}
+
+static int *g_rp, *g_ci; static int g_nnz; static const om_mesh *g_csr_mesh;
+static void build_csr(const om_mesh *m)
+{
+    if (g_csr_mesh == m && g_rp) return;
+    free(g_rp); free(g_ci);
+    size_t P = (size_t)m->elen*m->nn*m->nn;
+    int *pr = (int*)xcalloc(P*2, sizeof(int));
+    size_t p = 0;
+    for (int e = 0; e < m->elen; e++)
+        for (int i = 0; i < m->nn; i++) for (int j = 0; j < m->nn; j++) {
+            pr[2*p] = m->el[e*m->nn+i]; pr[2*p+1] = m->el[e*m->nn+j]; p++;
+        }
+    qsort(pr, P, 2*sizeof(int), pair_cmp);
+    int nz = 0;
+    for (size_t i = 0; i < P; i++)
+        if (!i || pr[2*i] != pr[2*i-2] || pr[2*i+1] != pr[2*i-1]) { pr[2*nz] = pr[2*i]; pr[2*nz+1] = pr[2*i+1]; nz++; }
+    g_rp = (int*)xcalloc(m->n+1, sizeof(int));
+    g_ci = (int*)xcalloc(nz, sizeof(int));
+    for (int i = 0; i < nz; i++) { g_rp[pr[2*i]+1]++; g_ci[i] = pr[2*i+1]; }
+    for (int i = 0; i < m->n; i++) g_rp[i+1] += g_rp[i];
+    g_nnz = nz; g_csr_mesh = m;
+    free(pr);
+}
+
+int om_csr_nnz(const om_mesh *m) { build_csr(m); return g_nnz; }
+void om_csr(const om_mesh *m, int *rowptr, int *colidx)
+{
+    build_csr(m);
+    memcpy(rowptr, g_rp, sizeof(int)*(size_t)(m->n+1));
+    memcpy(colidx, g_ci, sizeof(int)*(size_t)g_nnz);
+}
+
+/* TCompRowMatrix::operator() row search (crmatrix_imp.hpp:756-767) */
+static int slot(const int *rp, const int *ci, int r, int c)
+{
+    for (int k = rp[r]; k < rp[r+1]; k++) if (ci[k] == c) return k;
+    fprintf(stderr, "oracle: entry (%d,%d) not in pattern\n", r, c); exit(3);
+}
+
+/* ------------------------------------------------------------------------
+ * AssembleSystemMatrix (fwdsolver.cc:405-424 complex, :338-357 real):
+ * PFF(cmua), PDD(ckappa), BNDPFF(c2a) then iCFF(omega); elements ascending,
+ * local i then j (mesh.cc:2085-2129, :2252-2282).
+ * val: complex interleaved [re, im] per nonzero.
+ * ------------------------------------------------------------------------ */
+void om_assemble(const om_mesh *m, const int *rp, const int *ci, const double *cmua, const double *ckappa,
+                 const double *c2a, double omega, double *val)
+{
+    int nz = rp[m->n], nn = m->nn;
+    memset(val, 0, sizeof(double)*2*(size_t)nz);
+    for (int pass = 0; pass < 4; pass++)
+        for (int e = 0; e < m->elen; e++) {
+            const int *nd = m->el + e*nn;
+            for (int i = 0; i < nn; i++) for (int j = 0; j < nn; j++) {
+                int k = slot(rp, ci, nd[i], nd[j]);
+                switch (pass) {
+                case 0: val[2*k] += el_intpff(m, e, i, j, cmua); break;
+                case 1: val[2*k] += el_intpdd(m, e, i, j, ckappa); break;
+                case 2: val[2*k] += el_bndintpff(m, e, i, j, c2a); break;
+                case 3: val[2*k+1] += el_intff(m, e, i, j) * omega; break;
+                }
+            }
+        }
+}
+
+/* Mass matrix FF (ASSEMBLE_FF) used by closed-form checks */
+void om_assemble_ff(const om_mesh *m, const int *rp, const int *ci, double *val)
+{
+    memset(val, 0, sizeof(double)*(size_t)rp[m->n]);
+    for (int e = 0; e < m->elen; e++) {
+        const int *nd = m->el + e*m->nn;
+        for (int i = 0; i < m->nn; i++) for (int j = 0; j < m->nn; j++)
+            val[slot(rp, ci, nd[i], nd[j])] += el_intff(m, e, i, j);
+    }
+}
+
+/* Element-integral probes for closed-form tests */
+double om_el_intfff(const om_mesh *m, int e, int i, int j, int k) { return el_intfff(m, e, i, j, k); }
+double om_el_intfdd(const om_mesh *m, int e, int i, int j, int k) { return el_intfdd(m, e, i, j, k); }
+double om_el_intpff(const om_mesh *m, int e, int i, int j, const double *P) { return el_intpff(m, e, i, j, P); }
+double om_el_bndintpff(const om_mesh *m, int e, int i, int j, const double *P) { return el_bndintpff(m, e, i, j, P); }
+
+/* A_Keijzer (param.cc:24-32) */
+double om_a_keijzer(double n)
+{
+    double th = asin(1.0/n);
+    double costh = fabs(cos(th));
+    double R0 = ((n-1.0)*(n-1.0)) / ((n+1.0)*(n+1.0));
+    return (2.0/(1.0-R0) - 1.0 + costh*costh*costh) / (1.0 - costh*costh);
+}
+
+/* Solution coefficients (toastmodule.cc:1308-1318, fwdfem.cc:737-742) */
+void om_coeffs(int n, const double *mua, const double *mus, const double *ref, double *cmua, double *ckap, double *c2a)
+{
+    for (int i = 0; i < n; i++) {
+        cmua[i] = mua[i]*OM_C0/ref[i];
+        ckap[i] = OM_C0/(3.0*ref[i]*(mua[i]+mus[i]));
+        c2a[i] = OM_C0/(2.0*ref[i]*om_a_keijzer(ref[i]));
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * QVec_Gaussian (source.cc:4-41).  neumann != 0: boundary source distributed
+ * with BndIntFF; otherwise isotropic volume source with IntFF.
+ * ------------------------------------------------------------------------ */
+void om_qvec_gaussian(const om_mesh *m, const double *cnt, double w, int neumann, double *q)
+{
+    int n = m->n, nn = m->nn;
+    double w2 = w*w;
+    double fac1 = 1.0 / (sqrt(2.0*OM_PI) * w);
+    if (m->dim == 3) fac1 /= (sqrt(2.0*OM_PI) * w);
+    if (!neumann) fac1 /= (sqrt(2.0*OM_PI) * w);
+    double fac2 = -0.5/w2;
+    memset(q, 0, sizeof(double)*(size_t)n);
+    for (int e = 0; e < m->elen; e++) {
+        const int *nd = m->el + e*nn;
+        for (int i = 0; i < nn; i++) {
+            int is = nd[i];
+            if (neumann && !m->bnd[is]) continue;
+            double d = pdist(cnt, m->x + (size_t)is*m->dim, m->dim);
+            double qq = exp(d*d*fac2) * fac1;
+            for (int j = 0; j < nn; j++) {
+                int js = nd[j];
+                if (neumann) { if (m->bnd[js]) q[js] += qq * m->intbff[(size_t)e*nn*nn + i*nn + j]; }
+                else q[js] += qq * el_intff(m, e, i, j);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * QM file (qmmesh.cc:319-466) with measurement pulling (InitM
+ * qmmesh.cc:105-145 -> Mesh::PullToBoundary mesh.cc:1052-1111).
+ * ------------------------------------------------------------------------ */
+typedef struct { int dim, nq, nm, nqm; double *Q, *M; int *lptr, *lidx; } om_qm;
+
+static void el_local(const om_mesh *m, int e, const double *g, double *loc)  /* tet4.cc:186-197, tri3.cc:134-144 */
+{
+    int nn = m->nn;
+    const double *a = m->ga + e*nn, *b = m->gb + e*nn, *c = m->gc + e*nn, *d = m->gd + e*nn;
+    if (m->dim == 3) {
+        double scale = 1.0/(a[0]+a[1]+a[2]+a[3]);
+        loc[0] = (a[1] + b[1]*g[0] + c[1]*g[1] + d[1]*g[2]) * scale;
+        loc[1] = (a[2] + b[2]*g[0] + c[2]*g[1] + d[2]*g[2]) * scale;
+        loc[2] = (a[3] + b[3]*g[0] + c[3]*g[1] + d[3]*g[2]) * scale;
+    } else {
+        double scale = 1.0/(a[0]+a[1]+a[2]);
+        loc[0] = (a[1] + b[1]*g[0] + c[1]*g[1]) * scale;
+        loc[1] = (a[2] + b[2]*g[0] + c[2]*g[1]) * scale;
+    }
+}
+
+static int el_lcontains(int dim, const double *l)   /* tet4.cc:268-280, tri3.cc:191-201 (pad = true) */
+{
+    const double EPS = 1e-8;
+    if (dim == 3) return l[0]+EPS >= 0.0 && l[1]+EPS >= 0.0 && l[2]+EPS >= 0.0 && l[0]+l[1]+l[2]-EPS <= 1.0;
+    return l[0]+EPS >= 0.0 && l[1]+EPS >= 0.0 && l[0]+l[1]-EPS <= 1.0;
+}
+
+static void el_global(const om_mesh *m, int e, const double *l, double *g)   /* element.cc:164-176 */
+{
+    double u[4];
+    if (m->dim == 3) { u[0] = 1.0-l[0]-l[1]-l[2]; u[1] = l[0]; u[2] = l[1]; u[3] = l[2]; }
+    else { u[0] = 1.0-l[0]-l[1]; u[1] = l[0]; u[2] = l[1]; }
+    for (int i = 0; i < m->dim; i++) {
+        g[i] = 0.0;
+        for (int j = 0; j < m->nn; j++) g[i] += u[j] * m->x[(size_t)m->el[e*m->nn+j]*m->dim + i];
+    }
+}
+
+static int pull_to_boundary(const om_mesh *m, const double *p, double *pshift)
+{
+    static const double lnm3[4][3] = {{0,0,-1},{0,-1,0},{-1,0,0},{0.577350269,0.577350269,0.577350269}};  /* tet4.cc:244-256 */
+    static const double lnm2[3][2] = {{0,-1},{0.7071067814,0.7071067814},{-1,0}};                          /* tri3.cc:165-176 */
+    int dim = m->dim, nn = m->nn, nmin = -1;
+    double dmin = 1e10;
+    for (int n = 0; n < m->n; n++) {
+        if (!m->bnd[n]) continue;
+        double d = pdist(p, m->x + (size_t)n*dim, dim);
+        if (d < dmin) { dmin = d; nmin = n; }
+    }
+    for (int e = 0; e < m->elen; e++) {
+        const int *nd = m->el + e*nn;
+        int has = 0;
+        for (int k = 0; k < nn; k++) if (nd[k] == nmin) has = 1;
+        if (!has) continue;
+        int bsd[4], nb = 0;
+        for (int s = 0; s < m->nsd; s++) {          /* Element::BndSideList (element.cc:338-348) */
+            int ok = 1;
+            for (int k = 0; k < m->nsn; k++) if (!m->bnd[nd[dim == 3 ? TET_SN[s][k] : TRI_SN[s][k]]]) { ok = 0; break; }
+            if (ok) bsd[nb++] = s;
+        }
+        if (!nb) continue;
+        double ploc[3], nloc[3], pnew[3];
+        el_local(m, e, p, ploc);
+        for (int t = 0; t < nb; t++) {
+            int s = bsd[t];
+            const double *lnm = (dim == 3) ? lnm3[s] : lnm2[s];
+            int n0 = nd[dim == 3 ? TET_SN[s][0] : TRI_SN[s][0]];
+            el_local(m, e, m->x + (size_t)n0*dim, nloc);
+            double pd = 0.0;
+            for (int k = 0; k < dim; k++) pd += (ploc[k]-nloc[k]) * lnm[k];
+            for (int k = 0; k < dim; k++) pnew[k] = ploc[k] - lnm[k]*pd;
+            if (el_lcontains(dim, pnew)) { el_global(m, e, pnew, pshift); return 1; }
+        }
+    }
+    return 0;   /* PullToBoundary_old fallback is not restated */
+}
+
+om_qm *om_qm_load(const om_mesh *m, const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return NULL;
+    char line[4096], flag[128];
+    om_qm *qm = (om_qm*)xcalloc(1, sizeof(om_qm));
+    if (!fgets(line, sizeof line, f) || strncmp(line, "QM file", 7)) { fclose(f); return NULL; }
+    if (!fgets(line, sizeof line, f)) { fclose(f); return NULL; }
+    qm->dim = atoi(line+9);
+    int fixedq = 0, fixedm = 0;
+    while (fgets(line, sizeof line, f)) if (!strncmp(line, "SourceList", 10)) break;
+    flag[0] = 0; int ni = sscanf(line+10, "%d%127s", &qm->nq, flag);
+    fixedq = (ni > 1 && !strcasecmp(flag, "fixed")); (void)fixedq;
+    qm->Q = (double*)xcalloc((size_t)qm->nq*qm->dim, sizeof(double));
+    for (int i = 0; i < qm->nq; i++) {
+        if (!fgets(line, sizeof line, f)) { fclose(f); return NULL; }
+        char *p = line, *e;
+        for (int d = 0; d < qm->dim; d++) { qm->Q[i*qm->dim+d] = strtod(p, &e); p = e; }
+    }
+    while (fgets(line, sizeof line, f)) if (!strncmp(line, "MeasurementList", 15)) break;
+    flag[0] = 0; ni = sscanf(line+15, "%d%127s", &qm->nm, flag);
+    fixedm = (ni > 1 && !strcasecmp(flag, "fixed"));
+    qm->M = (double*)xcalloc((size_t)qm->nm*qm->dim, sizeof(double));
+    for (int i = 0; i < qm->nm; i++) {
+        if (!fgets(line, sizeof line, f)) { fclose(f); return NULL; }
+        char *p = line, *e;
+        for (int d = 0; d < qm->dim; d++) { qm->M[i*qm->dim+d] = strtod(p, &e); p = e; }
+    }
+    while (fgets(line, sizeof line, f)) if (!strncmp(line, "LinkList", 8)) break;
+    qm->lptr = (int*)xcalloc(qm->nq+1, sizeof(int));
+    int cap = qm->nq*qm->nm + 1;
+    qm->lidx = (int*)xcalloc(cap, sizeof(int));
+    for (int i = 0; i < qm->nq; i++) {
+        int cnt; if (fscanf(f, "%d", &cnt) != 1) { fclose(f); return NULL; }
+        int c; do { c = fgetc(f); } while (c != ':' && c != EOF);
+        qm->lptr[i+1] = qm->lptr[i] + cnt;
+        for (int j = 0; j < cnt; j++) if (fscanf(f, "%d", &qm->lidx[qm->lptr[i]+j]) != 1) { fclose(f); return NULL; }
+    }
+    qm->nqm = qm->lptr[qm->nq];
+    fclose(f);
+    if (!fixedm) {
+        for (int i = 0; i < qm->nm; i++) {
+            double ps[3];
+            if (pull_to_boundary(m, qm->M + i*qm->dim, ps)) memcpy(qm->M + i*qm->dim, ps, sizeof(double)*qm->dim);
+        }
+    }
+    return qm;
+}
+
+int om_qm_nq(const om_qm *q) { return q->nq; }
+int om_qm_nm(const om_qm *q) { return q->nm; }
+int om_qm_nqm(const om_qm *q) { return q->nqm; }
+void om_qm_get(const om_qm *q, double *Q, double *M, int *lptr, int *lidx)
+{
+    memcpy(Q, q->Q, sizeof(double)*(size_t)q->nq*q->dim);
+    memcpy(M, q->M, sizeof(double)*(size_t)q->nm*q->dim);
+    memcpy(lptr, q->lptr, sizeof(int)*(size_t)(q->nq+1));
+    memcpy(lidx, q->lidx, sizeof(int)*(size_t)q->nqm);
+}
+void om_qm_free(om_qm *q) { if (!q) return; free(q->Q); free(q->M); free(q->lptr); free(q->lidx); free(q); }
+
+/* Pull one point (exposed for tests) */
+int om_pull_to_boundary(const om_mesh *m, const double *p, double *out) { return pull_to_boundary(m, p, out); }
+
+/* ------------------------------------------------------------------------
+ * Krylov solvers.  CSR SpMV (crmatrix_imp.hpp:1173-1191); diagonal
+ * preconditioner (precon_imp.hpp:56-80: 1/diag, 1 where diag == 0).
+ * ------------------------------------------------------------------------ */
+static void spmv_r(int n, const int *rp, const int *ci, const double *v, const double *x, double *y)
+{
+    for (int r = 0; r < n; r++) {
+        double br = 0.0;
+        for (int i = rp[r]; i < rp[r+1]; i++) br += v[i] * x[ci[i]];
+        y[r] = br;
+    }
+}
+
+static double dot_r(int n, const double *a, const double *b) { double s = 0.0; for (int i = 0; i < n; i++) s += a[i]*b[i]; return s; }
+
+/* PCG (matrix_imp.hpp:93-154); val is real (one double per nonzero) */
+int om_pcg(int n, const int *rp, const int *ci, const double *val, const double *b, double *x,
+           double tol, int maxit, int diag, double *relerr)
+{
+    double *r = (double*)xcalloc(n, 8), *d = (double*)xcalloc(n, 8), *q = (double*)xcalloc(n, 8), *id = (double*)xcalloc(n, 8);
+    for (int i = 0; i < n; i++) {
+        double dg = 0.0;
+        for (int k = rp[i]; k < rp[i+1]; k++) if (ci[k] == i) dg = val[k];
+        id[i] = dg != 0.0 ? 1.0/dg : 1.0;
+    }
+    if (!maxit) maxit = n+1;
+    spmv_r(n, rp, ci, val, x, r);
+    for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
+    for (int i = 0; i < n; i++) d[i] = diag ? r[i]*id[i] : r[i];
+    double dnew = dot_r(n, r, d), dold, alpha, beta, err = 0.0;
+    double bnorm = sqrt(dot_r(n, b, b));
+    int it;
+    for (it = 0; it < maxit; it++) {
+        spmv_r(n, rp, ci, val, d, q);
+        alpha = dnew / dot_r(n, d, q);
+        if (alpha == 0.0 || isnan(alpha)) break;
+        for (int i = 0; i < n; i++) x[i] += d[i]*alpha;
+        for (int i = 0; i < n; i++) r[i] -= q[i]*alpha;
+        dold = dnew;
+        if (diag) {
+            for (int i = 0; i < n; i++) q[i] = r[i]*id[i];
+            dnew = dot_r(n, r, q);
+            beta = dnew/dold;
+            for (int i = 0; i < n; i++) d[i] = d[i]*beta + q[i];
+        } else {
+            dnew = dot_r(n, r, r);
+            beta = dnew/dold;
+            for (int i = 0; i < n; i++) d[i] = d[i]*beta + r[i];
+        }
+        if (beta == 0.0 || isnan(beta)) break;
+        err = sqrt(dot_r(n, r, r));
+        if (err <= tol*bnorm) break;
+    }
+    if (relerr) *relerr = err/bnorm;
+    free(r); free(d); free(q); free(id);
+    return it;
+}
+
+/* complex helpers on interleaved arrays */
+static void spmv_c(int n, const int *rp, const int *ci, const double *v, const double *x, double *y)
+{
+    for (int r = 0; r < n; r++) {
+        double br = 0.0, bi = 0.0;
+        for (int i = rp[r]; i < rp[r+1]; i++) {
+            double vr = v[2*i], vi = v[2*i+1], xr = x[2*ci[i]], xi = x[2*ci[i]+1];
+            br += vr*xr - vi*xi; bi += vr*xi + vi*xr;
+        }
+        y[2*r] = br; y[2*r+1] = bi;
+    }
+}
+static double dotR_c(int n, const double *a, const double *b) { double s = 0.0; for (int i = 0; i < n; i++) s += a[2*i]*b[2*i] + a[2*i+1]*b[2*i+1]; return s; }
+static void cmul(const double *a, const double *b, double *o) { double r = a[0]*b[0]-a[1]*b[1], i = a[0]*b[1]+a[1]*b[0]; o[0] = r; o[1] = i; }
+
+/* BiCGSTAB<complex<double>> (matrix_imp.hpp:350-413): real inner products */
+int om_bicgstab_c(int n, const int *rp, const int *ci, const double *val, const double *b, double *x,
+                  double tol, int maxit, int diag, double *relerr)
+{
+    size_t N = 2*(size_t)n*sizeof(double);
+    double *r = (double*)xcalloc(2*n, 8), *rd = (double*)xcalloc(2*n, 8), *p = (double*)xcalloc(2*n, 8), *pd = (double*)xcalloc(2*n, 8);
+    double *v = (double*)xcalloc(2*n, 8), *s = (double*)xcalloc(2*n, 8), *sd = (double*)xcalloc(2*n, 8), *t = (double*)xcalloc(2*n, 8);
+    double *id = (double*)xcalloc(2*n, 8);
+    for (int i = 0; i < n; i++) {
+        double dr = 0.0, di = 0.0;
+        for (int k = rp[i]; k < rp[i+1]; k++) if (ci[k] == i) { dr = val[2*k]; di = val[2*k+1]; }
+        if (dr != 0.0 || di != 0.0) { double den = dr*dr + di*di; id[2*i] = dr/den; id[2*i+1] = -di/den; }
+        else { id[2*i] = 1.0; id[2*i+1] = 0.0; }
+    }
+    if (!maxit) maxit = n+1;
+    spmv_c(n, rp, ci, val, x, r);
+    for (int i = 0; i < 2*n; i++) r[i] = b[i] - r[i];
+    memcpy(rd, r, N);
+    double bnorm = sqrt(dotR_c(n, b, b));
+    double rho = 0.0, rhop, alpha = 0.0, beta, omega = 0.0, aden, onum, oden, err = 0.0;
+    int it;
+    for (it = 0; it < maxit; it++) {
+        rhop = rho;
+        rho = dotR_c(n, rd, r);
+        if (rho == 0.0) break;
+        if (!it) memcpy(p, r, N);
+        else {
+            beta = (rho*alpha)/(rhop*omega);
+            for (int i = 0; i < 2*n; i++) p[i] = r[i] + (p[i]-v[i]*omega)*beta;
+        }
+        if (diag) for (int i = 0; i < n; i++) cmul(p+2*i, id+2*i, pd+2*i); else memcpy(pd, p, N);
+        spmv_c(n, rp, ci, val, pd, v);
+        aden = dotR_c(n, rd, v);
+        alpha = rho/aden;
+        for (int i = 0; i < 2*n; i++) s[i] = r[i] - v[i]*alpha;
+        if (diag) for (int i = 0; i < n; i++) cmul(s+2*i, id+2*i, sd+2*i); else memcpy(sd, s, N);
+        spmv_c(n, rp, ci, val, sd, t);
+        onum = dotR_c(n, t, s); oden = dotR_c(n, t, t);
+        omega = onum/oden;
+        for (int i = 0; i < 2*n; i++) { x[i] += pd[i]*alpha + sd[i]*omega; r[i] = s[i] - t[i]*omega; }
+        err = sqrt(dotR_c(n, r, r));
+        if (err <= tol*bnorm) break;
+        if (omega == 0.0) break;
+    }
+    if (relerr) *relerr = err/bnorm;
+    free(r); free(rd); free(p); free(pd); free(v); free(s); free(sd); free(t); free(id);
+    return it;
+}
+
+/* FP64 residual norm ||b - A x|| / ||b|| (complex interleaved) */
+double om_relres_c(int n, const int *rp, const int *ci, const double *val, const double *b, const double *x)
+{
+    double *y = (double*)xcalloc(2*n, 8);
+    spmv_c(n, rp, ci, val, x, y);
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < 2*n; i++) { double d = b[i]-y[i]; num += d*d; den += b[i]*b[i]; }
+    free(y);
+    return sqrt(num/den);
+}
+
+/* ------------------------------------------------------------------------
+ * ProjectAll (fwdsolver.cc:1415-1447): proj[Qofs[q]+i] = dot(phi_q, mvec_m)
+ * unconjugated (vector_imp.hpp:1045-1052), then complex log if logscale.
+ * phi: nq x n complex, mvec: nm x n complex (dense rows).
+ * ------------------------------------------------------------------------ */
+void om_project(int n, int nq, int nm, const double *phi, const double *mvec, const int *lptr, const int *lidx,
+                int logscale, double *proj)
+{
+    (void)nm;
+    for (int q = 0; q < nq; q++)
+        for (int k = lptr[q]; k < lptr[q+1]; k++) {
+            int mi = lidx[k];
+            double sr = 0.0, si = 0.0;
+            const double *f = phi + 2*(size_t)q*n, *g = mvec + 2*(size_t)mi*n;
+            for (int j = 0; j < n; j++) { sr += f[2*j]*g[2*j] - f[2*j+1]*g[2*j+1]; si += f[2*j]*g[2*j+1] + f[2*j+1]*g[2*j]; }
+            if (logscale) { double c = log(hypot(sr, si)), a = atan2(si, sr); sr = c; si = a; }
+            proj[2*k] = sr; proj[2*k+1] = si;
+        }
+}
+
+/* ------------------------------------------------------------------------
+ * Nodal Jacobian (jacobian.cc:441-497 with IntFG :806-835 and
+ * IntGradFGradG :839-872).  dphi: nq x n, aphi: nm x n (complex), proj: nQM
+ * complex linear projections; logscale divides by proj.  J: 2nQM x 2n
+ * row-major.  Rows: q ascending, linked m ascending in link-list order.
+ * ------------------------------------------------------------------------ */
+void om_jacobian_nodal(const om_mesh *m, int nq, int nm, const double *dphi, const double *aphi,
+                       const int *lptr, const int *lidx, const double *proj, int logscale, double *J)
+{
+    (void)nm;
+    int n = m->n, nn = m->nn, nqm = lptr[nq];
+    double *pm = (double*)xcalloc(2*(size_t)n, 8), *pk = (double*)xcalloc(2*(size_t)n, 8);
+    size_t rowlen = 2*(size_t)n;
+    for (int q = 0; q < nq; q++)
+        for (int k = lptr[q]; k < lptr[q+1]; k++) {
+            const double *f = dphi + 2*(size_t)q*n, *g = aphi + 2*(size_t)lidx[k]*n;
+            memset(pm, 0, 16*(size_t)n); memset(pk, 0, 16*(size_t)n);
+            for (int e = 0; e < m->elen; e++) {
+                const int *nd = m->el + e*nn;
+                for (int i = 0; i < nn; i++) {
+                    int bs = nd[i];
+                    for (int j = 0; j < nn; j++) {
+                        int nj = nd[j];
+                        double fg[2], t[2], sm[2], sk[2];
+                        cmul(f+2*nj, g+2*nj, fg);
+                        double w = el_intfff(m, e, i, j, j), wd = el_intfdd(m, e, i, j, j);
+                        sm[0] = fg[0]*w; sm[1] = fg[1]*w; sk[0] = fg[0]*wd; sk[1] = fg[1]*wd;
+                        for (int kk = 0; kk < j; kk++) {
+                            int nk = nd[kk];
+                            double a1[2], a2[2];
+                            cmul(f+2*nj, g+2*nk, a1); cmul(f+2*nk, g+2*nj, a2);
+                            t[0] = a1[0]+a2[0]; t[1] = a1[1]+a2[1];
+                            w = el_intfff(m, e, i, j, kk); wd = el_intfdd(m, e, i, j, kk);
+                            sm[0] += t[0]*w; sm[1] += t[1]*w; sk[0] += t[0]*wd; sk[1] += t[1]*wd;
+                        }
+                        pm[2*bs] += sm[0]; pm[2*bs+1] += sm[1];
+                        pk[2*bs] += sk[0]; pk[2*bs+1] += sk[1];
+                    }
+                }
+            }
+            double pr = proj[2*k], pi = proj[2*k+1], den = pr*pr + pi*pi;
+            double *R0 = J + (size_t)k*rowlen, *R1 = J + (size_t)(k+nqm)*rowlen;
+            for (int i = 0; i < n; i++) {
+                double a[2] = {pm[2*i], pm[2*i+1]}, b[2] = {pk[2*i], pk[2*i+1]};
+                if (logscale) {   /* complex division by proj */
+                    double ar = (a[0]*pr + a[1]*pi)/den, ai = (a[1]*pr - a[0]*pi)/den;
+                    double br = (b[0]*pr + b[1]*pi)/den, bi = (b[1]*pr - b[0]*pi)/den;
+                    a[0] = ar; a[1] = ai; b[0] = br; b[1] = bi;
+                }
+                R0[i] = -a[0]; R1[i] = -a[1]; R0[i+n] = -b[0]; R1[i+n] = -b[1];
+            }
+        }
+    free(pm); free(pk);
+}
