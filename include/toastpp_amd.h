@@ -1,0 +1,218 @@
+/*
+ * toastpp_amd.h -- C-ABI of the MI355X diffusion forward/adjoint solve,
+ * projection and Jacobian library (libtoastpp_amd.so).
+ *
+ * Plain C types only: pointers, sizes, int status codes.  Complex arrays are
+ * interleaved (re, im) doubles.  "Host" arrays are ordinary CPU memory; the
+ * *_device entry points take pointers to GPU memory (hipMalloc or a torch
+ * tensor's data_ptr) on the library's device.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * toastpp/toastpp).  Unlike the reference (xERROR -> exit(1)), no call ever
+ * terminates the process: errors come back as status codes and
+ * tmi_last_error() gives the message of the calling thread's last failure.
+ *
+ * Thread safety: distinct handles may be used from distinct threads; a
+ * single handle must not be used concurrently.
+ */
+#ifndef TOASTPP_AMD_H
+#define TOASTPP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TMI_OK                 0
+#define TMI_ERR_INVALID_ARG    1
+#define TMI_ERR_NOT_CONVERGED  2   /* solve finished, some column above tol */
+#define TMI_ERR_OOM            3
+#define TMI_ERR_DEVICE         4
+#define TMI_ERR_IO             5
+#define TMI_ERR_STATE          6   /* call out of order (e.g. CalcFields before Reset) */
+#define TMI_ERR_UNSUPPORTED    7
+
+/* element type ids (libfe/element.h:30-49) */
+#define TMI_ELID_TET4  3
+#define TMI_ELID_TRI3  15
+
+/* linear solvers (TFwdSolver LSOLVER / ITMETHOD, fwdsolver.h, gsmatrix.h:32-38) */
+#define TMI_LSOLVER_DIRECT    0   /* FP64-accurate solve: BiCGSTAB/PCG to tol 1e-14 on the GPU */
+#define TMI_LSOLVER_CG        1
+#define TMI_LSOLVER_BICGSTAB  2
+
+/* preconditioners (PreconType, precon.h) */
+#define TMI_PRECON_NONE  0
+#define TMI_PRECON_DIAG  1
+
+/* precision / batching mode of CalcFields (no reference counterpart) */
+#define TMI_MODE_FP64_SINGLE  0   /* FP64 Krylov, one right-hand side at a time */
+#define TMI_MODE_MIXED_BLOCK  1   /* complex-FP32 block Krylov inside FP64 iterative refinement */
+#define TMI_MODE_FP64_BLOCK   2   /* FP64 block Krylov (all columns in one block) */
+
+/* data scaling (DataScale, fwdsolver.h) */
+#define TMI_DATA_LIN  0
+#define TMI_DATA_LOG  1
+
+typedef struct tmi_mesh_s   *tmi_mesh;
+typedef struct tmi_solver_s *tmi_solver;
+typedef struct tmi_raster_s *tmi_raster;
+
+/* per right-hand-side convergence record (IterativeSolverResult, matrix.h:34-37,
+ * extended with refinement sweeps and a converged flag) */
+typedef struct {
+    int    iters;      /* Krylov iterations spent on this column (all sweeps) */
+    int    sweeps;     /* FP64 refinement sweeps (1 for FP64 modes) */
+    double relres;     /* final FP64 ||q - K phi|| / ||q|| */
+    int    converged;  /* relres <= tol */
+} tmi_colstat;
+
+/* ---------------------------------------------------------------- library */
+const char *tmi_last_error(void);
+/* Device name and HBM capacity of the library's GPU (device 0). */
+int tmi_device_info(char *name, int name_len, double *hbm_gib);
+/* Wait for all queued GPU work of the library. */
+int tmi_synchronize(void);
+/* Device buffers for the *_device entry points (no reference counterpart;
+ * plumbing for callers without their own allocator). */
+int tmi_device_alloc(unsigned long long bytes, void **ptr);
+int tmi_device_free(void *ptr);
+int tmi_memcpy_h2d(void *dst, const void *src, unsigned long long bytes);
+int tmi_memcpy_d2h(void *dst, const void *src, unsigned long long bytes);
+
+/* ------------------------------------------------------------------- mesh */
+/* Host-only parser of a "MeshData 5.0" file (Mesh operator>>, mesh.cc:1787-1825,
+ * ndlist.cc:146-159, node.cc:130-162, ellist.cc:154-190).  Call once with
+ * nodes == NULL to get the sizes, then again with buffers of those sizes.
+ * Only Tri3 and Tet4 elements are accepted (TMI_ERR_UNSUPPORTED otherwise). */
+int tmi_read_mesh_file(const char *path, int *dim, int *nlen, int *elen, int *nnode,
+                       double *nodes, int *elist, int *node_region);
+
+/* Create a device mesh and run Mesh::Setup on the GPU (mesh.cc:96-158):
+ * element geometry, MarkBoundary (mesh.cc:1614-1703), node-to-element
+ * adjacency.  nodes: nlen x dim, elist: elen x (dim+1), 0-based.
+ * Replaces toast.Mesh.Make (src/python/toast/mesh.py:71-89). */
+int tmi_mesh_make(int dim, int nlen, const double *nodes, int elen, const int *elist, tmi_mesh *out);
+/* tmi_read_mesh_file + tmi_mesh_make (toast.Mesh.Read, mesh.py:46-59) */
+int tmi_mesh_read(const char *path, tmi_mesh *out);
+int tmi_mesh_free(tmi_mesh m);
+int tmi_mesh_dims(tmi_mesh m, int *dim, int *nlen, int *elen, int *nbnd);
+int tmi_mesh_get_nodes(tmi_mesh m, double *nodes);
+int tmi_mesh_get_elements(tmi_mesh m, int *elist);
+/* node boundary flags after MarkBoundary (1 = boundary) */
+int tmi_mesh_get_bnd(tmi_mesh m, int *bnd);
+/* signed element sizes (Element_Unstructured::size, element.cc:516-530) */
+int tmi_mesh_element_size(tmi_mesh m, double *size);
+/* Mesh::SparseRowStructure (mesh.cc:422-549), built on the GPU.  With
+ * rowptr == NULL only *nnz is returned. */
+int tmi_mesh_sparse_structure(tmi_mesh m, int *rowptr, int *colidx, int *nnz);
+
+/* QM description: QMMesh::LoadQM (qmmesh.cc:319-466) including the pulling of
+ * non-fixed measurement positions onto the boundary (qmmesh.cc:105-145,
+ * mesh.cc:1052-1111). */
+int tmi_mesh_read_qm(tmi_mesh m, const char *path);
+/* Same from arrays: Q nq x dim, M nm x dim, link list in CSR form
+ * (lptr nq+1, lidx nqm).  pull_m != 0 pulls M onto the boundary. */
+int tmi_mesh_set_qm(tmi_mesh m, int nq, const double *Q, int nm, const double *M,
+                    const int *lptr, const int *lidx, int pull_m);
+int tmi_mesh_qm_dims(tmi_mesh m, int *nq, int *nm, int *nqm);
+int tmi_mesh_get_qm(tmi_mesh m, double *Q, double *M, int *lptr, int *lidx);
+
+/* Gaussian source/measurement profiles: QVec_Gaussian (source.cc:4-41) for
+ * npts centres, evaluated on the GPU.  neumann != 0: boundary source
+ * (SRCMODE_NEUMANN); else isotropic volume source.  out: npts x nlen
+ * (dense, real).  scale (nlen values or NULL) multiplies each profile
+ * nodally, as Mvec's c2a factor (fwdfem.cc:151-176, toastmodule.cc:1189-1220). */
+int tmi_qvec_gaussian(tmi_mesh m, int npts, const double *centres, double width, int neumann,
+                      const double *scale, double *out);
+
+/* Optical coefficients -> FEM coefficients: cmua = c mua / n,
+ * ckappa = c / (3 n (mua + mus')), c2a = c / (2 n A_Keijzer(n)), c = 0.3 mm/ps
+ * (toastmodule.cc:1308-1318, param.cc:24-32).  Host arrays of length nlen. */
+int tmi_optical_to_coeffs(int nlen, const double *mua, const double *mus, const double *ref,
+                          double *cmua, double *ckappa, double *c2a);
+
+/* --------------------------------------------------------- forward solver */
+/* TFwdSolver<T> (fwdsolver.h:77-508, fwdsolver.cc:17-1299).
+ * complex_valued = 0: TFwdSolver<double>, real CW system (omega must be 0);
+ * complex_valued = 1: TFwdSolver<std::complex<double>>.
+ * The mesh is borrowed and must outlive the solver. */
+int tmi_solver_create(tmi_mesh m, int complex_valued, int lsolver, double tol, tmi_solver *out);
+int tmi_solver_free(tmi_solver s);
+/* SetLinSolver("DIRECT"|"CG"|"BICGSTAB", tol) (fwdsolver.cc:1279-1299).
+ * CG on a complex solver is rejected (the reference's complex CG is a stub
+ * that returns the initial guess, gsmatrix_imp.hpp:289-296). */
+int tmi_solver_set_linsolver(tmi_solver s, const char *name, double tol);
+int tmi_solver_set_precon(tmi_solver s, int precon);            /* SetPrecon */
+int tmi_solver_set_maxit(tmi_solver s, int maxit);              /* SetLinSolverMaxit; 0 = n+1 */
+int tmi_solver_set_mode(tmi_solver s, int mode);                /* TMI_MODE_* */
+/* Column tile of the block modes (RHS columns per Krylov block; 0 = auto) */
+int tmi_solver_set_tile(tmi_solver s, int tile);
+/* Inner relative tolerance of the FP32 Krylov solve of the mixed mode */
+int tmi_solver_set_inner_tol(tmi_solver s, double inner_tol);
+/* Allocate (fwdsolver.cc:193-261): CSR pattern of the system matrix on the GPU */
+int tmi_solver_allocate(tmi_solver s);
+/* AssembleSystemMatrix (fwdsolver.cc:338-357, :405-424): K = PFF(cmua) +
+ * PDD(ckappa) + BNDPFF(c2a) [+ i omega FF] on the GPU.  Host arrays (nlen). */
+int tmi_solver_assemble(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a, double omega);
+/* Reset (fwdsolver.cc:494-505): assemble + preconditioner setup */
+int tmi_solver_reset(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a, double omega);
+int tmi_solver_nnz(tmi_solver s, int *nnz);
+/* Copy the FP64 system matrix back (public member F, toastmodule.cc:1069-1077).
+ * val: nnz complex (interleaved) for complex solvers, nnz real otherwise. */
+int tmi_solver_get_sysmat(tmi_solver s, int *rowptr, int *colidx, double *val);
+
+/* CalcFields (fwdsolver.cc:653-810).  qvec: nq x nlen CSR (row q = source q),
+ * values complex interleaved (complex solver) or real.  phi: nq x nlen
+ * (complex interleaved or real), input = initial guess (the reference's
+ * callers pass zeros), output = fields.  stats: nq records (may be NULL).
+ * Returns TMI_ERR_NOT_CONVERGED if a column ended above tol. */
+int tmi_solver_calc_fields(tmi_solver s, int nq, const int *qrowptr, const int *qcolidx,
+                           const double *qval, double *phi, tmi_colstat *stats);
+/* CalcFields on GPU-resident data.  d_rhs, d_phi: nlen x nq node-major
+ * (element [j, q] at j*nq + q), complex128 or float64.  d_phi is overwritten
+ * (zero initial guess).  Nothing is copied to the host except the stats. */
+int tmi_solver_calc_fields_device(tmi_solver s, int nq, const void *d_rhs, void *d_phi, tmi_colstat *stats);
+/* ProjectAll (fwdsolver.cc:1433-1447, :1415-1429): proj[Qofs[q]+i] =
+ * sum_j phi_q[j] mvec[QMref[q][i]][j] (unconjugated), log if
+ * dscale == TMI_DATA_LOG.  Uses the mesh's QM link list.  mvec: nm x nlen CSR. */
+int tmi_solver_project_all(tmi_solver s, int nm, const int *mrowptr, const int *mcolidx,
+                           const double *mval, const double *phi, int dscale, double *proj);
+/* Last CalcFields: GPU time (ms, HIP events) and mean Krylov iterations per column */
+int tmi_solver_last_timing(tmi_solver s, double *solve_ms, double *iters_per_col);
+/* Measurement hook (no reference counterpart): time `reps` launches of one
+ * kernel of the last CalcFields tile with HIP events on the library stream.
+ * which = 0: the inner block SpMM, 1: the FP64 residual SpMM.  Returns the
+ * average launch time (ms) and the algorithmic HBM bytes of one launch. */
+int tmi_solver_time_kernel(tmi_solver s, int which, int reps, double *avg_ms, double *bytes);
+
+/* --------------------------------------------------------------- Jacobian */
+/* GenerateJacobian, nodal basis (jacobian.cc:109-126 with RASTER_NDBASIS ->
+ * GenerateJacobian_mesh :441-497, IntFG :806-835, IntGradFGradG :839-872).
+ * dphi: nq x nlen complex, aphi: nm x nlen complex, proj: nqm complex linear
+ * projections (link-list order).  J: (2 nqm) x (2 nlen) row-major doubles
+ * [dlnA/dmua | dlnA/dkappa ; dphase/dmua | dphase/dkappa] for
+ * dscale == TMI_DATA_LOG, [Re ; Im] of the linear PMDFs otherwise. */
+int tmi_jacobian_nodal(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi,
+                       const double *proj, int dscale, double *J);
+
+/* ----------------------------------------------------------------- raster */
+/* Raster_Pixel with bdim == gdim (raster.cc:18-100, raster_px.cc:10-54):
+ * element-pixel reference (mesh.cc:2944-3015) and the mesh->grid map B
+ * (GridMapMatrix, mesh.cc:2486-2554) over the mesh bounding box. */
+int tmi_raster_create(tmi_mesh m, const int *gdim, tmi_raster *out);
+int tmi_raster_free(tmi_raster r);
+/* glen = prod(gdim), slen = grid points inside the mesh */
+int tmi_raster_dims(tmi_raster r, int *glen, int *slen);
+/* element reference per grid point (-1 outside), sol2basis index list */
+int tmi_raster_get_elref(tmi_raster r, int *elref, int *sol2basis);
+/* Map_MeshToGrid (raster.cc:188-199), complex: out glen complex */
+int tmi_raster_map_mesh_to_grid(tmi_raster r, const double *mvec, double *gvec);
+/* GenerateJacobian_grid (jacobian.cc:293-373): J (2 nqm) x (2 slen) */
+int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi,
+                        const double *proj, int dscale, double *J);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOASTPP_AMD_H */

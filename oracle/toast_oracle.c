@@ -627,7 +627,51 @@ static int pull_to_boundary(const om_mesh *m, const double *p, double *pshift)
             if (el_lcontains(dim, pnew)) { el_global(m, e, pnew, pshift); return 1; }
         }
     }
-    return 0;   /* PullToBoundary_old fallback is not restated */
+    /* PullToBoundary_old (mesh.cc:1114-1172): project onto the plane of the
+     * boundary side through nmin whose centre is nearest to p */
+    {
+        const double EPS = 1e-10;
+        double wmax = 0.0;
+        int found = 0;
+        for (int e = 0; e < m->elen; e++) {
+            const int *nd = m->el + e*nn;
+            int has = 0;
+            for (int k = 0; k < nn; k++) if (nd[k] == nmin) has = 1;
+            if (!has) continue;
+            for (int s = 0; s < m->nsd; s++) {
+                int containsmin = 0, isbnd = 1;
+                for (int k = 0; k < m->nsn; k++) {
+                    int v = nd[dim == 3 ? TET_SN[s][k] : TRI_SN[s][k]];
+                    if (v == nmin) containsmin = 1;
+                    else if (!m->bnd[v]) { isbnd = 0; break; }
+                }
+                if (!containsmin || !isbnd) continue;
+                double gc[3] = {0, 0, 0}, dc[3];
+                for (int k = 0; k < m->nsn; k++)
+                    for (int d = 0; d < dim; d++) gc[d] += m->x[(size_t)nd[dim == 3 ? TET_SN[s][k] : TRI_SN[s][k]]*dim + d] / m->nsn;
+                /* outward direction cosine: minus the gradient direction of the
+                 * shape function of the node opposite the side (tri3.cc:155-163, tet4.cc:230-241) */
+                static const int opp3[4] = {3, 2, 1, 0}, opp2[3] = {2, 0, 1};
+                int o = dim == 3 ? opp3[s] : opp2[s];
+                dc[0] = -m->gb[e*nn + o]; dc[1] = -m->gc[e*nn + o]; dc[2] = dim == 3 ? -m->gd[e*nn + o] : 0.0;
+                double len = 0.0;
+                for (int d = 0; d < dim; d++) len += dc[d]*dc[d];
+                len = sqrt(len);
+                for (int d = 0; d < dim; d++) dc[d] /= len;
+                double dist = pdist(p, gc, dim);
+                if (dist < EPS) dist = EPS;
+                double w = 1.0/dist;
+                if (w > wmax) {
+                    wmax = w;
+                    double pd = 0.0;
+                    for (int d = 0; d < dim; d++) pd += (p[d] - gc[d]) * dc[d];
+                    for (int d = 0; d < dim; d++) pshift[d] = p[d] - dc[d]*pd;
+                    found = 1;
+                }
+            }
+        }
+        return found;
+    }
 }
 
 om_qm *om_qm_load(const om_mesh *m, const char *path)

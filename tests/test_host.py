@@ -1,0 +1,104 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point the header declares, the host mesh reader agrees with the
+oracle's, the workload generators produce valid meshes, and bench.py refuses
+multi-GPU runs."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "toastpp_amd.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(tmi_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    from toastpp_amd import _lib
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) > 40
+    missing = [s for s in names if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python binding covers every declared entry point
+    assert set(names) <= set(_lib.exported_symbols())
+
+
+def test_host_mesh_reader_matches_oracle():
+    import ctypes as C
+    from oracle import oracle as O
+    from toastpp_amd import _lib
+    for sub, name in (("ref2d", "circle25_32.msh"), ("ref3d", "cyl2.msh")):
+        path = os.path.join(ROOT, "tests", "golden", sub, name).encode()
+        d, n, e, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.call("tmi_read_mesh_file", path, C.byref(d), C.byref(n), C.byref(e), C.byref(nn), None, None, None)
+        x = np.zeros((n.value, d.value))
+        el = np.zeros((e.value, nn.value), np.int32)
+        reg = np.zeros(n.value, np.int32)
+        _lib.call("tmi_read_mesh_file", path, C.byref(d), C.byref(n), C.byref(e), C.byref(nn), _lib.ptr(x),
+                  _lib.ptr(el), _lib.ptr(reg))
+        om = O.Mesh(path.decode())
+        assert np.array_equal(x, om.nodes())
+        assert np.array_equal(el, om.elements())
+        assert np.array_equal(reg, om.regions())
+
+
+def test_host_mesh_reader_errors(tmp_path):
+    import ctypes as C
+    from toastpp_amd import _lib
+    bad = tmp_path / "bad.msh"
+    bad.write_text("not a mesh\n")
+    d = C.c_int()
+    with pytest.raises(_lib.TmiError) as ei:
+        _lib.call("tmi_read_mesh_file", str(bad).encode(), C.byref(d), None, None, None, None, None, None)
+    assert ei.value.code == _lib.TMI_ERR_IO
+    tri6 = tmp_path / "tri6.msh"
+    tri6.write_text("MeshData 5.0\n\nNodeList 3 1\nN[0 0]\nN[1 0]\nN[0 1]\n\nElementList 1\np 1 2 3 1 2 3\n")
+    with pytest.raises(_lib.TmiError) as ei:
+        _lib.call("tmi_read_mesh_file", str(tri6).encode(), C.byref(d), None, None, None, None, None, None)
+    assert ei.value.code == _lib.TMI_ERR_UNSUPPORTED
+
+
+def test_no_gpu_call_fails_cleanly():
+    """Without a GPU, device calls return an error code instead of exiting."""
+    import toastpp_amd as T
+    from toastpp_amd import _lib
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except ImportError:
+        pass
+    with pytest.raises(_lib.TmiError) as ei:
+        T.Mesh(os.path.join(ROOT, "tests", "golden", "ref2d", "circle25_32.msh"))
+    assert ei.value.code == _lib.TMI_ERR_DEVICE
+
+
+@pytest.mark.parametrize("name,nodes,elems", [("disk2d", 3169, 6144), ("cyl3d", 31525, 172800),
+                                              ("ellipsoid3d", 250047, 1429968), ("cyl3d_jac", 103267, 583200)])
+def test_workloads(name, nodes, elems):
+    from toastpp_amd import meshgen as G
+    x, el, Q, M, mua, mus, ref, f, desc = G.workload(name)
+    assert x.shape[0] == nodes and el.shape[0] == elems
+    if x.shape[1] == 2:
+        p0, p1, p2 = x[el[:, 0]], x[el[:, 1]], x[el[:, 2]]
+        vol = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
+    else:
+        p0, p1, p2, p3 = (x[el[:, k]] for k in range(4))
+        vol = np.einsum("ij,ij->i", np.cross(p1 - p0, p2 - p0), p3 - p0)
+    assert np.all(vol > 0)
+    assert np.all(mua >= 0.025) and np.all(mus >= 2.0) and np.all(ref == 1.4)
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

@@ -1,0 +1,15 @@
+"""toastpp_amd: MI355X (gfx950) implementation of Toast++'s diffusion forward /
+adjoint solve, boundary projection and Jacobian.
+
+The compute path is libtoastpp_amd.so (hand-written HIP kernels behind the
+C-ABI in include/toastpp_amd.h); this package is its Python surface:
+  Mesh       -- toast.Mesh (src/python/toast/mesh.py)
+  FwdSolver  -- libstoast TFwdSolver<T> (src/libstoast/fwdsolver.h)
+  Solution   -- optical -> FEM coefficients (cmua, ckappa, c2a)
+  Raster     -- toast.Raster with a Raster_Pixel basis (bdim == gdim)
+"""
+from ._lib import TmiError, NotConverged  # noqa: F401
+from .fwdsolver import FwdSolver, Solution  # noqa: F401
+from .mesh import Mesh  # noqa: F401
+
+__all__ = ["Mesh", "FwdSolver", "Solution", "TmiError", "NotConverged"]

@@ -1,0 +1,108 @@
+// Block Krylov kernels: device-side declarations shared by the solver driver.
+//
+// Vectors of a block of K right-hand sides are stored node-major: element
+// (row i, column c) at i*K + c, so one CSR row of the SpMM reads K
+// consecutive values of each neighbour row (one coalesced 64-lane load for
+// K = 64 complex-FP32 columns: 512 bytes).
+//
+// Per-column scalars live in device memory (struct of arrays below); the
+// host only reads the active-column count every few iterations.
+#pragma once
+#include "common.hpp"
+
+namespace tmi {
+
+// offsets into the per-column double scalar array (each block of K doubles)
+enum ScalarSlot {
+    S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BNORM2, S_RES2, S_TOL2, S_SCALE, S_AUX, S_RHOP, S_BEST, S_NSLOT
+};
+// offsets into the per-column int array
+enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_NSLOT };
+
+// partial-sum slots written by the reduction epilogues
+constexpr int RED_SLOTS = 2;
+
+struct BlockDesc {
+    int n = 0;          // rows
+    int K = 0;          // columns (tile width)
+    int nblk = 0;       // reduction blocks of the row-parallel kernels
+    const int *rowptr = nullptr, *colidx = nullptr;
+};
+
+// precision / value-type selector
+enum VKind { VK_C32 = 0, VK_C64 = 1, VK_R32 = 2, VK_R64 = 3 };
+
+inline size_t vbytes(VKind k) { return k == VK_C64 ? 16 : (k == VK_R32 ? 4 : 8); }
+
+// ---- launchers (krylov_kernels.hip) ----
+// Y = A * D^-1 X (D^-1 = idiag, or identity if null); partials
+// red[0] = <W, Y>_R and, when want_yy, red[1] = <Y, Y>_R, per column.
+void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
+                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st);
+// R = B - A X (FP64), red[0] = ||R||^2
+void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
+                  const int *iflags, double *partial, hipStream_t st);
+
+// BiCGSTAB element-wise steps (real inner products, real alpha/beta/omega)
+void launch_bicg_p(VKind vk, const BlockDesc &b, const void *r, void *p, const void *v, const double *sc,
+                   const int *ic, hipStream_t st);
+void launch_bicg_s(VKind vk, const BlockDesc &b, const void *r, const void *v, void *s, const double *sc,
+                   const int *ic, hipStream_t st);
+void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *p, const void *s,
+                    const void *t, void *r, const void *rd, const double *sc, const int *ic, double *partial,
+                    hipStream_t st);
+// PCG element-wise steps
+void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
+                  const double *sc, const int *ic, double *partial, hipStream_t st);
+void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
+                 const int *ic, hipStream_t st);
+// first PCG step: d = D^-1 r, partials [<r, D^-1 r>, <r, r>]
+void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
+                    const int *ic, double *partial, hipStream_t st);
+
+// scalar updates (one thread per column)
+enum FinKind {
+    FIN_BICG_INIT = 0,  // red: [<r,r>, <rd,r>] -> bnorm2, rho, res2
+    FIN_BICG_ALPHA,     // red: [<rd,v>] -> alpha
+    FIN_BICG_OMEGA,     // red: [<t,s>, <t,t>] -> omega
+    FIN_BICG_RHO,       // red: [||r||^2, <rd,r>] -> res2, beta, rho, convergence
+    FIN_CG_INIT,        // red: [<r,z>, <r,r>]
+    FIN_CG_ALPHA,       // red: [<d,q>]
+    FIN_CG_BETA,        // red: [<r,z>, ||r||^2]
+    FIN_OUTER_BNORM,    // red: [<b,b>] -> outer bnorm2, scale = 1/||b||
+    FIN_OUTER_RES,      // red: [||b - A x||^2] -> outer res2, converged / stalled columns leave
+    FIN_BICG_RELIABLE,  // red: [<r,r>, <rd,r>] of the replaced residual -> res2, rho, beta
+    FIN_CG_RELIABLE,    // red: [<r,r>, <r,D^-1 r>] of the replaced residual -> res2, rho
+};
+// maxit: iteration cap of the Krylov loop (0 = none); tol2: squared outer
+// relative tolerance; inner_tol2: squared inner tolerance of the mixed mode
+// (<= 0: the inner solve reproduces the outer criterion, FP64 modes)
+// osc/oic: the outer (refinement) state, read by the *_RELIABLE kinds
+void launch_finalize(FinKind fk, const BlockDesc &b, const double *partial, double *sc, int *ic, int maxit,
+                     double tol2, double inner_tol2, hipStream_t st, const double *osc = nullptr,
+                     const int *oic = nullptr);
+// copy the outer active flags into the inner state and reset its counters
+void launch_inner_reset(const BlockDesc &b, const double *osc, const int *oic, double *sc, int *ic, hipStream_t st);
+
+// mixed-precision refinement helpers
+// r32 = r64 * scale[c]; rd32 = r32; x32 = 0 ; partial [<r32,r32>, <rd32,r32>]
+void launch_to_inner(VKind vk32, const BlockDesc &b, const void *r64, void *r32, void *rd32, void *x32,
+                     const double *scale, const int *oflags, double *partial, hipStream_t st);
+// reliable update of the inner residual: r_in = r64 * scale, x_in = 0,
+// partials [<r_in, r_in>, <rd, r_in>] (rd given) or [<r_in,r_in>, <r_in, D^-1 r_in>]
+void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
+                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st);
+// x64 += x32 / scale[c] for outer-active columns
+void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x32, const double *scale,
+                       const int *oflags, hipStream_t st);
+// per-column squared norm of X: partial [<X,X>, <X,X>]
+void launch_norm2(VKind vk, const BlockDesc &b, const void *X, const int *flags, double *partial, hipStream_t st);
+// inverse diagonal (precon_imp.hpp:56-65): idiag[i] = 1/K[i,i] (1 where 0)
+void launch_inv_diag(VKind vk, int n, const int *diagpos, const double *val64, void *idiag, hipStream_t st);
+// A D^-1: out64[k] = val64[k] * idiag[colidx[k]] (and an FP32 copy in out32)
+void launch_scale_cols(bool cplx, int nnz, const int *colidx, const double *val64, const void *idiag64, double *out64,
+                       void *out32, hipStream_t st);
+// copy with conversion FP64 -> FP32 (values)
+void launch_f64_to_f32(const double *src, float *dst, size_t count, hipStream_t st);
+
+}  // namespace tmi

@@ -1,0 +1,981 @@
+// Block Krylov kernels for gfx950.
+//
+// Layout: block vectors are node-major [n][K] (K = columns of the tile, a
+// power of two <= 64).  The SpMM maps one wavefront lane to one column and
+// 64/K rows to a wavefront, so each nonzero of a CSR row costs one coalesced
+// load of K neighbour values; matrix values and column indices are wave
+// uniform for K = 64 (readfirstlane -> scalar loads).  For a single column
+// (K = 1) a second SpMV maps LPR lanes to the nonzeros of one row.
+//
+// Every row-parallel kernel owns a contiguous row range per workgroup and
+// remaps workgroup ids so that each XCD (workgroups are dealt round-robin
+// over the 8 XCDs) walks one contiguous slice of the mesh: the neighbour
+// rows gathered by the SpMM then sit in that XCD's L2.
+//
+// Reductions (inner products per column) are accumulated in FP64 per lane,
+// combined across lanes, waves and workgroups in a fixed order: results are
+// run-to-run deterministic.
+#include <hip/hip_runtime.h>
+
+#include "krylov.hpp"
+
+namespace tmi {
+
+// ------------------------------------------------------------ value types
+__device__ inline float2 vadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 vadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline float vadd(float a, float b) { return a + b; }
+__device__ inline double vadd(double a, double b) { return a + b; }
+__device__ inline float2 vsub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 vsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline float vsub(float a, float b) { return a - b; }
+__device__ inline double vsub(double a, double b) { return a - b; }
+__device__ inline float2 vmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 vmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float vmul(float a, float b) { return a * b; }
+__device__ inline double vmul(double a, double b) { return a * b; }
+__device__ inline float2 vscale(float2 a, double s) { float f = (float)s; return make_float2(a.x * f, a.y * f); }
+__device__ inline double2 vscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ inline float vscale(float a, double s) { return a * (float)s; }
+__device__ inline double vscale(double a, double s) { return a * s; }
+__device__ inline double dotR(float2 a, float2 b) { return (double)a.x * (double)b.x + (double)a.y * (double)b.y; }
+__device__ inline double dotR(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+__device__ inline double dotR(float a, float b) { return (double)a * (double)b; }
+__device__ inline double dotR(double a, double b) { return a * b; }
+template <class V> __device__ inline V vzero();
+template <> __device__ inline float2 vzero<float2>() { return make_float2(0.f, 0.f); }
+template <> __device__ inline double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ inline float vzero<float>() { return 0.f; }
+template <> __device__ inline double vzero<double>() { return 0.0; }
+__device__ inline float2 to32(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ inline float to32(double a) { return (float)a; }
+__device__ inline double2 to64(float2 a) { return make_double2(a.x, a.y); }
+__device__ inline double to64(float a) { return a; }
+__device__ inline double2 to64(double2 a) { return a; }
+__device__ inline double to64(double a) { return a; }
+__device__ inline float2 cvt32(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ inline float cvt32(double a) { return (float)a; }
+template <class To> __device__ inline To cvt(double2 a);
+template <> __device__ inline float2 cvt<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+template <> __device__ inline double2 cvt<double2>(double2 a) { return a; }
+template <class To> __device__ inline To cvt(double a);
+template <> __device__ inline float cvt<float>(double a) { return (float)a; }
+template <> __device__ inline double cvt<double>(double a) { return a; }
+__device__ inline float2 shfl_add(float2 a, int off) { a.x += __shfl_xor(a.x, off); a.y += __shfl_xor(a.y, off); return a; }
+__device__ inline double2 shfl_add(double2 a, int off) { a.x += __shfl_xor(a.x, off); a.y += __shfl_xor(a.y, off); return a; }
+__device__ inline float shfl_add(float a, int off) { return a + __shfl_xor(a, off); }
+__device__ inline double shfl_add(double a, int off) { return a + __shfl_xor(a, off); }
+
+constexpr int NXCD = 8;
+// workgroup id -> logical id so that XCD x (which receives ids x, x+8, ...)
+// owns the contiguous logical range [x*G/8, (x+1)*G/8).  G is a multiple of 8.
+__device__ inline int xcd_block(int b, int G) { return (b % NXCD) * (G / NXCD) + b / NXCD; }
+
+// Row schedule of the SpMM kernels: the rows are cut into 8 contiguous
+// regions, one per XCD (workgroup b runs on XCD b % 8), and inside a region
+// the XCD's waves take rows round-robin.  All waves of an XCD therefore work
+// on one moving window of consecutive rows, so the neighbour rows a CSR row
+// gathers (i +- 1, i +- layer, i +- slab in a structured numbering) were
+// loaded into that XCD's L2 by the window just before.
+struct RowSched {
+    int start, end, first, stride;
+};
+__device__ inline RowSched row_sched(int n, int rows_per_wave)
+{
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
+    const int nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int per = (n + NXCD - 1) / NXCD;
+    RowSched r;
+    r.start = min(n, xcd * per);
+    r.end = min(n, r.start + per);
+    r.first = r.start + (bx * nw + wave) * rows_per_wave;
+    r.stride = nbx * nw * rows_per_wave;
+    return r;
+}
+
+// Reduce two per-lane sums over the lanes that share a column (lane % K) and
+// over the waves of the workgroup; write partial[(slot*G + blk)*K + c].
+template <int K>
+__device__ inline void block_reduce2(double d1, double d2, double *partial, int blk, int G, bool want2)
+{
+    for (int off = K; off < 64; off <<= 1) {
+        d1 += __shfl_xor(d1, off);
+        if (want2) d2 += __shfl_xor(d2, off);
+    }
+    __shared__ double s1[4][K], s2[4][K];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane < K) { s1[wave][lane] = d1; s2[wave][lane] = d2; }
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int w = 0; w < nw; w++) { t1 += s1[w][threadIdx.x]; t2 += s2[w][threadIdx.x]; }
+        partial[((size_t)0 * G + blk) * K + threadIdx.x] = t1;
+        if (want2) partial[((size_t)1 * G + blk) * K + threadIdx.x] = t2;
+    }
+}
+
+// ----------------------------------------------------------- block SpMM
+// Gathers are issued in chunks of CH nonzeros (indices and values first,
+// then all CH neighbour loads, then the multiply-adds) so that a wave keeps
+// CH independent memory requests in flight instead of one per nonzero.
+constexpr int CH = 16;
+
+template <class V, int K, bool PRE>
+__device__ inline V row_gather(int kb, int ke, int c, const int *__restrict__ colidx, const V *__restrict__ val,
+                               const V *__restrict__ idiag, const V *__restrict__ X)
+{
+    V acc = vzero<V>();
+    for (int k0 = kb; k0 < ke; k0 += CH) {
+        int jj[CH];
+        V aa[CH], xx[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            // tail of the row: repeat the last nonzero with a zero weight so
+            // every load stays inside the row
+            const int kk = min(k0 + u, ke - 1);
+            jj[u] = colidx[kk];
+            aa[u] = (k0 + u < ke) ? val[kk] : vzero<V>();
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            xx[u] = X[(size_t)jj[u] * K + c];
+            if (PRE) xx[u] = vmul(idiag[jj[u]], xx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) acc = vadd(acc, vmul(aa[u], xx[u]));
+    }
+    return acc;
+}
+
+template <class V, int K, bool PRE>
+__global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ colidx, const V *__restrict__ val,
+                                                  const V *__restrict__ idiag, const V *__restrict__ X,
+                                                  V *__restrict__ Y, const V *__restrict__ W, int want_yy,
+                                                  const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int R = 64 / K;
+    const int G = gridDim.x;
+    const int blk = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = lane % K, rsub = lane / K;
+    const bool act = active[c] != 0;
+    const RowSched rs = row_sched(n, R);
+    double d1 = 0.0, d2 = 0.0;
+    for (int base = rs.first; base < rs.end; base += rs.stride) {
+        const int i = base + rsub;
+        if (i < rs.end && act) {
+            V acc = row_gather<V, K, PRE>(rowptr[i], rowptr[i + 1], c, colidx, val, idiag, X);
+            Y[(size_t)i * K + c] = acc;
+            d1 += dotR(W[(size_t)i * K + c], acc);
+            if (want_yy) d2 += dotR(acc, acc);
+        }
+    }
+    block_reduce2<K>(d1, d2, partial, blk, G, want_yy != 0);
+}
+
+// ------------------------------------------- block SpMM, 64 columns
+// K = 64: one lane per column, one wave per contiguous run of rows.  The
+// row pointers, column indices and values of the run are fetched with
+// coalesced vector loads (64 entries per load) and broadcast with readlane,
+// so the only memory accesses on the critical path are the neighbour-row
+// gathers, issued B at a time.  Rows are dealt to waves in contiguous runs
+// inside one region per XCD.
+__device__ inline float2 rdlane(float2 v, int l)
+{
+    return make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.x), l)),
+                       __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.y), l)));
+}
+__device__ inline double rdlane_d(double v, int l)
+{
+    long long b = __builtin_bit_cast(long long, v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 rdlane(double2 v, int l) { return make_double2(rdlane_d(v.x, l), rdlane_d(v.y, l)); }
+__device__ inline float rdlane(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double rdlane(double v, int l) { return rdlane_d(v, l); }
+
+template <class V, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+__global__ void __launch_bounds__(256) k_spmm64(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                                                const V *__restrict__ val, const V *__restrict__ X,
+                                                V *__restrict__ Y, const V *__restrict__ W, int want_yy,
+                                                const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int B = 16;
+    const int lane = threadIdx.x & 63;
+    const int c = lane;
+    const bool act = active[c] != 0;
+    const int G = gridDim.x, nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
+    const int per_x = (n + NXCD - 1) / NXCD;
+    const int xs = min(n, xcd * per_x), xe = min(n, xs + per_x);
+    const int wpx = nbx * nw, wid = bx * nw + wave;
+    const int per_w = (xe - xs + wpx - 1) / wpx;
+    const int r0 = min(xe, xs + wid * per_w), r1 = min(xe, r0 + per_w);
+    double d1 = 0.0, d2 = 0.0;
+    for (int rb = r0; rb < r1; rb += 63) {
+        const int nr = min(63, r1 - rb);
+        const int rpv = rowptr[rb + min(lane, nr)];
+        const int kb = __builtin_amdgcn_readlane(rpv, 0), ke = __builtin_amdgcn_readlane(rpv, nr);
+        int row = 0;
+        int rend = __builtin_amdgcn_readlane(rpv, 1);
+        V acc = vzero<V>();
+        auto flush = [&]() {
+            const size_t at = (size_t)(rb + row) * 64 + c;
+            if (act) {
+                if (MODE == 0) {
+                    Y[at] = acc;
+                    d1 += dotR(W[at], acc);
+                    if (want_yy) d2 += dotR(acc, acc);
+                } else {
+                    V r = vsub(W[at], acc);
+                    Y[at] = r;
+                    d1 += dotR(r, r);
+                }
+            }
+            acc = vzero<V>();
+            row++;
+            rend = __builtin_amdgcn_readlane(rpv, min(row + 1, nr));
+        };
+        for (int seg = kb; seg < ke; seg += 64) {
+            const int kk = min(seg + lane, ke - 1);
+            const int cv = colidx[kk];
+            const V vv = val[kk];
+            const int m = min(64, ke - seg);
+            for (int t0 = 0; t0 < m; t0 += B) {
+                V xb[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    const int t = min(t0 + u, m - 1);
+                    const int j = __builtin_amdgcn_readlane(cv, t);
+                    xb[u] = act ? X[(size_t)j * 64 + c] : vzero<V>();
+                }
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    if (t0 + u < m) {
+                        const int k = seg + t0 + u;
+                        while (k >= rend && row < nr) flush();
+                        acc = vadd(acc, vmul(rdlane(vv, t0 + u), xb[u]));
+                    }
+                }
+            }
+        }
+        while (row < nr) flush();
+    }
+    block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
+}
+
+// single-column SpMV: LPR lanes share one row
+template <class V, int LPR>
+__global__ void __launch_bounds__(256) k_spmv_dot(int n, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ colidx, const V *__restrict__ val,
+                                                  const V *__restrict__ idiag, const V *__restrict__ X,
+                                                  V *__restrict__ Y, const V *__restrict__ W, int want_yy,
+                                                  const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int RPW = 64 / LPR;   // rows per wave
+    const int G = gridDim.x;
+    const int blk = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane % LPR, rw = lane / LPR;
+    const bool act = active[0] != 0;
+    const RowSched rs = row_sched(n, RPW);
+    double d1 = 0.0, d2 = 0.0;
+    for (int base = rs.first; base < rs.end; base += rs.stride) {
+        const int i = base + rw;
+        V acc = vzero<V>();
+        if (i < rs.end && act) {
+            const int kb = rowptr[i], ke = rowptr[i + 1];
+            for (int k = kb + sub; k < ke; k += LPR) {
+                const int j = colidx[k];
+                V xv = X[j];
+                if (idiag) xv = vmul(idiag[j], xv);
+                acc = vadd(acc, vmul(val[k], xv));
+            }
+        }
+        // sum the LPR partial rows (all lanes take part in the shuffles)
+        for (int off = LPR / 2; off > 0; off >>= 1) acc = shfl_add(acc, off);
+        if (i < rs.end && act && sub == 0) {
+            Y[i] = acc;
+            d1 += dotR(W[i], acc);
+            if (want_yy) d2 += dotR(acc, acc);
+        }
+    }
+    block_reduce2<1>(d1, d2, partial, blk, G, want_yy != 0);
+}
+
+// --------------------------------------------------------- FP64 residual
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_resid(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                                               const V *__restrict__ val, const V *__restrict__ X,
+                                               const V *__restrict__ B, V *__restrict__ Rr,
+                                               const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int R = 64 / K;
+    const int G = gridDim.x;
+    const int blk = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = lane % K, rsub = lane / K;
+    const bool act = active[c] != 0;
+    const RowSched rs = row_sched(n, R);
+    double d1 = 0.0;
+    for (int base = rs.first; base < rs.end; base += rs.stride) {
+        const int i = base + rsub;
+        if (i < rs.end && act) {
+            V acc = row_gather<V, K, false>(rowptr[i], rowptr[i + 1], c, colidx, val, nullptr, X);
+            V r = vsub(B[(size_t)i * K + c], acc);
+            Rr[(size_t)i * K + c] = r;
+            d1 += dotR(r, r);
+        }
+    }
+    block_reduce2<K>(d1, 0.0, partial, blk, G, false);
+}
+
+// -------------------------------------------------- element-wise updates
+// Each thread walks elements t = i*K + c with a stride that is a multiple of
+// K, so its column c is fixed.
+#define EW_PROLOGUE                                                      \
+    const int G = gridDim.x;                                             \
+    const int blk = xcd_block(blockIdx.x, G);                            \
+    const size_t tot = (size_t)n * K;                                    \
+    const size_t per = ((tot + G - 1) / G + 255) / 256 * 256;            \
+    const size_t t0 = (size_t)blk * per, t1 = min(tot, t0 + per);        \
+    const int c = (int)(threadIdx.x % K);
+
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_bicg_p(int n, const V *__restrict__ r, V *__restrict__ p,
+                                                const V *__restrict__ v, const double *__restrict__ sc,
+                                                const int *__restrict__ ic)
+{
+    EW_PROLOGUE
+    if (!ic[I_ACTIVE * K + c]) return;
+    const bool first = ic[I_FIRST * K + c] != 0;
+    const double beta = sc[S_BETA * K + c], omega = sc[S_OMEGA * K + c];
+    for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        V rv = r[t];
+        p[t] = first ? rv : vadd(rv, vscale(vsub(p[t], vscale(v[t], omega)), beta));
+    }
+}
+
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_bicg_s(int n, const V *__restrict__ r, const V *__restrict__ v,
+                                                V *__restrict__ s, const double *__restrict__ sc,
+                                                const int *__restrict__ ic)
+{
+    EW_PROLOGUE
+    if (!ic[I_ACTIVE * K + c]) return;
+    const double alpha = sc[S_ALPHA * K + c];
+    for (size_t t = t0 + threadIdx.x; t < t1; t += 256) s[t] = vsub(r[t], vscale(v[t], alpha));
+}
+
+// x += D^-1 (alpha p + omega s); r = s - omega t; partials ||r||^2, <rd, r>
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_bicg_xr(int n, const V *__restrict__ idiag, V *__restrict__ x,
+                                                 const V *__restrict__ p, const V *__restrict__ s,
+                                                 const V *__restrict__ tv, V *__restrict__ r,
+                                                 const V *__restrict__ rd, const double *__restrict__ sc,
+                                                 const int *__restrict__ ic, double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0;
+    if (ic[I_ACTIVE * K + c]) {
+        const double alpha = sc[S_ALPHA * K + c], omega = sc[S_OMEGA * K + c];
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+            V sv = s[t];
+            V upd = vadd(vscale(p[t], alpha), vscale(sv, omega));
+            if (idiag) upd = vmul(idiag[t / K], upd);
+            x[t] = vadd(x[t], upd);
+            V rn = vsub(sv, vscale(tv[t], omega));
+            r[t] = rn;
+            d1 += dotR(rn, rn);
+            d2 += dotR(rd[t], rn);
+        }
+    }
+    block_reduce2<K>(d1, d2, partial, blk, G, true);
+}
+
+// PCG: x += alpha d; r -= alpha q; partials <r, D^-1 r> and ||r||^2
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_cg_xr(int n, const V *__restrict__ idiag, V *__restrict__ x,
+                                               const V *__restrict__ d, const V *__restrict__ q, V *__restrict__ r,
+                                               const double *__restrict__ sc, const int *__restrict__ ic,
+                                               double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0;
+    if (ic[I_ACTIVE * K + c]) {
+        const double alpha = sc[S_ALPHA * K + c];
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+            x[t] = vadd(x[t], vscale(d[t], alpha));
+            V rn = vsub(r[t], vscale(q[t], alpha));
+            r[t] = rn;
+            V z = idiag ? vmul(idiag[t / K], rn) : rn;
+            d1 += dotR(rn, z);
+            d2 += dotR(rn, rn);
+        }
+    }
+    block_reduce2<K>(d1, d2, partial, blk, G, true);
+}
+
+// PCG: d = beta d + D^-1 r (beta = 0 on the first call); optional partials
+// <r, D^-1 r>, ||r||^2 (used to start the iteration)
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_cg_d(int n, const V *__restrict__ idiag, V *__restrict__ d,
+                                              const V *__restrict__ r, const double *__restrict__ sc,
+                                              const int *__restrict__ ic, double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0;
+    if (ic[I_ACTIVE * K + c]) {
+        const bool first = ic[I_FIRST * K + c] != 0;
+        const double beta = sc[S_BETA * K + c];
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+            V rv = r[t];
+            V z = idiag ? vmul(idiag[t / K], rv) : rv;
+            d[t] = first ? z : vadd(vscale(d[t], beta), z);
+            if (partial) { d1 += dotR(rv, z); d2 += dotR(rv, rv); }
+        }
+    }
+    if (partial) block_reduce2<K>(d1, d2, partial, blk, G, true);
+}
+
+// r_in = r64 * scale ; rd_in = r_in ; x_in = 0 ; partials <r_in, r_in> x2
+template <class V64, class V, int K>
+__global__ void __launch_bounds__(256) k_to_inner(int n, const V64 *__restrict__ r64, V *__restrict__ rin,
+                                                  V *__restrict__ rdin, V *__restrict__ xin,
+                                                  const double *__restrict__ scale, const int *__restrict__ oflags,
+                                                  double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0;
+    if (oflags[c]) {
+        const double s = scale[c];
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+            V v = cvt<V>(vscale(r64[t], s));
+            rin[t] = v;
+            if (rdin) rdin[t] = v;
+            if (xin) xin[t] = vzero<V>();
+            d1 += dotR(v, v);
+        }
+    }
+    block_reduce2<K>(d1, d1, partial, blk, G, true);
+}
+
+// reliable update: r_in = r64 * scale, x_in = 0; partials <r_in, r_in> and
+// <rd, r_in> (BiCGSTAB shadow residual) or <r_in, D^-1 r_in> (PCG)
+template <class V64, class V, int K>
+__global__ void __launch_bounds__(256) k_reliable(int n, const V64 *__restrict__ r64, V *__restrict__ rin,
+                                                  V *__restrict__ xin, const V *__restrict__ rd,
+                                                  const V *__restrict__ idiag, const double *__restrict__ scale,
+                                                  const int *__restrict__ oflags, double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0;
+    if (oflags[c]) {
+        const double s = scale[c];
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) {
+            V v = cvt<V>(vscale(r64[t], s));
+            rin[t] = v;
+            xin[t] = vzero<V>();
+            d1 += dotR(v, v);
+            if (rd) d2 += dotR(rd[t], v);
+            else d2 += dotR(v, idiag ? vmul(idiag[t / K], v) : v);
+        }
+    }
+    block_reduce2<K>(d1, d2, partial, blk, G, true);
+}
+
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_norm2(int n, const V *__restrict__ X, const int *__restrict__ flags,
+                                               double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0;
+    if (flags[c])
+        for (size_t t = t0 + threadIdx.x; t < t1; t += 256) d1 += dotR(X[t], X[t]);
+    block_reduce2<K>(d1, d1, partial, blk, G, true);
+}
+
+template <class V64, class V, int K>
+__global__ void __launch_bounds__(256) k_from_inner(int n, V64 *__restrict__ x64, const V *__restrict__ xin,
+                                                    const double *__restrict__ scale,
+                                                    const int *__restrict__ oflags)
+{
+    EW_PROLOGUE
+    if (!oflags[c]) return;
+    const double s = 1.0 / scale[c];
+    for (size_t t = t0 + threadIdx.x; t < t1; t += 256) x64[t] = vadd(x64[t], vscale(to64(xin[t]), s));
+}
+
+// --------------------------------------------------------------- finalize
+// One workgroup per column: fixed-order sum of the per-block partials, then
+// the scalar recurrence of that column.
+__global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const double *__restrict__ partial,
+                                                  double *__restrict__ sc, int *__restrict__ ic, int maxit,
+                                                  double tol2, double inner_tol2, const double *__restrict__ osc,
+                                                  const int *__restrict__ oic)
+{
+    const int c = blockIdx.x;
+    __shared__ double red[2][256];
+    for (int slot = 0; slot < 2; slot++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < G; b += 256) s += partial[((size_t)slot * G + b) * K + c];
+        red[slot][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double r0 = red[0][0], r1 = red[1][0];
+    double *S = sc;
+    int *I = ic;
+#define SC(slot) S[(slot) * K + c]
+#define IC(slot) I[(slot) * K + c]
+    switch (fk) {
+    case FIN_OUTER_BNORM:
+        SC(S_BNORM2) = r0;
+        SC(S_SCALE) = r0 > 0.0 ? 1.0 / sqrt(r0) : 1.0;
+        SC(S_BEST) = -1.0;
+        IC(I_STALL) = 0;
+        if (r0 == 0.0) IC(I_ACTIVE) = 0;   // zero right-hand side: the field stays at the initial guess
+        break;
+    case FIN_OUTER_RES: {
+        if (!IC(I_ACTIVE)) return;
+        SC(S_RES2) = r0;
+        IC(I_ITERS) += 1;   // true-residual evaluations
+        if (r0 <= tol2 * SC(S_BNORM2) || !isfinite(r0)) { IC(I_ACTIVE) = 0; break; }
+        // stagnation: no 2x drop of the true residual norm over 8 evaluations
+        if (SC(S_BEST) < 0.0 || r0 < 0.25 * SC(S_BEST)) { SC(S_BEST) = r0; IC(I_STALL) = 0; }
+        else if (++IC(I_STALL) >= 8) IC(I_ACTIVE) = 0;
+        break;
+    }
+    case FIN_BICG_RELIABLE:
+    case FIN_CG_RELIABLE: {
+        const int oact = oic[I_ACTIVE * K + c];
+        if (!oact) { IC(I_ACTIVE) = 0; break; }
+        SC(S_RES2) = r0;
+        if (fk == FIN_BICG_RELIABLE) {
+            // recompute beta of the coming iteration with the replaced residual
+            if (!IC(I_FIRST) && SC(S_RHOP) != 0.0 && SC(S_OMEGA) != 0.0 && r1 != 0.0 && isfinite(r1))
+                SC(S_BETA) = (r1 * SC(S_ALPHA)) / (SC(S_RHOP) * SC(S_OMEGA));
+            else
+                IC(I_FIRST) = 1;   // breakdown or not started: restart the directions from r
+            SC(S_RHO) = r1;
+        } else {
+            // PCG: the replacement sits between the beta step and the
+            // direction update d = beta d + D^-1 r, so beta is recomputed
+            // from the replaced residual
+            SC(S_RHO) = r1;
+            if (SC(S_RHOP) != 0.0 && isfinite(r1) && !IC(I_FIRST)) SC(S_BETA) = r1 / SC(S_RHOP);
+            else IC(I_FIRST) = 1;
+        }
+        if (IC(I_BREAK)) { IC(I_FIRST) = 1; IC(I_BREAK) = 0; }
+        IC(I_ACTIVE) = 1;
+        if (maxit > 0 && IC(I_ITERS) >= maxit) IC(I_ACTIVE) = 0;
+        break;
+    }
+    case FIN_BICG_INIT:
+    case FIN_CG_INIT:
+        if (!IC(I_ACTIVE)) return;
+        SC(S_BNORM2) = 1.0;   // the inner system is scaled by 1/||b||: absolute criterion
+        SC(S_RES2) = (fk == FIN_BICG_INIT) ? r0 : r1;
+        SC(S_RHO) = (fk == FIN_BICG_INIT) ? r1 : r0;
+        IC(I_FIRST) = 1;
+        IC(I_BREAK) = 0;
+        if (SC(S_RES2) <= SC(S_TOL2)) IC(I_ACTIVE) = 0;
+        break;
+    case FIN_BICG_ALPHA: {
+        if (!IC(I_ACTIVE)) return;
+        double alpha = SC(S_RHO) / r0;
+        if (r0 == 0.0 || !isfinite(alpha)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; alpha = 0.0; }
+        SC(S_ALPHA) = alpha;
+        break;
+    }
+    case FIN_BICG_OMEGA: {
+        if (!IC(I_ACTIVE)) return;
+        double om = (r1 == 0.0) ? 0.0 : r0 / r1;
+        if (!isfinite(om)) om = 0.0;
+        SC(S_OMEGA) = om;
+        break;
+    }
+    case FIN_BICG_RHO: {
+        if (!IC(I_ACTIVE)) return;
+        IC(I_ITERS) += 1;
+        IC(I_FIRST) = 0;
+        SC(S_RES2) = r0;
+        const double rho_new = r1;
+        SC(S_RHOP) = SC(S_RHO);
+        if (r0 <= SC(S_TOL2)) { IC(I_ACTIVE) = 0; break; }
+        if (SC(S_OMEGA) == 0.0 || rho_new == 0.0 || !isfinite(r0)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; break; }
+        if (maxit > 0 && IC(I_ITERS) >= maxit) { IC(I_ACTIVE) = 0; break; }
+        SC(S_BETA) = (rho_new * SC(S_ALPHA)) / (SC(S_RHO) * SC(S_OMEGA));
+        SC(S_RHOP) = SC(S_RHO);
+        SC(S_RHO) = rho_new;
+        break;
+    }
+    case FIN_CG_ALPHA: {
+        if (!IC(I_ACTIVE)) return;
+        double alpha = SC(S_RHO) / r0;
+        if (alpha == 0.0 || !isfinite(alpha)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; alpha = 0.0; }
+        SC(S_ALPHA) = alpha;
+        break;
+    }
+    case FIN_CG_BETA: {
+        if (!IC(I_ACTIVE)) return;
+        IC(I_ITERS) += 1;
+        IC(I_FIRST) = 0;
+        const double dold = SC(S_RHO), dnew = r0;
+        const double beta = dnew / dold;
+        SC(S_RHOP) = dold;
+        SC(S_RHO) = dnew;
+        SC(S_BETA) = beta;
+        SC(S_RES2) = r1;
+        if (beta == 0.0 || !isfinite(beta)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; break; }
+        if (r1 <= SC(S_TOL2)) { IC(I_ACTIVE) = 0; break; }
+        if (maxit > 0 && IC(I_ITERS) >= maxit) { IC(I_ACTIVE) = 0; break; }
+        break;
+    }
+    }
+#undef SC
+#undef IC
+}
+
+// ------------------------------------------------------------- utilities
+template <class V>
+__global__ void k_inv_diag(int n, const int *__restrict__ diagpos, const double *__restrict__ val64,
+                           V *__restrict__ idiag);
+template <>
+__global__ void k_inv_diag<double2>(int n, const int *__restrict__ diagpos, const double *__restrict__ val64,
+                                    double2 *__restrict__ idiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = diagpos[i];
+    double a = k >= 0 ? val64[2 * (size_t)k] : 0.0, b = k >= 0 ? val64[2 * (size_t)k + 1] : 0.0;
+    if (a == 0.0 && b == 0.0) { idiag[i] = make_double2(1.0, 0.0); return; }
+    double den = a * a + b * b;
+    idiag[i] = make_double2(a / den, -b / den);
+}
+template <>
+__global__ void k_inv_diag<float2>(int n, const int *__restrict__ diagpos, const double *__restrict__ val64,
+                                   float2 *__restrict__ idiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = diagpos[i];
+    double a = k >= 0 ? val64[2 * (size_t)k] : 0.0, b = k >= 0 ? val64[2 * (size_t)k + 1] : 0.0;
+    if (a == 0.0 && b == 0.0) { idiag[i] = make_float2(1.f, 0.f); return; }
+    double den = a * a + b * b;
+    idiag[i] = make_float2((float)(a / den), (float)(-b / den));
+}
+template <>
+__global__ void k_inv_diag<double>(int n, const int *__restrict__ diagpos, const double *__restrict__ val64,
+                                   double *__restrict__ idiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = diagpos[i];
+    double a = k >= 0 ? val64[k] : 0.0;
+    idiag[i] = a != 0.0 ? 1.0 / a : 1.0;
+}
+template <>
+__global__ void k_inv_diag<float>(int n, const int *__restrict__ diagpos, const double *__restrict__ val64,
+                                  float *__restrict__ idiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = diagpos[i];
+    double a = k >= 0 ? val64[k] : 0.0;
+    idiag[i] = a != 0.0 ? (float)(1.0 / a) : 1.f;
+}
+
+// right-preconditioned operator A D^-1 (column scaling), built once per Reset
+template <class V>
+__global__ void k_scale_cols(int nnz, const int *__restrict__ colidx, const V *__restrict__ val,
+                             const V *__restrict__ idiag, V *__restrict__ out64, void *__restrict__ out32)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    V v = vmul(val[k], idiag[colidx[k]]);
+    out64[k] = v;
+    if (out32) ((decltype(cvt32(v)) *)out32)[k] = cvt32(v);
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d, size_t cnt)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < cnt; i += (size_t)gridDim.x * blockDim.x) d[i] = (float)s[i];
+}
+
+// ================================================================ launchers
+static inline void check_K(int K)
+{
+    TMI_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64, TMI_ERR_INVALID_ARG,
+              "block width %d is not a power of two <= 64", K);
+}
+
+#define DISPATCH_K(K, MACRO)          \
+    switch (K) {                      \
+    case 1: MACRO(1); break;          \
+    case 2: MACRO(2); break;          \
+    case 4: MACRO(4); break;          \
+    case 8: MACRO(8); break;          \
+    case 16: MACRO(16); break;        \
+    case 32: MACRO(32); break;        \
+    default: MACRO(64); break;        \
+    }
+
+#define DISPATCH_V(vk, ...)                                    \
+    switch (vk) {                                              \
+    case VK_C32: { using V = float2; __VA_ARGS__; } break;     \
+    case VK_C64: { using V = double2; __VA_ARGS__; } break;    \
+    case VK_R32: { using V = float; __VA_ARGS__; } break;      \
+    default: { using V = double; __VA_ARGS__; } break;         \
+    }
+
+void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
+                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st)
+{
+    check_K(b.K);
+    dim3 g(b.nblk), blk(256);
+    if (b.K == 1) {
+        // lanes per row sized to the average row length
+        int avg = b.n ? 0 : 1;
+        (void)avg;
+#define SPMV1                                                                                                   \
+    hipLaunchKernelGGL((k_spmv_dot<V, 16>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,           \
+                       (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial);
+        DISPATCH_V(vk, SPMV1)
+#undef SPMV1
+    } else {
+#define SPMMK(KK)                                                                                             \
+    if (KK == 64 && !idiag && (vk == VK_C32 || vk == VK_C64))                                                \
+        hipLaunchKernelGGL((k_spmm64<V, 0>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
+                           (V *)Y, (const V *)W, (int)want_yy, iflags, partial);                             \
+    else if (idiag)                                                                                                \
+        hipLaunchKernelGGL((k_spmm_dot<V, KK, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
+                           (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial); \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_spmm_dot<V, KK, false>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, \
+                           (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial);
+        DISPATCH_V(vk, DISPATCH_K(b.K, SPMMK))
+#undef SPMMK
+    }
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
+                  const int *iflags, double *partial, hipStream_t st)
+{
+    check_K(b.K);
+    dim3 g(b.nblk), blk(256);
+#define RESK(KK)                                                                                              \
+    if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                           \
+        hipLaunchKernelGGL((k_spmm64<V, 1>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
+                           (V *)R, (const V *)B, 0, iflags, partial);                                        \
+    else                                                                                                      \
+    hipLaunchKernelGGL((k_resid<V, KK>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
+                       (const V *)B, (V *)R, iflags, partial);
+    DISPATCH_V(vk, DISPATCH_K(b.K, RESK))
+#undef RESK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_bicg_p(VKind vk, const BlockDesc &b, const void *r, void *p, const void *v, const double *sc,
+                   const int *ic, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define PK(KK) hipLaunchKernelGGL((k_bicg_p<V, KK>), g, blk, 0, st, b.n, (const V *)r, (V *)p, (const V *)v, sc, ic);
+    DISPATCH_V(vk, DISPATCH_K(b.K, PK))
+#undef PK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_bicg_s(VKind vk, const BlockDesc &b, const void *r, const void *v, void *s, const double *sc,
+                   const int *ic, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define SK(KK) hipLaunchKernelGGL((k_bicg_s<V, KK>), g, blk, 0, st, b.n, (const V *)r, (const V *)v, (V *)s, sc, ic);
+    DISPATCH_V(vk, DISPATCH_K(b.K, SK))
+#undef SK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *p, const void *s,
+                    const void *t, void *r, const void *rd, const double *sc, const int *ic, double *partial,
+                    hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define XK(KK)                                                                                                \
+    hipLaunchKernelGGL((k_bicg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)p,         \
+                       (const V *)s, (const V *)t, (V *)r, (const V *)rd, sc, ic, partial);
+    DISPATCH_V(vk, DISPATCH_K(b.K, XK))
+#undef XK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
+                  const double *sc, const int *ic, double *partial, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define CXK(KK)                                                                                               \
+    hipLaunchKernelGGL((k_cg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,           \
+                       (const V *)q, (V *)r, sc, ic, partial);
+    DISPATCH_V(vk, DISPATCH_K(b.K, CXK))
+#undef CXK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
+                 const int *ic, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define CDK(KK) \
+    hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, nullptr);
+    DISPATCH_V(vk, DISPATCH_K(b.K, CDK))
+#undef CDK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
+                    const int *ic, double *partial, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define CIK(KK) \
+    hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
+    DISPATCH_V(vk, DISPATCH_K(b.K, CIK))
+#undef CIK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_finalize(FinKind fk, const BlockDesc &b, const double *partial, double *sc, int *ic, int maxit,
+                     double tol2, double inner_tol2, hipStream_t st, const double *osc, const int *oic)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(b.K), dim3(256), 0, st, (int)fk, b.nblk, b.K, partial, sc, ic, maxit, tol2,
+                       inner_tol2, osc, oic);
+    TMI_LAUNCH_CHECK();
+}
+
+__global__ void k_inner_reset(int K, const double *__restrict__ osc, const int *__restrict__ oic,
+                              double *__restrict__ sc, int *__restrict__ ic)
+{
+    int c = threadIdx.x;
+    if (c >= K) return;
+    ic[I_ACTIVE * K + c] = oic[I_ACTIVE * K + c];
+    ic[I_ITERS * K + c] = 0;
+    ic[I_FIRST * K + c] = 1;
+    ic[I_BREAK * K + c] = 0;
+    sc[S_TOL2 * K + c] = osc[S_AUX * K + c];
+    sc[S_RHOP * K + c] = 0.0;
+    sc[S_BETA * K + c] = 0.0;
+    sc[S_OMEGA * K + c] = 0.0;
+    sc[S_ALPHA * K + c] = 0.0;
+}
+
+void launch_inner_reset(const BlockDesc &b, const double *osc, const int *oic, double *sc, int *ic, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inner_reset, dim3(1), dim3(64), 0, st, b.K, osc, oic, sc, ic);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_to_inner(VKind vk32, const BlockDesc &b, const void *r64, void *r32, void *rd32, void *x32,
+                     const double *scale, const int *oflags, double *partial, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define TIK(KK)                                                                                                \
+    hipLaunchKernelGGL((k_to_inner<V64, V, KK>), g, blk, 0, st, b.n, (const V64 *)r64, (V *)r32, (V *)rd32,   \
+                       (V *)x32, scale, oflags, partial);
+    switch (vk32) {
+    case VK_C32: { using V = float2; using V64 = double2; DISPATCH_K(b.K, TIK) break; }
+    case VK_C64: { using V = double2; using V64 = double2; DISPATCH_K(b.K, TIK) break; }
+    case VK_R32: { using V = float; using V64 = double; DISPATCH_K(b.K, TIK) break; }
+    default: { using V = double; using V64 = double; DISPATCH_K(b.K, TIK) break; }
+    }
+#undef TIK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x32, const double *scale,
+                       const int *oflags, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define FIK(KK)                                                                                              \
+    hipLaunchKernelGGL((k_from_inner<V64, V, KK>), g, blk, 0, st, b.n, (V64 *)x64, (const V *)x32, scale,   \
+                       oflags);
+    switch (vk32) {
+    case VK_C32: { using V = float2; using V64 = double2; DISPATCH_K(b.K, FIK) break; }
+    case VK_C64: { using V = double2; using V64 = double2; DISPATCH_K(b.K, FIK) break; }
+    case VK_R32: { using V = float; using V64 = double; DISPATCH_K(b.K, FIK) break; }
+    default: { using V = double; using V64 = double; DISPATCH_K(b.K, FIK) break; }
+    }
+#undef FIK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
+                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define RLK(KK)                                                                                                 \
+    hipLaunchKernelGGL((k_reliable<V64, V, KK>), g, blk, 0, st, b.n, (const V64 *)r64, (V *)rin, (V *)xin,      \
+                       (const V *)rd, (const V *)idiag, scale, oflags, partial);
+    switch (vk_in) {
+    case VK_C32: { using V = float2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }
+    case VK_C64: { using V = double2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }
+    case VK_R32: { using V = float; using V64 = double; DISPATCH_K(b.K, RLK) break; }
+    default: { using V = double; using V64 = double; DISPATCH_K(b.K, RLK) break; }
+    }
+#undef RLK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_norm2(VKind vk, const BlockDesc &b, const void *X, const int *flags, double *partial, hipStream_t st)
+{
+    dim3 g(b.nblk), blk(256);
+#define NK(KK) hipLaunchKernelGGL((k_norm2<V, KK>), g, blk, 0, st, b.n, (const V *)X, flags, partial);
+    DISPATCH_V(vk, DISPATCH_K(b.K, NK))
+#undef NK
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_inv_diag(VKind vk, int n, const int *diagpos, const double *val64, void *idiag, hipStream_t st)
+{
+    dim3 g(grid_for(n, 256)), blk(256);
+#define IDV hipLaunchKernelGGL((k_inv_diag<V>), g, blk, 0, st, n, diagpos, val64, (V *)idiag);
+    DISPATCH_V(vk, IDV)
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_scale_cols(bool cplx, int nnz, const int *colidx, const double *val64, const void *idiag64, double *out64,
+                       void *out32, hipStream_t st)
+{
+    dim3 g(grid_for(nnz, 256)), blk(256);
+    if (cplx)
+        hipLaunchKernelGGL((k_scale_cols<double2>), g, blk, 0, st, nnz, colidx, (const double2 *)val64,
+                           (const double2 *)idiag64, (double2 *)out64, out32);
+    else
+        hipLaunchKernelGGL((k_scale_cols<double>), g, blk, 0, st, nnz, colidx, (const double *)val64,
+                           (const double *)idiag64, (double *)out64, out32);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_f64_to_f32(const double *src, float *dst, size_t count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(count, 256, 4096)), dim3(256), 0, st, src, dst, count);
+    TMI_LAUNCH_CHECK();
+}
+
+}  // namespace tmi

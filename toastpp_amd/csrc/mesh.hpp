@@ -1,0 +1,59 @@
+// Device-resident finite-element mesh (Tri3 / Tet4) and its QM description.
+#pragma once
+#include "common.hpp"
+
+namespace tmi {
+
+// Per-element geometry record (doubles), one record per element:
+//   Tet4: a[0..3] b[4..7] c[8..11] d[12..15] size[16] side_area[17..20]
+//   Tri3: a[0..2] b[3..5] c[6..8] size[9] bndff[10..15] (lower triangle of
+//         the boundary mass matrix: (0,0) (1,0) (1,1) (2,0) (2,1) (2,2))
+constexpr int GEO3 = 24;
+constexpr int GEO2 = 16;
+// Per-element flag word: bits 0..3 boundary sides, bit 4 boundary element.
+constexpr int BNDEL_BIT = 16;
+
+struct QM {
+    int nq = 0, nm = 0, nqm = 0;
+    std::vector<double> Q, M;          // nq x dim, nm x dim
+    std::vector<int> lptr, lidx;       // link list (CSR)
+};
+
+struct Mesh {
+    int dim = 0, n = 0, elen = 0, nn = 0, nbnd = 0;
+    std::vector<double> h_nodes;       // host copies of the inputs
+    std::vector<int> h_elist;
+    DBuf<double> nodes;                // n x dim
+    DBuf<int> elist;                   // elen x nn
+    DBuf<double> geo;                  // elen x GEO{2,3}
+    DBuf<int> eflags;                  // elen
+    DBuf<int> bnd;                     // n (0/1)
+    DBuf<int> n2e_ptr, n2e;            // node -> elements, ascending
+    // CSR pattern of the system matrix (built lazily)
+    bool have_csr = false;
+    int nnz = 0;
+    DBuf<int> rowptr, colidx, rowidx, diagpos;
+    // host-side caches
+    std::vector<int> h_bnd;
+    QM qm;
+
+    int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }
+    void setup(hipStream_t st);        // Mesh::Setup on the GPU
+    void build_csr(hipStream_t st);    // Mesh::SparseRowStructure on the GPU
+};
+
+// host parser helpers
+void read_mesh_file(const char *path, int &dim, std::vector<double> &nodes, std::vector<int> &elist,
+                    std::vector<int> &region);
+void read_qm_file(const char *path, int dim, QM &qm, bool &fixed_m);
+// Mesh::PullToBoundary restated on the host (mesh.cc:1052-1111)
+bool pull_to_boundary(const Mesh &m, const std::vector<double> &geo, const double *p, double *out);
+
+// GPU kernels launched from mesh_kernels.hip
+void launch_exclusive_scan(const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries
+void launch_qvec_gaussian(const Mesh &m, int npts, const double *d_centres, double width, int neumann,
+                          const double *d_scale, double *d_out, hipStream_t st);
+void launch_assemble(const Mesh &m, const double *d_cmua, const double *d_ckap, const double *d_c2a, double omega,
+                     int complex_valued, double *d_val64, void *d_val32, hipStream_t st);
+
+}  // namespace tmi

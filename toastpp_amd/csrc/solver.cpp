@@ -1,0 +1,333 @@
+// Forward-solver driver: Allocate / Assemble / Reset / CalcFields on the GPU.
+//
+// CalcFields runs every mode as the same two-level loop over a tile of K
+// right-hand-side columns:
+//   outer (FP64):  r = q - K x  (FP64 CSR SpMM), per-column convergence test
+//                  ||r|| <= tol ||q|| (the reference criterion,
+//                  matrix_imp.hpp:350-413), then x += delta
+//   inner:         block BiCGSTAB (complex) or PCG (real) on K delta = r/||r||
+//                  -- complex FP32 in TMI_MODE_MIXED_BLOCK (to a loose inner
+//                  tolerance), FP64 in the FP64 modes (to the outer
+//                  tolerance, so one sweep normally suffices).
+// Columns leave the iteration individually (device-side active flags); the
+// host reads the flags every few iterations only.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace tmi {
+
+Solver::~Solver()
+{
+    if (h_flags) (void)hipHostFree(h_flags);
+    if (h_dbl) (void)hipHostFree(h_dbl);
+}
+
+void Solver::allocate()
+{
+    mesh->build_csr(st);
+    val64.alloc(nval());
+    val32.alloc(nval());
+    allocated = true;
+    assembled = false;
+}
+
+void Solver::assemble(const double *cmua, const double *ckap, const double *c2a, double om, bool host_ptrs)
+{
+    TMI_CHECK(allocated, TMI_ERR_STATE, "AssembleSystemMatrix called before Allocate");
+    TMI_CHECK(cplx || om == 0.0, TMI_ERR_INVALID_ARG, "nonzero omega on a real (CW) solver");
+    int n = mesh->n;
+    coef.alloc(3 * (size_t)n);
+    if (host_ptrs) {
+        TMI_HIP(hipMemcpyAsync(coef.p, cmua, n * sizeof(double), hipMemcpyHostToDevice, st));
+        TMI_HIP(hipMemcpyAsync(coef.p + n, ckap, n * sizeof(double), hipMemcpyHostToDevice, st));
+        TMI_HIP(hipMemcpyAsync(coef.p + 2 * n, c2a, n * sizeof(double), hipMemcpyHostToDevice, st));
+    } else {
+        TMI_HIP(hipMemcpyAsync(coef.p, cmua, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        TMI_HIP(hipMemcpyAsync(coef.p + n, ckap, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        TMI_HIP(hipMemcpyAsync(coef.p + 2 * n, c2a, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    omega = om;
+    precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
+    launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, val32.p, st);
+    assembled = true;
+}
+
+void Solver::reset_precon()
+{
+    TMI_CHECK(assembled, TMI_ERR_STATE, "preconditioner reset before assembly");
+    int n = mesh->n;
+    idiag64.alloc((size_t)n * vbytes(vk64()));
+    idiag32.alloc((size_t)n * vbytes(vk32()));
+    launch_inv_diag(vk64(), n, mesh->diagpos.p, val64.p, idiag64.p, st);
+    launch_inv_diag(vk32(), n, mesh->diagpos.p, val64.p, idiag32.p, st);
+    // BiCGSTAB applies M^-1 on the right (pd = M^-1 p, v = K pd): keep K D^-1
+    // as its own operator so the SpMM needs no per-nonzero diagonal gather
+    pval64.alloc(nval());
+    pval32.alloc(nval());
+    launch_scale_cols(cplx, mesh->nnz, mesh->colidx.p, val64.p, idiag64.p, pval64.p, pval32.p, st);
+    precon_valid = true;
+}
+
+void Solver::ensure_work(int K)
+{
+    int n = mesh->n;
+    VKind vin = (mode == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
+    if (w.K == K && w.n == n && w.vk_in == vin) return;
+    w.K = K;
+    w.n = n;
+    w.vk_in = vin;
+    // reduction workgroups: a multiple of 8 (one slice per XCD), enough to
+    // fill the GPU, few enough that the per-block partials stay small
+    long rows_work = ((long)n * K + 1023) / 1024;
+    int G = (int)std::min<long>(2048, std::max<long>(8, rows_work));
+    G = (G + 7) / 8 * 8;
+    w.G = G;
+    size_t nb64 = (size_t)n * K * vbytes(vk64()), nbi = (size_t)n * K * vbytes(vin);
+    w.x64.alloc(nb64); w.b64.alloc(nb64); w.r64.alloc(nb64);
+    w.xi.alloc(nbi); w.ri.alloc(nbi); w.rdi.alloc(nbi); w.p.alloc(nbi); w.v.alloc(nbi); w.s.alloc(nbi); w.t.alloc(nbi);
+    w.partial.alloc((size_t)2 * G * K);
+    w.sc.alloc((size_t)S_NSLOT * K); w.osc.alloc((size_t)S_NSLOT * K);
+    w.ic.alloc((size_t)I_NSLOT * K); w.oic.alloc((size_t)I_NSLOT * K);
+    if (!h_flags) TMI_HIP(hipHostMalloc((void **)&h_flags, sizeof(int) * I_NSLOT * 64));
+    if (!h_dbl) TMI_HIP(hipHostMalloc((void **)&h_dbl, sizeof(double) * S_NSLOT * 64));
+}
+
+int Solver::count_active(const int *d_flags)
+{
+    TMI_HIP(hipMemcpyAsync(h_flags, d_flags, sizeof(int) * w.K, hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    int c = 0;
+    for (int i = 0; i < w.K; i++) c += h_flags[i] != 0;
+    return c;
+}
+
+static constexpr int CHECK_EVERY = 8;
+
+// One tile: BiCGSTAB (complex) or PCG (real) on K columns with reliable
+// updates (Sleijpen & van der Vorst): the inner iteration runs in the inner
+// precision on the system scaled by 1/||q||; every `ru` iterations, and
+// whenever every column's recursive residual has met the tolerance, the
+// inner solution is folded into the FP64 solution, the true residual
+// q - K x is recomputed in FP64 and replaces the inner residual -- the
+// search directions are kept, so the iteration continues instead of
+// restarting.  A column is done when its FP64 true residual meets the
+// tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
+// improving (no 4x drop over 4 true-residual evaluations).
+int Solver::solve_tile(int kc, tmi_colstat *stats)
+{
+    int K = w.K, n = mesh->n;
+    BlockDesc bd{n, K, w.G, mesh->rowptr.p, mesh->colidx.p};
+    bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
+    const void *val_in = mixed ? (const void *)val32.p : (const void *)val64.p;
+    const void *idiag_in = nullptr;
+    bool use_cg = (lsolver == TMI_LSOLVER_CG);
+    // operator of the BiCGSTAB SpMMs: K D^-1 when diagonally preconditioned
+    const void *op_in = val_in;
+    if (precon == TMI_PRECON_DIAG) {
+        idiag_in = mixed ? (const void *)idiag32.p : (const void *)idiag64.p;
+        if (!use_cg) op_in = mixed ? (const void *)pval32.p : (const void *)pval64.p;
+    }
+    int maxit_total = maxit > 0 ? maxit : n + 1;
+    // reliable updates only in the mixed mode: in FP64 the recursive
+    // residual is the reference's own stopping test, and replacing it by the
+    // computed true residual would inject eps*||K||*||x|| noise into the
+    // far field, where the fields are ~1e-9 of their peak
+    int ru = mixed ? reliable_every : (1 << 30);
+    VKind vk = w.vk_in;
+    double *sc = w.sc.p, *osc = w.osc.p, *part = w.partial.p;
+    int *ic = w.ic.p, *oic = w.oic.p;
+    const int *oact = oic + I_ACTIVE * K;
+    double *scale = osc + S_SCALE * K;
+
+    std::vector<int> hic((size_t)I_NSLOT * K, 0);
+    for (int c = 0; c < kc; c++) hic[I_ACTIVE * K + c] = 1;
+    TMI_HIP(hipMemcpyAsync(oic, hic.data(), hic.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    TMI_HIP(hipMemsetAsync(osc, 0, w.osc.bytes(), st));
+    double tol2 = tol * tol;
+    // the recursive residual aims a little below the target so that the
+    // verifying FP64 residual usually passes at the first check
+    std::vector<double> itol2(K, mixed ? 0.25 * tol2 : tol2);
+
+    launch_norm2(vk64(), bd, w.b64.p, oact, part, st);
+    launch_finalize(FIN_OUTER_BNORM, bd, part, osc, oic, 0, 0, 0, st);
+    launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
+    // inner state
+    launch_inner_reset(bd, osc, oic, sc, ic, st);
+    TMI_HIP(hipMemcpyAsync(sc + S_TOL2 * K, itol2.data(), K * sizeof(double), hipMemcpyHostToDevice, st));
+    launch_to_inner(vk, bd, w.r64.p, w.ri.p, use_cg ? nullptr : w.rdi.p, w.xi.p, scale, oact, part, st);
+    if (use_cg) {
+        launch_cg_init(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, part, st);
+        launch_finalize(FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
+    } else {
+        launch_finalize(FIN_BICG_INIT, bd, part, sc, ic, 0, 0, 0, st);
+    }
+    static const bool verbose = getenv("TMI_VERBOSE") != nullptr;
+    int it = 0, nupd = 0, since = 0;
+    auto reliable_update = [&]() {
+        launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
+        launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+        launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
+        launch_reliable(vk, bd, w.r64.p, w.ri.p, w.xi.p, use_cg ? nullptr : w.rdi.p, idiag_in, scale, oact, part,
+                        st);
+        launch_finalize(use_cg ? FIN_CG_RELIABLE : FIN_BICG_RELIABLE, bd, part, sc, ic, maxit_total, 0, 0, st, osc,
+                        oic);
+        nupd++;
+        since = 0;
+    };
+    for (;;) {
+        if (count_active(oact) == 0) break;
+        if (count_active(ic + I_ACTIVE * K) == 0) {
+            // every column's recursive residual met the tolerance: FP64 modes
+            // stop here (as the reference does); the mixed mode verifies in FP64
+            if (!mixed) break;
+            reliable_update();
+            if (use_cg) launch_cg_d(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, st);
+            if (count_active(oact) == 0 || it >= maxit_total) break;
+            continue;
+        }
+        if (it >= maxit_total) break;
+        for (int k = 0; k < CHECK_EVERY; k++, it++, since++) {
+            if (use_cg) {
+                launch_spmm_dot(vk, bd, val_in, nullptr, w.p.p, w.v.p, w.p.p, false, ic + I_ACTIVE * K, part, st);
+                launch_finalize(FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
+                launch_cg_xr(vk, bd, idiag_in, w.xi.p, w.p.p, w.v.p, w.ri.p, sc, ic, part, st);
+                launch_finalize(FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
+                // PCG replaces the residual before the direction update
+                if (k == CHECK_EVERY - 1 && since + 1 >= ru) reliable_update();
+                launch_cg_d(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, st);
+            } else {
+                launch_bicg_p(vk, bd, w.ri.p, w.p.p, w.v.p, sc, ic, st);
+                launch_spmm_dot(vk, bd, op_in, nullptr, w.p.p, w.v.p, w.rdi.p, false, ic + I_ACTIVE * K, part, st);
+                launch_finalize(FIN_BICG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
+                launch_bicg_s(vk, bd, w.ri.p, w.v.p, w.s.p, sc, ic, st);
+                launch_spmm_dot(vk, bd, op_in, nullptr, w.s.p, w.t.p, w.s.p, true, ic + I_ACTIVE * K, part, st);
+                launch_finalize(FIN_BICG_OMEGA, bd, part, sc, ic, 0, 0, 0, st);
+                launch_bicg_xr(vk, bd, idiag_in, w.xi.p, w.p.p, w.s.p, w.t.p, w.ri.p, w.rdi.p, sc, ic, part, st);
+                launch_finalize(FIN_BICG_RHO, bd, part, sc, ic, maxit_total, 0, 0, st);
+            }
+        }
+        if (!use_cg && since >= ru) reliable_update();
+    }
+    // fold the last inner correction of columns still marked active (maxit
+    // exhausted) and record the final true residuals
+    launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
+    launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
+    TMI_HIP(hipMemcpyAsync(h_flags, ic, sizeof(int) * I_NSLOT * K, hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipMemcpyAsync(h_dbl, osc, sizeof(double) * S_NSLOT * K, hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    int notconv = 0;
+    for (int c = 0; c < kc; c++) {
+        double b2 = h_dbl[S_BNORM2 * K + c], r2 = h_dbl[S_RES2 * K + c];
+        double rel = b2 > 0.0 ? std::sqrt(r2 / b2) : 0.0;
+        bool conv = b2 == 0.0 || r2 <= tol2 * b2;
+        if (!conv) notconv++;
+        if (stats) {
+            stats[c].iters = h_flags[I_ITERS * K + c];
+            stats[c].sweeps = nupd;
+            stats[c].relres = rel;
+            stats[c].converged = conv ? 1 : 0;
+        }
+        last_iters += h_flags[I_ITERS * K + c];
+    }
+    if (verbose)
+        fprintf(stderr, "[tmi] tile K=%d: %d block iterations, %d reliable updates, %d not converged\n", K, it, nupd,
+                notconv);
+    return notconv;
+}
+
+int Solver::solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats)
+{
+    TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
+    TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
+              "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
+    if (precon == TMI_PRECON_DIAG && !precon_valid) reset_precon();
+    int K;
+    if (mode == TMI_MODE_FP64_SINGLE) K = 1;
+    else {
+        K = tile > 0 ? tile : 64;
+        while (K > 1 && K / 2 >= nq) K /= 2;   // smallest power of two >= nq, at most the tile
+    }
+    ensure_work(K);
+    size_t es = vbytes(vk64());
+    hipEvent_t e0, e1;
+    TMI_HIP(hipEventCreate(&e0));
+    TMI_HIP(hipEventCreate(&e1));
+    TMI_HIP(hipEventRecord(e0, st));
+    last_iters = 0.0;
+    int notconv = 0;
+    for (int q0 = 0; q0 < nq; q0 += K) {
+        int kc = std::min(K, nq - q0);
+        // gather the tile: columns q0..q0+kc of [n][ld] -> [n][K]
+        if (kc < K) {
+            TMI_HIP(hipMemsetAsync(w.b64.p, 0, w.b64.bytes(), st));
+            TMI_HIP(hipMemsetAsync(w.x64.p, 0, w.x64.bytes(), st));
+        }
+        TMI_HIP(hipMemcpy2DAsync(w.b64.p, K * es, (const char *)d_b + q0 * es, (size_t)ld * es, kc * es, mesh->n,
+                                 hipMemcpyDeviceToDevice, st));
+        TMI_HIP(hipMemcpy2DAsync(w.x64.p, K * es, (const char *)d_x + q0 * es, (size_t)ld * es, kc * es, mesh->n,
+                                 hipMemcpyDeviceToDevice, st));
+        notconv += solve_tile(kc, stats ? stats + q0 : nullptr);
+        TMI_HIP(hipMemcpy2DAsync((char *)d_x + q0 * es, (size_t)ld * es, w.x64.p, K * es, kc * es, mesh->n,
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    TMI_HIP(hipEventRecord(e1, st));
+    TMI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    last_solve_ms = ms;
+    last_iters = nq ? last_iters / nq : 0.0;
+    return notconv;
+}
+
+void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
+{
+    TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
+    TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
+    int K = w.K, n = mesh->n;
+    BlockDesc bd{n, K, w.G, mesh->rowptr.p, mesh->colidx.p};
+    std::vector<int> ones(K, 1);
+    DBuf<int> act;
+    act.upload(ones.data(), K, st);
+    bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
+    const void *idg = nullptr;
+    const void *op = mixed ? (const void *)val32.p : (const void *)val64.p;
+    if (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
+    auto launch = [&]() {
+        if (which == 0)
+            launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, w.rdi.p, false, act.p, w.partial.p, st);
+        else
+            launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, act.p, w.partial.p, st);
+    };
+    launch();   // warm
+    hipEvent_t e0, e1;
+    TMI_HIP(hipEventCreate(&e0));
+    TMI_HIP(hipEventCreate(&e1));
+    TMI_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) launch();
+    TMI_HIP(hipEventRecord(e1, st));
+    TMI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    // algorithmic bytes: matrix values + column indices once, row pointers,
+    // each X row read once, each Y row written once (+ the inverse diagonal)
+    double sv = (double)vbytes(which == 0 ? w.vk_in : vk64());
+    double nnz = mesh->nnz;
+    double b = nnz * (sv + 4.0) + 4.0 * (n + 1) + 2.0 * n * K * sv;
+
+    if (which == 1) b += (double)n * K * sv;   // the right-hand side block is read as well
+    *bytes = b;
+}
+
+}  // namespace tmi

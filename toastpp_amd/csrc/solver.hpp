@@ -1,0 +1,67 @@
+// Forward solver: the GPU counterpart of TFwdSolver<T> (fwdsolver.h:77-508).
+#pragma once
+#include "krylov.hpp"
+#include "mesh.hpp"
+
+namespace tmi {
+
+struct Solver {
+    Mesh *mesh = nullptr;
+    bool cplx = true;
+    int lsolver = TMI_LSOLVER_BICGSTAB;
+    double tol = 1e-10;
+    int precon = TMI_PRECON_NONE;
+    int maxit = 0;
+    int mode = TMI_MODE_MIXED_BLOCK;
+    int tile = 0;
+    double inner_tol = 1e-5;
+    int reliable_every = 16;   // inner iterations between FP64 residual replacements (mixed mode)
+    bool allocated = false, assembled = false, precon_valid = false;
+    double omega = 0.0;
+    hipStream_t st = nullptr;
+
+    DBuf<double> val64;    // nnz (real) or 2 nnz (complex)
+    DBuf<float> val32;
+    DBuf<char> idiag64, idiag32;
+    DBuf<double> pval64;   // A D^-1 (FP64) for the right-preconditioned inner iteration
+    DBuf<float> pval32;    // A D^-1 (FP32)
+    DBuf<double> coef;     // cmua | ckappa | c2a on the device
+
+    // workspace of one tile
+    struct Work {
+        int K = 0, n = 0, G = 0;
+        VKind vk_in = VK_C32;
+        DBuf<char> x64, b64, r64, xi, ri, rdi, p, v, s, t;
+        DBuf<double> partial, sc, osc;
+        DBuf<int> ic, oic;
+    } w;
+    int *h_flags = nullptr;        // pinned host mirror of per-column ints
+    double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
+    double last_solve_ms = 0.0, last_iters = 0.0;
+
+    ~Solver();
+    VKind vk64() const { return cplx ? VK_C64 : VK_R64; }
+    VKind vk32() const { return cplx ? VK_C32 : VK_R32; }
+    int nval() const { return cplx ? 2 * mesh->nnz : mesh->nnz; }
+    void allocate();
+    void assemble(const double *cmua, const double *ckap, const double *c2a, double omega, bool host_ptrs);
+    void reset_precon();
+    // Solve A X = B for nq columns.  d_b, d_x: [n][ld] node-major device
+    // arrays (FP64, complex or real); d_x holds the initial guess.
+    int solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
+    // Time `reps` launches of one kernel of the last tile (which: 0 = inner
+    // SpMM, 1 = FP64 residual SpMM) with HIP events on the solver stream.
+    // Returns the average ms and the algorithmic bytes of one launch.
+    void time_kernel(int which, int reps, double *avg_ms, double *bytes);
+
+   private:
+    void ensure_work(int K);
+    int solve_tile(int kc, tmi_colstat *stats);
+    int count_active(const int *d_flags);
+};
+
+// projections (mesh_kernels.hip): proj[k] = sum_j phi[q_k][j] * mvec[m_k][j]
+void launch_project(int nqm, const int *d_lq, const int *d_lm, const int *mrp, const int *mci, const double *mval,
+                    const double *phi, long sq, long sj, int cplx, double *proj, hipStream_t st);
+
+}  // namespace tmi

@@ -1,0 +1,153 @@
+"""FwdSolver: Python mirror of libstoast's TFwdSolver<T> (fwdsolver.h:77-508).
+
+Method names follow the reference (Allocate, AssembleSystemMatrix, Reset,
+CalcFields, ProjectAll, SetLinSolver, SetPrecon, ...); all work runs in
+libtoastpp_amd on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solution:
+    """FEM coefficients of the diffusion operator (Solution, solution.h:12-16):
+    cmua = c mua / n, ckappa = c / (3 n (mua + mus')), c2a = c / (2 n A(n))."""
+
+    def __init__(self, mua, mus, ref):
+        mua, mus, ref = _f64(mua), _f64(mus), _f64(ref)
+        n = len(mua)
+        if not (len(mus) == n and len(ref) == n):
+            raise ValueError("mua, mus and ref must have the same length")
+        self.cmua, self.ckappa, self.c2a = np.zeros(n), np.zeros(n), np.zeros(n)
+        L.call("tmi_optical_to_coeffs", n, mua, mus, ref, self.cmua, self.ckappa, self.c2a)
+
+
+class FwdSolver:
+    """TFwdSolver<double> (complex=False) or TFwdSolver<std::complex<double>>."""
+
+    def __init__(self, mesh, solver="BICGSTAB", tol=1e-10, complex=True):
+        self.mesh = mesh
+        self.complex = bool(complex)
+        h = C.c_void_p()
+        name = solver.upper()
+        ls = {"DIRECT": L.LSOLVER_DIRECT, "CG": L.LSOLVER_CG, "BICGSTAB": L.LSOLVER_BICGSTAB}.get(name)
+        if ls is None:
+            raise ValueError("unsupported linear solver %r (DIRECT, CG, BICGSTAB)" % solver)
+        L.call("tmi_solver_create", mesh.handle_ptr(), 1 if self.complex else 0, ls, float(tol), C.byref(h))
+        self._h = h
+        self.stats = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and L._lib is not None:
+            L._lib.tmi_solver_free(h)
+            self._h = None
+
+    # --- configuration (SetLinSolver / SetPrecon / SetLinSolverMaxit) ---
+    def SetLinSolver(self, solver, tol=1e-10):
+        L.call("tmi_solver_set_linsolver", self._h, solver.encode(), float(tol))
+
+    def SetPrecon(self, precon):
+        p = {"NONE": L.PRECON_NONE, "DIAG": L.PRECON_DIAG}.get(str(precon).upper(), precon)
+        L.call("tmi_solver_set_precon", self._h, int(p))
+
+    def SetLinSolverMaxit(self, maxit):
+        L.call("tmi_solver_set_maxit", self._h, int(maxit))
+
+    def SetMode(self, mode):
+        """'fp64' (single right-hand side), 'fp64_block' or 'mixed' (FP32 block + FP64 refinement)."""
+        m = {"fp64": L.MODE_FP64_SINGLE, "mixed": L.MODE_MIXED_BLOCK, "fp64_block": L.MODE_FP64_BLOCK}.get(mode, mode)
+        L.call("tmi_solver_set_mode", self._h, int(m))
+
+    def SetTile(self, tile):
+        L.call("tmi_solver_set_tile", self._h, int(tile))
+
+    def SetInnerTol(self, tol):
+        L.call("tmi_solver_set_inner_tol", self._h, float(tol))
+
+    # --- lifecycle ---
+    def Allocate(self):
+        L.call("tmi_solver_allocate", self._h)
+
+    def AssembleSystemMatrix(self, sol, omega=0.0):
+        L.call("tmi_solver_assemble", self._h, _f64(sol.cmua), _f64(sol.ckappa), _f64(sol.c2a), float(omega))
+
+    def Reset(self, sol, omega=0.0):
+        L.call("tmi_solver_reset", self._h, _f64(sol.cmua), _f64(sol.ckappa), _f64(sol.c2a), float(omega))
+
+    def nnz(self):
+        v = C.c_int()
+        L.call("tmi_solver_nnz", self._h, C.byref(v))
+        return v.value
+
+    def SystemMatrix(self):
+        """(rowptr, colidx, values) of the FP64 system matrix F."""
+        nz = self.nnz()
+        n = self.mesh.NodeCount()
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(nz, np.int32)
+        val = np.zeros(2 * nz if self.complex else nz)
+        L.call("tmi_solver_get_sysmat", self._h, L.ptr(rp), L.ptr(ci), L.ptr(val))
+        return rp, ci, (val.view(np.complex128) if self.complex else val)
+
+    def CalcFields(self, qvec, phi=None, allow_notconv=False):
+        """qvec: (nQ x n) scipy CSR matrix or dense array (row q = source q).
+        Returns phi (nQ x n); per-column records in self.stats."""
+        from scipy import sparse
+
+        q = sparse.csr_matrix(qvec)
+        nq, n = q.shape
+        dt = np.complex128 if self.complex else np.float64
+        vals = np.ascontiguousarray(q.data, dtype=dt)
+        if phi is None:
+            phi = np.zeros((nq, n), dt)
+        phi = np.ascontiguousarray(phi, dtype=dt)
+        stats = (L.ColStat * nq)()
+        st = L.call("tmi_solver_calc_fields", self._h, nq, np.ascontiguousarray(q.indptr, np.int32),
+                    np.ascontiguousarray(q.indices, np.int32), vals.view(np.float64), phi.view(np.float64),
+                    C.cast(stats, C.c_void_p), allow_notconv=allow_notconv)
+        self.stats = [(s.iters, s.sweeps, s.relres, bool(s.converged)) for s in stats]
+        self.converged = st == L.TMI_OK
+        return phi
+
+    def CalcFieldsDevice(self, nq, d_rhs, d_phi):
+        """GPU-resident CalcFields: d_rhs, d_phi are device pointers (int) to
+        [n][nq] node-major arrays; returns the per-column records."""
+        stats = (L.ColStat * nq)()
+        st = L.call("tmi_solver_calc_fields_device", self._h, nq, C.c_void_p(d_rhs), C.c_void_p(d_phi),
+                    C.cast(stats, C.c_void_p), allow_notconv=True)
+        self.stats = [(s.iters, s.sweeps, s.relres, bool(s.converged)) for s in stats]
+        self.converged = st == L.TMI_OK
+        return self.stats
+
+    def LastTiming(self):
+        ms, it = C.c_double(), C.c_double()
+        L.call("tmi_solver_last_timing", self._h, C.byref(ms), C.byref(it))
+        return ms.value, it.value
+
+    def TimeKernel(self, which=0, reps=20):
+        """(avg_ms, algorithmic_bytes) of the inner SpMM (0) or FP64 residual SpMM (1)."""
+        ms, b = C.c_double(), C.c_double()
+        L.call("tmi_solver_time_kernel", self._h, int(which), int(reps), C.byref(ms), C.byref(b))
+        return ms.value, b.value
+
+    def ProjectAll(self, mvec, phi, scale="log"):
+        """mvec: (nM x n) CSR/dense; phi: (nQ x n) -> nQM projections in link order."""
+        from scipy import sparse
+
+        m = sparse.csr_matrix(mvec)
+        dt = np.complex128 if self.complex else np.float64
+        nqm = self.mesh.qm_dims()[2]
+        proj = np.zeros(nqm, dt)
+        vals = np.ascontiguousarray(m.data, dtype=dt)
+        phi = np.ascontiguousarray(phi, dtype=dt)
+        L.call("tmi_solver_project_all", self._h, m.shape[0], np.ascontiguousarray(m.indptr, np.int32),
+               np.ascontiguousarray(m.indices, np.int32), vals.view(np.float64), phi.view(np.float64),
+               L.DATA_LOG if scale == "log" else L.DATA_LIN, proj.view(np.float64))
+        return proj
