@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-jacobian", action="store_true")
+    ap.add_argument("--jac-workload", default="cyl3d_jac")
+    ap.add_argument("--jac-reps", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +162,11 @@ def main():
         log("fp64 single-RHS: %.2f solves/s over %d columns, iters/col %.0f, err %.2e" %
             (k / t64, k, fp64["iters_per_col"], e64.max()))
 
+    del fm
+    jac = None
+    if not args.no_jacobian:
+        jac = jacobian_leg(args)
+
     # CPU baseline: FP64 oracle, one column, one thread
     cpu = None
     if not args.no_cpu:
@@ -189,9 +197,109 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_spmm_dot<float2,64> (inner block SpMM, complex FP32, 64 columns)",
                      "avg_launch_ms": round(spmm_ms, 4), "algorithmic_bytes": int(spmm_bytes)},
+        "jacobian": jac,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def jacobian_leg(args):
+    """Nodal Jacobian of log amplitude and phase w.r.t. cmua and ckappa on the
+    ~103k-node cylinder, 32 x 32 source-detector pairs, 100 MHz: direct and
+    adjoint fields from one 64-column mixed-precision block solve, linear
+    projections, then GenerateJacobian on the GPU (J = 2048 x 2n doubles,
+    resident in HBM).  The rows of two sources are checked against the FP64
+    oracle's Jacobian computed from the same fields."""
+    import ctypes as C
+
+    from scipy import sparse
+
+    import toastpp_amd as T
+    from toastpp_amd import _lib as L
+    from toastpp_amd import meshgen as G
+    from oracle import oracle as O
+
+    nodes, el, Q, M, mua, mus, ref, freq, desc = G.workload(args.jac_workload)
+    n = len(nodes)
+    log("jacobian workload", desc, "nodes", n, "elements", len(el))
+    mesh = T.Mesh(data=(nodes, el))
+    mesh.SetQM(Q, M)
+    nq, nm, nqm = mesh.qm_dims()
+    B = np.ascontiguousarray(np.hstack([mesh.Qvec(width=2.0).toarray(),
+                                        mesh.Mvec(width=2.0, ref=ref).toarray()]).astype(np.complex128))
+    dB = L.DeviceArray.from_host(B)
+    dX = L.DeviceArray(B.shape, np.complex128)
+    f = T.FwdSolver(mesh, "BICGSTAB", args.tol, complex=True)
+    f.SetMode("mixed")
+    f.SetPrecon("DIAG")
+    f.SetTile(64)
+    f.Allocate()
+    f.Reset(T.Solution(mua, mus, ref), freq * 2 * np.pi * 1e-6)
+    f.CalcFieldsDevice(nq + nm, dB.ptr, dX.ptr)            # warm
+    L.synchronize()
+    t0 = time.perf_counter()
+    st = f.CalcFieldsDevice(nq + nm, dB.ptr, dX.ptr)
+    L.synchronize()
+    t_fields = time.perf_counter() - t0
+    X = dX.to_host()
+    mrows = sparse.csr_matrix(B[:, nq:].T)
+    f.ProjectAllDevice(mrows, dX.ptr, nq + nm, scale="lin")
+    t0 = time.perf_counter()
+    proj = f.ProjectAllDevice(mrows, dX.ptr, nq + nm, scale="lin")
+    t_proj = time.perf_counter() - t0
+    dP = L.DeviceArray.from_host(proj)
+    dJ = L.DeviceArray((2 * nqm, 2 * n), np.float64)
+    h = mesh.handle_ptr()
+    base = dX.ptr
+
+    def run():
+        L.call("tmi_jacobian_nodal_device", h, nq, nm, C.c_void_p(base), nq + nm, C.c_void_p(base + 16 * nq),
+               nq + nm, C.c_void_p(dP.ptr), L.DATA_LOG, C.c_void_p(dJ.ptr))
+
+    run()
+    L.synchronize()
+    ts = []
+    for _ in range(max(1, args.jac_reps)):
+        t0 = time.perf_counter()
+        run()
+        L.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_jac = min(ts)
+    jbytes = 4.0 * nqm * n * 8 + n * (nq + nm) * 16
+    log("jacobian: fields %.1f ms (64 columns, iters/col %.0f), projections %.1f ms, nodal J %.1f ms "
+        "(%.0f GB/s of J + fields)" % (t_fields * 1e3, np.mean([s[0] for s in st]), t_proj * 1e3, t_jac * 1e3,
+                                       jbytes / t_jac / 1e9))
+    # parity: rows of sources 0 and nq-1 against the oracle on the same fields
+    lptr = np.arange(nq + 1, dtype=np.int32) * nm
+    rows = []
+    for q in (0, nq - 1):
+        rows += list(range(lptr[q], lptr[q + 1]))
+    rows = np.array(rows)
+    Jg = np.empty((2 * len(rows), 2 * n))
+    for t, k in enumerate(rows):
+        for blk in (0, 1):
+            r = k + blk * nqm
+            dst = Jg[t + blk * len(rows)]
+            L.call("tmi_memcpy_d2h", dst.ctypes.data_as(C.c_void_p), C.c_void_p(dJ.ptr + int(r) * 2 * n * 8), 2 * n * 8)
+    t0 = time.perf_counter()
+    om = O.Mesh(nodes=nodes, elems=el)
+    dsub = np.ascontiguousarray(X[:, [0, nq - 1]].T)
+    asub = np.ascontiguousarray(X[:, nq:].T)
+    lp2 = np.array([0, nm, 2 * nm], np.int32)
+    li2 = np.tile(np.arange(nm, dtype=np.int32), 2)
+    Jo = O.jacobian_nodal(om, dsub, asub, lp2, li2, proj[rows], logscale=True)
+    k2 = len(rows)
+    worst = 0.0
+    for r in (slice(0, k2), slice(k2, 2 * k2)):
+        for c in (slice(0, n), slice(n, 2 * n)):
+            worst = max(worst, np.max(np.abs(Jg[r, c] - Jo[r, c])) / np.max(np.abs(Jo[r, c])))
+    log("jacobian parity (%d rows vs FP64 oracle, %.1f s): worst block-relative diff %.2e" %
+        (2 * k2, time.perf_counter() - t0, worst))
+    del dJ, f
+    return {"workload": "%s: %s" % (args.jac_workload, desc), "nodes": n, "measurements": nqm,
+            "fields_s": round(t_fields, 4), "projections_s": round(t_proj, 4), "nodal_jacobian_s": round(t_jac, 4),
+            "total_s": round(t_fields + t_proj + t_jac, 4), "jacobian_GBps": round(jbytes / t_jac / 1e9, 1),
+            "parity_rows": int(2 * k2), "max_block_rel_diff_vs_oracle": float(worst)}
 
 
 def cpu_baseline(nodes, el, Q, mua, mus, ref, omega, tol, budget_s, gpu_iters):

@@ -178,6 +178,10 @@ int tmi_solver_calc_fields_device(tmi_solver s, int nq, const void *d_rhs, void 
  * dscale == TMI_DATA_LOG.  Uses the mesh's QM link list.  mvec: nm x nlen CSR. */
 int tmi_solver_project_all(tmi_solver s, int nm, const int *mrowptr, const int *mcolidx,
                            const double *mval, const double *phi, int dscale, double *proj);
+/* Same with GPU-resident fields d_phi: nlen rows of ld entries (first nq
+ * used), node-major as tmi_solver_calc_fields_device writes them. */
+int tmi_solver_project_all_device(tmi_solver s, int nm, const int *mrowptr, const int *mcolidx,
+                                  const double *mval, const void *d_phi, int ld, int dscale, double *proj);
 /* Last CalcFields: GPU time (ms, HIP events) and mean Krylov iterations per column */
 int tmi_solver_last_timing(tmi_solver s, double *solve_ms, double *iters_per_col);
 /* Measurement hook (no reference counterpart): time `reps` launches of one
@@ -189,12 +193,20 @@ int tmi_solver_time_kernel(tmi_solver s, int which, int reps, double *avg_ms, do
 /* --------------------------------------------------------------- Jacobian */
 /* GenerateJacobian, nodal basis (jacobian.cc:109-126 with RASTER_NDBASIS ->
  * GenerateJacobian_mesh :441-497, IntFG :806-835, IntGradFGradG :839-872).
- * dphi: nq x nlen complex, aphi: nm x nlen complex, proj: nqm complex linear
- * projections (link-list order).  J: (2 nqm) x (2 nlen) row-major doubles
- * [dlnA/dmua | dlnA/dkappa ; dphase/dmua | dphase/dkappa] for
- * dscale == TMI_DATA_LOG, [Re ; Im] of the linear PMDFs otherwise. */
+ * dphi: nlen x nq complex, aphi: nlen x nm complex, both node-major as
+ * toast.Mesh.Fields returns them (toastmodule.cc:1454-1485); proj: nqm complex
+ * linear projections in link-list order.  J: (2 nqm) x (2 nlen) row-major
+ * doubles [dlnA/dmua | dlnA/dkappa ; dphase/dmua | dphase/dkappa] for
+ * dscale == TMI_DATA_LOG, [Re ; Im] of the linear PMDFs otherwise.
+ * Columns are derivatives with respect to the nodal cmua and ckappa. */
 int tmi_jacobian_nodal(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi,
                        const double *proj, int dscale, double *J);
+/* Same on GPU-resident arrays: d_dphi nlen rows of ldd complex (first nq
+ * used), d_aphi nlen rows of lda complex (first nm used), so that direct and
+ * adjoint fields of one block solve can be passed in place; d_J must hold
+ * 4 nqm nlen doubles. */
+int tmi_jacobian_nodal_device(tmi_mesh m, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
+                              int lda, const void *d_proj, int dscale, void *d_J);
 
 /* ----------------------------------------------------------------- raster */
 /* Raster_Pixel with bdim == gdim (raster.cc:18-100, raster_px.cc:10-54):

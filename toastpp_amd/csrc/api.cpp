@@ -8,9 +8,8 @@
 
 #include "solver.hpp"
 
-struct tmi_mesh_s {
-    tmi::Mesh m;
-};
+#include "handles.hpp"
+
 struct tmi_solver_s {
     tmi::Solver s;
 };
@@ -293,6 +292,7 @@ static void set_qm(Mesh &m, QM qm, bool pull)
         }
     }
     m.qm = std::move(qm);
+    m.links_valid = false;
 }
 
 int tmi_mesh_read_qm(tmi_mesh h, const char *path)
@@ -611,31 +611,30 @@ int tmi_solver_calc_fields_device(tmi_solver h, int nq, const void *d_rhs, void 
     API_END
 }
 
-int tmi_solver_project_all(tmi_solver h, int nm, const int *mrp, const int *mci, const double *mval,
-                           const double *phi, int dscale, double *proj)
+// ProjectAll on fields held either on the host ([nq][n], source-major as the
+// reference's phi array) or on the device ([n][ld] node-major, block-solve
+// layout): sq / sj are the strides of source and node in complex entries
+static void project_all(Solver &s, int nm, const int *mrp, const int *mci, const double *mval, const double *d_phi,
+                        long sq, long sj, int dscale, double *proj)
 {
-    API_BEGIN
-    Solver &s = S(h);
-    NEED(mrp); NEED(mci); NEED(mval); NEED(phi); NEED(proj);
     Mesh &m = *s.mesh;
     const QM &qm = m.qm;
     TMI_CHECK(nm == qm.nm, TMI_ERR_INVALID_ARG, "mvec has %d rows, the QM description %d detectors", nm, qm.nm);
     int n = m.n, cw = s.cplx ? 2 : 1, nqm = qm.nqm;
-    if (nqm == 0) return TMI_OK;
+    if (nqm == 0) return;
     std::vector<int> lq(nqm), lm(nqm);
     for (int q = 0; q < qm.nq; q++)
         for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) { lq[k] = q; lm[k] = qm.lidx[k]; }
     int mnnz = mrp[nm];
     for (int k = 0; k < mnnz; k++) TMI_CHECK(mci[k] >= 0 && mci[k] < n, TMI_ERR_INVALID_ARG, "mvec column out of range");
     DBuf<int> dlq, dlm, drp, dci;
-    DBuf<double> dval, dphi, dproj((size_t)nqm * cw);
+    DBuf<double> dval, dproj((size_t)nqm * cw);
     dlq.upload(lq.data(), nqm, s.st);
     dlm.upload(lm.data(), nqm, s.st);
     drp.upload(mrp, nm + 1, s.st);
     dci.upload(mci, mnnz, s.st);
     dval.upload(mval, (size_t)mnnz * cw, s.st);
-    dphi.upload(phi, (size_t)qm.nq * n * cw, s.st);
-    launch_project(nqm, dlq.p, dlm.p, drp.p, dci.p, dval.p, dphi.p, n, 1, s.cplx ? 1 : 0, dproj.p, s.st);
+    launch_project(nqm, dlq.p, dlm.p, drp.p, dci.p, dval.p, d_phi, sq, sj, s.cplx ? 1 : 0, dproj.p, s.st);
     dproj.download(proj, (size_t)nqm * cw, s.st);
     TMI_HIP(hipStreamSynchronize(s.st));
     if (dscale == TMI_DATA_LOG) {   // complex log: Re = ln|y|, Im = arg y
@@ -649,6 +648,30 @@ int tmi_solver_project_all(tmi_solver h, int nm, const int *mrp, const int *mci,
             }
         }
     }
+}
+
+int tmi_solver_project_all(tmi_solver h, int nm, const int *mrp, const int *mci, const double *mval,
+                           const double *phi, int dscale, double *proj)
+{
+    API_BEGIN
+    Solver &s = S(h);
+    NEED(mrp); NEED(mci); NEED(mval); NEED(phi); NEED(proj);
+    Mesh &m = *s.mesh;
+    DBuf<double> dphi;
+    dphi.upload(phi, (size_t)m.qm.nq * m.n * (s.cplx ? 2 : 1), s.st);
+    project_all(s, nm, mrp, mci, mval, dphi.p, m.n, 1, dscale, proj);
+    API_END
+}
+
+int tmi_solver_project_all_device(tmi_solver h, int nm, const int *mrp, const int *mci, const double *mval,
+                                  const void *d_phi, int ld, int dscale, double *proj)
+{
+    API_BEGIN
+    Solver &s = S(h);
+    NEED(mrp); NEED(mci); NEED(mval); NEED(d_phi); NEED(proj);
+    TMI_CHECK(ld >= s.mesh->qm.nq, TMI_ERR_INVALID_ARG, "field row stride %d below the source count %d", ld,
+              s.mesh->qm.nq);
+    project_all(s, nm, mrp, mci, mval, (const double *)d_phi, 1, ld, dscale, proj);
     API_END
 }
 

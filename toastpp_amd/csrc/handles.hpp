@@ -1,0 +1,7 @@
+// Opaque C-ABI handle types shared by the entry-point translation units.
+#pragma once
+#include "mesh.hpp"
+
+struct tmi_mesh_s {
+    tmi::Mesh m;
+};

@@ -1,0 +1,17 @@
+// Jacobian (nodal and raster basis) launchers.
+#pragma once
+#include "mesh.hpp"
+
+namespace tmi {
+
+// d_dphi [n][ldd], d_aphi [n][lda] complex (node-major, the first nq / nm
+// entries of each row used); d_lptr: link-list
+// row pointer (nq+1); d_lq / d_lm: source and detector of every linked
+// measurement (link-list order, source-major); d_proj: nqm
+// complex linear projections; d_Pt: scratch of n*nqm*4 doubles; d_J: (2 nqm)
+// x (2 n) row-major.  dmax: longest CSR row of the mesh.
+void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                           const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
+                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st);
+
+}  // namespace tmi

@@ -1,0 +1,124 @@
+// Jacobian entry points of the C-ABI: GenerateJacobian for the nodal basis
+// (jacobian.cc:109-126 with RASTER_NDBASIS) and the Raster_Pixel basis.
+#include <cstring>
+#include <vector>
+
+#include "jacobian.hpp"
+#include "solver.hpp"
+
+#include "handles.hpp"
+
+namespace tmi {
+int mesh_dmax(Mesh &m, hipStream_t st);
+}
+
+using namespace tmi;
+
+#define API_BEGIN try {
+#define API_END                                   \
+    return TMI_OK;                                \
+    }                                             \
+    catch (const tmi::Error &e)                   \
+    {                                             \
+        set_last_error(e.what());                 \
+        return e.code;                            \
+    }                                             \
+    catch (const std::exception &e)               \
+    {                                             \
+        set_last_error(e.what());                 \
+        return TMI_ERR_INVALID_ARG;               \
+    }
+
+namespace tmi {
+
+// longest CSR row (neighbour count) of the mesh, cached
+int mesh_dmax(Mesh &m, hipStream_t st)
+{
+    m.build_csr(st);
+    if (m.dmax > 0) return m.dmax;
+    std::vector<int> rp = m.rowptr.to_host(st);
+    int d = 0;
+    for (int i = 0; i < m.n; i++) d = std::max(d, rp[i + 1] - rp[i]);
+    m.dmax = d;
+    return d;
+}
+
+// device copy of the link list: row pointer, source and detector per link
+static void ensure_links(Mesh &m, hipStream_t st)
+{
+    if (m.links_valid) return;
+    const QM &qm = m.qm;
+    std::vector<int> lq(qm.nqm), lm(qm.nqm);
+    for (int q = 0; q < qm.nq; q++)
+        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) { lq[k] = q; lm[k] = qm.lidx[k]; }
+    m.d_lptr.upload(qm.lptr.data(), qm.lptr.size(), st);
+    m.d_lq.upload(lq.data(), lq.size(), st);
+    m.d_lm.upload(lm.data(), lm.size(), st);
+    TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
+    m.links_valid = true;
+}
+
+static void jacobian_nodal_dev(Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi,
+                               int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st)
+{
+    const QM &qm = m.qm;
+    TMI_CHECK(qm.nq == nq && qm.nm == nm, TMI_ERR_INVALID_ARG,
+              "field counts (%d sources, %d detectors) do not match the QM description (%d, %d)", nq, nm, qm.nq,
+              qm.nm);
+    TMI_CHECK(qm.nqm > 0, TMI_ERR_INVALID_ARG, "the QM description links no measurements");
+    TMI_CHECK(ldd >= nq && lda >= nm, TMI_ERR_INVALID_ARG, "field row strides (%d, %d) below the field counts (%d, %d)",
+              ldd, lda, nq, nm);
+    TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
+    const int dmax = mesh_dmax(m, st);
+    ensure_links(m, st);
+    const size_t ws = (size_t)m.n * qm.nqm * 4;
+    if (m.jac_ws.n < ws) m.jac_ws.alloc(ws);
+    launch_jacobian_nodal(m, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lq.p, m.d_lm.p, qm.nqm, d_proj,
+                          dscale == TMI_DATA_LOG ? 1 : 0, m.jac_ws.p, d_J, dmax, st);
+}
+
+}  // namespace tmi
+
+extern "C" {
+
+int tmi_jacobian_nodal(tmi_mesh h, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                       int dscale, double *J)
+{
+    API_BEGIN
+    TMI_CHECK(h && dphi && aphi && proj && J, TMI_ERR_INVALID_ARG, "NULL argument");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    size_t n = m.n, nqm = m.qm.nqm;
+    DBuf<double> dd, da, dp, dJ(4 * nqm * n);
+    dd.upload(dphi, n * nq * 2, st);
+    da.upload(aphi, n * nm * 2, st);
+    dp.upload(proj, nqm * 2, st);
+    jacobian_nodal_dev(m, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
+    dJ.download(J, 4 * nqm * n, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_nodal_device(tmi_mesh h, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi, int lda,
+                              const void *d_proj, int dscale, void *d_J)
+{
+    API_BEGIN
+    TMI_CHECK(h && d_dphi && d_aphi && d_proj && d_J, TMI_ERR_INVALID_ARG, "NULL argument");
+    hipStream_t st = Context::get().stream;
+    jacobian_nodal_dev(h->m, nq, nm, (const double *)d_dphi, ldd, (const double *)d_aphi, lda,
+                       (const double *)d_proj, dscale, (double *)d_J, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_create(tmi_mesh, const int *, tmi_raster *) { return TMI_ERR_UNSUPPORTED; }
+int tmi_raster_free(tmi_raster) { return TMI_OK; }
+int tmi_raster_dims(tmi_raster, int *, int *) { return TMI_ERR_UNSUPPORTED; }
+int tmi_raster_get_elref(tmi_raster, int *, int *) { return TMI_ERR_UNSUPPORTED; }
+int tmi_raster_map_mesh_to_grid(tmi_raster, const double *, double *) { return TMI_ERR_UNSUPPORTED; }
+int tmi_jacobian_raster(tmi_raster, int, int, const double *, const double *, const double *, int, double *)
+{
+    return TMI_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

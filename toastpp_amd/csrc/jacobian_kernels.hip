@@ -1,0 +1,225 @@
+// Nodal-basis Jacobian (GenerateJacobian_mesh, jacobian.cc:441-497) on the GPU.
+//
+// For node i with neighbour set N(i) (its CSR row) the reference's PMDFs
+//   pmdf_mua_i(q,m) = sum_el sum_jk IntFFF(i,j,k) f_j g_k   (IntFG, jacobian.cc:806-835)
+//   pmdf_kap_i(q,m) = sum_el sum_jk IntFDD(i,j,k) f_j g_k   (IntGradFGradG, :839-872)
+// with f = dphi_q, g = aphi_m are bilinear forms f^T A_i g and f^T B_i g,
+// where A_i, B_i (|N(i)| x |N(i)|) collect the element integrals of the
+// elements holding i.  Over all source/detector pairs they are two small
+// GEMMs per node: P_i = F_i (A_i G_i^T), F_i = dphi on N(i) (nq x d),
+// G_i = aphi on N(i) (nm x d).  One wavefront per node: it builds A_i, B_i
+// in LDS from the element integrals, then for each chunk of detectors
+// gathers G into LDS and forms H = G A, K = G B, and for each chunk of
+// sources gathers F and forms P for the linked pairs of the two chunks,
+// written node-major (one contiguous row of the intermediate per node).  A tiled transpose kernel
+// then scales by 1/proj (log data), negates and writes J in the reference
+// layout [dlnA/dmua | dlnA/dkappa ; dphase/dmua | dphase/dkappa].
+//
+// This file is compiled without floating-point contraction so that A_i and
+// B_i use the exact element integrals of the assembly; the GEMM loops opt in
+// to FMA contraction with a pragma.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fe_integrals.hpp"
+#include "jacobian.hpp"
+
+namespace tmi {
+
+struct JacDims {
+    int n, nq, nm, nqm, dmax, chunk, ldd, lda;
+};
+
+__device__ inline int find_pos(const int *cols, int d, int v)
+{
+    for (int t = 0; t < d; t++)
+        if (cols[t] == v) return t;
+    return -1;
+}
+
+// LDS layout per wave (doubles): A[dmax^2] B[dmax^2] W[C dmax] (complex: the
+// detector chunk G, then the source chunk F) H[C dmax] Kk[C dmax] cols[dmax]
+// (ints), C = source/detector chunk
+static size_t jac_lds_doubles(int dmax, int chunk)
+{
+    return (size_t)2 * dmax * dmax + (size_t)6 * chunk * dmax + (dmax + 1) / 2 + 1;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ colidx, const int *__restrict__ n2e_ptr,
+                                                  const int *__restrict__ n2e, const int *__restrict__ el,
+                                                  const double *__restrict__ geo,
+                                                  const double2 *__restrict__ dphi,   // [n][ldd]
+                                                  const double2 *__restrict__ aphi,   // [n][lda]
+                                                  const int *__restrict__ lptr,       // [nq+1]
+                                                  const int *__restrict__ lq, const int *__restrict__ lm,
+                                                  double2 *__restrict__ Pt)           // [n][nqm][2]
+{
+    constexpr int NN = DIM + 1;
+    constexpr int GS = DIM == 3 ? GEO3 : GEO2;
+    extern __shared__ double lds[];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int dmax = jd.dmax, nq = jd.nq, nm = jd.nm, C = jd.chunk;
+    double *A = lds, *B = A + dmax * dmax;
+    double2 *W = (double2 *)(B + dmax * dmax);
+    double2 *H = W + C * dmax, *Kk = H + C * dmax;
+    int *cols = (int *)(Kk + C * dmax);
+    const int rb = rowptr[i], d = rowptr[i + 1] - rb;
+    for (int t = lane; t < d; t += 64) cols[t] = colidx[rb + t];
+    for (int t = lane; t < dmax * dmax; t += 64) { A[t] = 0.0; B[t] = 0.0; }
+    __syncthreads();
+    // element integrals, one element after another (deterministic sums):
+    // lane (a, b) adds IntFFF(li, a, b) and IntFDD(li, a, b) of the element
+    for (int q = n2e_ptr[i]; q < n2e_ptr[i + 1]; q++) {
+        const int e = n2e[q];
+        const int *nd = el + (long)NN * e;
+        const double *g = geo + (long)GS * e;
+        if (lane < NN * NN) {
+            const int a = lane / NN, b = lane % NN;
+            int li = 0;
+            for (int k = 0; k < NN; k++)
+                if (nd[k] == i) li = k;
+            const int pa = find_pos(cols, d, nd[a]), pb = find_pos(cols, d, nd[b]);
+            double fff = DIM == 3 ? tet_intfff(g, li, a, b) : tri_intfff(g, li, a, b);
+            double fdd = DIM == 3 ? tet_intfdd(g, a, b) : tri_intfdd(g, a, b);
+            if (pa >= 0 && pb >= 0) {   // always: the row of i holds every node of its elements
+                A[pa * dmax + pb] += fff;
+                B[pa * dmax + pb] += fdd;
+            }
+        }
+        __syncthreads();
+    }
+    double2 *out = Pt + (size_t)i * jd.nqm * 2;
+    for (int m0 = 0; m0 < nm; m0 += C) {
+        const int mc = min(C, nm - m0);
+        // detector chunk G on the neighbour set
+        for (int t = lane; t < mc * d; t += 64) {
+            const int mm = t / d, j = t % d;
+            W[mm * dmax + j] = aphi[(size_t)cols[j] * jd.lda + m0 + mm];
+        }
+        __syncthreads();
+        {
+#pragma clang fp contract(fast)
+            // H = G A^T, Kk = G B^T  (A, B symmetric)
+            for (int t = lane; t < mc * d; t += 64) {
+                const int mm = t / d, j = t % d;
+                double hr = 0.0, hi = 0.0, kr = 0.0, ki = 0.0;
+                for (int k = 0; k < d; k++) {
+                    const double2 gv = W[mm * dmax + k];
+                    const double a = A[j * dmax + k], b = B[j * dmax + k];
+                    hr += a * gv.x; hi += a * gv.y;
+                    kr += b * gv.x; ki += b * gv.y;
+                }
+                H[mm * dmax + j] = make_double2(hr, hi);
+                Kk[mm * dmax + j] = make_double2(kr, ki);
+            }
+        }
+        __syncthreads();
+        for (int q0 = 0; q0 < nq; q0 += C) {
+            const int qc = min(C, nq - q0);
+            for (int t = lane; t < qc * d; t += 64) {
+                const int qq = t / d, j = t % d;
+                W[qq * dmax + j] = dphi[(size_t)cols[j] * jd.ldd + q0 + qq];
+            }
+            __syncthreads();
+            {
+#pragma clang fp contract(fast)
+                // P(q,m) = F_q . H_m for the linked pairs of this chunk pair
+                for (int k = lptr[q0] + lane; k < lptr[q0 + qc]; k += 64) {
+                    const int mrel = lm[k] - m0;
+                    if (mrel < 0 || mrel >= mc) continue;
+                    const double2 *f = W + (lq[k] - q0) * dmax;
+                    const double2 *h = H + mrel * dmax, *kk = Kk + mrel * dmax;
+                    double mr = 0.0, mi = 0.0, pr = 0.0, pi = 0.0;
+                    for (int j = 0; j < d; j++) {
+                        const double2 fv = f[j], hv = h[j], kv = kk[j];
+                        mr += fv.x * hv.x - fv.y * hv.y;
+                        mi += fv.x * hv.y + fv.y * hv.x;
+                        pr += fv.x * kv.x - fv.y * kv.y;
+                        pi += fv.x * kv.y + fv.y * kv.x;
+                    }
+                    out[2 * k] = make_double2(mr, mi);
+                    out[2 * k + 1] = make_double2(pr, pi);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// J[k][i] = -Re(P_mua/proj_k), J[k+nqm][i] = -Im(.), J[k][n+i] = -Re(P_kap/proj_k),
+// J[k+nqm][n+i] = -Im(.)  (DATA_LOG; without log the division is skipped).
+// Tiles of 64 nodes x 16 measurements through LDS: the reads take 512-byte
+// runs of one node's measurements, the writes 512-byte runs of one J row.
+constexpr int TJK = 16;
+__global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const double2 *__restrict__ Pt,
+                                                       const double2 *__restrict__ proj, int logscale,
+                                                       double *__restrict__ J)
+{
+    __shared__ double2 tm[64][TJK + 1], tk[64][TJK + 1];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int t = threadIdx.x;
+    // read: node r, measurement c (2 values each)
+    for (int e = t; e < 64 * TJK; e += 256) {
+        const int r = e / TJK, c = e % TJK;
+        const int i = i0 + r, k = k0 + c;
+        if (i < n && k < nqm) {
+            tm[r][c] = Pt[((size_t)i * nqm + k) * 2];
+            tk[r][c] = Pt[((size_t)i * nqm + k) * 2 + 1];
+        }
+    }
+    __syncthreads();
+    const size_t ld = 2 * (size_t)n;
+    for (int e = t; e < 64 * TJK; e += 256) {
+        const int c = e / 64, r = e % 64;
+        const int k = k0 + c, i = i0 + r;
+        if (k < nqm && i < n) {
+            double2 a = tm[r][c], b = tk[r][c];
+            if (logscale) {
+                const double2 p = proj[k];
+                const double den = p.x * p.x + p.y * p.y;
+                a = make_double2((a.x * p.x + a.y * p.y) / den, (a.y * p.x - a.x * p.y) / den);
+                b = make_double2((b.x * p.x + b.y * p.y) / den, (b.y * p.x - b.x * p.y) / den);
+            }
+            J[(size_t)k * ld + i] = -a.x;
+            J[(size_t)(k + nqm) * ld + i] = -a.y;
+            J[(size_t)k * ld + n + i] = -b.x;
+            J[(size_t)(k + nqm) * ld + n + i] = -b.y;
+        }
+    }
+}
+
+void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                           const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
+                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st)
+{
+    // source/detector chunk: all of them if the wave's LDS stays within 64 KiB
+    // (two waves per CU), else the largest chunk that fits, up to 160 KiB
+    const int want = std::max(nq, nm);
+    int chunk = want;
+    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > 64 * 1024) chunk--;
+    if (chunk < std::min(want, 8))
+        while (chunk < want && jac_lds_doubles(dmax, chunk + 1) * sizeof(double) <= 160 * 1024) chunk++;
+    const size_t lds = jac_lds_doubles(dmax, chunk) * sizeof(double);
+    TMI_CHECK(lds <= 160 * 1024, TMI_ERR_UNSUPPORTED,
+              "nodal Jacobian: %d neighbours per node exceed the LDS of one CU", dmax);
+    JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda};
+    if (m.dim == 3)
+        hipLaunchKernelGGL(k_jac_nodes<3>, dim3(m.n), dim3(64), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
+                           m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
+                           (double2 *)d_Pt);
+    else
+        hipLaunchKernelGGL(k_jac_nodes<2>, dim3(m.n), dim3(64), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
+                           m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
+                           (double2 *)d_Pt);
+    TMI_LAUNCH_CHECK();
+    dim3 g((m.n + 63) / 64, (nqm + TJK - 1) / TJK);
+    hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
+                       (const double2 *)d_proj, logscale, d_J);
+    TMI_LAUNCH_CHECK();
+}
+
+}  // namespace tmi

@@ -32,10 +32,17 @@ struct Mesh {
     // CSR pattern of the system matrix (built lazily)
     bool have_csr = false;
     int nnz = 0;
+    int dmax = 0;                      // longest CSR row (0 = not yet computed)
     DBuf<int> rowptr, colidx, rowidx, diagpos;
     // host-side caches
     std::vector<int> h_bnd;
     QM qm;
+    // device copy of the QM link list for the Jacobian (row pointer, source
+    // and detector per link; rebuilt when the QM changes) and the Jacobian's
+    // node-major intermediate, kept between calls
+    bool links_valid = false;
+    DBuf<int> d_lptr, d_lq, d_lm;
+    DBuf<double> jac_ws;
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }
     void setup(hipStream_t st);        // Mesh::Setup on the GPU

@@ -137,6 +137,21 @@ class FwdSolver:
         L.call("tmi_solver_time_kernel", self._h, int(which), int(reps), C.byref(ms), C.byref(b))
         return ms.value, b.value
 
+    def ProjectAllDevice(self, mvec, d_phi, ld, scale="log"):
+        """ProjectAll on GPU-resident fields: d_phi is a device pointer (int)
+        to [n][ld] node-major fields (the first nQ entries of each row)."""
+        from scipy import sparse
+
+        m = sparse.csr_matrix(mvec)
+        dt = np.complex128 if self.complex else np.float64
+        nqm = self.mesh.qm_dims()[2]
+        proj = np.zeros(nqm, dt)
+        vals = np.ascontiguousarray(m.data, dtype=dt)
+        L.call("tmi_solver_project_all_device", self._h, m.shape[0], np.ascontiguousarray(m.indptr, np.int32),
+               np.ascontiguousarray(m.indices, np.int32), vals.view(np.float64), C.c_void_p(d_phi), ld,
+               L.DATA_LOG if scale == "log" else L.DATA_LIN, proj.view(np.float64))
+        return proj
+
     def ProjectAll(self, mvec, phi, scale="log"):
         """mvec: (nM x n) CSR/dense; phi: (nQ x n) -> nQM projections in link order."""
         from scipy import sparse

@@ -214,15 +214,18 @@ class Mesh:
         from .jacobian import jacobian
         return jacobian(self, raster, dphi, aphi, proj)
 
-    def JacobianOptical(self, raster, qvec, mvec, mua, mus, ref, freq=0, solver="bicgstab", tol=1e-12):
+    def JacobianOptical(self, raster, qvec, mvec, mua, mus, ref, freq=0, solver="bicgstab", tol=1e-12,
+                        mode="mixed"):
         """Fields, adjoint fields, projections and Jacobian from optical
-        parameters (toastmodule.cc:1514-1627)."""
+        parameters (toastmodule.cc:1514-1627).  mode selects the GPU solve
+        ('mixed', 'fp64', 'fp64_block'), as in Fields."""
         from .jacobian import jacobian
         q = sparse.csc_matrix(qvec)
         m = sparse.csc_matrix(mvec)
         qrows = sparse.csr_matrix((q.data, q.indices, q.indptr), shape=(q.shape[1], q.shape[0]))
         mrows = sparse.csr_matrix((m.data, m.indices, m.indptr), shape=(m.shape[1], m.shape[0]))
         fws = FwdSolver(self, solver, tol, complex=True)
+        fws.SetMode(mode)
         fws.Allocate()
         fws.Reset(Solution(mua, mus, ref), freq * 2.0 * np.pi * 1e-6)
         dphi = fws.CalcFields(qrows)
