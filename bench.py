@@ -31,6 +31,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the inner SpMM, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy; the file gives the commands
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmm_pmc_traffic_ellipsoid250k_64src.md")
+
+
+def measured_traffic(workload, tile):
+    if workload != "ellipsoid3d" or tile != 64 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1])
+    return None
 
 
 def log(*a):
@@ -58,6 +71,10 @@ def main():
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--jac-workload", default="cyl3d_jac")
     ap.add_argument("--jac-reps", type=int, default=3)
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="short run for hardware-counter passes: one mixed solve, 5 timed launches of the inner "
+                         "SpMM, then three 1 GiB device-to-device copies that calibrate FETCH_SIZE / WRITE_SIZE; "
+                         "prints no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -99,6 +116,15 @@ def main():
 
     fm, asm_ms = make_solver("mixed", args.tol, args.tile)
     log("assembly + preconditioner (host call incl. upload) %.1f ms" % asm_ms)
+    if args.pmc_pass:
+        fm.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
+        fm.TimeKernel(0, reps=5)
+        size = 1 << 30
+        a, b = L.DeviceArray((size,), np.uint8), L.DeviceArray((size,), np.uint8)
+        for _ in range(3):
+            L.call("tmi_memcpy_d2d", b._p, a._p, size)
+        log("pmc pass done: 5 SpMM launches, 3 copies of %d bytes" % size)
+        return
 
     # reference fields: FP64 block solve at tight tolerance (untimed)
     fr, _ = make_solver("fp64_block", 1e-13, args.tile)
@@ -194,7 +220,8 @@ def main():
         "fp64_single": fp64,
         "speedup_vs_fp64_single": (round(value / fp64["solves_per_s"], 3) if fp64 else None),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.workload, args.tile),
                      "kernel": "k_spmm_dot<float2,64> (inner block SpMM, complex FP32, 64 columns)",
                      "avg_launch_ms": round(spmm_ms, 4), "algorithmic_bytes": int(spmm_bytes)},
         "jacobian": jac,

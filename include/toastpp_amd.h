@@ -79,6 +79,7 @@ int tmi_device_alloc(unsigned long long bytes, void **ptr);
 int tmi_device_free(void *ptr);
 int tmi_memcpy_h2d(void *dst, const void *src, unsigned long long bytes);
 int tmi_memcpy_d2h(void *dst, const void *src, unsigned long long bytes);
+int tmi_memcpy_d2d(void *dst, const void *src, unsigned long long bytes);
 
 /* ------------------------------------------------------------------- mesh */
 /* Host-only parser of a "MeshData 5.0" file (Mesh operator>>, mesh.cc:1787-1825,

@@ -59,6 +59,7 @@ _SIGS = {
     "tmi_device_free": [P],
     "tmi_memcpy_h2d": [P, P, C.c_ulonglong],
     "tmi_memcpy_d2h": [P, P, C.c_ulonglong],
+    "tmi_memcpy_d2d": [P, P, C.c_ulonglong],
     "tmi_read_mesh_file": [C.c_char_p, IP, IP, IP, IP, P, P, P],
     "tmi_mesh_make": [C.c_int, C.c_int, D, C.c_int, I, PP],
     "tmi_mesh_read": [C.c_char_p, PP],

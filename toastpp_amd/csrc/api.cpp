@@ -142,6 +142,15 @@ int tmi_memcpy_d2h(void *dst, const void *src, unsigned long long bytes)
     API_END
 }
 
+int tmi_memcpy_d2d(void *dst, const void *src, unsigned long long bytes)
+{
+    API_BEGIN
+    hipStream_t st = Context::get().stream;
+    TMI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
 // ------------------------------------------------------------------ mesh
 int tmi_read_mesh_file(const char *path, int *dim, int *nlen, int *elen, int *nnode, double *nodes, int *elist,
                        int *node_region)
