@@ -151,6 +151,25 @@ int tmi_solver_set_mode(tmi_solver s, int mode);                /* TMI_MODE_* */
 int tmi_solver_set_tile(tmi_solver s, int tile);
 /* Inner relative tolerance of the FP32 Krylov solve of the mixed mode */
 int tmi_solver_set_inner_tol(tmi_solver s, double inner_tol);
+/* Node order of the solver's private copy of the system (no reference
+ * counterpart): TMI_ORDER_MORTON (default) relabels the nodes along a
+ * Z-order curve of their coordinates so that the sparse kernels' neighbour
+ * gathers stay in cache; TMI_ORDER_NATURAL keeps the mesh order.  Inputs,
+ * outputs and tmi_solver_get_sysmat always use the mesh order.  Changing it
+ * requires tmi_solver_allocate again. */
+#define TMI_ORDER_NATURAL 0
+#define TMI_ORDER_MORTON  1
+int tmi_solver_set_order(tmi_solver s, int order);
+/* Rows per run of the 64-column block SpMM (default 4): runs of this many
+ * consecutive rows are dealt round-robin to the waves of each XCD; 0 gives
+ * every wave one contiguous share instead.  Tuning parameter of the row
+ * schedule, no effect on results. */
+int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
+/* 64-column complex-FP32 SpMM variant: 1 (default) gathers neighbour rows
+ * from global memory per nonzero; 0 stages each block of rows' neighbour
+ * rows in LDS first.  Tuning parameter, no effect on results beyond
+ * rounding. */
+int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
 /* Allocate (fwdsolver.cc:193-261): CSR pattern of the system matrix on the GPU */
 int tmi_solver_allocate(tmi_solver s);
 /* AssembleSystemMatrix (fwdsolver.cc:338-357, :405-424): K = PFF(cmua) +

@@ -48,6 +48,7 @@ typedef struct {
     unsigned char *bndel;             /* elen */
     double *intdd;                    /* elen*nn*nn (full) */
     double *intbff;                   /* elen*nn*nn (full) */
+    int *csr_rp, *csr_ci, csr_nnz;    /* SparseRowStructure, built on first use */
 } om_mesh;
 
 static const int TET_SN[4][3] = {{0,1,2},{0,3,1},{0,2,3},{1,3,2}};   /* tet4.cc:177 */
@@ -133,6 +134,7 @@ void om_mesh_free(om_mesh *m)
     free(m->x); free(m->el); free(m->nreg); free(m->bnd);
     free(m->ga); free(m->gb); free(m->gc); free(m->gd); free(m->size); free(m->sside);
     free(m->bndside); free(m->bndel); free(m->intdd); free(m->intbff);
+    free(m->csr_rp); free(m->csr_ci);
     free(m);
 }
 
@@ -421,11 +423,9 @@ static int pair_cmp(const void *A, const void *B)
     return a[1] < b[1] ? -1 : (a[1] > b[1]);
 }
 
-static int *g_rp, *g_ci; static int g_nnz; static const om_mesh *g_csr_mesh;
-static void build_csr(const om_mesh *m)
+static void build_csr(om_mesh *m)
 {
-    if (g_csr_mesh == m && g_rp) return;
-    free(g_rp); free(g_ci);
+    if (m->csr_rp) return;
     size_t P = (size_t)m->elen*m->nn*m->nn;
     int *pr = (int*)xcalloc(P*2, sizeof(int));
     size_t p = 0;
@@ -437,20 +437,20 @@ static void build_csr(const om_mesh *m)
     int nz = 0;
     for (size_t i = 0; i < P; i++)
         if (!i || pr[2*i] != pr[2*i-2] || pr[2*i+1] != pr[2*i-1]) { pr[2*nz] = pr[2*i]; pr[2*nz+1] = pr[2*i+1]; nz++; }
-    g_rp = (int*)xcalloc(m->n+1, sizeof(int));
-    g_ci = (int*)xcalloc(nz, sizeof(int));
-    for (int i = 0; i < nz; i++) { g_rp[pr[2*i]+1]++; g_ci[i] = pr[2*i+1]; }
-    for (int i = 0; i < m->n; i++) g_rp[i+1] += g_rp[i];
-    g_nnz = nz; g_csr_mesh = m;
+    int *rp = (int*)xcalloc(m->n+1, sizeof(int));
+    int *ci = (int*)xcalloc(nz, sizeof(int));
+    for (int i = 0; i < nz; i++) { rp[pr[2*i]+1]++; ci[i] = pr[2*i+1]; }
+    for (int i = 0; i < m->n; i++) rp[i+1] += rp[i];
+    m->csr_rp = rp; m->csr_ci = ci; m->csr_nnz = nz;
     free(pr);
 }
 
-int om_csr_nnz(const om_mesh *m) { build_csr(m); return g_nnz; }
-void om_csr(const om_mesh *m, int *rowptr, int *colidx)
+int om_csr_nnz(om_mesh *m) { build_csr(m); return m->csr_nnz; }
+void om_csr(om_mesh *m, int *rowptr, int *colidx)
 {
     build_csr(m);
-    memcpy(rowptr, g_rp, sizeof(int)*(size_t)(m->n+1));
-    memcpy(colidx, g_ci, sizeof(int)*(size_t)g_nnz);
+    memcpy(rowptr, m->csr_rp, sizeof(int)*(size_t)(m->n+1));
+    memcpy(colidx, m->csr_ci, sizeof(int)*(size_t)m->csr_nnz);
 }
 
 /* TCompRowMatrix::operator() row search (crmatrix_imp.hpp:756-767) */

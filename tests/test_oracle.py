@@ -215,3 +215,18 @@ def test_nodal_jacobian_finite_differences():
             jc = J[:nqm, col] + 1j * J[nqm:, col]
             scale = np.max(np.abs(jc))
             assert np.max(np.abs(jc - fd)) <= 1e-5 * scale, (k, blk, jc, fd)
+
+
+def test_csr_is_per_mesh():
+    """Each mesh keeps its own pattern: a mesh created after another was
+    freed (possibly at the same address) gets its own CSR."""
+    for _ in range(3):
+        a = O.Mesh(nodes=G.disk(10.0, 4)[0], elems=G.disk(10.0, 4)[1])
+        rpa, _ = a.csr()
+        del a
+        nodes, el = G.cylinder(6.0, 6.0, 3, 2)
+        b = O.Mesh(nodes=nodes, elems=el)
+        rp, ci = b.csr()
+        pairs = sorted({(x, y) for e in el for x in e for y in e})
+        rows = np.repeat(np.arange(b.n), np.diff(rp))
+        assert list(zip(rows.tolist(), ci.tolist())) == pairs

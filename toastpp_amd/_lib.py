@@ -24,6 +24,7 @@ LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB = 0, 1, 2
 PRECON_NONE, PRECON_DIAG = 0, 1
 MODE_FP64_SINGLE, MODE_MIXED_BLOCK, MODE_FP64_BLOCK = 0, 1, 2
 DATA_LIN, DATA_LOG = 0, 1
+ORDER_NATURAL, ORDER_MORTON = 0, 1
 ELID_TET4, ELID_TRI3 = 3, 15
 
 _ERRNAMES = {1: "invalid argument", 2: "not converged", 3: "out of memory", 4: "device error", 5: "I/O error",
@@ -84,6 +85,9 @@ _SIGS = {
     "tmi_solver_set_mode": [P, C.c_int],
     "tmi_solver_set_tile": [P, C.c_int],
     "tmi_solver_set_inner_tol": [P, C.c_double],
+    "tmi_solver_set_order": [P, C.c_int],
+    "tmi_solver_set_rows_per_wave": [P, C.c_int],
+    "tmi_solver_set_spmm_variant": [P, C.c_int],
     "tmi_solver_allocate": [P],
     "tmi_solver_assemble": [P, D, D, D, C.c_double],
     "tmi_solver_reset": [P, D, D, D, C.c_double],

@@ -507,6 +507,35 @@ int tmi_solver_set_tile(tmi_solver h, int tile)
     API_END
 }
 
+int tmi_solver_set_order(tmi_solver h, int kind)
+{
+    API_BEGIN
+    TMI_CHECK(kind == TMI_ORDER_NATURAL || kind == TMI_ORDER_MORTON, TMI_ERR_INVALID_ARG, "unknown node order %d",
+              kind);
+    Solver &s = S(h);
+    if (s.order_kind != kind) {
+        s.order_kind = kind;
+        s.allocated = s.assembled = s.precon_valid = false;   // Allocate builds the solver's copy anew
+    }
+    API_END
+}
+
+int tmi_solver_set_rows_per_wave(tmi_solver h, int rows)
+{
+    API_BEGIN
+    TMI_CHECK(rows >= 0 && rows <= 63, TMI_ERR_INVALID_ARG, "rows per wave must be 0 .. 63");
+    S(h).rpw = rows;
+    API_END
+}
+
+int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
+{
+    API_BEGIN
+    TMI_CHECK(variant == 0 || variant == 1, TMI_ERR_INVALID_ARG, "SpMM variant must be 0 (LDS-staged) or 1 (direct)");
+    S(h).use_lds = variant == 0;
+    API_END
+}
+
 int tmi_solver_set_inner_tol(tmi_solver h, double t)
 {
     API_BEGIN

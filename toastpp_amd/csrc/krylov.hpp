@@ -22,11 +22,30 @@ enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_NSLOT };
 // partial-sum slots written by the reduction epilogues
 constexpr int RED_SLOTS = 2;
 
+// Row blocks of the LDS-staged 64-column SpMM: consecutive rows whose
+// neighbour rows (the X rows their nonzeros read) plus the rows themselves
+// (their W rows) number at most LDS_ROWS, so that one workgroup of
+// LDS_WAVES waves stages them in LDS once and reads them from there.
+// LDS_NZ_C32 bounds a block's nonzeros (its values and positions are staged
+// too) and a block holds fewer than LDS_ROWS_C32 rows.
+constexpr int LDS_ROWS_C32 = 136;   // 136 x 64 x 8 B = 68 KiB (+ 6 KiB metadata): two workgroups per CU
+constexpr int LDS_NZ_C32 = 576;
+constexpr int LDS_WAVES = 4;
+struct RowBlocks {
+    int nb = 0;                    // number of blocks (0: not available)
+    const int *brow = nullptr;     // nb+1 row bounds
+    const int *bu = nullptr;       // nb+1 bounds into ucols
+    const int *ucols = nullptr;    // neighbour rows of each block
+    const unsigned short *lcol = nullptr;   // per nonzero: position of its column in the block's ucols
+};
+
 struct BlockDesc {
     int n = 0;          // rows
     int K = 0;          // columns (tile width)
     int nblk = 0;       // reduction blocks of the row-parallel kernels
     const int *rowptr = nullptr, *colidx = nullptr;
+    int rpw = 0;        // rows per run of the 64-column SpMM (0: default)
+    RowBlocks lds;      // LDS-staged SpMM partition (complex FP32, 64 columns)
 };
 
 // precision / value-type selector
@@ -104,5 +123,17 @@ void launch_scale_cols(bool cplx, int nnz, const int *colidx, const double *val6
                        void *out32, hipStream_t st);
 // copy with conversion FP64 -> FP32 (values)
 void launch_f64_to_f32(const double *src, float *dst, size_t count, hipStream_t st);
+// matrix values into the solver's node order: out64[k] = val64[slot[k]]
+// (complex: both parts), out32 (optional) the FP32 copy
+void launch_permute_vals(bool cplx, int nnz, const int *slot, const double *val64, double *out64, float *out32,
+                         hipStream_t st);
+// tile of right-hand sides / solutions between the caller's [n][ld] layout
+// (reference node order) and the solver's [n][K] tile (solver node order):
+// gather dst[r][c] = src[perm[r]][q0 + c] for c < kc, 0 for kc <= c < K;
+// scatter dst[perm[r]][q0 + c] = src[r][c] for c < kc.  es: 16 (complex) or 8.
+void launch_gather_tile(int es, int n, int K, int kc, const int *perm, const void *src, long ld, int q0, void *dst,
+                        hipStream_t st);
+void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const void *src, void *dst, long ld, int q0,
+                         hipStream_t st);
 
 }  // namespace tmi

@@ -33,6 +33,25 @@ __device__ inline double vsub(double a, double b) { return a - b; }
 __device__ inline float2 vmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 vmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline float vmul(float a, float b) { return a * b; }
+// acc + a * b with four fused multiply-adds (complex) / one (real)
+__device__ inline float2 vfma(float2 acc, float2 a, float2 b)
+{
+    acc.x = __builtin_fmaf(a.x, b.x, acc.x);
+    acc.x = __builtin_fmaf(-a.y, b.y, acc.x);
+    acc.y = __builtin_fmaf(a.x, b.y, acc.y);
+    acc.y = __builtin_fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+__device__ inline double2 vfma(double2 acc, double2 a, double2 b)
+{
+    acc.x = __builtin_fma(a.x, b.x, acc.x);
+    acc.x = __builtin_fma(-a.y, b.y, acc.x);
+    acc.y = __builtin_fma(a.x, b.y, acc.y);
+    acc.y = __builtin_fma(a.y, b.x, acc.y);
+    return acc;
+}
+__device__ inline float vfma(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
+__device__ inline double vfma(double acc, double a, double b) { return __builtin_fma(a, b, acc); }
 __device__ inline double vmul(double a, double b) { return a * b; }
 __device__ inline float2 vscale(float2 a, double s) { float f = (float)s; return make_float2(a.x * f, a.y * f); }
 __device__ inline double2 vscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
@@ -181,8 +200,11 @@ __global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__
 // row pointers, column indices and values of the run are fetched with
 // coalesced vector loads (64 entries per load) and broadcast with readlane,
 // so the only memory accesses on the critical path are the neighbour-row
-// gathers, issued B at a time.  Rows are dealt to waves in contiguous runs
-// inside one region per XCD.
+// gathers, issued B at a time.  Each XCD owns one contiguous region of
+// rows; inside it, R > 0 deals runs of R rows round-robin over the XCD's
+// waves (the waves of an XCD then sweep the region together, so their
+// neighbour rows overlap in that XCD's L2), R <= 0 gives every wave one
+// contiguous share of the region, processed in runs of up to 63 rows.
 __device__ inline float2 rdlane(float2 v, int l)
 {
     return make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.x), l)),
@@ -206,7 +228,7 @@ template <class V, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1:
 __global__ void __launch_bounds__(256) k_spmm64(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
                                                 const V *__restrict__ val, const V *__restrict__ X,
                                                 V *__restrict__ Y, const V *__restrict__ W, int want_yy,
-                                                const int *__restrict__ active, double *__restrict__ partial)
+                                                const int *__restrict__ active, double *__restrict__ partial, int R)
 {
     constexpr int B = 16;
     const int lane = threadIdx.x & 63;
@@ -218,11 +240,21 @@ __global__ void __launch_bounds__(256) k_spmm64(int n, const int *__restrict__ r
     const int per_x = (n + NXCD - 1) / NXCD;
     const int xs = min(n, xcd * per_x), xe = min(n, xs + per_x);
     const int wpx = nbx * nw, wid = bx * nw + wave;
-    const int per_w = (xe - xs + wpx - 1) / wpx;
-    const int r0 = min(xe, xs + wid * per_w), r1 = min(xe, r0 + per_w);
+    int rstart, rend_all, rstep, rlen;
+    if (R > 0) {
+        rlen = min(R, 63);
+        rstart = xs + wid * rlen;
+        rend_all = xe;
+        rstep = wpx * rlen;
+    } else {
+        const int per_w = (xe - xs + wpx - 1) / wpx;
+        rstart = min(xe, xs + wid * per_w);
+        rend_all = min(xe, rstart + per_w);
+        rstep = rlen = 63;
+    }
     double d1 = 0.0, d2 = 0.0;
-    for (int rb = r0; rb < r1; rb += 63) {
-        const int nr = min(63, r1 - rb);
+    for (int rb = rstart; rb < rend_all; rb += rstep) {
+        const int nr = min(rlen, rend_all - rb);
         const int rpv = rowptr[rb + min(lane, nr)];
         const int kb = __builtin_amdgcn_readlane(rpv, 0), ke = __builtin_amdgcn_readlane(rpv, nr);
         int row = 0;
@@ -271,6 +303,139 @@ __global__ void __launch_bounds__(256) k_spmm64(int n, const int *__restrict__ r
         while (row < nr) flush();
     }
     block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
+}
+
+// ------------------------------- LDS-staged block SpMM, 64 columns
+// One workgroup per block of consecutive rows (blocks of one XCD's region
+// dealt round-robin to that XCD's workgroups).  The workgroup first copies
+// the block's neighbour rows of X -- each distinct row once -- and the
+// block's rows of W from global memory into LDS, all loads in flight
+// together, then every wave computes a contiguous share of the block's rows
+// from LDS: the vector-memory path carries each staged row once per block
+// instead of once per nonzero, and no row waits on a global load.
+template <int NW>
+__device__ inline void block_reduce2_nw(double d1, double d2, double *partial, int blk, int G, bool want2)
+{
+    __shared__ double s1[NW][64], s2[NW][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    s1[wave][lane] = d1;
+    s2[wave][lane] = d2;
+    __syncthreads();
+    if ((int)threadIdx.x < 64) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int w = 0; w < NW; w++) { t1 += s1[w][threadIdx.x]; t2 += s2[w][threadIdx.x]; }
+        partial[((size_t)0 * G + blk) * 64 + threadIdx.x] = t1;
+        if (want2) partial[((size_t)1 * G + blk) * 64 + threadIdx.x] = t2;
+    }
+}
+
+template <class V, int MODE, int UROWS, int NW, int NZMAX>
+__global__ void __launch_bounds__(64 * NW) k_spmm_lds(int nb, const int *__restrict__ brow, const int *__restrict__ bu,
+                                                       const int *__restrict__ ucols, const int *__restrict__ rowptr,
+                                                       const unsigned short *__restrict__ lcol,
+                                                       const V *__restrict__ val, const V *__restrict__ X,
+                                                       V *__restrict__ Y, const V *__restrict__ W, int want_yy,
+                                                       const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int NT = 64 * NW;
+    constexpr int SB = UROWS / NW;   // staged rows per wave, all in flight
+    constexpr int LB = 8;            // nonzeros per batch of LDS reads
+    constexpr int RPMAX = UROWS;     // row pointers of a block (rows < UROWS)
+    static_assert(SB * NW == UROWS && SB <= 64, "LDS rows must split evenly over the waves");
+    __shared__ V xs[UROWS * 64];                 // X neighbour rows, then W rows of the block
+    __shared__ V vs[NZMAX];                      // matrix values of the block
+    __shared__ unsigned short ls[NZMAX];         // their positions in xs
+    __shared__ int rs[RPMAX + 1];                // block-local row pointers
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool act = active[lane] != 0;
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x % NXCD, gi = blockIdx.x / NXCD, gstep = G / NXCD;
+    const int per = (nb + NXCD - 1) / NXCD;
+    const int b0 = min(nb, xcd * per), b1 = min(nb, b0 + per);
+    double d1 = 0.0, d2 = 0.0;
+    for (int b = b0 + gi; b < b1; b += gstep) {
+        const int u0 = bu[b], nu = bu[b + 1] - u0;
+        const int r0 = brow[b], nra = brow[b + 1] - r0;
+        const int z0 = rowptr[r0], nz = rowptr[r0 + nra] - z0;
+        const int ns = nu + nra;   // staged rows: X neighbours, then W of the block's rows
+        __syncthreads();           // the previous block's readers are done with the buffers
+        {
+            // row pointers, values and positions of the block (one pass of
+            // the workgroup each: nra < RPMAX <= NT, nz <= NZMAX)
+            int rp = 0;
+            if (tid <= nra) rp = rowptr[r0 + tid] - z0;
+            V vv[(NZMAX + NT - 1) / NT];
+            unsigned short lv[(NZMAX + NT - 1) / NT];
+#pragma unroll
+            for (int i = 0; i < (NZMAX + NT - 1) / NT; i++) {
+                const int k = min(tid + i * NT, nz - 1);
+                vv[i] = val[z0 + k];
+                lv[i] = lcol[z0 + k];
+            }
+            // wave w stages entries w, w + NW, ... w + NW (SB - 1) < UROWS;
+            // entries past ns repeat the last one into LDS rows never read,
+            // so that every load is unconditional and all are in flight together
+            const int e = min(wave + NW * lane, ns - 1);
+            const int src = e < nu ? ucols[u0 + e] : -1 - (r0 + e - nu);
+            V tmp[SB];
+#pragma unroll
+            for (int i = 0; i < SB; i++) {
+                const int sj = __builtin_amdgcn_readlane(src, i);
+                const V *p = sj >= 0 ? X + (size_t)sj * 64 : W + (size_t)(-1 - sj) * 64;
+                tmp[i] = p[lane];
+            }
+            if (tid <= nra) rs[tid] = rp;
+#pragma unroll
+            for (int i = 0; i < (NZMAX + NT - 1) / NT; i++) {
+                const int k = tid + i * NT;
+                if (k < nz) { vs[k] = vv[i]; ls[k] = lv[i]; }
+            }
+#pragma unroll
+            for (int i = 0; i < SB; i++) xs[(wave + NW * i) * 64 + lane] = tmp[i];
+        }
+        __syncthreads();
+        const int pw = (nra + NW - 1) / NW;
+        const int la = min(nra, wave * pw), lz = min(nra, (wave + 1) * pw);   // block-local rows
+        // one row at a time; a nonzero's value and position are broadcast
+        // LDS reads (every lane the same address), its X row a 512-byte read
+        for (int lr = la; lr < lz; lr++) {
+            const int kb = __builtin_amdgcn_readfirstlane(rs[lr]);
+            const int ke = __builtin_amdgcn_readfirstlane(rs[lr + 1]);
+            V acc = vzero<V>();
+            for (int k0 = kb; k0 < ke; k0 += LB) {
+                V xb[LB], vb[LB];
+#pragma unroll
+                for (int u = 0; u < LB; u++) {
+                    const int kk = min(k0 + u, ke - 1);
+                    vb[u] = vs[kk];
+                    xb[u] = xs[(int)ls[kk] * 64 + lane];
+                }
+                if (k0 + LB <= ke) {
+#pragma unroll
+                    for (int u = 0; u < LB; u++) acc = vfma(acc, vb[u], xb[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < LB; u++)
+                        if (k0 + u < ke) acc = vfma(acc, vb[u], xb[u]);
+                }
+            }
+            const V wv = xs[(nu + lr) * 64 + lane];
+            if (act) {
+                const size_t at = (size_t)(r0 + lr) * 64 + lane;
+                if (MODE == 0) {
+                    Y[at] = acc;
+                    d1 += dotR(wv, acc);
+                    if (want_yy) d2 += dotR(acc, acc);
+                } else {
+                    const V r = vsub(wv, acc);
+                    Y[at] = r;
+                    d1 += dotR(r, r);
+                }
+            }
+        }
+    }
+    block_reduce2_nw<NW>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
 }
 
 // single-column SpMV: LPR lanes share one row
@@ -720,6 +885,41 @@ __global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d
     for (; i < cnt; i += (size_t)gridDim.x * blockDim.x) d[i] = (float)s[i];
 }
 
+__global__ void k_permute_vals(int cnt, int w, const int *__restrict__ slot, const double *__restrict__ v64,
+                               double *__restrict__ o64, float *__restrict__ o32)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const size_t src = (size_t)slot[k] * w;
+    for (int t = 0; t < w; t++) {
+        const double v = v64[src + t];
+        o64[(size_t)k * w + t] = v;
+        if (o32) o32[(size_t)k * w + t] = (float)v;
+    }
+}
+
+template <class T>
+__global__ void k_gather_tile(int n, int K, int kc, const int *__restrict__ perm, const T *__restrict__ src, long ld,
+                              int q0, T *__restrict__ dst)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * K) return;
+    const int r = (int)(e / K), c = (int)(e % K);
+    T v{};
+    if (c < kc) v = src[(size_t)perm[r] * ld + q0 + c];
+    dst[e] = v;
+}
+
+template <class T>
+__global__ void k_scatter_tile(int n, int K, int kc, const int *__restrict__ perm, const T *__restrict__ src,
+                               T *__restrict__ dst, long ld, int q0)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * K) return;
+    const int r = (int)(e / K), c = (int)(e % K);
+    if (c < kc) dst[(size_t)perm[r] * ld + q0 + c] = src[e];
+}
+
 // ================================================================ launchers
 static inline void check_K(int K)
 {
@@ -762,9 +962,15 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
 #undef SPMV1
     } else {
 #define SPMMK(KK)                                                                                             \
-    if (KK == 64 && !idiag && (vk == VK_C32 || vk == VK_C64))                                                \
+    if (KK == 64 && !idiag && vk == VK_C32 && b.lds.nb > 0)                                                  \
+        hipLaunchKernelGGL((k_spmm_lds<float2, 0, LDS_ROWS_C32, LDS_WAVES, LDS_NZ_C32>), g, dim3(64 * LDS_WAVES), \
+                           0, st,                                                                             \
+                           b.lds.nb, b.lds.brow, b.lds.bu,                                                    \
+                           b.lds.ucols, b.rowptr, b.lds.lcol, (const float2 *)val, (const float2 *)X,          \
+                           (float2 *)Y, (const float2 *)W, (int)want_yy, iflags, partial);                    \
+    else if (KK == 64 && !idiag && (vk == VK_C32 || vk == VK_C64))                                           \
         hipLaunchKernelGGL((k_spmm64<V, 0>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
-                           (V *)Y, (const V *)W, (int)want_yy, iflags, partial);                             \
+                           (V *)Y, (const V *)W, (int)want_yy, iflags, partial, b.rpw);                      \
     else if (idiag)                                                                                                \
         hipLaunchKernelGGL((k_spmm_dot<V, KK, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
                            (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial); \
@@ -785,7 +991,7 @@ void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, 
 #define RESK(KK)                                                                                              \
     if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                           \
         hipLaunchKernelGGL((k_spmm64<V, 1>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
-                           (V *)R, (const V *)B, 0, iflags, partial);                                        \
+                           (V *)R, (const V *)B, 0, iflags, partial, b.rpw);                                 \
     else                                                                                                      \
     hipLaunchKernelGGL((k_resid<V, KK>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                        (const V *)B, (V *)R, iflags, partial);
@@ -975,6 +1181,42 @@ void launch_scale_cols(bool cplx, int nnz, const int *colidx, const double *val6
 void launch_f64_to_f32(const double *src, float *dst, size_t count, hipStream_t st)
 {
     hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(count, 256, 4096)), dim3(256), 0, st, src, dst, count);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_permute_vals(bool cplx, int nnz, const int *slot, const double *val64, double *out64, float *out32,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_permute_vals, dim3(grid_for(nnz, 256)), dim3(256), 0, st, nnz, cplx ? 2 : 1, slot, val64,
+                       out64, out32);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_gather_tile(int es, int n, int K, int kc, const int *perm, const void *src, long ld, int q0, void *dst,
+                        hipStream_t st)
+{
+    const size_t cnt = (size_t)n * K;
+    dim3 g((unsigned)((cnt + 255) / 256));
+    if (es == 16)
+        hipLaunchKernelGGL(k_gather_tile<double2>, g, dim3(256), 0, st, n, K, kc, perm, (const double2 *)src, ld, q0,
+                           (double2 *)dst);
+    else
+        hipLaunchKernelGGL(k_gather_tile<double>, g, dim3(256), 0, st, n, K, kc, perm, (const double *)src, ld, q0,
+                           (double *)dst);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const void *src, void *dst, long ld, int q0,
+                         hipStream_t st)
+{
+    const size_t cnt = (size_t)n * K;
+    dim3 g((unsigned)((cnt + 255) / 256));
+    if (es == 16)
+        hipLaunchKernelGGL(k_scatter_tile<double2>, g, dim3(256), 0, st, n, K, kc, perm, (const double2 *)src,
+                           (double2 *)dst, ld, q0);
+    else
+        hipLaunchKernelGGL(k_scatter_tile<double>, g, dim3(256), 0, st, n, K, kc, perm, (const double *)src,
+                           (double *)dst, ld, q0);
     TMI_LAUNCH_CHECK();
 }
 

@@ -43,11 +43,26 @@ struct Mesh {
     bool links_valid = false;
     DBuf<int> d_lptr, d_lq, d_lm;
     DBuf<double> jac_ws;
+    // node orders of the solvers' private copies of the system (order.cpp),
+    // one per kind (ORDER_NATURAL, ORDER_MORTON): row r of the relabelled
+    // system is node perm[r]; its CSR pattern with slot[k] = nonzero of the
+    // reference-order pattern held by entry k
+    struct NodeOrder {
+        bool valid = false;
+        DBuf<int> perm, rowptr, colidx, slot, diagpos;
+        // row blocks of the LDS-staged SpMM (complex FP32, 64 columns)
+        int nb = 0;
+        DBuf<int> brow, bu, ucols;
+        DBuf<unsigned short> lcol;
+    } order[2];
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }
     void setup(hipStream_t st);        // Mesh::Setup on the GPU
     void build_csr(hipStream_t st);    // Mesh::SparseRowStructure on the GPU
+    const NodeOrder &build_order(int kind, hipStream_t st);
 };
+
+enum { ORDER_NATURAL = 0, ORDER_MORTON = 1 };
 
 // host parser helpers
 void read_mesh_file(const char *path, int &dim, std::vector<double> &nodes, std::vector<int> &elist,

@@ -1,0 +1,168 @@
+// Locality order of the nodes for the solver's sparse kernels.
+//
+// The block SpMM gathers, for every row, the 64-column rows of X of its
+// neighbours (512 bytes each in complex FP32).  How often such a row is
+// fetched from beyond L2 depends on how far apart in row order its
+// neighbours are, so the solver works on a relabelled copy of the system:
+// nodes sorted along a Morton (Z-order) curve of their coordinates, which
+// keeps spatially close nodes close in row order whatever order the mesh
+// file or generator used.  Only the solver's private copy is relabelled; the
+// mesh, its CSR pattern and the assembled matrix keep the reference order.
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "krylov.hpp"
+#include "mesh.hpp"
+
+namespace tmi {
+
+// spread the low 21 bits of v to every third bit (3D) / 32 bits to every
+// second bit (2D)
+static uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+static uint64_t spread2(uint64_t v)
+{
+    v &= 0xffffffffULL;
+    v = (v | v << 16) & 0x0000ffff0000ffffULL;
+    v = (v | v << 8) & 0x00ff00ff00ff00ffULL;
+    v = (v | v << 4) & 0x0f0f0f0f0f0f0f0fULL;
+    v = (v | v << 2) & 0x3333333333333333ULL;
+    v = (v | v << 1) & 0x5555555555555555ULL;
+    return v;
+}
+
+std::vector<int> morton_order(int dim, int n, const double *x)
+{
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+        lo[d] = hi[d] = x[d];
+        for (int i = 1; i < n; i++) {
+            lo[d] = std::min(lo[d], x[(size_t)i * dim + d]);
+            hi[d] = std::max(hi[d], x[(size_t)i * dim + d]);
+        }
+    }
+    // one scale for all axes keeps the curve's cells cubic
+    double ext = 0.0;
+    for (int d = 0; d < dim; d++) ext = std::max(ext, hi[d] - lo[d]);
+    const double levels = dim == 3 ? 2097151.0 : 4294967295.0;
+    const double s = ext > 0.0 ? levels / ext : 0.0;
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; i++) {
+        uint64_t k = 0;
+        for (int d = 0; d < dim; d++) {
+            double q = (x[(size_t)i * dim + d] - lo[d]) * s;
+            uint64_t c = (uint64_t)std::min(levels, std::max(0.0, q));
+            k |= (dim == 3 ? spread3(c) : spread2(c)) << d;
+        }
+        key[i] = k;
+    }
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    return perm;
+}
+
+// Greedy partition of the rows (in order) into blocks whose distinct
+// columns plus rows number at most umax and whose nonzeros number at most
+// nzmax: brow / bu bound the blocks' rows and their entries of ucols;
+// lcol[k] is the position of colidx[k] in its block's list.  Returns false
+// if a single row does not fit.
+static bool row_blocks(int n, const std::vector<int> &rp, const std::vector<int> &ci, int umax, int nzmax,
+                       std::vector<int> &brow,
+                       std::vector<int> &bu, std::vector<int> &ucols, std::vector<unsigned short> &lcol)
+{
+    std::vector<int> mark(n, -1), pos(n, 0);
+    brow.assign(1, 0);
+    bu.assign(1, 0);
+    ucols.clear();
+    lcol.assign(ci.size(), 0);
+    int blk = 0, cur = 0;
+    for (int r = 0; r < n; r++) {
+        if (rp[r + 1] - rp[r] + 1 > umax || rp[r + 1] - rp[r] > nzmax) return false;
+        int fresh = 0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) fresh += mark[ci[k]] != blk;
+        const int rows = r - brow.back();
+        if (cur + fresh + rows + 1 > umax || rp[r + 1] - rp[brow.back()] > nzmax) {
+            brow.push_back(r);
+            bu.push_back((int)ucols.size());
+            blk++;
+            cur = 0;
+        }
+        for (int k = rp[r]; k < rp[r + 1]; k++) {
+            const int c = ci[k];
+            if (mark[c] != blk) {
+                mark[c] = blk;
+                pos[c] = cur++;
+                ucols.push_back(c);
+            }
+            lcol[k] = (unsigned short)pos[c];
+        }
+    }
+    brow.push_back(n);
+    bu.push_back((int)ucols.size());
+    return true;
+}
+
+const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
+{
+    TMI_CHECK(kind == ORDER_NATURAL || kind == ORDER_MORTON, TMI_ERR_INVALID_ARG, "unknown node order %d", kind);
+    build_csr(st);
+    NodeOrder &order = this->order[kind];
+    if (order.valid) return order;
+    std::vector<int> perm;
+    if (kind == ORDER_MORTON) perm = morton_order(dim, n, h_nodes.data());
+    else {
+        perm.resize(n);
+        std::iota(perm.begin(), perm.end(), 0);
+    }
+    std::vector<int> iperm(n);
+    for (int r = 0; r < n; r++) iperm[perm[r]] = r;
+    const std::vector<int> rp = rowptr.to_host(st), ci = colidx.to_host(st);
+    std::vector<int> prp(n + 1), pci(nnz), slot(nnz), pdiag(n, -1);
+    std::vector<std::pair<int, int>> row;
+    prp[0] = 0;
+    for (int r = 0; r < n; r++) {
+        const int o = perm[r];
+        row.clear();
+        for (int k = rp[o]; k < rp[o + 1]; k++) row.emplace_back(iperm[ci[k]], k);
+        std::sort(row.begin(), row.end());
+        int at = prp[r];
+        for (auto &e : row) {
+            if (e.first == r) pdiag[r] = at;
+            pci[at] = e.first;
+            slot[at] = e.second;
+            at++;
+        }
+        prp[r + 1] = at;
+    }
+    order.perm.upload(perm.data(), n, st);
+    order.rowptr.upload(prp.data(), n + 1, st);
+    order.colidx.upload(pci.data(), nnz, st);
+    order.slot.upload(slot.data(), nnz, st);
+    order.diagpos.upload(pdiag.data(), n, st);
+    std::vector<int> brow, bu, ucols;
+    std::vector<unsigned short> lcol;
+    order.nb = 0;
+    if (row_blocks(n, prp, pci, LDS_ROWS_C32, LDS_NZ_C32, brow, bu, ucols, lcol)) {
+        order.nb = (int)brow.size() - 1;
+        order.brow.upload(brow.data(), brow.size(), st);
+        order.bu.upload(bu.data(), bu.size(), st);
+        order.ucols.upload(ucols.data(), ucols.size(), st);
+        order.lcol.upload(lcol.data(), lcol.size(), st);
+    }
+    TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
+    order.valid = true;
+    return order;
+}
+
+}  // namespace tmi

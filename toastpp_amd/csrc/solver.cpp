@@ -30,10 +30,21 @@ Solver::~Solver()
 void Solver::allocate()
 {
     mesh->build_csr(st);
+    mesh->build_order(order_kind, st);
     val64.alloc(nval());
-    val32.alloc(nval());
+    sval64.alloc(nval());
+    sval32.alloc(nval());
     allocated = true;
     assembled = false;
+}
+
+BlockDesc Solver::block(int K) const
+{
+    const Mesh::NodeOrder &o = ord();
+    BlockDesc b{mesh->n, K, w.G, o.rowptr.p, o.colidx.p};
+    b.rpw = rpw;
+    if (use_lds && o.nb > 0) b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
+    return b;
 }
 
 void Solver::assemble(const double *cmua, const double *ckap, const double *c2a, double om, bool host_ptrs)
@@ -53,7 +64,9 @@ void Solver::assemble(const double *cmua, const double *ckap, const double *c2a,
     }
     omega = om;
     precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
-    launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, val32.p, st);
+    launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st);
+    // the solver's copy in its own node order
+    launch_permute_vals(cplx, mesh->nnz, ord().slot.p, val64.p, sval64.p, sval32.p, st);
     assembled = true;
 }
 
@@ -63,13 +76,13 @@ void Solver::reset_precon()
     int n = mesh->n;
     idiag64.alloc((size_t)n * vbytes(vk64()));
     idiag32.alloc((size_t)n * vbytes(vk32()));
-    launch_inv_diag(vk64(), n, mesh->diagpos.p, val64.p, idiag64.p, st);
-    launch_inv_diag(vk32(), n, mesh->diagpos.p, val64.p, idiag32.p, st);
+    launch_inv_diag(vk64(), n, ord().diagpos.p, sval64.p, idiag64.p, st);
+    launch_inv_diag(vk32(), n, ord().diagpos.p, sval64.p, idiag32.p, st);
     // BiCGSTAB applies M^-1 on the right (pd = M^-1 p, v = K pd): keep K D^-1
     // as its own operator so the SpMM needs no per-nonzero diagonal gather
     pval64.alloc(nval());
     pval32.alloc(nval());
-    launch_scale_cols(cplx, mesh->nnz, mesh->colidx.p, val64.p, idiag64.p, pval64.p, pval32.p, st);
+    launch_scale_cols(cplx, mesh->nnz, ord().colidx.p, sval64.p, idiag64.p, pval64.p, pval32.p, st);
     precon_valid = true;
 }
 
@@ -121,9 +134,9 @@ static constexpr int CHECK_EVERY = 8;
 int Solver::solve_tile(int kc, tmi_colstat *stats)
 {
     int K = w.K, n = mesh->n;
-    BlockDesc bd{n, K, w.G, mesh->rowptr.p, mesh->colidx.p};
+    BlockDesc bd = block(K);
     bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
-    const void *val_in = mixed ? (const void *)val32.p : (const void *)val64.p;
+    const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     const void *idiag_in = nullptr;
     bool use_cg = (lsolver == TMI_LSOLVER_CG);
     // operator of the BiCGSTAB SpMMs: K D^-1 when diagonally preconditioned
@@ -155,7 +168,7 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
 
     launch_norm2(vk64(), bd, w.b64.p, oact, part, st);
     launch_finalize(FIN_OUTER_BNORM, bd, part, osc, oic, 0, 0, 0, st);
-    launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
     // inner state
     launch_inner_reset(bd, osc, oic, sc, ic, st);
@@ -171,7 +184,7 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
     int it = 0, nupd = 0, since = 0;
     auto reliable_update = [&]() {
         launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
-        launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+        launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
         launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
         launch_reliable(vk, bd, w.r64.p, w.ri.p, w.xi.p, use_cg ? nullptr : w.rdi.p, idiag_in, scale, oact, part,
                         st);
@@ -217,7 +230,7 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
     // fold the last inner correction of columns still marked active (maxit
     // exhausted) and record the final true residuals
     launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
-    launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
     TMI_HIP(hipMemcpyAsync(h_flags, ic, sizeof(int) * I_NSLOT * K, hipMemcpyDeviceToHost, st));
     TMI_HIP(hipMemcpyAsync(h_dbl, osc, sizeof(double) * S_NSLOT * K, hipMemcpyDeviceToHost, st));
@@ -264,18 +277,13 @@ int Solver::solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats
     int notconv = 0;
     for (int q0 = 0; q0 < nq; q0 += K) {
         int kc = std::min(K, nq - q0);
-        // gather the tile: columns q0..q0+kc of [n][ld] -> [n][K]
-        if (kc < K) {
-            TMI_HIP(hipMemsetAsync(w.b64.p, 0, w.b64.bytes(), st));
-            TMI_HIP(hipMemsetAsync(w.x64.p, 0, w.x64.bytes(), st));
-        }
-        TMI_HIP(hipMemcpy2DAsync(w.b64.p, K * es, (const char *)d_b + q0 * es, (size_t)ld * es, kc * es, mesh->n,
-                                 hipMemcpyDeviceToDevice, st));
-        TMI_HIP(hipMemcpy2DAsync(w.x64.p, K * es, (const char *)d_x + q0 * es, (size_t)ld * es, kc * es, mesh->n,
-                                 hipMemcpyDeviceToDevice, st));
+        // gather the tile: columns q0..q0+kc of [n][ld] -> [n][K] in the
+        // solver's node order (columns kc..K-1 zero)
+        const int *perm = ord().perm.p;
+        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, st);
+        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, st);
         notconv += solve_tile(kc, stats ? stats + q0 : nullptr);
-        TMI_HIP(hipMemcpy2DAsync((char *)d_x + q0 * es, (size_t)ld * es, w.x64.p, K * es, kc * es, mesh->n,
-                                 hipMemcpyDeviceToDevice, st));
+        launch_scatter_tile((int)es, mesh->n, K, kc, perm, w.x64.p, d_x, ld, q0, st);
     }
     TMI_HIP(hipEventRecord(e1, st));
     TMI_HIP(hipEventSynchronize(e1));
@@ -293,19 +301,19 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
     int K = w.K, n = mesh->n;
-    BlockDesc bd{n, K, w.G, mesh->rowptr.p, mesh->colidx.p};
+    BlockDesc bd = block(K);
     std::vector<int> ones(K, 1);
     DBuf<int> act;
     act.upload(ones.data(), K, st);
     bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
     const void *idg = nullptr;
-    const void *op = mixed ? (const void *)val32.p : (const void *)val64.p;
+    const void *op = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     if (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
     auto launch = [&]() {
         if (which == 0)
             launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, w.rdi.p, false, act.p, w.partial.p, st);
         else
-            launch_resid(vk64(), bd, val64.p, w.x64.p, w.b64.p, w.r64.p, act.p, w.partial.p, st);
+            launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, act.p, w.partial.p, st);
     };
     launch();   // warm
     hipEvent_t e0, e1;

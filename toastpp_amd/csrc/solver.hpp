@@ -16,15 +16,19 @@ struct Solver {
     int tile = 0;
     double inner_tol = 1e-5;
     int reliable_every = 16;   // inner iterations between FP64 residual replacements (mixed mode)
+    int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
+    int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
+    bool use_lds = false;            // LDS-staged 64-column SpMM instead of per-nonzero gathers
     bool allocated = false, assembled = false, precon_valid = false;
     double omega = 0.0;
     hipStream_t st = nullptr;
 
-    DBuf<double> val64;    // nnz (real) or 2 nnz (complex)
-    DBuf<float> val32;
+    DBuf<double> val64;    // nnz (real) or 2 nnz (complex), reference node order
+    DBuf<double> sval64;   // the same in the solver's node order (Mesh::order)
+    DBuf<float> sval32;
     DBuf<char> idiag64, idiag32;
-    DBuf<double> pval64;   // A D^-1 (FP64) for the right-preconditioned inner iteration
-    DBuf<float> pval32;    // A D^-1 (FP32)
+    DBuf<double> pval64;   // A D^-1 (FP64, solver order) for the right-preconditioned inner iteration
+    DBuf<float> pval32;    // A D^-1 (FP32, solver order)
     DBuf<double> coef;     // cmua | ckappa | c2a on the device
 
     // workspace of one tile
@@ -43,6 +47,7 @@ struct Solver {
     VKind vk64() const { return cplx ? VK_C64 : VK_R64; }
     VKind vk32() const { return cplx ? VK_C32 : VK_R32; }
     int nval() const { return cplx ? 2 * mesh->nnz : mesh->nnz; }
+    const Mesh::NodeOrder &ord() const { return mesh->order[order_kind]; }
     void allocate();
     void assemble(const double *cmua, const double *ckap, const double *c2a, double omega, bool host_ptrs);
     void reset_precon();
@@ -55,6 +60,7 @@ struct Solver {
     void time_kernel(int which, int reps, double *avg_ms, double *bytes);
 
    private:
+    BlockDesc block(int K) const;
     void ensure_work(int K);
     int solve_tile(int kc, tmi_colstat *stats);
     int count_active(const int *d_flags);

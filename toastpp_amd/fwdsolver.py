@@ -68,6 +68,18 @@ class FwdSolver:
     def SetTile(self, tile):
         L.call("tmi_solver_set_tile", self._h, int(tile))
 
+    def SetOrder(self, order):
+        """'morton' (default) or 'natural' node order of the solver's copy of
+        the system; call Allocate afterwards."""
+        L.call("tmi_solver_set_order", self._h, {"natural": L.ORDER_NATURAL, "morton": L.ORDER_MORTON}[order])
+
+    def SetRowsPerWave(self, rows):
+        L.call("tmi_solver_set_rows_per_wave", self._h, int(rows))
+
+    def SetSpmmVariant(self, variant):
+        """'direct' (default): per-nonzero gathers; 'lds': LDS-staged 64-column SpMM."""
+        L.call("tmi_solver_set_spmm_variant", self._h, {"lds": 0, "direct": 1}[variant])
+
     def SetInnerTol(self, tol):
         L.call("tmi_solver_set_inner_tol", self._h, float(tol))
 
