@@ -229,20 +229,38 @@ int tmi_jacobian_nodal_device(tmi_mesh m, int nq, int nm, const void *d_dphi, in
                               int lda, const void *d_proj, int dscale, void *d_J);
 
 /* ----------------------------------------------------------------- raster */
-/* Raster_Pixel with bdim == gdim (raster.cc:18-100, raster_px.cc:10-54):
- * element-pixel reference (mesh.cc:2944-3015) and the mesh->grid map B
- * (GridMapMatrix, mesh.cc:2486-2554) over the mesh bounding box. */
-int tmi_raster_create(tmi_mesh m, const int *gdim, tmi_raster *out);
+/* Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,
+ * raster_px.cc:10-54; toast.Raster(mesh, grd), raster.py): the element of
+ * every grid point (GenerateElementPixelRef, mesh.cc:2944-3015), the
+ * mesh -> grid map B (GridMapMatrix, mesh.cc:2486-2554) and the solution
+ * mask (grid points inside the mesh).  gdim: dim grid sizes (each >= 2);
+ * bb: NULL for the mesh bounding box (Mesh::BoundingBox, pad 0), else
+ * [bbmin(dim), bbmax(dim)].  Built on the GPU. */
+int tmi_raster_create(tmi_mesh m, const int *gdim, const double *bb, tmi_raster *out);
 int tmi_raster_free(tmi_raster r);
-/* glen = prod(gdim), slen = grid points inside the mesh */
+/* glen = prod(gdim) (= blen), slen = grid points inside the mesh */
 int tmi_raster_dims(tmi_raster r, int *glen, int *slen);
-/* element reference per grid point (-1 outside), sol2basis index list */
+/* bounding box: bbmin(dim), bbmax(dim) */
+int tmi_raster_bbox(tmi_raster r, double *bbmin, double *bbmax);
+/* element per grid point (-1 outside), glen; sol2basis (Raster::sol2basis,
+ * raster.cc:64-77), slen; either may be NULL */
 int tmi_raster_get_elref(tmi_raster r, int *elref, int *sol2basis);
-/* Map_MeshToGrid (raster.cc:188-199), complex: out glen complex */
-int tmi_raster_map_mesh_to_grid(tmi_raster r, const double *mvec, double *gvec);
-/* GenerateJacobian_grid (jacobian.cc:293-373): J (2 nqm) x (2 slen) */
+/* rows of B: glen x (dim+1) node indices (-1 outside) and weights */
+int tmi_raster_get_bmap(tmi_raster r, int *cols, double *w);
+/* Map_MeshToGrid (raster.cc:188-199, Ax_cplx of B) for ncols complex
+ * columns: mvec nlen x ncols (node-major), gvec glen x ncols */
+int tmi_raster_map_mesh_to_grid(tmi_raster r, int ncols, const double *mvec, double *gvec);
+/* GenerateJacobian_grid (jacobian.cc:293-373): dphi nlen x nq, aphi
+ * nlen x nm complex (node-major, as toast.Mesh.Fields returns them), proj
+ * nqm complex linear projections; J (2 nqm) x (2 slen) row-major
+ * [dlnA/dmua | dlnA/dkappa ; dphase/dmua | dphase/dkappa] on the solution
+ * voxels, scaled by the voxel size prod gsize/(gdim-1). */
 int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi,
                         const double *proj, int dscale, double *J);
+/* Same on GPU-resident arrays (field rows of ldd / lda complex entries);
+ * d_J must hold 4 nqm slen doubles. */
+int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
+                               int lda, const void *d_proj, int dscale, void *d_J);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,11 @@ def lib():
             "om_qm_nq": (C.c_int, [P]), "om_qm_nm": (C.c_int, [P]), "om_qm_nqm": (C.c_int, [P]),
             "om_qm_get": (None, [P, D, D, I, I]), "om_qm_free": (None, [P]),
             "om_pull_to_boundary": (C.c_int, [P, D, D]),
+            "om_bbox": (None, [P, D, D]),
+            "om_raster_elref": (None, [P, I, D, D, I]),
+            "om_raster_bmap": (None, [P, I, D, D, I, I, D]),
+            "om_image_gradient": (None, [C.c_int, I, D, D, I, D]),
+            "om_jacobian_grid": (None, [P, I, D, D, I, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
             "om_pcg": (C.c_int, [C.c_int, I, I, D, D, D, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "om_bicgstab_c": (C.c_int, [C.c_int, I, I, D, D, D, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "om_relres_c": (C.c_double, [C.c_int, I, I, D, D, D]),
@@ -220,6 +225,58 @@ def jacobian_nodal(mesh, dphi, aphi, lptr, lidx, proj, logscale=True):
     J = np.zeros((2 * nqm, 2 * mesh.n))
     lib().om_jacobian_nodal(mesh.h, nq, nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
                             _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, J)
+    return J
+
+
+class Raster:
+    """Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,
+    raster_px.cc:10-54) restated: bounding box of the mesh, element of every
+    grid point, the grid map B and the solution mask."""
+
+    def __init__(self, mesh, gdim, bb=None):
+        self.mesh = mesh
+        self.dim = mesh.dim
+        self.gdim = np.ascontiguousarray(gdim, np.int32)
+        if bb is None:
+            lo, hi = np.zeros(self.dim), np.zeros(self.dim)
+            lib().om_bbox(mesh.h, lo, hi)
+        else:
+            lo, hi = np.ascontiguousarray(bb[0], np.float64), np.ascontiguousarray(bb[1], np.float64)
+        self.bbmin, self.bbmax = lo, hi
+        self.glen = int(np.prod(self.gdim))
+        self.elref = np.zeros(self.glen, np.int32)
+        lib().om_raster_elref(mesh.h, self.gdim, lo, hi, self.elref)
+        self.sol2basis = np.nonzero(self.elref >= 0)[0].astype(np.int32)
+        self.slen = len(self.sol2basis)
+
+    def bmap(self):
+        nn = self.dim + 1
+        cols = np.zeros(self.glen * nn, np.int32)
+        w = np.zeros(self.glen * nn)
+        lib().om_raster_bmap(self.mesh.h, self.gdim, self.bbmin, self.bbmax, self.elref, cols, w)
+        return cols.reshape(self.glen, nn), w.reshape(self.glen, nn)
+
+    def gsize(self):
+        return self.bbmax - self.bbmin
+
+
+def image_gradient(gdim, gsize, im, mask):
+    gdim = np.ascontiguousarray(gdim, np.int32)
+    im = np.ascontiguousarray(im, np.complex128)
+    out = np.zeros(len(gdim) * im.size, np.complex128)
+    lib().om_image_gradient(len(gdim), gdim, np.ascontiguousarray(gsize, np.float64), im.view(np.float64),
+                            np.ascontiguousarray(mask, np.int32), out.view(np.float64))
+    return out.reshape(len(gdim), im.size)
+
+
+def jacobian_grid(raster, dphi, aphi, lptr, lidx, proj, logscale=True):
+    dphi = np.ascontiguousarray(dphi, np.complex128); aphi = np.ascontiguousarray(aphi, np.complex128)
+    nq, nm = dphi.shape[0], aphi.shape[0]
+    nqm = int(lptr[-1])
+    J = np.zeros((2 * nqm, 2 * raster.slen))
+    lib().om_jacobian_grid(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, nq, nm,
+                           dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32), _c(lidx, np.int32),
+                           _cplx(proj), 1 if logscale else 0, J)
     return J
 
 

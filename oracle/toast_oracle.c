@@ -945,3 +945,217 @@ void om_jacobian_nodal(const om_mesh *m, int nq, int nm, const double *dphi, con
         }
     free(pm); free(pk);
 }
+
+/* ------------------------------------------------------------------------
+ * Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,
+ * raster_px.cc:10-54): grid points p = bbmin + i*dW, dW = (bbmax-bbmin)/(g-1).
+ * ------------------------------------------------------------------------ */
+
+/* Mesh::BoundingBox (pad 0): node coordinate extremes */
+void om_bbox(const om_mesh *m, double *bbmin, double *bbmax)
+{
+    for (int d = 0; d < m->dim; d++) { bbmin[d] = 1e300; bbmax[d] = -1e300; }
+    for (int i = 0; i < m->n; i++)
+        for (int d = 0; d < m->dim; d++) {
+            double v = m->x[(size_t)i*m->dim + d];
+            if (v < bbmin[d]) bbmin[d] = v;
+            if (v > bbmax[d]) bbmax[d] = v;
+        }
+}
+
+/* Element_Unstructured::GContains (element.cc:553-563): element bounding box
+ * test with EPS 1e-8, then Local + LContains (pad = true) */
+static int el_gcontains(const om_mesh *m, int e, const double *p)
+{
+    const double EPS = 1e-8;
+    int dim = m->dim, nn = m->nn;
+    for (int d = 0; d < dim; d++) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < nn; i++) {
+            double v = m->x[(size_t)m->el[e*nn+i]*dim + d];
+            if (v < lo) lo = v;
+            if (v > hi) hi = v;
+        }
+        if (p[d] < lo-EPS || p[d] > hi+EPS) return 0;
+    }
+    double loc[3];
+    el_local(m, e, p, loc);
+    return el_lcontains(dim, loc);
+}
+
+/* GenerateElementPixelRef (mesh.cc:2944-3015): for every grid point the first
+ * element (ascending) whose EPS-inflated bounding box range holds it and
+ * which contains it; -1 outside the mesh */
+void om_raster_elref(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, int *elref)
+{
+    const double EPS = 1e-6;
+    int dim = m->dim, nn = m->nn;
+    int nx = gdim[0], ny = gdim[1], nz = dim > 2 ? gdim[2] : 1;
+    double dW[3];
+    for (int d = 0; d < dim; d++) dW[d] = (bbmax[d]-bbmin[d])/(double)(gdim[d]-1);
+    for (long i = 0; i < (long)nx*ny*nz; i++) elref[i] = -1;
+    for (int e = 0; e < m->elen; e++) {
+        double lo[3], hi[3];
+        int rmin[3] = {0, 0, 0}, rmax[3] = {1, 1, 1};
+        for (int d = 0; d < dim; d++) { lo[d] = 1e10; hi[d] = -1e10; }
+        for (int i = 0; i < nn; i++)
+            for (int d = 0; d < dim; d++) {
+                double v = m->x[(size_t)m->el[e*nn+i]*dim + d];
+                if (v < lo[d]) lo[d] = v;
+                if (v > hi[d]) hi[d] = v;
+            }
+        int overlap = 1;
+        for (int d = 0; d < dim; d++) {
+            lo[d] -= EPS; hi[d] += EPS;
+            if (lo[d] > bbmax[d] || hi[d] < bbmin[d]) { overlap = 0; break; }
+            int a = (int)ceil((lo[d]-bbmin[d])/dW[d]), b = (int)floor((hi[d]-bbmin[d])/dW[d]) + 1;
+            rmin[d] = a > 0 ? a : 0;
+            rmax[d] = b < gdim[d] ? b : gdim[d];
+        }
+        if (!overlap) continue;
+        double p[3];
+        for (int iz = rmin[2]; iz < rmax[2]; iz++) {
+            if (dim > 2) p[2] = iz*dW[2] + bbmin[2];
+            for (int iy = rmin[1]; iy < rmax[1]; iy++) {
+                p[1] = iy*dW[1] + bbmin[1];
+                for (int ix = rmin[0]; ix < rmax[0]; ix++) {
+                    long idx = ix + (long)nx*(iy + (long)ny*iz);
+                    if (elref[idx] >= 0) continue;
+                    p[0] = ix*dW[0] + bbmin[0];
+                    if (el_gcontains(m, e, p)) elref[idx] = e;
+                }
+            }
+        }
+    }
+}
+
+/* GridMapMatrix (mesh.cc:2486-2554): row per grid point with the element's
+ * global shape functions at the point (tet4.cc:302-313, tri3.cc:231-241);
+ * cols/w: glen x nn, col -1 / weight 0 where the point is outside. */
+void om_raster_bmap(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                    int *cols, double *w)
+{
+    int dim = m->dim, nn = m->nn;
+    int nx = gdim[0], ny = gdim[1], nz = dim > 2 ? gdim[2] : 1;
+    double dW[3];
+    for (int d = 0; d < dim; d++) dW[d] = (bbmax[d]-bbmin[d])/(double)(gdim[d]-1);
+    long i = 0;
+    double p[3];
+    for (int iz = 0; iz < nz; iz++) {
+        if (dim > 2) p[2] = iz*dW[2] + bbmin[2];
+        for (int iy = 0; iy < ny; iy++) {
+            p[1] = iy*dW[1] + bbmin[1];
+            for (int ix = 0; ix < nx; ix++, i++) {
+                for (int j = 0; j < nn; j++) { cols[i*nn+j] = -1; w[i*nn+j] = 0.0; }
+                int e = elref[i];
+                if (e < 0) continue;
+                p[0] = ix*dW[0] + bbmin[0];
+                const double *a = m->ga + e*nn, *b = m->gb + e*nn, *c = m->gc + e*nn, *dd = m->gd + e*nn;
+                double scale = dim == 3 ? 1.0/(6.0*m->size[e]) : 1.0/(2.0*m->size[e]);
+                for (int j = 0; j < nn; j++) {
+                    cols[i*nn+j] = m->el[e*nn+j];
+                    w[i*nn+j] = dim == 3 ? scale * (a[j] + b[j]*p[0] + c[j]*p[1] + dd[j]*p[2])
+                                         : scale * (a[j] + b[j]*p[0] + c[j]*p[1]);
+                }
+            }
+        }
+    }
+}
+
+/* ImageGradient (jacobian.cc:944-1060) of a complex grid image; spacing
+ * gsize/n per axis; mask = elref.  grad: dim x glen complex */
+void om_image_gradient(int dim, const int *gdim, const double *gsize, const double *im, const int *mask, double *grad)
+{
+    int nx = gdim[0], ny = gdim[1], nz = dim > 2 ? gdim[2] : 1;
+    long n = (long)nx*ny*nz;
+    long stride[3] = {1, nx, (long)nx*ny};
+    int len[3] = {nx, ny, nz};
+    for (int d = 0; d < dim; d++) {
+        double h = gsize[d]/len[d], ih = 1.0/h, i2h = 0.5*ih;
+        double *g = grad + 2*(size_t)d*n;
+        long s = stride[d];
+        for (long idx = 0; idx < n; idx++) {
+            int pos = (int)((idx / s) % len[d]);
+            int bcnt = mask[idx] >= 0;
+            int blo = pos > 0 && mask[idx-s] >= 0;
+            int bhi = pos < len[d]-1 && mask[idx+s] >= 0;
+            double *o = g + 2*idx;
+            if (blo && bhi) { o[0] = (im[2*(idx+s)]-im[2*(idx-s)]) * i2h; o[1] = (im[2*(idx+s)+1]-im[2*(idx-s)+1]) * i2h; }
+            else if (!bcnt) { o[0] = 0.0; o[1] = 0.0; }
+            else if (blo) { o[0] = (im[2*idx]-im[2*(idx-s)]) * ih; o[1] = (im[2*idx+1]-im[2*(idx-s)+1]) * ih; }
+            else if (bhi) { o[0] = (im[2*(idx+s)]-im[2*idx]) * ih; o[1] = (im[2*(idx+s)+1]-im[2*idx+1]) * ih; }
+            else { o[0] = 0.0; o[1] = 0.0; }
+        }
+    }
+}
+
+/* Raster::Map_MeshToGrid (raster.cc:188-199, Ax_cplx of the real B) */
+static void map_mesh_to_grid(long glen, int nn, const int *cols, const double *w, const double *f, double *out)
+{
+    for (long i = 0; i < glen; i++) {
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < nn; j++) {
+            if (cols[i*nn+j] < 0) continue;
+            re += w[i*nn+j] * f[2*(size_t)cols[i*nn+j]];
+            im += w[i*nn+j] * f[2*(size_t)cols[i*nn+j]+1];
+        }
+        out[2*i] = re; out[2*i+1] = im;
+    }
+}
+
+/* GenerateJacobian_grid (jacobian.cc:293-373) for Raster_Pixel with
+ * bdim == gdim: PMDF_mua = -d a, PMDF_kap = -sum_d grad_d d grad_d a
+ * (jacobian.cc:57-79), divided by the projection for log data, gathered
+ * into the solution basis (grid points inside the mesh, ascending), scaled
+ * by prod gsize/(bdim-1).  dphi: nq x n, aphi: nm x n complex;
+ * J: (2 nqm) x (2 slen). */
+void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                      int nq, int nm, const double *dphi, const double *aphi, const int *lptr, const int *lidx,
+                      const double *proj, int logscale, double *J)
+{
+    int dim = m->dim, nn = m->nn;
+    long glen = 1;
+    for (int d = 0; d < dim; d++) glen *= gdim[d];
+    double gsize[3], sz = 1.0;
+    for (int d = 0; d < dim; d++) { gsize[d] = bbmax[d]-bbmin[d]; sz *= gsize[d]/(gdim[d]-1.0); }
+    int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
+    double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
+    om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
+    long slen = 0;
+    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
+    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
+    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    int nqm = lptr[nq];
+    double *ca = (double*)xcalloc(2*(size_t)glen*nm, 8);
+    for (int j = 0; j < nm; j++) map_mesh_to_grid(glen, nn, cols, w, aphi + 2*(size_t)j*m->n, ca + 2*(size_t)j*glen);
+    double *cd = (double*)xcalloc(2*(size_t)glen, 8);
+    double *gd = (double*)xcalloc(2*(size_t)glen*dim, 8), *ga = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    size_t rowlen = 2*(size_t)slen;
+    for (int q = 0; q < nq; q++) {
+        map_mesh_to_grid(glen, nn, cols, w, dphi + 2*(size_t)q*m->n, cd);
+        om_image_gradient(dim, gdim, gsize, cd, elref, gd);
+        for (int k = lptr[q]; k < lptr[q+1]; k++) {
+            const double *a = ca + 2*(size_t)lidx[k]*glen;
+            om_image_gradient(dim, gdim, gsize, a, elref, ga);
+            double pr = proj[2*k], pi = proj[2*k+1], den = pr*pr + pi*pi;
+            double *R0 = J + (size_t)k*rowlen, *R1 = J + (size_t)(k+nqm)*rowlen;
+            for (long s = 0; s < slen; s++) {
+                long g = s2b[s];
+                double pm[2], pk[2] = {0.0, 0.0}, t[2];
+                cmul(cd + 2*g, a + 2*g, t);
+                pm[0] = -t[0]; pm[1] = -t[1];
+                for (int d = 0; d < dim; d++) {
+                    cmul(gd + 2*((size_t)d*glen + g), ga + 2*((size_t)d*glen + g), t);
+                    pk[0] -= t[0]; pk[1] -= t[1];
+                }
+                if (logscale) {
+                    double ar = (pm[0]*pr + pm[1]*pi)/den, ai = (pm[1]*pr - pm[0]*pi)/den;
+                    double br = (pk[0]*pr + pk[1]*pi)/den, bi = (pk[1]*pr - pk[0]*pi)/den;
+                    pm[0] = ar; pm[1] = ai; pk[0] = br; pk[1] = bi;
+                }
+                R0[s] = pm[0]*sz; R1[s] = pm[1]*sz; R0[s+slen] = pk[0]*sz; R1[s+slen] = pk[1]*sz;
+            }
+        }
+    }
+    free(cols); free(w); free(s2b); free(ca); free(cd); free(gd); free(ga);
+}

@@ -1,0 +1,140 @@
+"""The oracle's raster basis and grid Jacobian (Raster_Pixel with basis grid
+== intermediate grid, GenerateJacobian_grid), pinned before it is trusted:
+
+  * element of every grid point against a brute-force barycentric search;
+  * grid map B: partition of unity and exact reproduction of linear fields;
+  * ImageGradient: closed form on linear fields (spacing gsize/n, the
+    reference's choice, jacobian.cc:961) and its mask rules on a hand case;
+  * grid Jacobian: its voxel sums are a quadrature of the same integrals the
+    nodal Jacobian's node sums give exactly (sum_i phi_i = 1), so the two
+    must agree to quadrature error -- this fixes sign, Re/Im split, link
+    order, 1/proj and the voxel-size scale.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from toastpp_amd import meshgen as G
+
+
+def _barycentric_elref(nodes, el, pts, tol=1e-8):
+    """min element index whose barycentric coordinates at p are >= -tol."""
+    dim = nodes.shape[1]
+    out = np.full(len(pts), -1)
+    for e in range(len(el)):
+        X = nodes[el[e]]
+        T = (X[1:] - X[0]).T
+        lam = np.linalg.solve(T, (pts - X[0]).T).T
+        full = np.c_[1.0 - lam.sum(1), lam]
+        inside = np.all(full >= -tol * 10, axis=1)
+        sel = inside & (out < 0)
+        out[sel] = e
+    return out
+
+
+@pytest.mark.parametrize("case", ["disk", "cyl"])
+def test_elref_brute_force(case):
+    if case == "disk":
+        nodes, el = G.disk(10.0, 4)
+        gdim = [13, 11]
+    else:
+        nodes, el = G.cylinder(6.0, 6.0, 2, 3)
+        gdim = [7, 6, 5]
+    m = O.Mesh(nodes=nodes, elems=el)
+    R = O.Raster(m, gdim)
+    axes = [R.bbmin[d] + np.arange(gdim[d]) * (R.bbmax[d] - R.bbmin[d]) / (gdim[d] - 1) for d in range(len(gdim))]
+    mesh_ = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([g.transpose().ravel() for g in mesh_], axis=1)   # x fastest
+    ref = _barycentric_elref(nodes, el, pts)
+    # points well inside or well outside agree exactly; points on shared
+    # faces belong to the lowest-numbered element in both
+    assert np.array_equal(R.elref, ref)
+    assert R.slen == int(np.count_nonzero(ref >= 0))
+    assert np.all(np.diff(R.sol2basis) > 0)
+
+
+@pytest.mark.parametrize("case", ["disk", "cyl"])
+def test_grid_map_linear_reproduction(case):
+    nodes, el = (G.disk(10.0, 5) if case == "disk" else G.cylinder(6.0, 6.0, 3, 3))
+    m = O.Mesh(nodes=nodes, elems=el)
+    gdim = [21, 17] if case == "disk" else [9, 9, 8]
+    R = O.Raster(m, gdim)
+    cols, w = R.bmap()
+    inside = R.elref >= 0
+    assert np.allclose(w[inside].sum(1), 1.0, atol=1e-12)
+    assert np.all(w[~inside] == 0) and np.all(cols[~inside] == -1)
+    coef = np.array([0.7, -1.3, 0.4][:nodes.shape[1]])
+    f = nodes @ coef + 2.5
+    axes = [R.bbmin[d] + np.arange(gdim[d]) * (R.bbmax[d] - R.bbmin[d]) / (gdim[d] - 1) for d in range(len(gdim))]
+    pts = np.stack([g.transpose().ravel() for g in np.meshgrid(*axes, indexing="ij")], axis=1)
+    mapped = (w * np.where(cols >= 0, f[np.maximum(cols, 0)], 0.0)).sum(1)
+    assert np.allclose(mapped[inside], pts[inside] @ coef + 2.5, rtol=1e-12, atol=1e-12)
+
+
+def test_image_gradient_closed_form_and_mask_rules():
+    gdim = [8, 6, 5]
+    gsize = np.array([7.0, 10.0, 4.0])
+    x = [np.arange(g) * s / (g - 1) for g, s in zip(gdim, gsize)]
+    X = np.meshgrid(*x, indexing="ij")
+    a = np.array([1.5 + 0.5j, -2.0, 0.25j])
+    im = sum(a[d] * X[d] for d in range(3)).transpose().ravel()
+    mask = np.zeros(im.size, np.int32)
+    g = O.image_gradient(gdim, gsize, im, mask)
+    for d in range(3):
+        # grid spacing gsize/(n-1) over a finite-difference step gsize/n
+        assert np.allclose(g[d], a[d] * gdim[d] / (gdim[d] - 1), rtol=1e-13, atol=1e-13)
+    # mask rules along x on a 5 x 2 grid: centre masked with both neighbours
+    # inside still takes the central difference (the reference tests the
+    # neighbours first); a point with one masked neighbour uses the other
+    gdim = [5, 2]
+    im = np.arange(10, dtype=float) ** 2 + 0j
+    mask = np.array([0, -1, 0, 0, -1,   0, 0, 0, -1, -1], np.int32)
+    g = O.image_gradient(gdim, np.array([5.0, 2.0]), im, mask)[0].real
+    ih = 5 / 5.0
+    exp = np.array([0.0,                          # x=0: right neighbour masked, no left -> 0
+                    (im[2] - im[0]).real * ih / 2,   # x=1 masked, both neighbours in mask -> central
+                    (im[3] - im[2]).real * ih,      # x=2: left masked -> forward
+                    (im[3] - im[2]).real * ih,      # x=3: right masked -> backward
+                    0.0,                          # x=4 masked, left in mask but no right -> 0
+                    (im[6] - im[5]).real * ih,
+                    (im[7] - im[5]).real * ih / 2,
+                    (im[7] - im[6]).real * ih,
+                    0.0, 0.0])
+    assert np.allclose(g, exp, rtol=1e-14)
+
+
+def _fields(m, Q, M, mua, mus, ref, freq):
+    cm, ck, c2 = O.coeffs(mua, mus, ref)
+    rp, ci, val = m.sysmat(cm, ck, c2, freq * 2 * np.pi * 1e-6)
+    qv = m.qvec_gaussian(Q, 2.0)
+    mv = m.qvec_gaussian(M, 2.0) * c2[None, :]
+    d = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-14, 20000, True)[0] for q in qv])
+    a = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-14, 20000, True)[0] for q in mv])
+    return d, a, mv
+
+
+def test_grid_jacobian_quadrature_of_nodal_jacobian():
+    nodes, el = G.disk(25.0, 16)
+    m = O.Mesh(nodes=nodes, elems=el)
+    n = m.n
+    Q, M = G.ring(4, 25.0), G.ring(4, 25.0, offset=0.5)
+    lp = np.array([0, 2, 4, 6, 8], np.int32)
+    li = np.array([1, 2, 2, 3, 3, 0, 0, 1], np.int32)
+    mua, mus, ref = np.full(n, 0.01), np.full(n, 1.0), np.full(n, 1.4)
+    d, a, mv = _fields(m, Q, M, mua, mus, ref, 100.0)
+    proj = O.project(d, mv.astype(complex), lp, li, logscale=False)
+    Jn = O.jacobian_nodal(m, d, a, lp, li, proj, logscale=True)
+    R = O.Raster(m, [96, 96])
+    Jg = O.jacobian_grid(R, d, a, lp, li, proj, logscale=True)
+    nqm = len(li)
+    assert Jg.shape == (2 * nqm, 2 * R.slen)
+    sn_mua, sg_mua = Jn[:, :n].sum(1), Jg[:, :R.slen].sum(1)
+    sn_kap, sg_kap = Jn[:, n:].sum(1), Jg[:, R.slen:].sum(1)
+    # the mua integrand is sampled exactly at the voxels: quadrature error only
+    # (0.13 % at 96 x 96, 0.09 % at 192 x 192)
+    assert np.max(np.abs(sg_mua / sn_mua - 1)) < 0.005
+    # the kappa integrand uses finite differences with step gsize/n
+    # (4.6 % at 96 x 96, 2.2 % at 192 x 192)
+    assert np.max(np.abs(sg_kap / sn_kap - 1)) < 0.07
+    # the wrong sign, a swapped Re/Im block or a missing voxel scale fails
+    assert not np.allclose(-sg_mua, sn_mua, rtol=0.5)

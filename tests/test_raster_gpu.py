@@ -1,0 +1,113 @@
+"""GPU raster basis and grid Jacobian (Raster_Pixel, GenerateJacobian_grid)
+against the FP64 oracle, through the C-ABI.  The oracle's raster is pinned
+in test_oracle_raster.py."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import CASES
+
+pytestmark = pytest.mark.gpu
+
+TMI = pytest.importorskip("toastpp_amd")
+
+GRIDS = {"2d": [64, 64], "3d": [24, 22, 20]}
+
+
+@pytest.fixture(scope="module", params=["2d", "3d"])
+def pair(request):
+    case = CASES["2d_real_cg" if request.param == "2d" else "3d_cplx_direct"]
+    om = O.Mesh(case["mesh"])
+    gm = TMI.Mesh(case["mesh"])
+    g = GRIDS[request.param]
+    return om, gm, O.Raster(om, g), TMI.Raster(gm, np.array(g))
+
+
+def test_raster_structure(pair):
+    om, gm, orr, gr = pair
+    assert gr.GLen() == orr.glen and gr.SLen() == orr.slen
+    assert np.array_equal(gr.bbmin, orr.bbmin) and np.array_equal(gr.bbmax, orr.bbmax)
+    assert np.array_equal(gr.Elref(), orr.elref)
+    assert np.array_equal(gr.Sol2Basis(), orr.sol2basis)
+    cols, w = gr._bmap()
+    ocols, ow = orr.bmap()
+    assert np.array_equal(cols, ocols)
+    assert np.max(np.abs(w - ow)) <= 1e-15
+
+
+def test_map_mesh_to_grid(pair):
+    om, gm, orr, gr = pair
+    rng = np.random.default_rng(2)
+    n = om.n
+    f = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    g = gr.MeshToGrid(f)
+    cols, w = orr.bmap()
+    ref = np.einsum("gj,gjc->gc", w, np.where(cols[:, :, None] >= 0, f[np.maximum(cols, 0)], 0))
+    assert np.max(np.abs(g - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # real input stays real; M->S is the gather of M->B
+    fr = rng.standard_normal(n)
+    assert not np.iscomplexobj(gr.Map("M->B", fr))
+    assert np.array_equal(gr.Map("M->S", fr), gr.Map("M->B", fr)[gr.Sol2Basis()])
+    s = gr.Map("B->S", gr.Map("M->B", fr))
+    assert np.array_equal(gr.Map("S->B", s)[gr.Sol2Basis()], s)
+
+
+@pytest.mark.parametrize("scale", ["log", "lin"])
+def test_grid_jacobian_vs_oracle(pair, scale):
+    """Arbitrary complex fields on a reduced link list."""
+    om, gm, orr, gr = pair
+    n = om.n
+    dim = om.dim
+    rng = np.random.default_rng(7)
+    nq, nm = 3, 4
+    Q = rng.standard_normal((nq, dim))
+    M = rng.standard_normal((nm, dim))
+    links = [[0, 2, 3], [1], [0, 1, 2, 3]]
+    gm.SetQM(Q, M, links=links)
+    lp = np.array([0, 3, 4, 8], np.int32)
+    li = np.array([0, 2, 3, 1, 0, 1, 2, 3], np.int32)
+    nqm = len(li)
+    dphi = (rng.standard_normal((n, nq)) + 1j * rng.standard_normal((n, nq)))
+    aphi = (rng.standard_normal((n, nm)) + 1j * rng.standard_normal((n, nm)))
+    proj = rng.standard_normal(nqm) + 1j * rng.standard_normal(nqm)
+    from toastpp_amd.jacobian import jacobian
+    Jg = jacobian(gm, gr, dphi, aphi, proj, scale=scale)
+    Jo = O.jacobian_grid(orr, dphi.T, aphi.T, lp, li, proj, logscale=scale == "log")
+    assert Jg.shape == Jo.shape == (2 * nqm, 2 * orr.slen)
+    rel = np.max(np.abs(Jg - Jo)) / np.max(np.abs(Jo))
+    print("grid Jacobian %dD %s: max diff / max |J| = %.3g" % (dim, scale, rel))
+    assert rel <= 1e-13
+
+
+def test_raster_jacobian_optical_2d():
+    """JacobianOptical in the raster basis: GPU fields and Jacobian against
+    the oracle's fields and grid Jacobian (1e-6 block Frobenius)."""
+    case = CASES["2d_cplx_bicgstab"]
+    om = O.Mesh(case["mesh"])
+    Q, Mp, lp, li = om.read_qm(case["qm"])
+    gm = TMI.Mesh(case["mesh"])
+    gm.ReadQM(case["qm"])
+    n = om.n
+    rng = np.random.default_rng(5)
+    mua = 0.01 + 0.01 * rng.random(n)
+    mus = 1.0 + 0.5 * rng.random(n)
+    ref = np.full(n, 1.4)
+    freq = 100.0
+    gr = TMI.Raster(gm, np.array([48, 48]))
+    Jg = gm.JacobianOptical(gr, gm.Qvec(width=2.0), gm.Mvec(width=2.0, ref=1.4), mua, mus, ref, freq,
+                            solver="bicgstab", tol=1e-13, mode="fp64")
+    cm, ck, c2 = O.coeffs(mua, mus, ref)
+    rp, ci, val = om.sysmat(cm, ck, c2, freq * 2 * np.pi * 1e-6)
+    qv = om.qvec_gaussian(Q, 2.0)
+    mv = om.qvec_gaussian(Mp, 2.0) * c2[None, :]
+    d = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-15, 20000, True)[0] for q in qv])
+    a = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-15, 20000, True)[0] for q in mv])
+    proj = O.project(d, mv.astype(complex), lp, li, logscale=False)
+    Jo = O.jacobian_grid(O.Raster(om, [48, 48]), d, a, lp, li, proj, logscale=True)
+    nqm, slen = int(lp[-1]), gr.SLen()
+    worst = 0.0
+    for r in (slice(0, nqm), slice(nqm, 2 * nqm)):
+        for c in (slice(0, slen), slice(slen, 2 * slen)):
+            worst = max(worst, np.linalg.norm(Jg[r, c] - Jo[r, c]) / np.linalg.norm(Jo[r, c]))
+    print("raster JacobianOptical 2D: block-relative Frobenius %.3g" % worst)
+    assert worst <= 1e-6

@@ -11,5 +11,6 @@ C-ABI in include/toastpp_amd.h); this package is its Python surface:
 from ._lib import TmiError, NotConverged  # noqa: F401
 from .fwdsolver import FwdSolver, Solution  # noqa: F401
 from .mesh import Mesh  # noqa: F401
+from .raster import Raster  # noqa: F401
 
-__all__ = ["Mesh", "FwdSolver", "Solution", "TmiError", "NotConverged"]
+__all__ = ["Mesh", "FwdSolver", "Solution", "Raster", "TmiError", "NotConverged"]

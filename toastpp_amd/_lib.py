@@ -101,12 +101,15 @@ _SIGS = {
     "tmi_solver_time_kernel": [P, C.c_int, C.c_int, DP, DP],
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
-    "tmi_raster_create": [P, I, PP],
+    "tmi_raster_create": [P, I, P, PP],
     "tmi_raster_free": [P],
     "tmi_raster_dims": [P, IP, IP],
-    "tmi_raster_get_elref": [P, I, I],
-    "tmi_raster_map_mesh_to_grid": [P, D, D],
+    "tmi_raster_bbox": [P, D, D],
+    "tmi_raster_get_elref": [P, P, P],
+    "tmi_raster_get_bmap": [P, P, P],
+    "tmi_raster_map_mesh_to_grid": [P, C.c_int, D, D],
     "tmi_jacobian_raster": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
+    "tmi_jacobian_raster_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
 }
 
 _lib = None

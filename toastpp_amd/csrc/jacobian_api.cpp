@@ -3,10 +3,17 @@
 #include <cstring>
 #include <vector>
 
+#include <algorithm>
+
 #include "jacobian.hpp"
+#include "raster.hpp"
 #include "solver.hpp"
 
 #include "handles.hpp"
+
+struct tmi_raster_s {
+    tmi::Raster r;
+};
 
 namespace tmi {
 int mesh_dmax(Mesh &m, hipStream_t st);
@@ -111,14 +118,158 @@ int tmi_jacobian_nodal_device(tmi_mesh h, int nq, int nm, const void *d_dphi, in
     API_END
 }
 
-int tmi_raster_create(tmi_mesh, const int *, tmi_raster *) { return TMI_ERR_UNSUPPORTED; }
-int tmi_raster_free(tmi_raster) { return TMI_OK; }
-int tmi_raster_dims(tmi_raster, int *, int *) { return TMI_ERR_UNSUPPORTED; }
-int tmi_raster_get_elref(tmi_raster, int *, int *) { return TMI_ERR_UNSUPPORTED; }
-int tmi_raster_map_mesh_to_grid(tmi_raster, const double *, double *) { return TMI_ERR_UNSUPPORTED; }
-int tmi_jacobian_raster(tmi_raster, int, int, const double *, const double *, const double *, int, double *)
+int tmi_raster_create(tmi_mesh h, const int *gdim, const double *bb, tmi_raster *out)
 {
-    return TMI_ERR_UNSUPPORTED;
+    API_BEGIN
+    TMI_CHECK(h && gdim && out, TMI_ERR_INVALID_ARG, "NULL argument");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    tmi_raster r = new tmi_raster_s;
+    try {
+        Raster &R = r->r;
+        R.mesh = &m;
+        R.dim = m.dim;
+        R.glen = 1;
+        for (int d = 0; d < m.dim; d++) {
+            TMI_CHECK(gdim[d] >= 2, TMI_ERR_INVALID_ARG, "grid dimension %d must be >= 2", gdim[d]);
+            R.gdim[d] = gdim[d];
+            R.glen *= gdim[d];
+        }
+        if (bb) {
+            for (int d = 0; d < m.dim; d++) { R.bbmin[d] = bb[d]; R.bbmax[d] = bb[m.dim + d]; }
+        } else {   // Mesh::BoundingBox with pad 0
+            for (int d = 0; d < m.dim; d++) { R.bbmin[d] = 1e300; R.bbmax[d] = -1e300; }
+            for (int i = 0; i < m.n; i++)
+                for (int d = 0; d < m.dim; d++) {
+                    const double v = m.h_nodes[(size_t)i * m.dim + d];
+                    R.bbmin[d] = std::min(R.bbmin[d], v);
+                    R.bbmax[d] = std::max(R.bbmax[d], v);
+                }
+        }
+        for (int d = 0; d < m.dim; d++) {
+            TMI_CHECK(R.bbmax[d] > R.bbmin[d], TMI_ERR_INVALID_ARG, "invalid bounding box");
+            R.dW[d] = (R.bbmax[d] - R.bbmin[d]) / (double)(R.gdim[d] - 1);
+        }
+        R.build(st);
+    } catch (...) {
+        delete r;
+        throw;
+    }
+    *out = r;
+    API_END
+}
+
+int tmi_raster_free(tmi_raster r)
+{
+    API_BEGIN
+    if (r) {
+        (void)hipStreamSynchronize(Context::get().stream);
+        delete r;
+    }
+    API_END
+}
+
+int tmi_raster_dims(tmi_raster r, int *glen, int *slen)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    if (glen) *glen = (int)r->r.glen;
+    if (slen) *slen = r->r.slen;
+    API_END
+}
+
+int tmi_raster_bbox(tmi_raster r, double *bbmin, double *bbmax)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    for (int d = 0; d < r->r.dim; d++) {
+        if (bbmin) bbmin[d] = r->r.bbmin[d];
+        if (bbmax) bbmax[d] = r->r.bbmax[d];
+    }
+    API_END
+}
+
+int tmi_raster_get_elref(tmi_raster r, int *elref, int *sol2basis)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    hipStream_t st = Context::get().stream;
+    if (elref) r->r.elref.download(elref, r->r.glen, st);
+    if (sol2basis) r->r.sol2basis.download(sol2basis, r->r.slen, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_get_bmap(tmi_raster r, int *cols, double *w)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    hipStream_t st = Context::get().stream;
+    const size_t cnt = (size_t)r->r.glen * (r->r.dim + 1);
+    if (cols) r->r.bcols.download(cols, cnt, st);
+    if (w) r->r.bw.download(w, cnt, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_map_mesh_to_grid(tmi_raster r, int ncols, const double *mvec, double *gvec)
+{
+    API_BEGIN
+    TMI_CHECK(r && mvec && gvec && ncols > 0, TMI_ERR_INVALID_ARG, "invalid argument");
+    hipStream_t st = Context::get().stream;
+    const Raster &R = r->r;
+    DBuf<double> df, dg((size_t)R.glen * ncols * 2);
+    df.upload(mvec, (size_t)R.mesh->n * ncols * 2, st);
+    launch_raster_map(R, ncols, df.p, ncols, dg.p, st);
+    dg.download(gvec, (size_t)R.glen * ncols * 2, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+static void jacobian_raster_dev(Raster &R, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi,
+                                int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st)
+{
+    Mesh &m = *R.mesh;
+    const QM &qm = m.qm;
+    TMI_CHECK(qm.nq == nq && qm.nm == nm, TMI_ERR_INVALID_ARG,
+              "field counts (%d sources, %d detectors) do not match the QM description (%d, %d)", nq, nm, qm.nq,
+              qm.nm);
+    TMI_CHECK(qm.nqm > 0, TMI_ERR_INVALID_ARG, "the QM description links no measurements");
+    TMI_CHECK(ldd >= nq && lda >= nm, TMI_ERR_INVALID_ARG, "field row strides below the field counts");
+    TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
+    ensure_links(m, st);
+    launch_jacobian_grid(R, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lm.p, qm.nqm, d_proj,
+                         dscale == TMI_DATA_LOG ? 1 : 0, d_J, st);
+}
+
+int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                        int dscale, double *J)
+{
+    API_BEGIN
+    TMI_CHECK(r && dphi && aphi && proj && J, TMI_ERR_INVALID_ARG, "NULL argument");
+    Raster &R = r->r;
+    hipStream_t st = Context::get().stream;
+    const size_t n = R.mesh->n, nqm = R.mesh->qm.nqm;
+    DBuf<double> dd, da, dp, dJ(4 * nqm * (size_t)R.slen);
+    dd.upload(dphi, n * nq * 2, st);
+    da.upload(aphi, n * nm * 2, st);
+    dp.upload(proj, nqm * 2, st);
+    jacobian_raster_dev(R, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
+    dJ.download(J, 4 * nqm * (size_t)R.slen, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi, int lda,
+                               const void *d_proj, int dscale, void *d_J)
+{
+    API_BEGIN
+    TMI_CHECK(r && d_dphi && d_aphi && d_proj && d_J, TMI_ERR_INVALID_ARG, "NULL argument");
+    hipStream_t st = Context::get().stream;
+    jacobian_raster_dev(r->r, nq, nm, (const double *)d_dphi, ldd, (const double *)d_aphi, lda,
+                        (const double *)d_proj, dscale, (double *)d_J, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
 }
 
 }  // extern "C"

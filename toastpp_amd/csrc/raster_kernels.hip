@@ -1,0 +1,342 @@
+// Raster basis (Raster_Pixel, basis grid == intermediate grid) and the grid
+// Jacobian (GenerateJacobian_grid, jacobian.cc:293-373) on the GPU.
+//
+// Compiled without floating-point contraction: the grid coordinates, local
+// coordinates and shape functions follow the reference operation by
+// operation (mesh.cc:2486-2554, 2944-3015; tet4.cc:186-196, 268-279,
+// 302-313; tri3.cc:134-144, 191-201, 231-241).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "raster.hpp"
+
+namespace tmi {
+
+struct GridDesc {
+    int nx, ny, nz;
+    double bx, by, bz;     // bbmin
+    double hx, hy, hz;     // spacing (bbmax - bbmin) / (g - 1)
+    double tx, ty, tz;     // bbmax
+};
+
+__global__ void k_fill_int(int *p, long n, int v)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// GenerateElementPixelRef: one thread per element visits the grid points of
+// its bounding box (inflated by 1e-6) and claims those it contains; the
+// reference keeps the first (lowest-numbered) containing element, which is
+// the minimum over all containing elements.
+template <int DIM>
+__global__ void k_elref(int elen, const double *__restrict__ nodes, const int *__restrict__ el,
+                        const double *__restrict__ geo, GridDesc gd, int *__restrict__ elref)
+{
+    constexpr int NN = DIM + 1;
+    constexpr int GS = DIM == 3 ? GEO3 : GEO2;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= elen) return;
+    const double EPS = 1e-6, EPSB = 1e-8, EPSL = 1e-8;
+    double lo[3] = {1e10, 1e10, 1e10}, hi[3] = {-1e10, -1e10, -1e10};   // element bounding box
+    for (int i = 0; i < NN; i++) {
+        const double *x = nodes + (long)el[(long)e * NN + i] * DIM;
+        for (int d = 0; d < DIM; d++) {
+            if (x[d] < lo[d]) lo[d] = x[d];
+            if (x[d] > hi[d]) hi[d] = x[d];
+        }
+    }
+    const double bmin[3] = {gd.bx, gd.by, gd.bz}, bmax[3] = {gd.tx, gd.ty, gd.tz}, h[3] = {gd.hx, gd.hy, gd.hz};
+    const int g[3] = {gd.nx, gd.ny, gd.nz};
+    int rmin[3] = {0, 0, 0}, rmax[3] = {1, 1, 1};
+    for (int d = 0; d < DIM; d++) {
+        const double l = lo[d] - EPS, u = hi[d] + EPS;
+        if (l > bmax[d] || u < bmin[d]) return;
+        rmin[d] = max(0, (int)ceil((l - bmin[d]) / h[d]));
+        rmax[d] = min(g[d], (int)floor((u - bmin[d]) / h[d]) + 1);
+    }
+    const double *gg = geo + (long)e * GS;
+    const double *a = gg, *b = gg + NN, *c = gg + 2 * NN, *dd = gg + 3 * NN;
+    double sum = a[0] + a[1] + a[2];
+    if (DIM == 3) sum = sum + a[3];
+    const double scale = 1.0 / sum;
+    double p[3] = {0, 0, 0};
+    for (int iz = rmin[2]; iz < rmax[2]; iz++) {
+        if (DIM > 2) p[2] = iz * gd.hz + gd.bz;
+        for (int iy = rmin[1]; iy < rmax[1]; iy++) {
+            p[1] = iy * gd.hy + gd.by;
+            for (int ix = rmin[0]; ix < rmax[0]; ix++) {
+                p[0] = ix * gd.hx + gd.bx;
+                bool in = true;
+                for (int d = 0; d < DIM; d++)
+                    if (p[d] < lo[d] - EPSB || p[d] > hi[d] + EPSB) in = false;
+                if (!in) continue;
+                double l0, l1, l2 = 0.0;
+                if (DIM == 3) {
+                    l0 = (a[1] + b[1] * p[0] + c[1] * p[1] + dd[1] * p[2]) * scale;
+                    l1 = (a[2] + b[2] * p[0] + c[2] * p[1] + dd[2] * p[2]) * scale;
+                    l2 = (a[3] + b[3] * p[0] + c[3] * p[1] + dd[3] * p[2]) * scale;
+                    in = l0 + EPSL >= 0.0 && l1 + EPSL >= 0.0 && l2 + EPSL >= 0.0 && l0 + l1 + l2 - EPSL <= 1.0;
+                } else {
+                    l0 = (a[1] + b[1] * p[0] + c[1] * p[1]) * scale;
+                    l1 = (a[2] + b[2] * p[0] + c[2] * p[1]) * scale;
+                    in = l0 + EPSL >= 0.0 && l1 + EPSL >= 0.0 && l0 + l1 - EPSL <= 1.0;
+                }
+                if (in) atomicMin(&elref[ix + (long)gd.nx * (iy + (long)gd.ny * iz)], e);
+            }
+        }
+    }
+}
+
+__global__ void k_elref_finish(int *elref, long n, int *flag)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = elref[i] == INT_MAX ? -1 : elref[i];
+    elref[i] = v;
+    flag[i] = v >= 0;
+}
+
+__global__ void k_compact(const int *__restrict__ flag, const int *__restrict__ pos, long n, int *__restrict__ out)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = (int)i;
+}
+
+// GridMapMatrix rows: global shape functions of the holding element
+template <int DIM>
+__global__ void k_bmap(long glen, const int *__restrict__ elref, const int *__restrict__ el,
+                       const double *__restrict__ geo, GridDesc gd, int *__restrict__ cols, double *__restrict__ w)
+{
+    constexpr int NN = DIM + 1;
+    constexpr int GS = DIM == 3 ? GEO3 : GEO2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= glen) return;
+    const int e = elref[i];
+    if (e < 0) {
+        for (int j = 0; j < NN; j++) { cols[i * NN + j] = -1; w[i * NN + j] = 0.0; }
+        return;
+    }
+    const int ix = (int)(i % gd.nx), iy = (int)((i / gd.nx) % gd.ny), iz = (int)(i / ((long)gd.nx * gd.ny));
+    double p[3];
+    p[0] = ix * gd.hx + gd.bx;
+    p[1] = iy * gd.hy + gd.by;
+    p[2] = DIM > 2 ? iz * gd.hz + gd.bz : 0.0;
+    const double *gg = geo + (long)e * GS;
+    const double *a = gg, *b = gg + NN, *c = gg + 2 * NN, *dd = gg + 3 * NN;
+    const double size = DIM == 3 ? gg[16] : gg[9];
+    const double scale = DIM == 3 ? 1.0 / (6.0 * size) : 1.0 / (2.0 * size);
+    for (int j = 0; j < NN; j++) {
+        cols[i * NN + j] = el[(long)e * NN + j];
+        w[i * NN + j] = DIM == 3 ? scale * (a[j] + b[j] * p[0] + c[j] * p[1] + dd[j] * p[2])
+                                 : scale * (a[j] + b[j] * p[0] + c[j] * p[1]);
+    }
+}
+
+void Raster::build(hipStream_t st)
+{
+    Mesh &m = *mesh;
+    const int nn = m.nn;
+    GridDesc gd{gdim[0], gdim[1], gdim[2], bbmin[0], bbmin[1], bbmin[2], dW[0], dW[1], dW[2],
+                bbmax[0], bbmax[1], bbmax[2]};
+    elref.alloc(glen);
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((glen + 255) / 256)), dim3(256), 0, st, elref.p, glen, INT_MAX);
+    TMI_LAUNCH_CHECK();
+    if (dim == 3)
+        hipLaunchKernelGGL(k_elref<3>, dim3((m.elen + 127) / 128), dim3(128), 0, st, m.elen, m.nodes.p, m.elist.p,
+                           m.geo.p, gd, elref.p);
+    else
+        hipLaunchKernelGGL(k_elref<2>, dim3((m.elen + 127) / 128), dim3(128), 0, st, m.elen, m.nodes.p, m.elist.p,
+                           m.geo.p, gd, elref.p);
+    TMI_LAUNCH_CHECK();
+    DBuf<int> flag(glen), pos(glen + 1);
+    const dim3 gg((unsigned)((glen + 255) / 256));
+    hipLaunchKernelGGL(k_elref_finish, gg, dim3(256), 0, st, elref.p, glen, flag.p);
+    TMI_LAUNCH_CHECK();
+    TMI_CHECK(glen < INT_MAX, TMI_ERR_UNSUPPORTED, "grid of %ld points is too large", glen);
+    launch_exclusive_scan(flag.p, pos.p, (int)glen, st);
+    TMI_HIP(hipMemcpyAsync(&slen, pos.p + glen, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    sol2basis.alloc(slen > 0 ? slen : 1);
+    hipLaunchKernelGGL(k_compact, gg, dim3(256), 0, st, flag.p, pos.p, glen, sol2basis.p);
+    TMI_LAUNCH_CHECK();
+    bcols.alloc((size_t)glen * nn);
+    bw.alloc((size_t)glen * nn);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_bmap<3>, gg, dim3(256), 0, st, glen, elref.p, m.elist.p, m.geo.p, gd, bcols.p, bw.p);
+    else
+        hipLaunchKernelGGL(k_bmap<2>, gg, dim3(256), 0, st, glen, elref.p, m.elist.p, m.geo.p, gd, bcols.p, bw.p);
+    TMI_LAUNCH_CHECK();
+    TMI_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- fields
+// out[g][c] = sum_j w[g][j] f[col[g][j]][c] (complex), j ascending as the
+// CSR row of B; thread per (g, c) with c fastest
+template <int NN>
+__global__ void k_raster_map(long glen, int ncols, const int *__restrict__ cols, const double *__restrict__ w,
+                             const double2 *__restrict__ f, int ld, double2 *__restrict__ out, long ldo)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= glen * ncols) return;
+    const long g = t / ncols;
+    const int c = (int)(t % ncols);
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < NN; j++) {
+        const int nd = cols[g * NN + j];
+        if (nd < 0) continue;
+        const double2 v = f[(long)nd * ld + c];
+        re += w[g * NN + j] * v.x;
+        im += w[g * NN + j] * v.y;
+    }
+    out[g * ldo + c] = make_double2(re, im);
+}
+
+void launch_raster_map(const Raster &r, int ncols, const double *d_f, int ld, double *d_out, hipStream_t st)
+{
+    const long cnt = r.glen * ncols;
+    const dim3 g((unsigned)((cnt + 255) / 256));
+    if (r.dim == 3)
+        hipLaunchKernelGGL(k_raster_map<4>, g, dim3(256), 0, st, r.glen, ncols, r.bcols.p, r.bw.p,
+                           (const double2 *)d_f, ld, (double2 *)d_out, (long)ncols);
+    else
+        hipLaunchKernelGGL(k_raster_map<3>, g, dim3(256), 0, st, r.glen, ncols, r.bcols.p, r.bw.p,
+                           (const double2 *)d_f, ld, (double2 *)d_out, (long)ncols);
+    TMI_LAUNCH_CHECK();
+}
+
+// ImageGradient (jacobian.cc:944-1060) at the solution voxels: for column
+// c and solution voxel s, fgs[c][s] = (value, d/dx, d/dy, d/dz) with the
+// reference's spacing gsize/n and mask rules (neighbours tested first).
+__device__ inline double2 grad1(const double2 *__restrict__ v, long ldv, const int *__restrict__ mask, long g,
+                                int pos, int len, long s, double ih, double i2h)
+{
+    const bool bcnt = mask[g] >= 0;
+    const bool blo = pos > 0 && mask[g - s] >= 0;
+    const bool bhi = pos < len - 1 && mask[g + s] >= 0;
+    if (blo && bhi) {
+        const double2 a = v[(g + s) * ldv], b = v[(g - s) * ldv];
+        return make_double2((a.x - b.x) * i2h, (a.y - b.y) * i2h);
+    }
+    if (!bcnt) return make_double2(0.0, 0.0);
+    if (blo) {
+        const double2 a = v[g * ldv], b = v[(g - s) * ldv];
+        return make_double2((a.x - b.x) * ih, (a.y - b.y) * ih);
+    }
+    if (bhi) {
+        const double2 a = v[(g + s) * ldv], b = v[g * ldv];
+        return make_double2((a.x - b.x) * ih, (a.y - b.y) * ih);
+    }
+    return make_double2(0.0, 0.0);
+}
+
+struct GradDesc {
+    int nx, ny, nz, dim;
+    double ih[3], i2h[3];
+};
+
+__global__ void k_grid_grad(int slen, int ncols, const int *__restrict__ s2b, const int *__restrict__ mask,
+                            const double2 *__restrict__ fgv, GradDesc gd, double2 *__restrict__ fgs)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)slen * ncols) return;
+    const int s = (int)(t % slen), c = (int)(t / slen);
+    const long g = s2b[s];
+    const double2 *v = fgv + c;    // column c of [g][ncols]
+    const long ldv = ncols;
+    const int pos[3] = {(int)(g % gd.nx), (int)((g / gd.nx) % gd.ny), (int)(g / ((long)gd.nx * gd.ny))};
+    const int len[3] = {gd.nx, gd.ny, gd.nz};
+    const long str[3] = {1, gd.nx, (long)gd.nx * gd.ny};
+    double2 *o = fgs + ((long)c * slen + s) * 4;
+    o[0] = v[g * ldv];
+    for (int d = 0; d < 3; d++)
+        o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, g, pos[d], len[d], str[d], gd.ih[d], gd.i2h[d])
+                              : make_double2(0.0, 0.0);
+}
+
+// J rows of one source: block (voxel tile of 64, source q); thread
+// (voxel, link lane of 4) walks the links of q.  PMDF_mua = -d a,
+// PMDF_kap = -sum_d grad_d d grad_d a (jacobian.cc:57-79), divided by the
+// projection for log data, times the voxel size sz.
+__global__ void __launch_bounds__(256) k_jac_grid(int slen, int nq, int dim, const double2 *__restrict__ fgs,
+                                                  const int *__restrict__ lptr, const int *__restrict__ lm,
+                                                  const double2 *__restrict__ proj, int logscale, double sz,
+                                                  int nqm, double *__restrict__ J)
+{
+    const int q = blockIdx.y;
+    const int vs = threadIdx.x & 63, lk = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + vs;
+    if (s >= slen) return;
+    const double2 *D = fgs + ((long)q * slen + s) * 4;
+    const double2 d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
+    const size_t ld = 2 * (size_t)slen;
+    for (int k = lptr[q] + lk; k < lptr[q + 1]; k += 4) {
+        const double2 *A = fgs + ((long)(nq + lm[k]) * slen + s) * 4;
+        const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3];
+        double pm_r = -(d0.x * a0.x - d0.y * a0.y), pm_i = -(d0.x * a0.y + d0.y * a0.x);
+        double pk_r = 0.0, pk_i = 0.0;
+        pk_r -= d1.x * a1.x - d1.y * a1.y;
+        pk_i -= d1.x * a1.y + d1.y * a1.x;
+        pk_r -= d2.x * a2.x - d2.y * a2.y;
+        pk_i -= d2.x * a2.y + d2.y * a2.x;
+        if (dim > 2) {
+            pk_r -= d3.x * a3.x - d3.y * a3.y;
+            pk_i -= d3.x * a3.y + d3.y * a3.x;
+        }
+        if (logscale) {
+            const double2 p = proj[k];
+            const double den = p.x * p.x + p.y * p.y;
+            const double ar = (pm_r * p.x + pm_i * p.y) / den, ai = (pm_i * p.x - pm_r * p.y) / den;
+            const double br = (pk_r * p.x + pk_i * p.y) / den, bi = (pk_i * p.x - pk_r * p.y) / den;
+            pm_r = ar; pm_i = ai; pk_r = br; pk_i = bi;
+        }
+        J[(size_t)k * ld + s] = pm_r * sz;
+        J[(size_t)(k + nqm) * ld + s] = pm_i * sz;
+        J[(size_t)k * ld + slen + s] = pk_r * sz;
+        J[(size_t)(k + nqm) * ld + slen + s] = pk_i * sz;
+    }
+}
+
+void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                          const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
+                          double *d_J, hipStream_t st)
+{
+    const int ncols = nq + nm;
+    TMI_CHECK(r.slen > 0, TMI_ERR_INVALID_ARG, "the raster has no voxel inside the mesh");
+    r.fgv.alloc((size_t)r.glen * ncols * 2);
+    r.fgs.alloc((size_t)r.slen * ncols * 8);
+    // fields on the grid: direct columns 0..nq-1, adjoint nq..nq+nm-1
+    const long cnt_q = r.glen * nq, cnt_m = r.glen * nm;
+    const int NN = r.dim + 1;
+    if (NN == 4) {
+        hipLaunchKernelGGL(k_raster_map<4>, dim3((unsigned)((cnt_q + 255) / 256)), dim3(256), 0, st, r.glen, nq,
+                           r.bcols.p, r.bw.p, (const double2 *)d_dphi, ldd, (double2 *)r.fgv.p, (long)ncols);
+        hipLaunchKernelGGL(k_raster_map<4>, dim3((unsigned)((cnt_m + 255) / 256)), dim3(256), 0, st, r.glen, nm,
+                           r.bcols.p, r.bw.p, (const double2 *)d_aphi, lda, (double2 *)r.fgv.p + nq, (long)ncols);
+    } else {
+        hipLaunchKernelGGL(k_raster_map<3>, dim3((unsigned)((cnt_q + 255) / 256)), dim3(256), 0, st, r.glen, nq,
+                           r.bcols.p, r.bw.p, (const double2 *)d_dphi, ldd, (double2 *)r.fgv.p, (long)ncols);
+        hipLaunchKernelGGL(k_raster_map<3>, dim3((unsigned)((cnt_m + 255) / 256)), dim3(256), 0, st, r.glen, nm,
+                           r.bcols.p, r.bw.p, (const double2 *)d_aphi, lda, (double2 *)r.fgv.p + nq, (long)ncols);
+    }
+    TMI_LAUNCH_CHECK();
+    GradDesc gd{r.gdim[0], r.gdim[1], r.gdim[2], r.dim, {0, 0, 0}, {0, 0, 0}};
+    double sz = 1.0;
+    for (int d = 0; d < r.dim; d++) {
+        const double gsize = r.bbmax[d] - r.bbmin[d];
+        const double h = gsize / r.gdim[d];
+        gd.ih[d] = 1.0 / h;
+        gd.i2h[d] = 0.5 * gd.ih[d];
+        sz *= gsize / (r.gdim[d] - 1.0);
+    }
+    const long cnt_s = (long)r.slen * ncols;
+    hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.slen, ncols,
+                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p);
+    TMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_jac_grid, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq, r.dim,
+                       (const double2 *)r.fgs.p, d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
+    TMI_LAUNCH_CHECK();
+}
+
+}  // namespace tmi
