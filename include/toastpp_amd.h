@@ -31,6 +31,7 @@ extern "C" {
 #define TMI_ERR_IO             5
 #define TMI_ERR_STATE          6   /* call out of order (e.g. CalcFields before Reset) */
 #define TMI_ERR_UNSUPPORTED    7
+#define TMI_ERR_TIMEOUT        8   /* tmi_server_wait: request not finished yet */
 
 /* element type ids (libfe/element.h:30-49) */
 #define TMI_ELID_TET4  3
@@ -57,6 +58,7 @@ extern "C" {
 typedef struct tmi_mesh_s   *tmi_mesh;
 typedef struct tmi_solver_s *tmi_solver;
 typedef struct tmi_raster_s *tmi_raster;
+typedef struct tmi_server_s *tmi_server;
 
 /* per right-hand-side convergence record (IterativeSolverResult, matrix.h:34-37,
  * extended with refinement sweeps and a converged flag) */
@@ -170,6 +172,9 @@ int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
  * rows in LDS first.  Tuning parameter, no effect on results beyond
  * rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
+/* Device bytes of the workspace a CalcFields of nq columns uses (no
+ * reference counterpart; the serving loop's admission unit). */
+int tmi_solver_work_bytes(tmi_solver s, int nq, double *bytes);
 /* Allocate (fwdsolver.cc:193-261): CSR pattern of the system matrix on the GPU */
 int tmi_solver_allocate(tmi_solver s);
 /* AssembleSystemMatrix (fwdsolver.cc:338-357, :405-424): K = PFF(cmua) +
@@ -261,6 +266,29 @@ int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const 
  * d_J must hold 4 nqm slen doubles. */
 int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
                                int lda, const void *d_proj, int dscale, void *d_J);
+
+/* ---------------------------------------------------------------- serving */
+/* Serving loop (no reference counterpart; the reference's CalcFields runs
+ * one caller's sources to completion, fwdsolver.cc:653-810).  A server owns
+ * `slots` worker threads, each with its own HIP stream and workspace.
+ * Requests -- a prepared solver (Allocate + Reset done, matrix resident)
+ * and nq right-hand sides in device memory, node-major [nlen][ld] --
+ * are admitted in arrival order as soon as a slot is free and the
+ * workspaces of the running requests fit hbm_budget_bytes; admitted
+ * requests run concurrently.  One solver may serve several requests at
+ * once.  Results do not depend on what else runs (fixed-order reductions).
+ * The device buffers must stay valid until tmi_server_wait returns. */
+int tmi_server_create(int slots, double hbm_budget_bytes, tmi_server *out);
+int tmi_server_free(tmi_server s);          /* finishes queued requests first */
+int tmi_server_submit(tmi_server s, tmi_solver solver, int nq, const void *d_rhs, void *d_phi, int ld,
+                      long long *id);
+/* Wait for a request (timeout_s < 0: no limit; TMI_ERR_TIMEOUT if not done).
+ * stats: nq records; times: [queued, started, finished] seconds on the
+ * server clock; status: the request's own status (TMI_OK,
+ * TMI_ERR_NOT_CONVERGED or an error). */
+int tmi_server_wait(tmi_server s, long long id, double timeout_s, tmi_colstat *stats, double *times, int *status);
+int tmi_server_counts(tmi_server s, int *running, int *queued, double *hbm_bytes);
+double tmi_server_clock(tmi_server s);      /* seconds since creation */
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ TMI_ERR_DEVICE = 4
 TMI_ERR_IO = 5
 TMI_ERR_STATE = 6
 TMI_ERR_UNSUPPORTED = 7
+TMI_ERR_TIMEOUT = 8
 
 LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB = 0, 1, 2
 PRECON_NONE, PRECON_DIAG = 0, 1
@@ -28,7 +29,7 @@ ORDER_NATURAL, ORDER_MORTON = 0, 1
 ELID_TET4, ELID_TRI3 = 3, 15
 
 _ERRNAMES = {1: "invalid argument", 2: "not converged", 3: "out of memory", 4: "device error", 5: "I/O error",
-             6: "invalid state", 7: "unsupported"}
+             6: "invalid state", 7: "unsupported", 8: "timeout"}
 
 
 class TmiError(RuntimeError):
@@ -88,6 +89,7 @@ _SIGS = {
     "tmi_solver_set_order": [P, C.c_int],
     "tmi_solver_set_rows_per_wave": [P, C.c_int],
     "tmi_solver_set_spmm_variant": [P, C.c_int],
+    "tmi_solver_work_bytes": [P, C.c_int, DP],
     "tmi_solver_allocate": [P],
     "tmi_solver_assemble": [P, D, D, D, C.c_double],
     "tmi_solver_reset": [P, D, D, D, C.c_double],
@@ -110,6 +112,11 @@ _SIGS = {
     "tmi_raster_map_mesh_to_grid": [P, C.c_int, D, D],
     "tmi_jacobian_raster": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_raster_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
+    "tmi_server_create": [C.c_int, C.c_double, PP],
+    "tmi_server_free": [P],
+    "tmi_server_submit": [P, P, C.c_int, P, P, C.c_int, C.POINTER(C.c_longlong)],
+    "tmi_server_wait": [P, C.c_longlong, C.c_double, P, DP, IP],
+    "tmi_server_counts": [P, IP, IP, DP],
 }
 
 _lib = None
@@ -125,6 +132,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.tmi_last_error.restype = C.c_char_p
         L.tmi_last_error.argtypes = []
+        L.tmi_server_clock.restype = C.c_double
+        L.tmi_server_clock.argtypes = [C.c_void_p]
         for name, args in _SIGS.items():
             f = getattr(L, name)
             f.restype = C.c_int
@@ -134,7 +143,7 @@ def lib():
 
 
 def exported_symbols():
-    return ["tmi_last_error"] + list(_SIGS)
+    return ["tmi_last_error", "tmi_server_clock"] + list(_SIGS)
 
 
 def check(status, allow_notconv=False):

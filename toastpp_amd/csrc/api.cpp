@@ -10,10 +10,6 @@
 
 #include "handles.hpp"
 
-struct tmi_solver_s {
-    tmi::Solver s;
-};
-
 namespace tmi {
 
 static thread_local std::string g_last_error;
@@ -536,6 +532,16 @@ int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
     API_END
 }
 
+int tmi_solver_work_bytes(tmi_solver h, int nq, double *bytes)
+{
+    API_BEGIN
+    TMI_CHECK(bytes && nq > 0, TMI_ERR_INVALID_ARG, "invalid argument");
+    Solver &s = S(h);
+    TMI_CHECK(s.allocated, TMI_ERR_STATE, "work_bytes before Allocate");
+    *bytes = (double)s.work_bytes(nq);
+    API_END
+}
+
 int tmi_solver_set_inner_tol(tmi_solver h, double t)
 {
     API_BEGIN
@@ -725,8 +731,8 @@ int tmi_solver_last_timing(tmi_solver h, double *solve_ms, double *iters_per_col
 {
     API_BEGIN
     Solver &s = S(h);
-    if (solve_ms) *solve_ms = s.last_solve_ms;
-    if (iters_per_col) *iters_per_col = s.last_iters;
+    if (solve_ms) *solve_ms = s.ex.last_solve_ms;
+    if (iters_per_col) *iters_per_col = s.ex.last_iters;
     API_END
 }
 

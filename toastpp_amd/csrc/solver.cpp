@@ -21,10 +21,44 @@
 
 namespace tmi {
 
-Solver::~Solver()
+Solver::Exec::Exec(bool new_stream)
 {
+    if (new_stream) {
+        TMI_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own_stream = true;
+    }
+}
+
+Solver::Exec::~Exec()
+{
+    if (st) (void)hipStreamSynchronize(st);
     if (h_flags) (void)hipHostFree(h_flags);
     if (h_dbl) (void)hipHostFree(h_dbl);
+    if (own_stream) (void)hipStreamDestroy(st);
+}
+
+size_t Solver::Exec::bytes() const
+{
+    return w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
+           w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
+           w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+}
+
+Solver::~Solver() {}
+
+int Solver::tile_for(int nq) const
+{
+    if (mode == TMI_MODE_FP64_SINGLE) return 1;
+    int K = tile > 0 ? tile : 64;
+    while (K > 1 && K / 2 >= nq) K /= 2;   // smallest power of two >= nq, at most the tile
+    return K;
+}
+
+size_t Solver::work_bytes(int nq) const
+{
+    const size_t K = tile_for(nq), n = mesh->n;
+    const size_t s64 = vbytes(vk64()), sin = vbytes(mode == TMI_MODE_MIXED_BLOCK ? vk32() : vk64());
+    return n * K * (3 * s64 + 7 * sin) + 2 * 2048 * K * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * K;
 }
 
 void Solver::allocate()
@@ -38,10 +72,10 @@ void Solver::allocate()
     assembled = false;
 }
 
-BlockDesc Solver::block(int K) const
+BlockDesc Solver::block(const Exec &e, int K) const
 {
     const Mesh::NodeOrder &o = ord();
-    BlockDesc b{mesh->n, K, w.G, o.rowptr.p, o.colidx.p};
+    BlockDesc b{mesh->n, K, e.w.G, o.rowptr.p, o.colidx.p};
     b.rpw = rpw;
     if (use_lds && o.nb > 0) b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
     return b;
@@ -86,8 +120,9 @@ void Solver::reset_precon()
     precon_valid = true;
 }
 
-void Solver::ensure_work(int K)
+void Solver::ensure_work(Exec &e, int K)
 {
+    auto &w = e.w;
     int n = mesh->n;
     VKind vin = (mode == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
     if (w.K == K && w.n == n && w.vk_in == vin) return;
@@ -106,16 +141,16 @@ void Solver::ensure_work(int K)
     w.partial.alloc((size_t)2 * G * K);
     w.sc.alloc((size_t)S_NSLOT * K); w.osc.alloc((size_t)S_NSLOT * K);
     w.ic.alloc((size_t)I_NSLOT * K); w.oic.alloc((size_t)I_NSLOT * K);
-    if (!h_flags) TMI_HIP(hipHostMalloc((void **)&h_flags, sizeof(int) * I_NSLOT * 64));
-    if (!h_dbl) TMI_HIP(hipHostMalloc((void **)&h_dbl, sizeof(double) * S_NSLOT * 64));
+    if (!e.h_flags) TMI_HIP(hipHostMalloc((void **)&e.h_flags, sizeof(int) * I_NSLOT * 64));
+    if (!e.h_dbl) TMI_HIP(hipHostMalloc((void **)&e.h_dbl, sizeof(double) * S_NSLOT * 64));
 }
 
-int Solver::count_active(const int *d_flags)
+int Solver::count_active(Exec &e, const int *d_flags)
 {
-    TMI_HIP(hipMemcpyAsync(h_flags, d_flags, sizeof(int) * w.K, hipMemcpyDeviceToHost, st));
-    TMI_HIP(hipStreamSynchronize(st));
+    TMI_HIP(hipMemcpyAsync(e.h_flags, d_flags, sizeof(int) * e.w.K, hipMemcpyDeviceToHost, e.st));
+    TMI_HIP(hipStreamSynchronize(e.st));
     int c = 0;
-    for (int i = 0; i < w.K; i++) c += h_flags[i] != 0;
+    for (int i = 0; i < e.w.K; i++) c += e.h_flags[i] != 0;
     return c;
 }
 
@@ -131,10 +166,14 @@ static constexpr int CHECK_EVERY = 8;
 // restarting.  A column is done when its FP64 true residual meets the
 // tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
 // improving (no 4x drop over 4 true-residual evaluations).
-int Solver::solve_tile(int kc, tmi_colstat *stats)
+int Solver::solve_tile(Exec &e, int kc, tmi_colstat *stats)
 {
+    auto &w = e.w;
+    hipStream_t st = e.st;
+    int *h_flags = e.h_flags;
+    double *h_dbl = e.h_dbl;
     int K = w.K, n = mesh->n;
-    BlockDesc bd = block(K);
+    BlockDesc bd = block(e, K);
     bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
     const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     const void *idiag_in = nullptr;
@@ -194,14 +233,14 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
         since = 0;
     };
     for (;;) {
-        if (count_active(oact) == 0) break;
-        if (count_active(ic + I_ACTIVE * K) == 0) {
+        if (count_active(e, oact) == 0) break;
+        if (count_active(e, ic + I_ACTIVE * K) == 0) {
             // every column's recursive residual met the tolerance: FP64 modes
             // stop here (as the reference does); the mixed mode verifies in FP64
             if (!mixed) break;
             reliable_update();
             if (use_cg) launch_cg_d(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, st);
-            if (count_active(oact) == 0 || it >= maxit_total) break;
+            if (count_active(e, oact) == 0 || it >= maxit_total) break;
             continue;
         }
         if (it >= maxit_total) break;
@@ -247,7 +286,7 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
             stats[c].relres = rel;
             stats[c].converged = conv ? 1 : 0;
         }
-        last_iters += h_flags[I_ITERS * K + c];
+        e.last_iters += h_flags[I_ITERS * K + c];
     }
     if (verbose)
         fprintf(stderr, "[tmi] tile K=%d: %d block iterations, %d reliable updates, %d not converged\n", K, it, nupd,
@@ -255,53 +294,59 @@ int Solver::solve_tile(int kc, tmi_colstat *stats)
     return notconv;
 }
 
-int Solver::solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats)
+int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats)
 {
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
     TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
-    if (precon == TMI_PRECON_DIAG && !precon_valid) reset_precon();
-    int K;
-    if (mode == TMI_MODE_FP64_SINGLE) K = 1;
-    else {
-        K = tile > 0 ? tile : 64;
-        while (K > 1 && K / 2 >= nq) K /= 2;   // smallest power of two >= nq, at most the tile
+    if (precon == TMI_PRECON_DIAG) {
+        std::lock_guard<std::mutex> lk(prep_mu);
+        if (!precon_valid) {
+            reset_precon();
+            TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
+        }
     }
-    ensure_work(K);
+    if (!e.st) e.st = st;
+    const int K = tile_for(nq);
+    ensure_work(e, K);
+    auto &w = e.w;
+    hipStream_t es_ = e.st;
     size_t es = vbytes(vk64());
     hipEvent_t e0, e1;
     TMI_HIP(hipEventCreate(&e0));
     TMI_HIP(hipEventCreate(&e1));
-    TMI_HIP(hipEventRecord(e0, st));
-    last_iters = 0.0;
+    TMI_HIP(hipEventRecord(e0, es_));
+    e.last_iters = 0.0;
     int notconv = 0;
     for (int q0 = 0; q0 < nq; q0 += K) {
         int kc = std::min(K, nq - q0);
         // gather the tile: columns q0..q0+kc of [n][ld] -> [n][K] in the
         // solver's node order (columns kc..K-1 zero)
         const int *perm = ord().perm.p;
-        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, st);
-        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, st);
-        notconv += solve_tile(kc, stats ? stats + q0 : nullptr);
-        launch_scatter_tile((int)es, mesh->n, K, kc, perm, w.x64.p, d_x, ld, q0, st);
+        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, es_);
+        launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, es_);
+        notconv += solve_tile(e, kc, stats ? stats + q0 : nullptr);
+        launch_scatter_tile((int)es, mesh->n, K, kc, perm, w.x64.p, d_x, ld, q0, es_);
     }
-    TMI_HIP(hipEventRecord(e1, st));
+    TMI_HIP(hipEventRecord(e1, es_));
     TMI_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     TMI_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    last_solve_ms = ms;
-    last_iters = nq ? last_iters / nq : 0.0;
+    e.last_solve_ms = ms;
+    e.last_iters = nq ? e.last_iters / nq : 0.0;
     return notconv;
 }
 
 void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
 {
+    auto &w = ex.w;
+    hipStream_t st = ex.st ? ex.st : this->st;
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
     int K = w.K, n = mesh->n;
-    BlockDesc bd = block(K);
+    BlockDesc bd = block(ex, K);
     std::vector<int> ones(K, 1);
     DBuf<int> act;
     act.upload(ones.data(), K, st);

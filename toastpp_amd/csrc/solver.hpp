@@ -1,5 +1,7 @@
 // Forward solver: the GPU counterpart of TFwdSolver<T> (fwdsolver.h:77-508).
 #pragma once
+#include <mutex>
+
 #include "krylov.hpp"
 #include "mesh.hpp"
 
@@ -31,17 +33,34 @@ struct Solver {
     DBuf<float> pval32;    // A D^-1 (FP32, solver order)
     DBuf<double> coef;     // cmua | ckappa | c2a on the device
 
-    // workspace of one tile
-    struct Work {
-        int K = 0, n = 0, G = 0;
-        VKind vk_in = VK_C32;
-        DBuf<char> x64, b64, r64, xi, ri, rdi, p, v, s, t;
-        DBuf<double> partial, sc, osc;
-        DBuf<int> ic, oic;
-    } w;
-    int *h_flags = nullptr;        // pinned host mirror of per-column ints
-    double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
-    double last_solve_ms = 0.0, last_iters = 0.0;
+    // per-solve state: a stream, the workspace of one tile, pinned host
+    // mirrors of the per-column flags and the last solve's timing.  The
+    // solver's own context runs on the library stream; a serving loop gives
+    // each in-flight request its own context, so that solves of one or
+    // several solvers run concurrently (the assembled matrix is read-only
+    // while solving).
+    struct Exec {
+        hipStream_t st = nullptr;
+        bool own_stream = false;
+        struct Work {
+            int K = 0, n = 0, G = 0;
+            VKind vk_in = VK_C32;
+            DBuf<char> x64, b64, r64, xi, ri, rdi, p, v, s, t;
+            DBuf<double> partial, sc, osc;
+            DBuf<int> ic, oic;
+        } w;
+        int *h_flags = nullptr;        // pinned host mirror of per-column ints
+        double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
+        double last_solve_ms = 0.0, last_iters = 0.0;
+        Exec() = default;
+        explicit Exec(bool new_stream);
+        Exec(const Exec &) = delete;
+        Exec &operator=(const Exec &) = delete;
+        ~Exec();
+        size_t bytes() const;          // device bytes held by the workspace
+    };
+    Exec ex;                           // default context (library stream)
+    std::mutex prep_mu;                // lazy preconditioner setup
 
     ~Solver();
     VKind vk64() const { return cplx ? VK_C64 : VK_R64; }
@@ -53,17 +72,21 @@ struct Solver {
     void reset_precon();
     // Solve A X = B for nq columns.  d_b, d_x: [n][ld] node-major device
     // arrays (FP64, complex or real); d_x holds the initial guess.
-    int solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
+    int solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats) { return solve_on(ex, nq, d_b, d_x, ld, stats); }
+    int solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
+    // device bytes a solve of nq columns needs for its workspace
+    size_t work_bytes(int nq) const;
+    int tile_for(int nq) const;
     // Time `reps` launches of one kernel of the last tile (which: 0 = inner
     // SpMM, 1 = FP64 residual SpMM) with HIP events on the solver stream.
     // Returns the average ms and the algorithmic bytes of one launch.
     void time_kernel(int which, int reps, double *avg_ms, double *bytes);
 
    private:
-    BlockDesc block(int K) const;
-    void ensure_work(int K);
-    int solve_tile(int kc, tmi_colstat *stats);
-    int count_active(const int *d_flags);
+    BlockDesc block(const Exec &e, int K) const;
+    void ensure_work(Exec &e, int K);
+    int solve_tile(Exec &e, int kc, tmi_colstat *stats);
+    int count_active(Exec &e, const int *d_flags);
 };
 
 // projections (mesh_kernels.hip): proj[k] = sum_j phi[q_k][j] * mvec[m_k][j]

@@ -138,6 +138,12 @@ class FwdSolver:
         self.converged = st == L.TMI_OK
         return self.stats
 
+    def WorkBytes(self, nq):
+        """device bytes of the workspace a solve of nq columns uses"""
+        b = C.c_double()
+        L.call("tmi_solver_work_bytes", self._h, int(nq), C.byref(b))
+        return b.value
+
     def LastTiming(self):
         ms, it = C.c_double(), C.c_double()
         L.call("tmi_solver_last_timing", self._h, C.byref(ms), C.byref(it))
