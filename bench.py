@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--jac-workload", default="cyl3d_jac")
     ap.add_argument("--jac-reps", type=int, default=3)
     ap.add_argument("--raster", type=int, default=64, help="raster basis grid size per axis")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--pmc-pass", action="store_true",
                     help="short run for hardware-counter passes: one mixed solve, 5 timed launches of the inner "
                          "SpMM, then three 1 GiB device-to-device copies that calibrate FETCH_SIZE / WRITE_SIZE; "
@@ -194,6 +198,11 @@ def main():
     if not args.no_jacobian:
         jac = jacobian_leg(args)
 
+    batch = serve = None
+    if not args.no_serve:
+        batch = multi_mesh_leg(args)
+        serve = serving_leg(args)
+
     # CPU baseline: FP64 oracle, one column, one thread
     cpu = None
     if not args.no_cpu:
@@ -223,9 +232,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload, args.tile),
-                     "kernel": "k_spmm_dot<float2,64> (inner block SpMM, complex FP32, 64 columns)",
+                     "kernel": "k_spmm64<float2,0> (inner block SpMM + <W,Y> dots, complex FP32, 64 columns)",
                      "avg_launch_ms": round(spmm_ms, 4), "algorithmic_bytes": int(spmm_bytes)},
         "jacobian": jac,
+        "multi_mesh": batch,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -373,6 +384,135 @@ def jacobian_leg(args):
     out.update({"raster": "%d^3" % g, "raster_solution_voxels": int(slen), "raster_build_s": round(t_rb, 4),
                 "raster_jacobian_s": round(t_jr, 4), "raster_jacobian_GBps": round(rbytes / t_jr / 1e9, 1),
                 "raster_parity_rows": int(2 * nm), "raster_max_rel_diff_vs_oracle": float(wr)})
+    return out
+
+
+def _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, tol):
+    """one device mesh, its source block (complex and real) and a prepared
+    solver per frequency (BiCGSTAB complex, PCG real for CW)"""
+    mesh = T.Mesh(data=(nodes, el))
+    mesh.SetQM(Q, Q[:1])
+    B = mesh.Qvec(width=2.0).toarray()
+    dBc = L.DeviceArray.from_host(np.ascontiguousarray(B))
+    dBr = L.DeviceArray.from_host(np.ascontiguousarray(B.real))
+    solvers = {}
+    for freq in freqs:
+        cplx = freq != 0
+        f = T.FwdSolver(mesh, "BICGSTAB" if cplx else "CG", tol, complex=cplx)
+        f.SetPrecon("DIAG")
+        f.Allocate()
+        f.Reset(T.Solution(mua, mus, ref), freq * 2 * np.pi * 1e-6)
+        solvers[freq] = (f, dBc if cplx else dBr)
+    return mesh, solvers
+
+
+def multi_mesh_leg(args):
+    """Three ellipsoids (20k, 80k, 250k nodes, 64 sources each, 100 MHz):
+    one after another, then as three concurrent requests of the serving
+    loop (one stream each)."""
+    import ctypes as C
+
+    import toastpp_amd as T
+    from toastpp_amd import _lib as L
+    from toastpp_amd import meshgen as G
+    from toastpp_amd.serve import Server
+
+    meshes = []
+    for name, nodes, el, Q, mua, mus, ref in G.ellipsoid_batch():
+        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, [100.0], args.tol)
+        f, dB = solvers[100.0]
+        meshes.append((mesh, f, dB, name))
+    outs = [L.DeviceArray((dB.shape[0], 64), np.complex128) for _, _, dB, _ in meshes]
+    srv = Server(slots=3, hbm_budget_gib=200.0)
+
+    def run(concurrent):
+        ids = []
+        for dX in outs:
+            L.call("tmi_memcpy_h2d", dX._p, np.zeros(dX.shape, np.complex128).ctypes.data_as(C.c_void_p), dX.nbytes)
+        t0 = time.perf_counter()
+        for (mesh, f, dB, name), dX in zip(meshes, outs):
+            ids.append(srv.submit(f, 64, dB.ptr, dX.ptr, 64))
+            if not concurrent:
+                srv.wait(ids[-1])
+        if concurrent:
+            for r in ids:
+                srv.wait(r)
+        return time.perf_counter() - t0
+
+    run(True)   # warm
+    t_seq = run(False)
+    t_con = run(True)
+    srv.close()
+    cols = 3 * 64
+    log("multi-mesh 20k+80k+250k x 64 sources: one after another %.1f ms, concurrent %.1f ms (%.2fx), %.0f solves/s"
+        % (t_seq * 1e3, t_con * 1e3, t_seq / t_con, cols / t_con))
+    return {"meshes": [m[3] for m in meshes], "sources_each": 64, "sequential_s": round(t_seq, 4),
+            "concurrent_s": round(t_con, 4), "speedup": round(t_seq / t_con, 3),
+            "field_solves_per_s": round(cols / t_con, 2)}
+
+
+def serving_leg(args):
+    """Poisson arrivals (seeded) of CalcFields requests: meshes from 3k-node
+    disks to 250k-node ellipsoids, frequency in {0, 50, 100, 200} MHz, 8-64
+    sources, tol as the main leg; matrices resident, continuous batching in
+    the serving loop.  Latency = finish - arrival."""
+    import ctypes as C
+
+    import toastpp_amd as T
+    from toastpp_amd import _lib as L
+    from toastpp_amd import meshgen as G
+    from toastpp_amd.serve import Server, latency_stats
+
+    pool = G.serving_pool()
+    freqs = [0.0, 50.0, 100.0, 200.0]
+    prepared, keep = {}, []
+    t0 = time.perf_counter()
+    for mi, (name, nodes, el, Q, mua, mus, ref) in enumerate(pool):
+        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, args.tol)
+        keep.append(mesh)
+        for f in freqs:
+            prepared[(mi, f)] = solvers[f]
+    log("serving: %d meshes x %d frequencies prepared in %.1f s" % (len(pool), len(freqs), time.perf_counter() - t0))
+    rng = np.random.default_rng(4)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / args.serve_rate)
+        if t > args.serve_seconds:
+            break
+        arrivals.append((t, int(rng.integers(len(pool))), freqs[int(rng.integers(len(freqs)))],
+                         int(rng.integers(8, 65))))
+    srv = Server(slots=args.serve_slots, hbm_budget_gib=200.0)
+    reqs = []
+    start = time.perf_counter()
+    for (ta, mi, f, nq) in arrivals:
+        solver, dB = prepared[(mi, f)]
+        n = dB.shape[0]
+        dX = L.DeviceArray((n, 64), np.complex128 if solver.complex else np.float64)
+        L.call("tmi_memcpy_h2d", dX._p, np.zeros(dX.shape, dX.dtype).ctypes.data_as(C.c_void_p), dX.nbytes)
+        delay = ta - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        reqs.append((srv.submit(solver, nq, dB.ptr, dX.ptr, 64), dX, nq, mi, f))
+    times, worst, cols, notconv = [], 0.0, 0, 0
+    for rid, dX, nq, mi, f in reqs:
+        st, tm, status = srv.wait(rid)
+        times.append(tm)
+        worst = max(worst, max(x[2] for x in st))
+        notconv += sum(1 for x in st if not x[3])
+        cols += nq
+    makespan = max(tm[2] for tm in times) - min(tm[0] for tm in times)
+    srv.close()
+    lat = latency_stats(times)
+    svc = np.array([tm[2] - tm[1] for tm in times])
+    out = {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate, "slots": args.serve_slots,
+           "latency_p50_ms": round(lat["p50_ms"], 2), "latency_p99_ms": round(lat["p99_ms"], 2),
+           "latency_mean_ms": round(lat["mean_ms"], 2), "service_p50_ms": round(float(np.median(svc)) * 1e3, 2),
+           "throughput_requests_per_s": round(len(reqs) / makespan, 2),
+           "field_solves_per_s": round(cols / makespan, 1), "max_relres": worst, "not_converged": notconv,
+           "meshes": [p[0] for p in pool], "freqs_MHz": freqs}
+    log("serving: %d requests at %.0f/s: latency p50 %.1f ms p99 %.1f ms, %.1f req/s, %.0f solves/s, max relres %.1e"
+        % (len(reqs), args.serve_rate, lat["p50_ms"], lat["p99_ms"], out["throughput_requests_per_s"],
+           out["field_solves_per_s"], worst))
     return out
 
 

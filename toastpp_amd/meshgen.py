@@ -153,6 +153,39 @@ def workload(name):
     raise KeyError(name)
 
 
+def serving_pool():
+    """Meshes of the serving workload: 2D disks of 3k-20k nodes and 3D
+    cylinders / ellipsoids of 30k-250k nodes, each with 64 boundary sources
+    (name, nodes, elements, sources, mua, mus, ref)."""
+    out = []
+    for name, K in (("disk3k", 32), ("disk19k", 80)):
+        nodes, el = disk(25.0, K)
+        out.append((name, nodes, el, ring(64, 25.0)))
+    nodes, el = cylinder(25.0, 50.0, 20, 24)
+    out.append(("cyl31k", nodes, el, np.vstack([ring(32, 25.0, z) for z in (-12.5, 12.5)])))
+    for name, N in (("ellipsoid80k", 42), ("ellipsoid250k", 62)):
+        nodes, el = ellipsoid(80.0, 65.0, 90.0, N)
+        out.append((name, nodes, el, _ellipsoid_ring_points(80.0, 65.0, 90.0, [-45.0, -15.0, 15.0, 45.0], 16, 0.0)))
+    nodes, el = cylinder(25.0, 50.0, 30, 36)
+    out.append(("cyl103k", nodes, el, np.vstack([ring(32, 25.0, z) for z in (-12.5, 12.5)])))
+    res = []
+    for i, (name, nodes, el, Q) in enumerate(out):
+        mua, mus, ref = params(nodes, 10 + i)
+        res.append((name, nodes, el, Q, mua, mus, ref))
+    return res
+
+
+def ellipsoid_batch():
+    """Three ellipsoids (20k, 80k, 250k nodes) of the multi-mesh batch, 64 sources each."""
+    res = []
+    for name, N in (("ellipsoid20k", 26), ("ellipsoid80k", 42), ("ellipsoid250k", 62)):
+        nodes, el = ellipsoid(80.0, 65.0, 90.0, N)
+        Q = _ellipsoid_ring_points(80.0, 65.0, 90.0, [-45.0, -15.0, 15.0, 45.0], 16, 0.0)
+        mua, mus, ref = params(nodes, 2)
+        res.append((name, nodes, el, Q, mua, mus, ref))
+    return res
+
+
 def _ellipsoid_ring_points(a, b, c, zs, n, offset):
     pts = []
     for z in zs:
