@@ -129,6 +129,13 @@ int tmi_mesh_get_qm(tmi_mesh m, double *Q, double *M, int *lptr, int *lidx);
 int tmi_qvec_gaussian(tmi_mesh m, int npts, const double *centres, double width, int neumann,
                       const double *scale, double *out);
 
+/* Point source profiles: QVec_Point (source.cc:157-196), evaluated on the
+ * host: shape functions of the element holding each centre (nearest node
+ * pair if outside the mesh), on boundary nodes only if neumann != 0.
+ * out: npts x nlen (dense, real).  The reference's measurement vectors have
+ * no point profile (CalcMvec, toastmodule.cc:1189-1220, leaves them zero). */
+int tmi_qvec_point(tmi_mesh m, int npts, const double *centres, int neumann, double *out);
+
 /* Optical coefficients -> FEM coefficients: cmua = c mua / n,
  * ckappa = c / (3 n (mua + mus')), c2a = c / (2 n A_Keijzer(n)), c = 0.3 mm/ps
  * (toastmodule.cc:1308-1318, param.cc:24-32).  Host arrays of length nlen. */

@@ -55,6 +55,7 @@ def lib():
             "om_qm_get": (None, [P, D, D, I, I]), "om_qm_free": (None, [P]),
             "om_pull_to_boundary": (C.c_int, [P, D, D]),
             "om_bbox": (None, [P, D, D]),
+            "om_qvec_point": (None, [P, D, C.c_int, D]),
             "om_raster_elref": (None, [P, I, D, D, I]),
             "om_raster_bmap": (None, [P, I, D, D, I, I, D]),
             "om_image_gradient": (None, [C.c_int, I, D, D, I, D]),
@@ -226,6 +227,17 @@ def jacobian_nodal(mesh, dphi, aphi, lptr, lidx, proj, logscale=True):
     lib().om_jacobian_nodal(mesh.h, nq, nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
                             _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, J)
     return J
+
+
+def qvec_point(mesh, pts, neumann=True):
+    """QVec_Point for each row of pts -> (npts x n)."""
+    pts = np.atleast_2d(np.asarray(pts, np.float64))
+    out = np.zeros((pts.shape[0], mesh.n))
+    for i, p in enumerate(pts):
+        row = np.zeros(mesh.n)
+        lib().om_qvec_point(mesh.h, np.ascontiguousarray(p), 1 if neumann else 0, row)
+        out[i] = row
+    return out
 
 
 class Raster:

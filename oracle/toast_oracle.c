@@ -1159,3 +1159,38 @@ void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, co
     }
     free(cols); free(w); free(s2b); free(ca); free(cd); free(gd); free(ga);
 }
+
+/* QVec_Point (source.cc:157-196): the element holding p (first in
+ * ascending order that contains it, GContains with its padding), else the
+ * element of the nearest node pair; shape functions at p, boundary nodes
+ * only for Neumann sources.  out: n values. */
+void om_qvec_point(const om_mesh *m, const double *p, int neumann, double *out)
+{
+    int el = -1, dim = m->dim, nn = m->nn;
+    for (int e = 0; e < m->elen && el < 0; e++)
+        if (el_gcontains(m, e, p)) el = e;
+    if (el < 0) {
+        double dmin = 1e10, dmin2 = 1e10;
+        const double eps = 1e-8;
+        for (int e = 0; e < m->elen; e++)
+            for (int j = 0; j < nn; j++) {
+                double d = pdist(m->x + (size_t)m->el[e*nn+j]*dim, p, dim);
+                if (d < dmin+eps)
+                    for (int k = 0; k < nn; k++) {
+                        if (k == j) continue;
+                        double d2 = pdist(m->x + (size_t)m->el[e*nn+k]*dim, p, dim);
+                        if (d2 < dmin2) { dmin2 = d2; dmin = d; el = e; }
+                    }
+            }
+    }
+    for (int i = 0; i < m->n; i++) out[i] = 0.0;
+    if (el < 0) return;
+    double loc[3], fun[4];
+    el_local(m, el, p, loc);
+    if (dim == 3) { fun[0] = 1.0-loc[0]-loc[1]-loc[2]; fun[1] = loc[0]; fun[2] = loc[1]; fun[3] = loc[2]; }
+    else { fun[0] = 1.0-loc[0]-loc[1]; fun[1] = loc[0]; fun[2] = loc[1]; }
+    for (int i = 0; i < nn; i++) {
+        int nd = m->el[el*nn+i];
+        if (m->bnd[nd] || !neumann) out[nd] = fun[i];
+    }
+}

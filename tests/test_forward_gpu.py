@@ -136,3 +136,14 @@ def test_fields_vs_oracle_3d(mode):
         x, it, err = O.bicgstab(rp, ci, val, qv[i].astype(np.complex128), 1e-15, 20000, True)
         rel = np.linalg.norm(phi[i] - x) / np.linalg.norm(x)
         assert rel <= 1e-8, "column %d rel L2 %.3g" % (i, rel)
+
+
+def test_point_sources_match_oracle(pair):
+    case, om, gm = pair
+    gm.ReadQM(case["qm"])
+    Q = om.read_qm(case["qm"])[0]
+    for neumann in (True, False):
+        gq = gm.Qvec(type="Neumann" if neumann else "Isotropic", shape="Point").toarray().T.real
+        oq = O.qvec_point(om, Q, neumann=neumann)
+        assert np.array_equal(gq, oq)
+    assert gm.Mvec(shape="Point").nnz == 0   # the reference's CalcMvec has no point profile

@@ -230,3 +230,25 @@ def test_csr_is_per_mesh():
         pairs = sorted({(x, y) for e in el for x in e for y in e})
         rows = np.repeat(np.arange(b.n), np.diff(rp))
         assert list(zip(rows.tolist(), ci.tolist())) == pairs
+
+
+def test_qvec_point_closed_forms(disk, cyl):
+    rng = np.random.default_rng(9)
+    for m in (disk, cyl):
+        x = m.nodes()
+        el = m.elements()
+        # at a node: the unit vector of that node (isotropic source)
+        k = int(rng.integers(m.n))
+        q = O.qvec_point(m, x[k], neumann=False)[0]
+        assert q[k] == pytest.approx(1.0, abs=1e-12) and np.abs(np.delete(q, k)).max() <= 1e-12
+        # inside an element: partition of unity and linear reproduction
+        e = int(rng.integers(m.elen))
+        w = rng.random(m.nn); w /= w.sum()
+        p = w @ x[el[e]]
+        q = O.qvec_point(m, p, neumann=False)[0]
+        assert q.sum() == pytest.approx(1.0, abs=1e-12)
+        assert np.allclose(q @ x, p, atol=1e-12)
+        # Neumann sources keep boundary nodes only
+        qn = O.qvec_point(m, p, neumann=True)[0]
+        bnd = m.bnd().astype(bool)
+        assert np.all(qn[~bnd] == 0) and np.allclose(qn[bnd], q[bnd])

@@ -77,6 +77,7 @@ _SIGS = {
     "tmi_mesh_qm_dims": [P, IP, IP, IP],
     "tmi_mesh_get_qm": [P, P, P, P, P],
     "tmi_qvec_gaussian": [P, C.c_int, D, C.c_double, C.c_int, P, D],
+    "tmi_qvec_point": [P, C.c_int, D, C.c_int, D],
     "tmi_optical_to_coeffs": [C.c_int, D, D, D, D, D, D],
     "tmi_solver_create": [P, C.c_int, C.c_int, C.c_double, PP],
     "tmi_solver_free": [P],

@@ -389,6 +389,15 @@ static double a_keijzer(double n)   // param.cc:24-32
     return (2.0 / (1.0 - R0) - 1.0 + costh * costh * costh) / (1.0 - costh * costh);
 }
 
+int tmi_qvec_point(tmi_mesh h, int npts, const double *centres, int neumann, double *out)
+{
+    API_BEGIN
+    NEED(centres); NEED(out);
+    Mesh &m = M(h);
+    for (int i = 0; i < npts; i++) qvec_point(m, centres + (size_t)i * m.dim, neumann != 0, out + (size_t)i * m.n);
+    API_END
+}
+
 int tmi_optical_to_coeffs(int nlen, const double *mua, const double *mus, const double *ref, double *cmua,
                           double *ckappa, double *c2a)
 {

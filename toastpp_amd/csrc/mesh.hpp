@@ -70,6 +70,8 @@ void read_mesh_file(const char *path, int &dim, std::vector<double> &nodes, std:
 void read_qm_file(const char *path, int dim, QM &qm, bool &fixed_m);
 // Mesh::PullToBoundary restated on the host (mesh.cc:1052-1111)
 bool pull_to_boundary(const Mesh &m, const std::vector<double> &geo, const double *p, double *out);
+// QVec_Point (source.cc:157-196) on the host: out has n entries
+void qvec_point(const Mesh &m, const double *p, bool neumann, double *out);
 
 // GPU kernels launched from mesh_kernels.hip
 void launch_exclusive_scan(const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries

@@ -265,4 +265,73 @@ bool pull_to_boundary(const Mesh &m, const std::vector<double> &, const double *
     return found;
 }
 
+// QVec_Point (source.cc:157-196): shape functions of the element holding
+// p at p; with Neumann mode only on boundary nodes.  Elements are searched
+// in ascending order (the reference's Mesh::ElFind starts next to the last
+// element found; on a shared face both elements give the same values).  A
+// point outside the mesh takes the element of the nearest node pair, as the
+// reference does.
+void qvec_point(const Mesh &m, const double *p, bool neumann, double *out)
+{
+    const int dim = m.dim, nn = m.nn;
+    const double *X = m.h_nodes.data();
+    int el = -1;
+    double g[GEO3], P[4][3], loc[3];
+    auto geom = [&](int e) {
+        const int *nd = m.h_elist.data() + (size_t)e * nn;
+        for (int k = 0; k < nn; k++)
+            for (int d = 0; d < dim; d++) P[k][d] = X[(size_t)nd[k] * dim + d];
+        if (dim == 3) tet_geometry(P[0], P[1], P[2], P[3], g);
+        else tri_geometry(P[0], P[1], P[2], g);
+    };
+    for (int e = 0; e < m.elen && el < 0; e++) {
+        geom(e);
+        bool in = true;   // Element_Unstructured::GContains: bounding box (EPS 1e-8) then LContains
+        for (int d = 0; d < dim && in; d++) {
+            double lo = P[0][d], hi = P[0][d];
+            for (int k = 1; k < nn; k++) { lo = std::min(lo, P[k][d]); hi = std::max(hi, P[k][d]); }
+            if (p[d] < lo - 1e-8 || p[d] > hi + 1e-8) in = false;
+        }
+        if (!in) continue;
+        el_local(dim, g, p, loc);
+        const double EPS = 1e-8;
+        in = dim == 3 ? (loc[0] + EPS >= 0.0 && loc[1] + EPS >= 0.0 && loc[2] + EPS >= 0.0 &&
+                         loc[0] + loc[1] + loc[2] - EPS <= 1.0)
+                      : (loc[0] + EPS >= 0.0 && loc[1] + EPS >= 0.0 && loc[0] + loc[1] - EPS <= 1.0);
+        if (in) el = e;
+    }
+    if (el < 0) {
+        double dmin = 1e10, dmin2 = 1e10;
+        const double eps = 1e-8;
+        for (int e = 0; e < m.elen; e++) {
+            const int *nd = m.h_elist.data() + (size_t)e * nn;
+            for (int j = 0; j < nn; j++) {
+                double d = 0.0;
+                for (int k = 0; k < dim; k++) d += (X[(size_t)nd[j] * dim + k] - p[k]) * (X[(size_t)nd[j] * dim + k] - p[k]);
+                d = sqrt(d);
+                if (d < dmin + eps) {
+                    for (int kk = 0; kk < nn; kk++) {
+                        if (kk == j) continue;
+                        double d2 = 0.0;
+                        for (int k = 0; k < dim; k++)
+                            d2 += (X[(size_t)nd[kk] * dim + k] - p[k]) * (X[(size_t)nd[kk] * dim + k] - p[k]);
+                        d2 = sqrt(d2);
+                        if (d2 < dmin2) { dmin2 = d2; dmin = d; el = e; }
+                    }
+                }
+            }
+        }
+        TMI_CHECK(el >= 0, TMI_ERR_INVALID_ARG, "no element found for the point source");
+        geom(el);
+        el_local(dim, g, p, loc);
+    }
+    for (int i = 0; i < m.n; i++) out[i] = 0.0;
+    double fun[4];
+    if (dim == 3) { fun[0] = 1.0 - loc[0] - loc[1] - loc[2]; fun[1] = loc[0]; fun[2] = loc[1]; fun[3] = loc[2]; }
+    else { fun[0] = 1.0 - loc[0] - loc[1]; fun[1] = loc[0]; fun[2] = loc[1]; }
+    const int *nd = m.h_elist.data() + (size_t)el * nn;
+    for (int i = 0; i < nn; i++)
+        if (m.h_bnd[nd[i]] || !neumann) out[nd[i]] = fun[i];
+}
+
 }  // namespace tmi

@@ -170,6 +170,14 @@ class FwdSolver:
                L.DATA_LOG if scale == "log" else L.DATA_LIN, proj.view(np.float64))
         return proj
 
+    def ProjectAllReal(self, mvec, phi, scale="log"):
+        """ProjectAll_real without phase unwrapping (fwdsolver.cc:940-992):
+        [Re ..., Im ...] of the projections (log: ln amplitude, phase)."""
+        p = self.ProjectAll(mvec, phi, scale)
+        if not self.complex:
+            return np.asarray(p, np.float64)
+        return np.concatenate([p.real, p.imag])
+
     def ProjectAll(self, mvec, phi, scale="log"):
         """mvec: (nM x n) CSR/dense; phi: (nQ x n) -> nQM projections in link order."""
         from scipy import sparse

@@ -152,22 +152,36 @@ class Mesh:
                L.ptr(sc), out)
         return out
 
+    def _points(self, pts, neumann):
+        pts = np.ascontiguousarray(np.atleast_2d(pts), np.float64)
+        out = np.zeros((pts.shape[0], self.NodeCount()))
+        L.call("tmi_qvec_point", self.handle_ptr(), pts.shape[0], pts, 1 if neumann else 0, out)
+        return out
+
     def Qvec(self, type="Neumann", shape="Gaussian", width=1):
-        """Source vectors as a CSC (n x nQ) matrix (toastmodule.cc:1098-1170)."""
-        if shape.lower() != "gaussian":
-            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "only Gaussian source profiles are supported")
+        """Source vectors as a CSC (n x nQ) matrix (toastmodule.cc:1098-1170);
+        shape 'Gaussian' (source.cc:4-41) or 'Point' (source.cc:157-196)."""
         t = type.lower()
         if t not in ("neumann", "isotropic"):
             raise ValueError("invalid source type %r" % type)
         Q = self.QMPositions()[0]
-        q = self._profiles(Q, width, t == "neumann")
+        if shape.lower() == "gaussian":
+            q = self._profiles(Q, width, t == "neumann")
+        elif shape.lower() == "point":
+            q = self._points(Q, t == "neumann")
+        else:
+            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "source profile %r is not supported" % shape)
         return sparse.csc_matrix(sparse.csr_matrix(q.astype(np.complex128)).T)
 
     def Mvec(self, shape="Gaussian", width=1, ref=1):
-        """Measurement vectors scaled by c0/(2 n A(n)) (toastmodule.cc:1189-1259)."""
-        if shape.lower() != "gaussian":
-            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "only Gaussian measurement profiles are supported")
+        """Measurement vectors scaled by c0/(2 n A(n)) (toastmodule.cc:1189-1259).
+        As in the reference (CalcMvec has no point case), 'Point' gives zero
+        vectors."""
         n = self.NodeCount()
+        if shape.lower() == "point":
+            return sparse.csc_matrix((n, self.qm_dims()[1]), dtype=np.complex128)
+        if shape.lower() != "gaussian":
+            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "measurement profile %r is not supported" % shape)
         refv = np.broadcast_to(np.asarray(ref, np.float64), (n,)).copy()
         c2a = Solution(np.zeros(n), np.ones(n), refv).c2a
         M = self.QMPositions()[1]
