@@ -179,6 +179,12 @@ int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
  * rows in LDS first.  Tuning parameter, no effect on results beyond
  * rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
+/* Column compaction of the block solve (no reference counterpart): when at
+ * most half of a tile's columns are still iterating, their state moves to
+ * a tile of the smallest power-of-two width >= max(active, min_width) and
+ * the finished columns are written out.  min_width 0 turns it off; default
+ * 4.  Results are unchanged up to the rounding of narrower kernels. */
+int tmi_solver_set_compaction(tmi_solver s, int min_width);
 /* Device bytes of the workspace a CalcFields of nq columns uses (no
  * reference counterpart; the serving loop's admission unit). */
 int tmi_solver_work_bytes(tmi_solver s, int nq, double *bytes);

@@ -541,6 +541,16 @@ int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
     API_END
 }
 
+int tmi_solver_set_compaction(tmi_solver h, int min_width)
+{
+    API_BEGIN
+    TMI_CHECK(min_width == 0 || min_width == 1 || min_width == 2 || min_width == 4 || min_width == 8 ||
+                  min_width == 16 || min_width == 32,
+              TMI_ERR_INVALID_ARG, "compaction width must be 0 (off) or a power of two <= 32");
+    S(h).compact_min = min_width;
+    API_END
+}
+
 int tmi_solver_work_bytes(tmi_solver h, int nq, double *bytes)
 {
     API_BEGIN

@@ -135,5 +135,12 @@ void launch_gather_tile(int es, int n, int K, int kc, const int *perm, const voi
                         hipStream_t st);
 void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const void *src, void *dst, long ld, int q0,
                          hipStream_t st);
+// column compaction of a [rows][ks] array into [rows][kd]: column j of dst
+// is column sel[j] of src for j < nsel, zero beyond (elem_bytes 4, 8, 16)
+void launch_select_cols(int elem_bytes, long rows, int ks, int kd, int nsel, const int *sel, const void *src,
+                        void *dst, hipStream_t st);
+// scatter with a column map: dst[perm[r]][q0 + cmap[j]] = src[r][j] where cmap[j] >= 0
+void launch_scatter_map(int es, int n, int K, const int *perm, const int *cmap, const void *src, void *dst, long ld,
+                        int q0, hipStream_t st);
 
 }  // namespace tmi

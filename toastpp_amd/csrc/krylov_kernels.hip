@@ -920,6 +920,33 @@ __global__ void k_scatter_tile(int n, int K, int kc, const int *__restrict__ per
     if (c < kc) dst[(size_t)perm[r] * ld + q0 + c] = src[e];
 }
 
+// column compaction: dst[r][j] = src[r][sel[j]] for j < nsel, 0 otherwise
+template <class T>
+__global__ void k_select_cols(long rows, int ks, int kd, int nsel, const int *__restrict__ sel,
+                              const T *__restrict__ src, T *__restrict__ dst)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * kd) return;
+    const long r = e / kd;
+    const int j = (int)(e % kd);
+    T v{};
+    if (j < nsel) v = src[r * ks + sel[j]];
+    dst[e] = v;
+}
+
+// scatter tile columns with a column map: dst[perm[r]][q0 + cmap[j]] = src[r][j]
+// for every slot j with cmap[j] >= 0
+template <class T>
+__global__ void k_scatter_map(int n, int K, const int *__restrict__ perm, const int *__restrict__ cmap,
+                              const T *__restrict__ src, T *__restrict__ dst, long ld, int q0)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * K) return;
+    const int r = (int)(e / K), j = (int)(e % K);
+    const int c = cmap[j];
+    if (c >= 0) dst[(size_t)perm[r] * ld + q0 + c] = src[e];
+}
+
 // ================================================================ launchers
 static inline void check_K(int K)
 {
@@ -1216,6 +1243,44 @@ void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const vo
                            (double2 *)dst, ld, q0);
     else
         hipLaunchKernelGGL(k_scatter_tile<double>, g, dim3(256), 0, st, n, K, kc, perm, (const double *)src,
+                           (double *)dst, ld, q0);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_select_cols(int elem_bytes, long rows, int ks, int kd, int nsel, const int *sel, const void *src,
+                        void *dst, hipStream_t st)
+{
+    const long cnt = rows * kd;
+    const dim3 g((unsigned)((cnt + 255) / 256));
+    switch (elem_bytes) {
+    case 16:
+        hipLaunchKernelGGL(k_select_cols<double2>, g, dim3(256), 0, st, rows, ks, kd, nsel, sel, (const double2 *)src,
+                           (double2 *)dst);
+        break;
+    case 8:
+        hipLaunchKernelGGL(k_select_cols<double>, g, dim3(256), 0, st, rows, ks, kd, nsel, sel, (const double *)src,
+                           (double *)dst);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_select_cols<int>, g, dim3(256), 0, st, rows, ks, kd, nsel, sel, (const int *)src,
+                           (int *)dst);
+        break;
+    default:
+        TMI_CHECK(false, TMI_ERR_INVALID_ARG, "column selection of %d-byte entries", elem_bytes);
+    }
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_scatter_map(int es, int n, int K, const int *perm, const int *cmap, const void *src, void *dst, long ld,
+                        int q0, hipStream_t st)
+{
+    const size_t cnt = (size_t)n * K;
+    const dim3 g((unsigned)((cnt + 255) / 256));
+    if (es == 16)
+        hipLaunchKernelGGL(k_scatter_map<double2>, g, dim3(256), 0, st, n, K, perm, cmap, (const double2 *)src,
+                           (double2 *)dst, ld, q0);
+    else
+        hipLaunchKernelGGL(k_scatter_map<double>, g, dim3(256), 0, st, n, K, perm, cmap, (const double *)src,
                            (double *)dst, ld, q0);
     TMI_LAUNCH_CHECK();
 }

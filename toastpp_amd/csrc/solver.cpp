@@ -39,9 +39,19 @@ Solver::Exec::~Exec()
 
 size_t Solver::Exec::bytes() const
 {
-    return w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
-           w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
-           w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+    size_t b = 0;
+    for (const Work &w : wk)
+        b += w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
+             w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
+             w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+    return b;
+}
+
+static int log2i(int K)
+{
+    int l = 0;
+    while ((1 << l) < K) l++;
+    return l;
 }
 
 Solver::~Solver() {}
@@ -56,9 +66,10 @@ int Solver::tile_for(int nq) const
 
 size_t Solver::work_bytes(int nq) const
 {
-    const size_t K = tile_for(nq), n = mesh->n;
+    // the tile and, with compaction, its narrower copies (K/2 + K/4 + ... < K)
+    const size_t K = tile_for(nq), n = mesh->n, Kall = compact_min > 0 ? 2 * K : K;
     const size_t s64 = vbytes(vk64()), sin = vbytes(mode == TMI_MODE_MIXED_BLOCK ? vk32() : vk64());
-    return n * K * (3 * s64 + 7 * sin) + 2 * 2048 * K * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * K;
+    return n * Kall * (3 * s64 + 7 * sin) + 2 * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
 }
 
 void Solver::allocate()
@@ -72,10 +83,10 @@ void Solver::allocate()
     assembled = false;
 }
 
-BlockDesc Solver::block(const Exec &e, int K) const
+BlockDesc Solver::block(const Exec::Work &w, int K) const
 {
     const Mesh::NodeOrder &o = ord();
-    BlockDesc b{mesh->n, K, e.w.G, o.rowptr.p, o.colidx.p};
+    BlockDesc b{mesh->n, K, w.G, o.rowptr.p, o.colidx.p};
     b.rpw = rpw;
     if (use_lds && o.nb > 0) b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
     return b;
@@ -120,12 +131,15 @@ void Solver::reset_precon()
     precon_valid = true;
 }
 
-void Solver::ensure_work(Exec &e, int K)
+Solver::Exec::Work &Solver::ensure_work(Exec &e, int K)
 {
-    auto &w = e.w;
+    auto &w = e.wk[log2i(K)];
     int n = mesh->n;
     VKind vin = (mode == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
-    if (w.K == K && w.n == n && w.vk_in == vin) return;
+    if (!e.h_flags) TMI_HIP(hipHostMalloc((void **)&e.h_flags, sizeof(int) * I_NSLOT * 64 * 2));
+    if (!e.h_dbl) TMI_HIP(hipHostMalloc((void **)&e.h_dbl, sizeof(double) * S_NSLOT * 64));
+    if (!e.cols.p) e.cols.alloc(128);
+    if (w.K == K && w.n == n && w.vk_in == vin) return w;
     w.K = K;
     w.n = n;
     w.vk_in = vin;
@@ -141,16 +155,15 @@ void Solver::ensure_work(Exec &e, int K)
     w.partial.alloc((size_t)2 * G * K);
     w.sc.alloc((size_t)S_NSLOT * K); w.osc.alloc((size_t)S_NSLOT * K);
     w.ic.alloc((size_t)I_NSLOT * K); w.oic.alloc((size_t)I_NSLOT * K);
-    if (!e.h_flags) TMI_HIP(hipHostMalloc((void **)&e.h_flags, sizeof(int) * I_NSLOT * 64));
-    if (!e.h_dbl) TMI_HIP(hipHostMalloc((void **)&e.h_dbl, sizeof(double) * S_NSLOT * 64));
+    return w;
 }
 
-int Solver::count_active(Exec &e, const int *d_flags)
+int Solver::count_active(Exec &e, const int *d_flags, int K)
 {
-    TMI_HIP(hipMemcpyAsync(e.h_flags, d_flags, sizeof(int) * e.w.K, hipMemcpyDeviceToHost, e.st));
+    TMI_HIP(hipMemcpyAsync(e.h_flags, d_flags, sizeof(int) * K, hipMemcpyDeviceToHost, e.st));
     TMI_HIP(hipStreamSynchronize(e.st));
     int c = 0;
-    for (int i = 0; i < e.w.K; i++) c += e.h_flags[i] != 0;
+    for (int i = 0; i < K; i++) c += e.h_flags[i] != 0;
     return c;
 }
 
@@ -166,101 +179,193 @@ static constexpr int CHECK_EVERY = 8;
 // restarting.  A column is done when its FP64 true residual meets the
 // tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
 // improving (no 4x drop over 4 true-residual evaluations).
-int Solver::solve_tile(Exec &e, int kc, tmi_colstat *stats)
+int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0)
 {
-    auto &w = e.w;
     hipStream_t st = e.st;
-    int *h_flags = e.h_flags;
+    int *h_flags = e.h_flags, *h_oflags = e.h_flags + I_NSLOT * 64;
     double *h_dbl = e.h_dbl;
-    int K = w.K, n = mesh->n;
-    BlockDesc bd = block(e, K);
-    bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
+    const int n = mesh->n;
+    const bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
     const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     const void *idiag_in = nullptr;
-    bool use_cg = (lsolver == TMI_LSOLVER_CG);
+    const bool use_cg = (lsolver == TMI_LSOLVER_CG);
     // operator of the BiCGSTAB SpMMs: K D^-1 when diagonally preconditioned
     const void *op_in = val_in;
     if (precon == TMI_PRECON_DIAG) {
         idiag_in = mixed ? (const void *)idiag32.p : (const void *)idiag64.p;
         if (!use_cg) op_in = mixed ? (const void *)pval32.p : (const void *)pval64.p;
     }
-    int maxit_total = maxit > 0 ? maxit : n + 1;
+    const int maxit_total = maxit > 0 ? maxit : n + 1;
     // reliable updates only in the mixed mode: in FP64 the recursive
     // residual is the reference's own stopping test, and replacing it by the
     // computed true residual would inject eps*||K||*||x|| noise into the
     // far field, where the fields are ~1e-9 of their peak
-    int ru = mixed ? reliable_every : (1 << 30);
-    VKind vk = w.vk_in;
-    double *sc = w.sc.p, *osc = w.osc.p, *part = w.partial.p;
-    int *ic = w.ic.p, *oic = w.oic.p;
-    const int *oact = oic + I_ACTIVE * K;
-    double *scale = osc + S_SCALE * K;
+    const int ru = mixed ? reliable_every : (1 << 30);
+    const VKind vk = mixed ? vk32() : vk64();
+    const int s64 = (int)vbytes(vk64()), sin = (int)vbytes(vk);
+    const int *perm = ord().perm.p;
+
+    // current workspace and the caller's tile column of each of its slots
+    Exec::Work *wp = &ensure_work(e, K0);
+    std::vector<int> cmap(K0, -1);
+    for (int c = 0; c < kc; c++) cmap[c] = c;
+    int K = K0;
+    BlockDesc bd;
+    double *sc, *osc, *part, *scale;
+    int *ic, *oic;
+    const int *oact;
+    auto bind = [&]() {
+        K = wp->K;
+        bd = block(*wp, K);
+        sc = wp->sc.p;
+        osc = wp->osc.p;
+        part = wp->partial.p;
+        ic = wp->ic.p;
+        oic = wp->oic.p;
+        oact = oic + I_ACTIVE * K;
+        scale = osc + S_SCALE * K;
+    };
+    bind();
 
     std::vector<int> hic((size_t)I_NSLOT * K, 0);
     for (int c = 0; c < kc; c++) hic[I_ACTIVE * K + c] = 1;
     TMI_HIP(hipMemcpyAsync(oic, hic.data(), hic.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    TMI_HIP(hipMemsetAsync(osc, 0, w.osc.bytes(), st));
-    double tol2 = tol * tol;
+    TMI_HIP(hipMemsetAsync(osc, 0, wp->osc.bytes(), st));
+    const double tol2 = tol * tol;
     // the recursive residual aims a little below the target so that the
     // verifying FP64 residual usually passes at the first check
     std::vector<double> itol2(K, mixed ? 0.25 * tol2 : tol2);
 
-    launch_norm2(vk64(), bd, w.b64.p, oact, part, st);
+    launch_norm2(vk64(), bd, wp->b64.p, oact, part, st);
     launch_finalize(FIN_OUTER_BNORM, bd, part, osc, oic, 0, 0, 0, st);
-    launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
     // inner state
     launch_inner_reset(bd, osc, oic, sc, ic, st);
     TMI_HIP(hipMemcpyAsync(sc + S_TOL2 * K, itol2.data(), K * sizeof(double), hipMemcpyHostToDevice, st));
-    launch_to_inner(vk, bd, w.r64.p, w.ri.p, use_cg ? nullptr : w.rdi.p, w.xi.p, scale, oact, part, st);
+    launch_to_inner(vk, bd, wp->r64.p, wp->ri.p, use_cg ? nullptr : wp->rdi.p, wp->xi.p, scale, oact, part, st);
     if (use_cg) {
-        launch_cg_init(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, part, st);
+        launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st);
         launch_finalize(FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
     } else {
         launch_finalize(FIN_BICG_INIT, bd, part, sc, ic, 0, 0, 0, st);
     }
     static const bool verbose = getenv("TMI_VERBOSE") != nullptr;
-    int it = 0, nupd = 0, since = 0;
+    int it = 0, nupd = 0, since = 0, notconv = 0, ncompact = 0;
     auto reliable_update = [&]() {
-        launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
-        launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+        launch_from_inner(vk, bd, wp->x64.p, wp->xi.p, scale, oact, st);
+        launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
         launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
-        launch_reliable(vk, bd, w.r64.p, w.ri.p, w.xi.p, use_cg ? nullptr : w.rdi.p, idiag_in, scale, oact, part,
-                        st);
+        launch_reliable(vk, bd, wp->r64.p, wp->ri.p, wp->xi.p, use_cg ? nullptr : wp->rdi.p, idiag_in, scale, oact,
+                        part, st);
         launch_finalize(use_cg ? FIN_CG_RELIABLE : FIN_BICG_RELIABLE, bd, part, sc, ic, maxit_total, 0, 0, st, osc,
                         oic);
         nupd++;
         since = 0;
     };
+    // per-column records of the slots whose caller column is in `which`,
+    // from the device state; their solutions are written to d_x
+    auto finish = [&](const std::vector<int> &which) {
+        TMI_HIP(hipMemcpyAsync(h_flags, ic, sizeof(int) * I_NSLOT * K, hipMemcpyDeviceToHost, st));
+        TMI_HIP(hipMemcpyAsync(h_oflags, oic, sizeof(int) * I_NSLOT * K, hipMemcpyDeviceToHost, st));
+        TMI_HIP(hipMemcpyAsync(h_dbl, osc, sizeof(double) * S_NSLOT * K, hipMemcpyDeviceToHost, st));
+        TMI_HIP(hipMemcpyAsync(e.cols.p, which.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        launch_scatter_map(s64, n, K, perm, e.cols.p, wp->x64.p, d_x, ld, q0, st);
+        TMI_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < K; j++) {
+            const int c = which[j];
+            if (c < 0) continue;
+            const double b2 = h_dbl[S_BNORM2 * K + j], r2 = h_dbl[S_RES2 * K + j];
+            const bool conv = b2 == 0.0 || r2 <= tol2 * b2;
+            if (!conv) notconv++;
+            if (stats) {
+                stats[c].iters = h_flags[I_ITERS * K + j];
+                stats[c].sweeps = nupd;
+                stats[c].relres = b2 > 0.0 ? std::sqrt(r2 / b2) : 0.0;
+                stats[c].converged = conv ? 1 : 0;
+            }
+            e.last_iters += h_flags[I_ITERS * K + j];
+        }
+    };
+    // narrow the tile to the columns still running: finished columns are
+    // recorded and written out, the state of the others moves to the
+    // workspace of the smallest power-of-two width that holds them
+    auto compact = [&]() -> bool {
+        TMI_HIP(hipMemcpyAsync(h_oflags, oact, sizeof(int) * K, hipMemcpyDeviceToHost, st));
+        TMI_HIP(hipStreamSynchronize(st));
+        std::vector<int> done(K, -1), sel;
+        for (int j = 0; j < K; j++) {
+            if (cmap[j] < 0) continue;
+            if (h_oflags[j]) sel.push_back(j);
+            else done[j] = cmap[j];
+        }
+        int K2 = 1;
+        while (K2 < (int)sel.size()) K2 *= 2;
+        K2 = std::max(K2, compact_min);
+        if (K2 >= K) return false;
+        finish(done);
+        Exec::Work &w2 = ensure_work(e, K2);
+        const int ns = (int)sel.size();
+        TMI_HIP(hipMemcpyAsync(e.cols.p + 64, sel.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+        const int *d_sel = e.cols.p + 64;
+        auto mv = [&](int bytes, long rows, const void *src, void *dst) {
+            launch_select_cols(bytes, rows, K, K2, ns, d_sel, src, dst, st);
+        };
+        mv(s64, n, wp->x64.p, w2.x64.p);
+        mv(s64, n, wp->b64.p, w2.b64.p);
+        mv(s64, n, wp->r64.p, w2.r64.p);
+        mv(sin, n, wp->xi.p, w2.xi.p);
+        mv(sin, n, wp->ri.p, w2.ri.p);
+        mv(sin, n, wp->rdi.p, w2.rdi.p);
+        mv(sin, n, wp->p.p, w2.p.p);
+        mv(sin, n, wp->v.p, w2.v.p);
+        mv(sin, n, wp->s.p, w2.s.p);
+        mv(sin, n, wp->t.p, w2.t.p);
+        mv(8, S_NSLOT, sc, w2.sc.p);
+        mv(8, S_NSLOT, osc, w2.osc.p);
+        mv(4, I_NSLOT, ic, w2.ic.p);
+        mv(4, I_NSLOT, oic, w2.oic.p);
+        std::vector<int> cm2(K2, -1);
+        for (int j = 0; j < ns; j++) cm2[j] = cmap[sel[j]];
+        cmap.swap(cm2);
+        wp = &w2;
+        bind();
+        ncompact++;
+        return true;
+    };
     for (;;) {
-        if (count_active(e, oact) == 0) break;
-        if (count_active(e, ic + I_ACTIVE * K) == 0) {
+        const int nact = count_active(e, oact, K);
+        if (nact == 0) break;
+        if (compact_min > 0 && nact <= K / 2 && K / 2 >= compact_min) compact();
+        if (count_active(e, ic + I_ACTIVE * K, K) == 0) {
             // every column's recursive residual met the tolerance: FP64 modes
             // stop here (as the reference does); the mixed mode verifies in FP64
             if (!mixed) break;
             reliable_update();
-            if (use_cg) launch_cg_d(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, st);
-            if (count_active(e, oact) == 0 || it >= maxit_total) break;
+            if (use_cg) launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st);
+            if (count_active(e, oact, K) == 0 || it >= maxit_total) break;
             continue;
         }
         if (it >= maxit_total) break;
         for (int k = 0; k < CHECK_EVERY; k++, it++, since++) {
             if (use_cg) {
-                launch_spmm_dot(vk, bd, val_in, nullptr, w.p.p, w.v.p, w.p.p, false, ic + I_ACTIVE * K, part, st);
+                launch_spmm_dot(vk, bd, val_in, nullptr, wp->p.p, wp->v.p, wp->p.p, false, ic + I_ACTIVE * K, part, st);
                 launch_finalize(FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_cg_xr(vk, bd, idiag_in, w.xi.p, w.p.p, w.v.p, w.ri.p, sc, ic, part, st);
+                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st);
                 launch_finalize(FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
                 // PCG replaces the residual before the direction update
                 if (k == CHECK_EVERY - 1 && since + 1 >= ru) reliable_update();
-                launch_cg_d(vk, bd, idiag_in, w.p.p, w.ri.p, sc, ic, st);
+                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st);
             } else {
-                launch_bicg_p(vk, bd, w.ri.p, w.p.p, w.v.p, sc, ic, st);
-                launch_spmm_dot(vk, bd, op_in, nullptr, w.p.p, w.v.p, w.rdi.p, false, ic + I_ACTIVE * K, part, st);
+                launch_bicg_p(vk, bd, wp->ri.p, wp->p.p, wp->v.p, sc, ic, st);
+                launch_spmm_dot(vk, bd, op_in, nullptr, wp->p.p, wp->v.p, wp->rdi.p, false, ic + I_ACTIVE * K, part,
+                                st);
                 launch_finalize(FIN_BICG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_bicg_s(vk, bd, w.ri.p, w.v.p, w.s.p, sc, ic, st);
-                launch_spmm_dot(vk, bd, op_in, nullptr, w.s.p, w.t.p, w.s.p, true, ic + I_ACTIVE * K, part, st);
+                launch_bicg_s(vk, bd, wp->ri.p, wp->v.p, wp->s.p, sc, ic, st);
+                launch_spmm_dot(vk, bd, op_in, nullptr, wp->s.p, wp->t.p, wp->s.p, true, ic + I_ACTIVE * K, part, st);
                 launch_finalize(FIN_BICG_OMEGA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_bicg_xr(vk, bd, idiag_in, w.xi.p, w.p.p, w.s.p, w.t.p, w.ri.p, w.rdi.p, sc, ic, part, st);
+                launch_bicg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->s.p, wp->t.p, wp->ri.p, wp->rdi.p, sc, ic,
+                               part, st);
                 launch_finalize(FIN_BICG_RHO, bd, part, sc, ic, maxit_total, 0, 0, st);
             }
         }
@@ -268,29 +373,13 @@ int Solver::solve_tile(Exec &e, int kc, tmi_colstat *stats)
     }
     // fold the last inner correction of columns still marked active (maxit
     // exhausted) and record the final true residuals
-    launch_from_inner(vk, bd, w.x64.p, w.xi.p, scale, oact, st);
-    launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, oact, part, st);
+    launch_from_inner(vk, bd, wp->x64.p, wp->xi.p, scale, oact, st);
+    launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
-    TMI_HIP(hipMemcpyAsync(h_flags, ic, sizeof(int) * I_NSLOT * K, hipMemcpyDeviceToHost, st));
-    TMI_HIP(hipMemcpyAsync(h_dbl, osc, sizeof(double) * S_NSLOT * K, hipMemcpyDeviceToHost, st));
-    TMI_HIP(hipStreamSynchronize(st));
-    int notconv = 0;
-    for (int c = 0; c < kc; c++) {
-        double b2 = h_dbl[S_BNORM2 * K + c], r2 = h_dbl[S_RES2 * K + c];
-        double rel = b2 > 0.0 ? std::sqrt(r2 / b2) : 0.0;
-        bool conv = b2 == 0.0 || r2 <= tol2 * b2;
-        if (!conv) notconv++;
-        if (stats) {
-            stats[c].iters = h_flags[I_ITERS * K + c];
-            stats[c].sweeps = nupd;
-            stats[c].relres = rel;
-            stats[c].converged = conv ? 1 : 0;
-        }
-        e.last_iters += h_flags[I_ITERS * K + c];
-    }
+    finish(cmap);
     if (verbose)
-        fprintf(stderr, "[tmi] tile K=%d: %d block iterations, %d reliable updates, %d not converged\n", K, it, nupd,
-                notconv);
+        fprintf(stderr, "[tmi] tile K=%d: %d block iterations, %d reliable updates, %d compactions (final width %d), "
+                "%d not converged\n", K0, it, nupd, ncompact, K, notconv);
     return notconv;
 }
 
@@ -308,8 +397,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     }
     if (!e.st) e.st = st;
     const int K = tile_for(nq);
-    ensure_work(e, K);
-    auto &w = e.w;
+    auto &w = ensure_work(e, K);
     hipStream_t es_ = e.st;
     size_t es = vbytes(vk64());
     hipEvent_t e0, e1;
@@ -325,8 +413,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
         const int *perm = ord().perm.p;
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, es_);
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, es_);
-        notconv += solve_tile(e, kc, stats ? stats + q0 : nullptr);
-        launch_scatter_tile((int)es, mesh->n, K, kc, perm, w.x64.p, d_x, ld, q0, es_);
+        notconv += solve_tile(e, K, kc, stats ? stats + q0 : nullptr, d_x, ld, q0);
     }
     TMI_HIP(hipEventRecord(e1, es_));
     TMI_HIP(hipEventSynchronize(e1));
@@ -341,12 +428,12 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
 
 void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
 {
-    auto &w = ex.w;
+    auto &w = ex.wk[log2i(tile_for(64))];
     hipStream_t st = ex.st ? ex.st : this->st;
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
     int K = w.K, n = mesh->n;
-    BlockDesc bd = block(ex, K);
+    BlockDesc bd = block(w, K);
     std::vector<int> ones(K, 1);
     DBuf<int> act;
     act.upload(ones.data(), K, st);

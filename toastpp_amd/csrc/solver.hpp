@@ -21,6 +21,7 @@ struct Solver {
     int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
     bool use_lds = false;            // LDS-staged 64-column SpMM instead of per-nonzero gathers
+    int compact_min = 4;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
     double omega = 0.0;
     hipStream_t st = nullptr;
@@ -48,8 +49,11 @@ struct Solver {
             DBuf<char> x64, b64, r64, xi, ri, rdi, p, v, s, t;
             DBuf<double> partial, sc, osc;
             DBuf<int> ic, oic;
-        } w;
-        int *h_flags = nullptr;        // pinned host mirror of per-column ints
+        };
+        Work wk[7];                    // one workspace per tile width 1, 2, 4, ..., 64
+        Work &w = wk[6];               // the 64-column workspace
+        DBuf<int> cols;                // column maps of compaction (2 x 64)
+        int *h_flags = nullptr;        // pinned host mirror of per-column ints (inner, outer)
         double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
         double last_solve_ms = 0.0, last_iters = 0.0;
         Exec() = default;
@@ -83,10 +87,10 @@ struct Solver {
     void time_kernel(int which, int reps, double *avg_ms, double *bytes);
 
    private:
-    BlockDesc block(const Exec &e, int K) const;
-    void ensure_work(Exec &e, int K);
-    int solve_tile(Exec &e, int kc, tmi_colstat *stats);
-    int count_active(Exec &e, const int *d_flags);
+    BlockDesc block(const Exec::Work &w, int K) const;
+    Exec::Work &ensure_work(Exec &e, int K);
+    int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0);
+    int count_active(Exec &e, const int *d_flags, int K);
 };
 
 // projections (mesh_kernels.hip): proj[k] = sum_j phi[q_k][j] * mvec[m_k][j]

@@ -138,6 +138,10 @@ class FwdSolver:
         self.converged = st == L.TMI_OK
         return self.stats
 
+    def SetCompaction(self, min_width):
+        """narrowest tile of the column compaction (0: off, default 4)"""
+        L.call("tmi_solver_set_compaction", self._h, int(min_width))
+
     def WorkBytes(self, nq):
         """device bytes of the workspace a solve of nq columns uses"""
         b = C.c_double()
