@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--jac-reps", type=int, default=3)
     ap.add_argument("--raster", type=int, default=64, help="raster basis grid size per axis")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -121,6 +122,11 @@ def main():
 
     fm, asm_ms = make_solver("mixed", args.tol, args.tile)
     log("assembly + preconditioner (host call incl. upload) %.1f ms" % asm_ms)
+    tuned = None
+    if not args.no_autotune and not args.pmc_pass:
+        t0 = time.perf_counter()
+        tuned = fm.Autotune(nq, dB.ptr, dX.ptr, nq)
+        log("autotune (%.1f s): %s" % (time.perf_counter() - t0, tuned))
     if args.pmc_pass:
         fm.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
         fm.TimeKernel(0, reps=5)
@@ -223,7 +229,8 @@ def main():
         "dtype": "complex-fp32 Krylov + fp64 refinement (c64/c128)",
         "data": "synthetic (generated mesh, seed-drawn parameter maps)",
         "config": {"workload": "%s: %s" % (args.workload, desc), "nodes": n, "elements": int(len(el)),
-                   "sources": nq, "tile": args.tile, "tol": args.tol, "precon": "diag", "solver": "bicgstab"},
+                   "sources": nq, "tile": tuned["tile"] if tuned else args.tile, "tol": args.tol, "precon": "diag",
+                   "solver": "bicgstab", "autotuned": tuned},
         "max_rel_err_vs_fp64": float(err.max()),
         "iters_per_col": float(np.mean(iters)),
         "converged": bool(conv),

@@ -179,6 +179,13 @@ int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
  * rows in LDS first.  Tuning parameter, no effect on results beyond
  * rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
+/* Autotuner (no reference counterpart): times the 64-column SpMM for the
+ * row-schedule / variant candidates and whole CalcFields of the caller's nq
+ * right-hand sides (device, node-major [nlen][ld]) for tile widths 8..64,
+ * keeps the fastest settings on the solver and reports them (any output
+ * pointer may be NULL).  d_phi is scratch and ends holding a solution. */
+int tmi_solver_autotune(tmi_solver s, int nq, const void *d_rhs, void *d_phi, int ld, int *tile,
+                        int *rows_per_wave, int *spmm_variant, double *solve_ms);
 /* Column compaction of the block solve (no reference counterpart): when at
  * most half of a tile's columns are still iterating, their state moves to
  * a tile of the smallest power-of-two width >= max(active, min_width) and

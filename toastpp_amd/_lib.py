@@ -92,6 +92,7 @@ _SIGS = {
     "tmi_solver_set_spmm_variant": [P, C.c_int],
     "tmi_solver_work_bytes": [P, C.c_int, DP],
     "tmi_solver_set_compaction": [P, C.c_int],
+    "tmi_solver_autotune": [P, C.c_int, P, P, C.c_int, IP, IP, IP, DP],
     "tmi_solver_allocate": [P],
     "tmi_solver_assemble": [P, D, D, D, C.c_double],
     "tmi_solver_reset": [P, D, D, D, C.c_double],

@@ -541,6 +541,15 @@ int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
     API_END
 }
 
+int tmi_solver_autotune(tmi_solver h, int nq, const void *d_rhs, void *d_phi, int ld, int *tile, int *rows_per_wave,
+                        int *spmm_variant, double *solve_ms)
+{
+    API_BEGIN
+    TMI_CHECK(d_rhs && d_phi && nq > 0 && ld >= nq, TMI_ERR_INVALID_ARG, "invalid autotune request");
+    S(h).autotune(nq, d_rhs, d_phi, ld, tile, rows_per_wave, spmm_variant, solve_ms);
+    API_END
+}
+
 int tmi_solver_set_compaction(tmi_solver h, int min_width)
 {
     API_BEGIN

@@ -56,9 +56,10 @@ static int log2i(int K)
 
 Solver::~Solver() {}
 
-int Solver::tile_for(int nq) const
+int Solver::tile_for(int nq, int md) const
 {
-    if (mode == TMI_MODE_FP64_SINGLE) return 1;
+    if (md < 0) md = mode;
+    if (md == TMI_MODE_FP64_SINGLE) return 1;
     int K = tile > 0 ? tile : 64;
     while (K > 1 && K / 2 >= nq) K /= 2;   // smallest power of two >= nq, at most the tile
     return K;
@@ -131,11 +132,11 @@ void Solver::reset_precon()
     precon_valid = true;
 }
 
-Solver::Exec::Work &Solver::ensure_work(Exec &e, int K)
+Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md)
 {
     auto &w = e.wk[log2i(K)];
     int n = mesh->n;
-    VKind vin = (mode == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
+    VKind vin = (md == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
     if (!e.h_flags) TMI_HIP(hipHostMalloc((void **)&e.h_flags, sizeof(int) * I_NSLOT * 64 * 2));
     if (!e.h_dbl) TMI_HIP(hipHostMalloc((void **)&e.h_dbl, sizeof(double) * S_NSLOT * 64));
     if (!e.cols.p) e.cols.alloc(128);
@@ -179,13 +180,13 @@ static constexpr int CHECK_EVERY = 8;
 // restarting.  A column is done when its FP64 true residual meets the
 // tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
 // improving (no 4x drop over 4 true-residual evaluations).
-int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0)
+int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md)
 {
     hipStream_t st = e.st;
     int *h_flags = e.h_flags, *h_oflags = e.h_flags + I_NSLOT * 64;
     double *h_dbl = e.h_dbl;
     const int n = mesh->n;
-    const bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
+    const bool mixed = (md == TMI_MODE_MIXED_BLOCK);
     const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     const void *idiag_in = nullptr;
     const bool use_cg = (lsolver == TMI_LSOLVER_CG);
@@ -206,7 +207,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     const int *perm = ord().perm.p;
 
     // current workspace and the caller's tile column of each of its slots
-    Exec::Work *wp = &ensure_work(e, K0);
+    Exec::Work *wp = &ensure_work(e, K0, md);
     std::vector<int> cmap(K0, -1);
     for (int c = 0; c < kc; c++) cmap[c] = c;
     int K = K0;
@@ -304,7 +305,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         K2 = std::max(K2, compact_min);
         if (K2 >= K) return false;
         finish(done);
-        Exec::Work &w2 = ensure_work(e, K2);
+        Exec::Work &w2 = ensure_work(e, K2, md);
         const int ns = (int)sel.size();
         TMI_HIP(hipMemcpyAsync(e.cols.p + 64, sel.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
         const int *d_sel = e.cols.p + 64;
@@ -383,11 +384,12 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     return notconv;
 }
 
-int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats)
+int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md)
 {
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
     TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
+    if (md < 0) md = mode;
     if (precon == TMI_PRECON_DIAG) {
         std::lock_guard<std::mutex> lk(prep_mu);
         if (!precon_valid) {
@@ -396,8 +398,12 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
         }
     }
     if (!e.st) e.st = st;
-    const int K = tile_for(nq);
-    auto &w = ensure_work(e, K);
+    std::vector<tmi_colstat> own;
+    if (!stats) {
+        own.resize(nq);
+        stats = own.data();
+    }
+    const int K = tile_for(nq, md);
     hipStream_t es_ = e.st;
     size_t es = vbytes(vk64());
     hipEvent_t e0, e1;
@@ -410,10 +416,43 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
         int kc = std::min(K, nq - q0);
         // gather the tile: columns q0..q0+kc of [n][ld] -> [n][K] in the
         // solver's node order (columns kc..K-1 zero)
+        auto &w = ensure_work(e, K, md);
         const int *perm = ord().perm.p;
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, es_);
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, es_);
-        notconv += solve_tile(e, K, kc, stats ? stats + q0 : nullptr, d_x, ld, q0);
+        notconv += solve_tile(e, K, kc, stats + q0, d_x, ld, q0, md);
+    }
+    double iters = e.last_iters;
+    // a column the mixed-precision iteration could not bring to the
+    // tolerance (a BiCGSTAB breakdown in FP32) is solved again by the FP64
+    // block iteration, starting from where the mixed solve left it
+    if (notconv && md == TMI_MODE_MIXED_BLOCK && fp64_fallback) {
+        std::vector<int> bad;
+        for (int c = 0; c < nq; c++)
+            if (!stats[c].converged) bad.push_back(c);
+        const int nb = (int)bad.size();
+        DBuf<char> tb((size_t)mesh->n * nb * es), tx((size_t)mesh->n * nb * es);
+        for (int j = 0; j < nb; j++) {
+            TMI_HIP(hipMemcpy2DAsync(tb.p + j * es, nb * es, (const char *)d_b + bad[j] * es, (size_t)ld * es, es,
+                                     mesh->n, hipMemcpyDeviceToDevice, es_));
+            TMI_HIP(hipMemcpy2DAsync(tx.p + j * es, nb * es, (const char *)d_x + bad[j] * es, (size_t)ld * es, es,
+                                     mesh->n, hipMemcpyDeviceToDevice, es_));
+        }
+        std::vector<tmi_colstat> st2(nb);
+        solve_on(e, nb, tb.p, tx.p, nb, st2.data(), TMI_MODE_FP64_BLOCK);
+        iters += e.last_iters * nb;
+        notconv = 0;
+        for (int j = 0; j < nb; j++) {
+            TMI_HIP(hipMemcpy2DAsync((char *)d_x + bad[j] * es, (size_t)ld * es, tx.p + j * es, nb * es, es, mesh->n,
+                                     hipMemcpyDeviceToDevice, es_));
+            tmi_colstat &c = stats[bad[j]];
+            c.iters += st2[j].iters;
+            c.sweeps += 1;
+            c.relres = st2[j].relres;
+            c.converged = st2[j].converged;
+            if (!c.converged) notconv++;
+        }
+        TMI_HIP(hipStreamSynchronize(es_));
     }
     TMI_HIP(hipEventRecord(e1, es_));
     TMI_HIP(hipEventSynchronize(e1));
@@ -422,13 +461,13 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     e.last_solve_ms = ms;
-    e.last_iters = nq ? e.last_iters / nq : 0.0;
+    e.last_iters = nq ? iters / nq : 0.0;
     return notconv;
 }
 
 void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
 {
-    auto &w = ex.wk[log2i(tile_for(64))];
+    auto &w = ex.wk[log2i(tile_for(64, mode))];
     hipStream_t st = ex.st ? ex.st : this->st;
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
@@ -468,6 +507,53 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
 
     if (which == 1) b += (double)n * K * sv;   // the right-hand side block is read as well
     *bytes = b;
+}
+
+void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out, int *rpw_out, int *variant_out,
+                      double *ms_out)
+{
+    TMI_CHECK(assembled, TMI_ERR_STATE, "autotune before Reset");
+    const size_t es = vbytes(vk64());
+    auto zero_x = [&]() { TMI_HIP(hipMemset2DAsync(d_x, (size_t)ld * es, 0, (size_t)nq * es, mesh->n, st)); };
+    // 1. row schedule and SpMM variant of the 64-column kernel (mixed mode)
+    if (mode == TMI_MODE_MIXED_BLOCK && cplx) {
+        const int tile0 = tile;
+        tile = 64;
+        zero_x();
+        solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
+        struct Cand { bool lds; int rpw; };
+        const Cand cands[] = {{false, 0}, {false, 2}, {false, 4}, {false, 8}, {false, 16}, {true, 0}};
+        double best = 1e30;
+        Cand pick{use_lds, rpw};
+        for (const Cand &c : cands) {
+            if (c.lds && ord().nb == 0) continue;
+            use_lds = c.lds;
+            rpw = c.rpw;
+            double ms = 0, bytes = 0;
+            time_kernel(0, 10, &ms, &bytes);
+            if (ms < best) { best = ms; pick = c; }
+        }
+        use_lds = pick.lds;
+        rpw = pick.rpw;
+        tile = tile0;
+    }
+    // 2. tile width: whole solves of the caller's right-hand sides
+    if (mode != TMI_MODE_FP64_SINGLE) {
+        double best = 1e30;
+        int pick = tile;
+        for (int t = 64; t >= 8; t /= 2) {
+            if (t < 64 && t >= tile_for(nq) * 2) continue;   // a wider tile than the columns need
+            tile = t;
+            zero_x();
+            solve(nq, d_b, d_x, ld, nullptr);
+            if (ex.last_solve_ms < best) { best = ex.last_solve_ms; pick = t; }
+        }
+        tile = pick;
+        if (ms_out) *ms_out = best;
+    }
+    if (tile_out) *tile_out = tile;
+    if (rpw_out) *rpw_out = rpw;
+    if (variant_out) *variant_out = use_lds ? 0 : 1;
 }
 
 }  // namespace tmi

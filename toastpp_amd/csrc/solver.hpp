@@ -77,19 +77,26 @@ struct Solver {
     // Solve A X = B for nq columns.  d_b, d_x: [n][ld] node-major device
     // arrays (FP64, complex or real); d_x holds the initial guess.
     int solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats) { return solve_on(ex, nq, d_b, d_x, ld, stats); }
-    int solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
+    // md: precision / batching mode of this solve (-1: the solver's mode)
+    int solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md = -1);
     // device bytes a solve of nq columns needs for its workspace
     size_t work_bytes(int nq) const;
-    int tile_for(int nq) const;
+    int tile_for(int nq, int md = -1) const;
+    bool fp64_fallback = true;       // mixed mode: columns that do not converge are re-solved in FP64
     // Time `reps` launches of one kernel of the last tile (which: 0 = inner
     // SpMM, 1 = FP64 residual SpMM) with HIP events on the solver stream.
     // Returns the average ms and the algorithmic bytes of one launch.
     void time_kernel(int which, int reps, double *avg_ms, double *bytes);
+    // Pick the row schedule / SpMM variant (timed 64-column SpMM launches)
+    // and the tile width (timed solves of the caller's right-hand sides,
+    // d_x used as scratch and left holding the last solve) and keep them.
+    void autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out, int *rpw_out, int *variant_out,
+                  double *ms_out);
 
    private:
     BlockDesc block(const Exec::Work &w, int K) const;
-    Exec::Work &ensure_work(Exec &e, int K);
-    int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0);
+    Exec::Work &ensure_work(Exec &e, int K, int md);
+    int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md);
     int count_active(Exec &e, const int *d_flags, int K);
 };
 

@@ -138,6 +138,15 @@ class FwdSolver:
         self.converged = st == L.TMI_OK
         return self.stats
 
+    def Autotune(self, nq, d_rhs, d_phi, ld=None):
+        """Pick tile width, row schedule and SpMM variant for nq device
+        right-hand sides ([n][ld] node-major); returns them as a dict."""
+        t, r, v, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        L.call("tmi_solver_autotune", self._h, int(nq), C.c_void_p(d_rhs), C.c_void_p(d_phi), int(ld or nq),
+               C.byref(t), C.byref(r), C.byref(v), C.byref(ms))
+        return {"tile": t.value, "rows_per_wave": r.value, "spmm_variant": "lds" if v.value == 0 else "direct",
+                "solve_ms": ms.value}
+
     def SetCompaction(self, min_width):
         """narrowest tile of the column compaction (0: off, default 4)"""
         L.call("tmi_solver_set_compaction", self._h, int(min_width))
