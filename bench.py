@@ -78,9 +78,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--pmc-pass", action="store_true",
-                    help="short run for hardware-counter passes: one mixed solve, 5 timed launches of the inner "
-                         "SpMM, then three 1 GiB device-to-device copies that calibrate FETCH_SIZE / WRITE_SIZE; "
-                         "prints no JSON")
+                    help="short run for hardware-counter passes: the autotuner, one mixed solve, 5 timed "
+                         "launches of the inner SpMM, then three 1 GiB device-to-device copies that calibrate "
+                         "FETCH_SIZE / WRITE_SIZE; prints no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,7 +123,7 @@ def main():
     fm, asm_ms = make_solver("mixed", args.tol, args.tile)
     log("assembly + preconditioner (host call incl. upload) %.1f ms" % asm_ms)
     tuned = None
-    if not args.no_autotune and not args.pmc_pass:
+    if not args.no_autotune:
         t0 = time.perf_counter()
         tuned = fm.Autotune(nq, dB.ptr, dX.ptr, nq)
         log("autotune (%.1f s): %s" % (time.perf_counter() - t0, tuned))
