@@ -147,3 +147,23 @@ def test_point_sources_match_oracle(pair):
         oq = O.qvec_point(om, Q, neumann=neumann)
         assert np.array_equal(gq, oq)
     assert gm.Mvec(shape="Point").nnz == 0   # the reference's CalcMvec has no point profile
+
+
+def test_project_all_real_phase_unwrap():
+    """ProjectAll_real with SetPhaseUnwrap (fwdsolver.cc:951-989): the
+    log-amplitude half is unchanged, every phase ends up in (-2 pi, 0] and
+    differs from the wrapped phase by exactly 0 or -2 pi."""
+    case, fws, phi, proj = _run_case("2d_cplx_bicgstab", "mixed", tol=1e-10)
+    gm = fws.mesh
+    mv = sparse.csr_matrix(gm.Mvec(width=2.0, ref=case["n"]).T)
+    wrapped = fws.ProjectAllReal(mv, phi)
+    fws.SetPhaseUnwrap(True)
+    unwrapped = fws.ProjectAllReal(mv, phi)
+    nqm = len(proj)
+    assert np.array_equal(unwrapped[:nqm], wrapped[:nqm])
+    shift = unwrapped[nqm:] - wrapped[nqm:]
+    assert np.all((shift == 0.0) | (shift == -2.0 * np.pi))
+    assert np.all(unwrapped[nqm:] <= 0.0) and np.all(unwrapped[nqm:] > -2.0 * np.pi)
+    assert np.any(shift != 0.0) or np.all(wrapped[nqm:] <= 0.0)
+    assert np.array_equal(fws.ProjectAllReal(mv, phi, scale="lin"), np.concatenate(
+        [fws.ProjectAll(mv, phi, "lin").real, fws.ProjectAll(mv, phi, "lin").imag]))

@@ -536,8 +536,9 @@ int tmi_solver_set_rows_per_wave(tmi_solver h, int rows)
 int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
 {
     API_BEGIN
-    TMI_CHECK(variant == 0 || variant == 1, TMI_ERR_INVALID_ARG, "SpMM variant must be 0 (LDS-staged) or 1 (direct)");
-    S(h).use_lds = variant == 0;
+    TMI_CHECK(variant >= 0 && variant <= 2, TMI_ERR_INVALID_ARG,
+              "SpMM variant must be 0 (LDS, register staging), 1 (direct) or 2 (LDS DMA, double-buffered)");
+    S(h).spmm_variant = variant;
     API_END
 }
 

@@ -37,7 +37,11 @@ struct RowBlocks {
     const int *bu = nullptr;       // nb+1 bounds into ucols
     const int *ucols = nullptr;    // neighbour rows of each block
     const unsigned short *lcol = nullptr;   // per nonzero: position of its column in the block's ucols
+    bool dma = false;              // double-buffered LDS-DMA kernel instead of the register-staged one
 };
+// workgroups of the LDS-DMA SpMM: one per CU (each holds two 74 KiB buffers),
+// at most the reduction grid (a multiple of 8)
+constexpr int DMA_GRID = 256;
 
 struct BlockDesc {
     int n = 0;          // rows

@@ -146,7 +146,9 @@ const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
         prp[r + 1] = at;
     }
     order.perm.upload(perm.data(), n, st);
-    order.rowptr.upload(prp.data(), n + 1, st);
+    // 4 trailing copies of nnz: the LDS-DMA SpMM stages row pointers in 16-byte chunks
+    prp.insert(prp.end(), 4, prp[n]);
+    order.rowptr.upload(prp.data(), n + 5, st);
     order.colidx.upload(pci.data(), nnz, st);
     order.slot.upload(slot.data(), nnz, st);
     order.diagpos.upload(pdiag.data(), n, st);
@@ -158,6 +160,7 @@ const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
         order.brow.upload(brow.data(), brow.size(), st);
         order.bu.upload(bu.data(), bu.size(), st);
         order.ucols.upload(ucols.data(), ucols.size(), st);
+        lcol.insert(lcol.end(), 8, (unsigned short)0);   // 16-byte chunk slack, as for the row pointers
         order.lcol.upload(lcol.data(), lcol.size(), st);
     }
     TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope

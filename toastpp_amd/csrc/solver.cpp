@@ -79,7 +79,7 @@ void Solver::allocate()
     mesh->build_order(order_kind, st);
     val64.alloc(nval());
     sval64.alloc(nval());
-    sval32.alloc(nval());
+    sval32.alloc(nval() + 8);   // + 16-byte-chunk slack for the LDS-DMA SpMM's value staging
     allocated = true;
     assembled = false;
 }
@@ -89,7 +89,10 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
     const Mesh::NodeOrder &o = ord();
     BlockDesc b{mesh->n, K, w.G, o.rowptr.p, o.colidx.p};
     b.rpw = rpw;
-    if (use_lds && o.nb > 0) b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
+    if (spmm_variant != 1 && o.nb > 0) {
+        b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
+        b.lds.dma = spmm_variant == 2;
+    }
     return b;
 }
 
@@ -127,7 +130,7 @@ void Solver::reset_precon()
     // BiCGSTAB applies M^-1 on the right (pd = M^-1 p, v = K pd): keep K D^-1
     // as its own operator so the SpMM needs no per-nonzero diagonal gather
     pval64.alloc(nval());
-    pval32.alloc(nval());
+    pval32.alloc(nval() + 8);
     launch_scale_cols(cplx, mesh->nnz, ord().colidx.p, sval64.p, idiag64.p, pval64.p, pval32.p, st);
     precon_valid = true;
 }
@@ -521,19 +524,19 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
         tile = 64;
         zero_x();
         solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
-        struct Cand { bool lds; int rpw; };
-        const Cand cands[] = {{false, 0}, {false, 2}, {false, 4}, {false, 8}, {false, 16}, {true, 0}};
+        struct Cand { int variant; int rpw; };
+        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}};
         double best = 1e30;
-        Cand pick{use_lds, rpw};
+        Cand pick{spmm_variant, rpw};
         for (const Cand &c : cands) {
-            if (c.lds && ord().nb == 0) continue;
-            use_lds = c.lds;
+            if (c.variant != 1 && ord().nb == 0) continue;
+            spmm_variant = c.variant;
             rpw = c.rpw;
             double ms = 0, bytes = 0;
             time_kernel(0, 10, &ms, &bytes);
             if (ms < best) { best = ms; pick = c; }
         }
-        use_lds = pick.lds;
+        spmm_variant = pick.variant;
         rpw = pick.rpw;
         tile = tile0;
     }
@@ -553,7 +556,7 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
     }
     if (tile_out) *tile_out = tile;
     if (rpw_out) *rpw_out = rpw;
-    if (variant_out) *variant_out = use_lds ? 0 : 1;
+    if (variant_out) *variant_out = spmm_variant;
 }
 
 }  // namespace tmi

@@ -20,7 +20,7 @@ struct Solver {
     int reliable_every = 16;   // inner iterations between FP64 residual replacements (mixed mode)
     int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
-    bool use_lds = false;            // LDS-staged 64-column SpMM instead of per-nonzero gathers
+    int spmm_variant = 1;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA
     int compact_min = 4;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
     double omega = 0.0;

@@ -10,6 +10,9 @@ import numpy as np
 
 from . import _lib as L
 
+# SpMM variant names, in the order of their C-ABI codes
+SPMM_VARIANTS = ("lds", "direct", "dma")
+
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
@@ -42,6 +45,7 @@ class FwdSolver:
         L.call("tmi_solver_create", mesh.handle_ptr(), 1 if self.complex else 0, ls, float(tol), C.byref(h))
         self._h = h
         self.stats = None
+        self.unwrap_phase = False   # fwdsolver.cc:74
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -77,8 +81,9 @@ class FwdSolver:
         L.call("tmi_solver_set_rows_per_wave", self._h, int(rows))
 
     def SetSpmmVariant(self, variant):
-        """'direct' (default): per-nonzero gathers; 'lds': LDS-staged 64-column SpMM."""
-        L.call("tmi_solver_set_spmm_variant", self._h, {"lds": 0, "direct": 1}[variant])
+        """'direct' (default): per-nonzero gathers; 'lds': LDS-staged 64-column
+        SpMM; 'dma': LDS-staged with the LDS DMA, double-buffered."""
+        L.call("tmi_solver_set_spmm_variant", self._h, SPMM_VARIANTS.index(variant))
 
     def SetInnerTol(self, tol):
         L.call("tmi_solver_set_inner_tol", self._h, float(tol))
@@ -144,7 +149,7 @@ class FwdSolver:
         t, r, v, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         L.call("tmi_solver_autotune", self._h, int(nq), C.c_void_p(d_rhs), C.c_void_p(d_phi), int(ld or nq),
                C.byref(t), C.byref(r), C.byref(v), C.byref(ms))
-        return {"tile": t.value, "rows_per_wave": r.value, "spmm_variant": "lds" if v.value == 0 else "direct",
+        return {"tile": t.value, "rows_per_wave": r.value, "spmm_variant": SPMM_VARIANTS[v.value],
                 "solve_ms": ms.value}
 
     def SetCompaction(self, min_width):
@@ -183,13 +188,23 @@ class FwdSolver:
                L.DATA_LOG if scale == "log" else L.DATA_LIN, proj.view(np.float64))
         return proj
 
+    def SetPhaseUnwrap(self, unwrap):
+        """fwdsolver.h:142-143."""
+        self.unwrap_phase = bool(unwrap)
+
     def ProjectAllReal(self, mvec, phi, scale="log"):
-        """ProjectAll_real without phase unwrapping (fwdsolver.cc:940-992):
-        [Re ..., Im ...] of the projections (log: ln amplitude, phase)."""
+        """ProjectAll_real (fwdsolver.cc:940-992): [Re ..., Im ...] of the
+        projections (log: ln amplitude, phase).  With phase unwrapping on and
+        log scaling, positive phases are shifted by -2 pi (the branch the
+        reference compiles, fwdsolver.cc:985-989)."""
         p = self.ProjectAll(mvec, phi, scale)
         if not self.complex:
             return np.asarray(p, np.float64)
-        return np.concatenate([p.real, p.imag])
+        out = np.concatenate([p.real, p.imag])
+        if self.unwrap_phase and scale != "lin":
+            ph = out[len(p):]
+            ph[ph > 0.0] -= 2.0 * np.pi
+        return out
 
     def ProjectAll(self, mvec, phi, scale="log"):
         """mvec: (nM x n) CSR/dense; phi: (nQ x n) -> nQM projections in link order."""
