@@ -80,3 +80,27 @@ def test_variants_agree_on_cylinder():
     for variant in ("lds", "dma"):
         rel = np.linalg.norm(out[variant] - out["direct"], axis=0) / np.linalg.norm(out["direct"], axis=0)
         assert rel.max() <= 1e-8, "%s vs direct: worst column rel L2 %.3g" % (variant, rel.max())
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64_block"])
+@pytest.mark.parametrize("tile", [2, 4, 8, 16, 32])
+def test_narrow_tiles_vs_oracle_3d(golden_3d, mode, tile):
+    """Tiles narrower than 64 columns (the narrow-SpMM kernel, complex FP32
+    and FP64, and its FP64 residual form) against the FP64 oracle."""
+    case, ref = golden_3d
+    gm = TMI.Mesh(case["mesh"])
+    gm.ReadQM(case["qm"])
+    n = gm.NodeCount()
+    fws = TMI.FwdSolver(gm, "BICGSTAB", 1e-12, complex=True)
+    fws.SetMode(mode)
+    fws.SetPrecon("DIAG")
+    fws.SetTile(tile)
+    fws.Allocate()
+    fws.Reset(TMI.Solution(np.full(n, case["mua"]), np.full(n, case["mus"]), np.full(n, case["n"])),
+              case["freq"] * 2 * np.pi * 1e-6)
+    qv = sparse.csr_matrix(gm.Qvec(width=2.0).T)[:4]
+    phi = fws.CalcFields(qv)
+    assert all(s[3] for s in fws.stats)
+    for i in range(4):
+        rel = np.linalg.norm(phi[i] - ref[i]) / np.linalg.norm(ref[i])
+        assert rel <= 1e-8, "%s tile %d column %d rel L2 %.3g" % (mode, tile, i, rel)

@@ -17,6 +17,8 @@
 // run-to-run deterministic.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "krylov.hpp"
 
 namespace tmi {
@@ -307,6 +309,125 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *
         while (row < nr) flush();
     }
     block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
+}
+
+// ------------------------------------ block SpMM, K < 64 columns
+// 64 / K row streams per wave: stream s (lanes sK .. sK+K-1, lane = column)
+// works through a run of RS consecutive rows as one flat list of nonzeros.
+// Row pointers of the wave's S*RS rows come with one coalesced load; each
+// stream loads the column indices and values of its next K nonzeros with one
+// coalesced load and hands them to its lanes with lane shuffles, so the
+// memory accesses per nonzero step are the one gather of the neighbour row
+// (K values per stream, S rows per instruction).  Shuffles run with every
+// lane active; only the row flushes branch per stream.  Each row accumulates
+// acc + val * x in ascending order, like the 64-column kernel.
+template <class V>
+__device__ inline V shfl_v(V v, int src);
+template <>
+__device__ inline float2 shfl_v<float2>(float2 v, int src)
+{
+    return make_float2(__shfl(v.x, src), __shfl(v.y, src));
+}
+template <>
+__device__ inline double2 shfl_v<double2>(double2 v, int src)
+{
+    return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+}
+template <>
+__device__ inline float shfl_v<float>(float v, int src) { return __shfl(v, src); }
+template <>
+__device__ inline double shfl_v<double>(double v, int src) { return __shfl(v, src); }
+
+constexpr int NK_RSMAX = 4;   // rows per stream per run (at most)
+// K of the narrow kernel for a dispatch over all tile widths (the 1- and
+// 64-column cases never reach it).  Below NK_KMIN columns the per-lane
+// gather kernel is faster on MI355X (K = 4: 23 us against 46 us per launch
+// at 250k rows: with 16 streams per wave the per-stream row flushes diverge
+// at almost every nonzero).
+constexpr int NK_KMIN = 16;
+__host__ __device__ constexpr int nk_k(int K) { return K >= NK_KMIN && K <= 32 ? K : NK_KMIN; }
+__host__ __device__ constexpr int nk_rows(int K) { return (64 / K) * NK_RSMAX <= 63 ? NK_RSMAX : 63 / (64 / K); }
+
+template <class V, int K, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+__global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                                                 const V *__restrict__ val, const V *__restrict__ X,
+                                                 V *__restrict__ Y, const V *__restrict__ W, int want_yy,
+                                                 const int *__restrict__ active, double *__restrict__ partial)
+{
+    constexpr int S = 64 / K, RS = nk_rows(K), RW = S * RS, B = K < 16 ? K : 16;
+    static_assert(K >= 2 && K <= 32 && RW <= 63, "narrow SpMM: 2 <= K <= 32");
+    const int lane = threadIdx.x & 63, c = lane % K, s = lane / K, sb = s * K;
+    const bool act = active[c] != 0;
+    const int G = gridDim.x, nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
+    const int per_x = (n + NXCD - 1) / NXCD;
+    const int xs = min(n, xcd * per_x), xe = min(n, xs + per_x);
+    const int wpx = nbx * nw, wid = bx * nw + wave;
+    double d1 = 0.0, d2 = 0.0;
+    for (int rb = xs + wid * RW; rb < xe; rb += wpx * RW) {
+        const int nrun = min(RW, xe - rb);
+        const int rpv = rowptr[rb + min(lane, nrun)];   // row pointers of the run
+        // stream s: rows rb + s RS .. + nr (nr may be 0 at the region's end)
+        const int f = min(nrun, s * RS), nr = min(nrun, f + RS) - f;
+        int rpa[RS + 1];
+#pragma unroll
+        for (int i = 0; i <= RS; i++) rpa[i] = __shfl(rpv, min(f + i, f + nr));
+        const int kb = rpa[0], len = rpa[RS] - kb;   // rpa[i] for i > nr repeats the end
+        int row = 0;
+        auto rend_of = [&](int r) {
+            int e = rpa[RS];
+#pragma unroll
+            for (int i = RS; i >= 1; i--)
+                if (r + 1 == i) e = rpa[i];
+            return e;
+        };
+        int rend = rend_of(0);
+        V acc = vzero<V>();
+        auto flush = [&]() {
+            const size_t at = (size_t)(rb + f + row) * K + c;
+            if (act) {
+                if (MODE == 0) {
+                    Y[at] = acc;
+                    d1 += dotR(W[at], acc);
+                    if (want_yy) d2 += dotR(acc, acc);
+                } else {
+                    V r = vsub(W[at], acc);
+                    Y[at] = r;
+                    d1 += dotR(r, r);
+                }
+            }
+            acc = vzero<V>();
+            row++;
+            rend = rend_of(row);
+        };
+        for (int o = 0; __any(o < len); o += K) {
+            const int m = min(K, len - o);   // nonzeros of this segment (<= 0: stream done)
+            const int kk = max(0, min(kb + o + c, kb + len - 1));
+            const int cv = colidx[kk];
+            const V vv = val[kk];
+            for (int t0 = 0; t0 < K && __any(t0 < m); t0 += B) {
+                V xb[B], ab[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    const int src = sb + max(0, min(t0 + u, m - 1));
+                    const int j = __shfl(cv, src);
+                    ab[u] = shfl_v(vv, src);
+                    xb[u] = X[(size_t)j * K + c];
+                }
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    if (t0 + u < m) {
+                        const int k = kb + o + t0 + u;
+                        while (k >= rend && row < nr) flush();
+                        acc = vadd(acc, vmul(ab[u], xb[u]));
+                    }
+                }
+            }
+        }
+        while (row < nr) flush();
+    }
+    block_reduce2<K>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
 }
 
 // ------------------------------- LDS-staged block SpMM, 64 columns
@@ -765,6 +886,38 @@ __device__ __forceinline__ void ew_for(size_t tot, LD ld, USE use)
     }
 }
 
+// Nontemporal loads and stores of the streamed block vectors.  Measured on
+// MI355X for 3 reads + 1 write over 128 MB arrays (one element per thread,
+// working set beyond the last-level cache): 6.8 TB/s with both loads and
+// stores nontemporal, 6.1 TB/s with neither or only one of them.
+#ifndef TMI_EW_NT
+#define TMI_EW_NT 1
+#endif
+typedef double tmi_dv2 __attribute__((ext_vector_type(2)));
+template <class V>
+__device__ inline V ldnt(const V *p)
+{
+    if constexpr (!TMI_EW_NT) return *p;
+    else if constexpr (sizeof(V) == 8 && alignof(V) == 8 && !std::is_same<V, double>::value)
+        return __builtin_bit_cast(V, __builtin_nontemporal_load((const unsigned long long *)p));
+    else if constexpr (sizeof(V) == 16) {
+        const tmi_dv2 u = __builtin_nontemporal_load((const tmi_dv2 *)p);
+        return __builtin_bit_cast(V, u);
+    } else
+        return __builtin_nontemporal_load(p);
+}
+template <class V>
+__device__ inline void stnt(V *p, V v)
+{
+    if constexpr (!TMI_EW_NT) *p = v;
+    else if constexpr (sizeof(V) == 8 && alignof(V) == 8 && !std::is_same<V, double>::value)
+        __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)p);
+    else if constexpr (sizeof(V) == 16)
+        __builtin_nontemporal_store(__builtin_bit_cast(tmi_dv2, v), (tmi_dv2 *)p);
+    else
+        __builtin_nontemporal_store(v, p);
+}
+
 template <class V, int K>
 __global__ void __launch_bounds__(256) k_bicg_p(int n, const V *__restrict__ r, V *__restrict__ p,
                                                 const V *__restrict__ v, const double *__restrict__ sc,
@@ -772,7 +925,7 @@ __global__ void __launch_bounds__(256) k_bicg_p(int n, const V *__restrict__ r, 
 {
     EW1_PROLOGUE
     __shared__ ColPar<K, 2, 2> cp;
-    const V rv = r[tl], pv = p[tl], vv = v[tl];
+    const V rv = ldnt(r + tl), pv = ldnt(p + tl), vv = ldnt(v + tl);
     if (threadIdx.x < K) {
         cp.i[0][threadIdx.x] = ic[I_ACTIVE * K + threadIdx.x];
         cp.i[1][threadIdx.x] = ic[I_FIRST * K + threadIdx.x];
@@ -781,7 +934,7 @@ __global__ void __launch_bounds__(256) k_bicg_p(int n, const V *__restrict__ r, 
     }
     __syncthreads();
     if (t >= tot || !cp.i[0][c]) return;
-    p[t] = cp.i[1][c] ? rv : vadd(rv, vscale(vsub(pv, vscale(vv, cp.d[1][c])), cp.d[0][c]));
+    stnt(p + t, cp.i[1][c] ? rv : vadd(rv, vscale(vsub(pv, vscale(vv, cp.d[1][c])), cp.d[0][c])));
 }
 
 template <class V, int K>
@@ -791,14 +944,14 @@ __global__ void __launch_bounds__(256) k_bicg_s(int n, const V *__restrict__ r, 
 {
     EW1_PROLOGUE
     __shared__ ColPar<K, 1, 1> cp;
-    const V rv = r[tl], vv = v[tl];
+    const V rv = ldnt(r + tl), vv = ldnt(v + tl);
     if (threadIdx.x < K) {
         cp.i[0][threadIdx.x] = ic[I_ACTIVE * K + threadIdx.x];
         cp.d[0][threadIdx.x] = sc[S_ALPHA * K + threadIdx.x];
     }
     __syncthreads();
     if (t >= tot || !cp.i[0][c]) return;
-    s[t] = vsub(rv, vscale(vv, cp.d[0][c]));
+    stnt(s + t, vsub(rv, vscale(vv, cp.d[0][c])));
 }
 
 // x += D^-1 (alpha p + omega s); r = s - omega t; partials ||r||^2, <rd, r>
@@ -816,13 +969,16 @@ __global__ void __launch_bounds__(256) k_bicg_xr(int n, const V *__restrict__ id
         struct L { V s, p, x, t, rd, di; };
         ew_for(
             tot,
-            [&](size_t t) { return L{s[t], p[t], x[t], tv[t], rd[t], idiag ? idiag[t / K] : V{}}; },
+            [&](size_t t) {
+                return L{ldnt(s + t), ldnt(p + t), ldnt(x + t), ldnt(tv + t), ldnt(rd + t),
+                         idiag ? idiag[t / K] : V{}};
+            },
             [&](size_t t, const L &a) {
                 V upd = vadd(vscale(a.p, alpha), vscale(a.s, omega));
                 if (idiag) upd = vmul(a.di, upd);
-                x[t] = vadd(a.x, upd);
+                stnt(x + t, vadd(a.x, upd));
                 V rn = vsub(a.s, vscale(a.t, omega));
-                r[t] = rn;
+                stnt(r + t, rn);
                 d1 += dotR(rn, rn);
                 d2 += dotR(a.rd, rn);
             });
@@ -950,46 +1106,24 @@ __global__ void __launch_bounds__(256) k_from_inner(int n, V64 *__restrict__ x64
 {
     EW1_PROLOGUE
     __shared__ ColPar<K, 1, 1> cp;
-    const V64 xv = x64[tl];
-    const V xi = xin[tl];
+    const V64 xv = ldnt(x64 + tl);
+    const V xi = ldnt(xin + tl);
     if (threadIdx.x < K) {
         cp.i[0][threadIdx.x] = oflags[threadIdx.x];
         cp.d[0][threadIdx.x] = scale[threadIdx.x];
     }
     __syncthreads();
     if (t >= tot || !cp.i[0][c]) return;
-    x64[t] = vadd(xv, vscale(to64(xi), 1.0 / cp.d[0][c]));
+    stnt(x64 + t, vadd(xv, vscale(to64(xi), 1.0 / cp.d[0][c])));
 }
 
-// --------------------------------------------------------------- finalize
-// One workgroup per column: fixed-order sum of the per-block partials, then
-// the scalar recurrence of that column.
-__global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const double *__restrict__ partial,
-                                                  double *__restrict__ sc, int *__restrict__ ic, int maxit,
-                                                  double tol2, double inner_tol2, const double *__restrict__ osc,
-                                                  const int *__restrict__ oic)
+// The scalar recurrence of one column after a reduction (r0, r1: the two
+// summed partials); S and I hold the column's scalar and flag slots
+// (I[I_NSLOT]: the outer solve's active flag).
+__device__ inline void fin_column(int fk, double r0, double r1, double *S, int *I, int maxit, double tol2)
 {
-    const int c = blockIdx.x;
-    __shared__ double red[2][256];
-    for (int slot = 0; slot < 2; slot++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < G; b += 256) s += partial[((size_t)slot * K + c) * G + b];
-        red[slot][threadIdx.x] = s;
-    }
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + off];
-            red[1][threadIdx.x] += red[1][threadIdx.x + off];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    const double r0 = red[0][0], r1 = red[1][0];
-    double *S = sc;
-    int *I = ic;
-#define SC(slot) S[(slot) * K + c]
-#define IC(slot) I[(slot) * K + c]
+#define SC(slot) S[slot]
+#define IC(slot) I[slot]
     switch (fk) {
     case FIN_OUTER_BNORM:
         SC(S_BNORM2) = r0;
@@ -999,7 +1133,7 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
         if (r0 == 0.0) IC(I_ACTIVE) = 0;   // zero right-hand side: the field stays at the initial guess
         break;
     case FIN_OUTER_RES: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         SC(S_RES2) = r0;
         IC(I_ITERS) += 1;   // true-residual evaluations
         if (r0 <= tol2 * SC(S_BNORM2) || !isfinite(r0)) { IC(I_ACTIVE) = 0; break; }
@@ -1010,7 +1144,7 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
     }
     case FIN_BICG_RELIABLE:
     case FIN_CG_RELIABLE: {
-        const int oact = oic[I_ACTIVE * K + c];
+        const int oact = I[I_NSLOT];
         if (!oact) { IC(I_ACTIVE) = 0; break; }
         SC(S_RES2) = r0;
         if (fk == FIN_BICG_RELIABLE) {
@@ -1035,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
     }
     case FIN_BICG_INIT:
     case FIN_CG_INIT:
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         SC(S_BNORM2) = 1.0;   // the inner system is scaled by 1/||b||: absolute criterion
         SC(S_RES2) = (fk == FIN_BICG_INIT) ? r0 : r1;
         SC(S_RHO) = (fk == FIN_BICG_INIT) ? r1 : r0;
@@ -1044,21 +1178,21 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
         if (SC(S_RES2) <= SC(S_TOL2)) IC(I_ACTIVE) = 0;
         break;
     case FIN_BICG_ALPHA: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         double alpha = SC(S_RHO) / r0;
         if (r0 == 0.0 || !isfinite(alpha)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; alpha = 0.0; }
         SC(S_ALPHA) = alpha;
         break;
     }
     case FIN_BICG_OMEGA: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         double om = (r1 == 0.0) ? 0.0 : r0 / r1;
         if (!isfinite(om)) om = 0.0;
         SC(S_OMEGA) = om;
         break;
     }
     case FIN_BICG_RHO: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         IC(I_ITERS) += 1;
         IC(I_FIRST) = 0;
         SC(S_RES2) = r0;
@@ -1073,14 +1207,14 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
         break;
     }
     case FIN_CG_ALPHA: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         double alpha = SC(S_RHO) / r0;
         if (alpha == 0.0 || !isfinite(alpha)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; alpha = 0.0; }
         SC(S_ALPHA) = alpha;
         break;
     }
     case FIN_CG_BETA: {
-        if (!IC(I_ACTIVE)) return;
+        if (!IC(I_ACTIVE)) break;
         IC(I_ITERS) += 1;
         IC(I_FIRST) = 0;
         const double dold = SC(S_RHO), dnew = r0;
@@ -1098,6 +1232,43 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
 #undef SC
 #undef IC
 }
+
+// --------------------------------------------------------------- finalize
+// One workgroup per column: fixed-order sum of the per-block partials, then
+// the scalar recurrence of that column.
+__global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const double *__restrict__ partial,
+                                                  double *__restrict__ sc, int *__restrict__ ic, int maxit,
+                                                  double tol2, double inner_tol2, const double *__restrict__ osc,
+                                                  const int *__restrict__ oic)
+{
+    const int c = blockIdx.x;
+    __shared__ double red[2][256];
+    __shared__ double sS[S_NSLOT];
+    __shared__ int sI[I_NSLOT + 1];
+    // the column's scalars and flags come in with one parallel load, beside
+    // the partial sums, instead of one dependent load per use below
+    if ((int)threadIdx.x < S_NSLOT) sS[threadIdx.x] = sc[threadIdx.x * K + c];
+    else if ((int)threadIdx.x >= 64 && (int)threadIdx.x < 64 + I_NSLOT) sI[threadIdx.x - 64] = ic[(threadIdx.x - 64) * K + c];
+    else if ((int)threadIdx.x == 64 + I_NSLOT && oic) sI[I_NSLOT] = oic[I_ACTIVE * K + c];
+    for (int slot = 0; slot < 2; slot++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < G; b += 256) s += partial[((size_t)slot * K + c) * G + b];
+        red[slot][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) fin_column(fk, red[0][0], red[1][0], sS, sI, maxit, tol2);
+    __syncthreads();
+    if ((int)threadIdx.x < S_NSLOT) sc[threadIdx.x * K + c] = sS[threadIdx.x];
+    else if ((int)threadIdx.x >= 64 && (int)threadIdx.x < 64 + I_NSLOT) ic[(threadIdx.x - 64) * K + c] = sI[threadIdx.x - 64];
+}
+
 
 // ------------------------------------------------------------- utilities
 template <class V>
@@ -1291,6 +1462,9 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
     else if (idiag)                                                                                                \
         hipLaunchKernelGGL((k_spmm_dot<V, KK, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
                            (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial); \
+    else if (KK >= NK_KMIN && KK <= 32)                                                                       \
+        hipLaunchKernelGGL((k_spmm_nk<V, nk_k(KK), 0>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
+                           (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial);               \
     else                                                                                                      \
         hipLaunchKernelGGL((k_spmm_dot<V, KK, false>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, \
                            (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial);
@@ -1309,6 +1483,9 @@ void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, 
     if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                           \
         hipLaunchKernelGGL((k_spmm64<V, 1>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                            (V *)R, (const V *)B, 0, iflags, partial, b.rpw);                                 \
+    else if (KK >= NK_KMIN && KK <= 32)                                                                       \
+        hipLaunchKernelGGL((k_spmm_nk<V, nk_k(KK), 1>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
+                           (const V *)X, (V *)R, (const V *)B, 0, iflags, partial);                          \
     else                                                                                                      \
     hipLaunchKernelGGL((k_resid<V, KK>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                        (const V *)B, (V *)R, iflags, partial);
