@@ -174,12 +174,14 @@ int tmi_solver_set_order(tmi_solver s, int order);
  * every wave one contiguous share instead.  Tuning parameter of the row
  * schedule, no effect on results. */
 int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
-/* 64-column complex-FP32 SpMM variant: 1 (default) gathers neighbour rows
- * from global memory per nonzero; 0 stages each block of rows' neighbour
+/* 64-column complex-FP32 SpMM variant: 1 gathers neighbour rows from
+ * global memory per nonzero; 0 stages each block of rows' neighbour
  * rows in LDS first (through registers); 2 stages them with the LDS DMA
  * into two alternating buffers, the next block's copy overlapping the
- * current block's compute.  Tuning parameter, no effect on results beyond
- * rounding. */
+ * current block's compute; 3 (default) walks blocks of 8 consecutive rows
+ * over the union of their columns and gathers each shared neighbour row
+ * once.
+ * Tuning parameter, no effect on results beyond rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
 /* Autotuner (no reference counterpart): times the 64-column SpMM for the
  * row-schedule / variant candidates and whole CalcFields of the caller's nq

@@ -1,7 +1,7 @@
 """The 64-column SpMM variants (direct gathers, LDS staging through
-registers, LDS DMA with double buffering) inside the mixed-precision block
-solve: each against the FP64 oracle on the 3D golden case, and against each
-other on a mesh large enough that every workgroup of the DMA kernel walks
+registers, LDS DMA with double buffering, row blocks with shared
+gathers) inside the mixed-precision block solve: each against the FP64
+oracle on the 3D golden case, and against each other on a mesh large enough that every workgroup of the DMA kernel walks
 several row blocks and owns more than one reduction slot.  Through the C-ABI
 on the GPU."""
 import numpy as np
@@ -17,7 +17,7 @@ TMI = pytest.importorskip("toastpp_amd")
 from toastpp_amd import _lib as L  # noqa: E402
 from toastpp_amd import meshgen as G  # noqa: E402
 
-VARIANTS = ["lds", "direct", "dma"]
+VARIANTS = ["lds", "direct", "dma", "rowblock"]
 
 
 @pytest.fixture(scope="module")
@@ -77,7 +77,7 @@ def test_variants_agree_on_cylinder():
         stats = f.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
         assert all(s[3] for s in stats), variant
         out[variant] = dX.to_host()
-    for variant in ("lds", "dma"):
+    for variant in ("lds", "dma", "rowblock"):
         rel = np.linalg.norm(out[variant] - out["direct"], axis=0) / np.linalg.norm(out["direct"], axis=0)
         assert rel.max() <= 1e-8, "%s vs direct: worst column rel L2 %.3g" % (variant, rel.max())
 

@@ -536,8 +536,9 @@ int tmi_solver_set_rows_per_wave(tmi_solver h, int rows)
 int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
 {
     API_BEGIN
-    TMI_CHECK(variant >= 0 && variant <= 2, TMI_ERR_INVALID_ARG,
-              "SpMM variant must be 0 (LDS, register staging), 1 (direct) or 2 (LDS DMA, double-buffered)");
+    TMI_CHECK(variant >= 0 && variant <= 3, TMI_ERR_INVALID_ARG,
+              "SpMM variant must be 0 (LDS, register staging), 1 (direct), 2 (LDS DMA, double-buffered) or 3 "
+              "(row blocks with shared gathers)");
     S(h).spmm_variant = variant;
     API_END
 }

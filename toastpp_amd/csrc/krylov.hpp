@@ -43,6 +43,17 @@ struct RowBlocks {
 // at most the reduction grid (a multiple of 8)
 constexpr int DMA_GRID = 256;
 
+// rows per block of the row-block SpMM (one bit each in the column masks)
+constexpr int RB_ROWS = 8;
+struct RowUnions {
+    int nb = 0;                          // blocks (0: not used)
+    const int *ofs = nullptr;            // nb+1 bounds into col / mask
+    const int *col = nullptr;            // distinct columns of each block, ascending
+    const unsigned char *mask = nullptr; // rows of the block holding each column
+    const void *val = nullptr;           // coefficients in (column, row) order per block
+    const void *src = nullptr;           // the operator (CSR values) they were taken from
+};
+
 struct BlockDesc {
     int n = 0;          // rows
     int K = 0;          // columns (tile width)
@@ -50,6 +61,7 @@ struct BlockDesc {
     const int *rowptr = nullptr, *colidx = nullptr;
     int rpw = 0;        // rows per run of the 64-column SpMM (0: default)
     RowBlocks lds;      // LDS-staged SpMM partition (complex FP32, 64 columns)
+    RowUnions rb;       // row-block SpMM (complex FP32, 64 columns)
 };
 
 // precision / value-type selector
@@ -131,6 +143,8 @@ void launch_f64_to_f32(const double *src, float *dst, size_t count, hipStream_t 
 // (complex: both parts), out32 (optional) the FP32 copy
 void launch_permute_vals(bool cplx, int nnz, const int *slot, const double *val64, double *out64, float *out32,
                          hipStream_t st);
+// FP32 values into the row-block SpMM's order: out[k] = in[slot[k]] (w floats each)
+void launch_gather_vals32(int nnz, int w, const int *slot, const float *in, float *out, hipStream_t st);
 // tile of right-hand sides / solutions between the caller's [n][ld] layout
 // (reference node order) and the solver's [n][K] tile (solver node order):
 // gather dst[r][c] = src[perm[r]][q0 + c] for c < kc, 0 for kc <= c < K;

@@ -54,6 +54,14 @@ struct Mesh {
         int nb = 0;
         DBuf<int> brow, bu, ucols;
         DBuf<unsigned short> lcol;
+        // blocks of RB_ROWS consecutive rows with the union of their columns
+        // (row-block SpMM): per block the ascending distinct columns
+        // (rb_col[rb_ofs[b] .. rb_ofs[b+1])), per distinct column the mask
+        // of block rows holding it, and the CSR slot of each coefficient in
+        // the kernel's order (distinct column ascending, then row)
+        int rb_nb = 0;
+        DBuf<int> rb_ofs, rb_col, rb_slot;
+        DBuf<unsigned char> rb_mask;
     } order[2];
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }

@@ -9,6 +9,7 @@
 // file or generator used.  Only the solver's private copy is relabelled; the
 // mesh, its CSR pattern and the assembled matrix keep the reference order.
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <numeric>
 #include <vector>
@@ -113,6 +114,38 @@ static bool row_blocks(int n, const std::vector<int> &rp, const std::vector<int>
     return true;
 }
 
+// Row blocks of RB_ROWS consecutive rows: the ascending union of their
+// columns with a row mask per column, and the CSR slots of the block's
+// coefficients in (column, row) order.  The block's coefficients occupy the
+// block's own CSR range, so the kernel finds them at rowptr[first row].
+static void row_block_unions(int n, const std::vector<int> &rp, const std::vector<int> &ci, std::vector<int> &ofs,
+                             std::vector<int> &col, std::vector<unsigned char> &mask, std::vector<int> &slot)
+{
+    const int nb = (n + RB_ROWS - 1) / RB_ROWS;
+    ofs.assign(1, 0);
+    col.clear();
+    mask.clear();
+    slot.assign(ci.size(), 0);
+    std::vector<std::array<int, 3>> e;   // (column, row in block, CSR slot)
+    for (int b = 0; b < nb; b++) {
+        const int r0 = b * RB_ROWS, r1 = std::min(n, r0 + RB_ROWS);
+        e.clear();
+        for (int r = r0; r < r1; r++)
+            for (int k = rp[r]; k < rp[r + 1]; k++) e.push_back({ci[k], r - r0, k});
+        std::sort(e.begin(), e.end());
+        int at = rp[r0];
+        for (size_t i = 0; i < e.size(); i++) {
+            if (i == 0 || e[i][0] != e[i - 1][0]) {
+                col.push_back(e[i][0]);
+                mask.push_back(0);
+            }
+            mask.back() |= (unsigned char)(1u << e[i][1]);
+            slot[at++] = e[i][2];
+        }
+        ofs.push_back((int)col.size());
+    }
+}
+
 const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
 {
     TMI_CHECK(kind == ORDER_NATURAL || kind == ORDER_MORTON, TMI_ERR_INVALID_ARG, "unknown node order %d", kind);
@@ -162,6 +195,17 @@ const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
         order.ucols.upload(ucols.data(), ucols.size(), st);
         lcol.insert(lcol.end(), 8, (unsigned short)0);   // 16-byte chunk slack, as for the row pointers
         order.lcol.upload(lcol.data(), lcol.size(), st);
+    }
+    {
+        std::vector<int> ofs, col, rslot;
+        std::vector<unsigned char> mask;
+        row_block_unions(n, prp, pci, ofs, col, mask, rslot);
+        order.rb_nb = (int)ofs.size() - 1;
+        order.rb_ofs.upload(ofs.data(), ofs.size(), st);
+        order.rb_col.upload(col.data(), col.size(), st);
+        order.rb_mask.upload(mask.data(), mask.size(), st);
+        order.rb_slot.upload(rslot.data(), rslot.size(), st);
+        TMI_HIP(hipStreamSynchronize(st));
     }
     TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
     order.valid = true;

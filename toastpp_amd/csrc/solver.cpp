@@ -89,10 +89,12 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
     const Mesh::NodeOrder &o = ord();
     BlockDesc b{mesh->n, K, w.G, o.rowptr.p, o.colidx.p};
     b.rpw = rpw;
-    if (spmm_variant != 1 && o.nb > 0) {
+    if ((spmm_variant == 0 || spmm_variant == 2) && o.nb > 0) {
         b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
         b.lds.dma = spmm_variant == 2;
     }
+    if (spmm_variant == 3 && o.rb_nb > 0 && rb_src)
+        b.rb = RowUnions{o.rb_nb, o.rb_ofs.p, o.rb_col.p, o.rb_mask.p, rbval32.p, rb_src};
     return b;
 }
 
@@ -113,6 +115,7 @@ void Solver::assemble(const double *cmua, const double *ckap, const double *c2a,
     }
     omega = om;
     precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
+    rb_src = nullptr;       // ... and so do the row-block coefficients
     launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st);
     // the solver's copy in its own node order
     launch_permute_vals(cplx, mesh->nnz, ord().slot.p, val64.p, sval64.p, sval32.p, st);
@@ -387,6 +390,18 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     return notconv;
 }
 
+void Solver::prepare_row_blocks()
+{
+    // the FP32 operator of the mixed BiCGSTAB SpMMs
+    const float *op = (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) ? pval32.p : sval32.p;
+    if (rb_src == op) return;
+    const Mesh::NodeOrder &o = ord();
+    rbval32.alloc(nval() + 8);
+    launch_gather_vals32(mesh->nnz, cplx ? 2 : 1, o.rb_slot.p, op, rbval32.p, st);
+    TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
+    rb_src = op;
+}
+
 int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md)
 {
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
@@ -399,6 +414,10 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
             reset_precon();
             TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
         }
+    }
+    if (spmm_variant == 3 && cplx) {
+        std::lock_guard<std::mutex> lk(prep_mu);
+        prepare_row_blocks();
     }
     if (!e.st) e.st = st;
     std::vector<tmi_colstat> own;
@@ -525,12 +544,16 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
         zero_x();
         solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
         struct Cand { int variant; int rpw; };
-        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}};
+        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}};
         double best = 1e30;
         Cand pick{spmm_variant, rpw};
         for (const Cand &c : cands) {
-            if (c.variant != 1 && ord().nb == 0) continue;
+            if ((c.variant == 0 || c.variant == 2) && ord().nb == 0) continue;
             spmm_variant = c.variant;
+            if (c.variant == 3) {
+                std::lock_guard<std::mutex> lk(prep_mu);
+                prepare_row_blocks();
+            }
             rpw = c.rpw;
             double ms = 0, bytes = 0;
             time_kernel(0, 10, &ms, &bytes);

@@ -20,9 +20,11 @@ struct Solver {
     int reliable_every = 16;   // inner iterations between FP64 residual replacements (mixed mode)
     int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
-    int spmm_variant = 1;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA
+    int spmm_variant = 3;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
+                                     // 3 row blocks with shared gathers (default)
     int compact_min = 4;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
+    void prepare_row_blocks();   // rbval32 for the current operator (caller holds prep_mu)
     double omega = 0.0;
     hipStream_t st = nullptr;
 
@@ -32,6 +34,8 @@ struct Solver {
     DBuf<char> idiag64, idiag32;
     DBuf<double> pval64;   // A D^-1 (FP64, solver order) for the right-preconditioned inner iteration
     DBuf<float> pval32;    // A D^-1 (FP32, solver order)
+    DBuf<float> rbval32;   // the FP32 operator in the row-block SpMM's coefficient order
+    const void *rb_src = nullptr;   // the operator rbval32 holds (null: none)
     DBuf<double> coef;     // cmua | ckappa | c2a on the device
 
     // per-solve state: a stream, the workspace of one tile, pinned host

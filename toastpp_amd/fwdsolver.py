@@ -11,7 +11,7 @@ import numpy as np
 from . import _lib as L
 
 # SpMM variant names, in the order of their C-ABI codes
-SPMM_VARIANTS = ("lds", "direct", "dma")
+SPMM_VARIANTS = ("lds", "direct", "dma", "rowblock")
 
 
 def _f64(a):
@@ -81,8 +81,9 @@ class FwdSolver:
         L.call("tmi_solver_set_rows_per_wave", self._h, int(rows))
 
     def SetSpmmVariant(self, variant):
-        """'direct' (default): per-nonzero gathers; 'lds': LDS-staged 64-column
-        SpMM; 'dma': LDS-staged with the LDS DMA, double-buffered."""
+        """'direct': per-nonzero gathers; 'lds': LDS-staged 64-column
+        SpMM; 'dma': LDS-staged with the LDS DMA, double-buffered; 'rowblock':
+        blocks of 8 rows gathering each shared neighbour row once (default)."""
         L.call("tmi_solver_set_spmm_variant", self._h, SPMM_VARIANTS.index(variant))
 
     def SetInnerTol(self, tol):
