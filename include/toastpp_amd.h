@@ -256,6 +256,15 @@ int tmi_jacobian_nodal(tmi_mesh m, int nq, int nm, const double *dphi, const dou
  * 4 nqm nlen doubles. */
 int tmi_jacobian_nodal_device(tmi_mesh m, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
                               int lda, const void *d_proj, int dscale, void *d_J);
+/* GenerateJacobian_cw, nodal basis (jacobian.cc:132-186 with raster ==
+ * RASTER_NDBASIS, GenerateJacobian_cw_mesh :727-790): CW (real) fields,
+ * dphi nlen x nq and aphi nlen x nm node-major; proj: nqm real linear
+ * projections (may be NULL for TMI_DATA_LIN).  Jmua = -IntFG(dphi_q, aphi_m)
+ * and Jkap = -IntGradFGradG(dphi_q, aphi_m), one row per linked measurement
+ * (nqm x nlen row-major), each row times 1/proj for TMI_DATA_LOG; either
+ * output may be NULL. */
+int tmi_jacobian_cw(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                    int dscale, double *Jmua, double *Jkap);
 
 /* ----------------------------------------------------------------- raster */
 /* Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,
@@ -290,6 +299,14 @@ int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const 
  * d_J must hold 4 nqm slen doubles. */
 int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
                                int lda, const void *d_proj, int dscale, void *d_J);
+/* GenerateJacobian_cw on the raster basis (jacobian.cc:132-186,
+ * GenerateJacobian_cw_grid :610-721): real fields as for tmi_jacobian_cw;
+ * Jmua / Jkap nqm x slen row-major, PMDFs -d a and -grad d . grad a on the
+ * solution voxels (the adjoint gradients without the inside-mesh mask, as the
+ * reference takes them), times the voxel size, each row times 1/proj for
+ * TMI_DATA_LOG; either output may be NULL. */
+int tmi_jacobian_raster_cw(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                           int dscale, double *Jmua, double *Jkap);
 
 /* ---------------------------------------------------------------- serving */
 /* Serving loop (no reference counterpart; the reference's CalcFields runs

@@ -65,6 +65,7 @@ def lib():
             "om_relres_c": (C.c_double, [C.c_int, I, I, D, D, D]),
             "om_project": (None, [C.c_int, C.c_int, C.c_int, D, D, I, I, C.c_int, D]),
             "om_jacobian_nodal": (None, [P, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
+            "om_jacobian_grid_cw": (None, [P, I, D, D, I, C.c_int, C.c_int, D, D, I, I, P, C.c_int, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -229,6 +230,20 @@ def jacobian_nodal(mesh, dphi, aphi, lptr, lidx, proj, logscale=True):
     return J
 
 
+def jacobian_cw_nodal(mesh, dphi, aphi, lptr, lidx, proj=None, logscale=True):
+    """GenerateJacobian_cw_mesh (jacobian.cc:727-790) + the log row scaling
+    (:160-186) for real fields (nq x n, nm x n): the element-integral loops of
+    om_jacobian_nodal on the fields with zero imaginary parts, whose real
+    blocks are -IntFG/proj and -IntGradFGradG/proj.  Returns (Jmua, Jkap),
+    each nqm x n."""
+    dphi = np.asarray(dphi, np.float64).astype(np.complex128)
+    aphi = np.asarray(aphi, np.float64).astype(np.complex128)
+    nqm = int(lptr[-1])
+    p = np.ones(nqm) if proj is None else np.asarray(proj, np.float64)
+    J = jacobian_nodal(mesh, dphi, aphi, lptr, lidx, p.astype(np.complex128), logscale)
+    return J[:nqm, :mesh.n].copy(), J[:nqm, mesh.n:].copy()
+
+
 def qvec_point(mesh, pts, neumann=True):
     """QVec_Point for each row of pts -> (npts x n)."""
     pts = np.atleast_2d(np.asarray(pts, np.float64))
@@ -290,6 +305,21 @@ def jacobian_grid(raster, dphi, aphi, lptr, lidx, proj, logscale=True):
                            dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32), _c(lidx, np.int32),
                            _cplx(proj), 1 if logscale else 0, J)
     return J
+
+
+def jacobian_grid_cw(raster, dphi, aphi, lptr, lidx, proj=None, logscale=True):
+    """GenerateJacobian_cw_grid (jacobian.cc:610-721) + log row scaling:
+    real fields (nq x n, nm x n) -> (Jmua, Jkap), each nqm x slen."""
+    dphi = np.ascontiguousarray(dphi, np.float64); aphi = np.ascontiguousarray(aphi, np.float64)
+    nq, nm = dphi.shape[0], aphi.shape[0]
+    nqm = int(lptr[-1])
+    Jm = np.zeros((nqm, raster.slen)); Jk = np.zeros((nqm, raster.slen))
+    pj = None if proj is None else np.ascontiguousarray(proj, np.float64)
+    lib().om_jacobian_grid_cw(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, nq, nm, dphi, aphi,
+                              _c(lptr, np.int32), _c(lidx, np.int32),
+                              None if pj is None else pj.ctypes.data_as(C.c_void_p), 1 if logscale else 0,
+                              Jm.ctypes.data_as(C.c_void_p), Jk.ctypes.data_as(C.c_void_p))
+    return Jm, Jk
 
 
 def forward(mesh, qm, mua, mus, ref, freq, qwidth=2.0, mwidth=2.0, solver="bicgstab", tol=1e-15,

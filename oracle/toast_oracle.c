@@ -1160,6 +1160,62 @@ void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, co
     free(cols); free(w); free(s2b); free(ca); free(cd); free(gd); free(ga);
 }
 
+/* GenerateJacobian_cw_grid (jacobian.cc:610-721) for Raster_Pixel with
+ * bdim == gdim, followed by the log row scaling of GenerateJacobian_cw
+ * (:160-186): real fields dphi (nq x n), aphi (nm x n); PMDF_mua = -d a,
+ * PMDF_kap = -sum_d grad_d d grad_d a with the direct gradients masked by
+ * the element reference and the adjoint gradients unmasked (ImageGradient
+ * with mask 0, as the reference calls it there), gathered into the solution
+ * basis, times prod gsize/(bdim-1), then each row times 1/proj for log data.
+ * Jmua, Jkap: nqm x slen (either may be NULL). */
+void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax,
+                         const int *elref, int nq, int nm, const double *dphi, const double *aphi, const int *lptr,
+                         const int *lidx, const double *proj, int logscale, double *Jmua, double *Jkap)
+{
+    int dim = m->dim, nn = m->nn;
+    long glen = 1;
+    for (int d = 0; d < dim; d++) glen *= gdim[d];
+    double gsize[3], sz = 1.0;
+    for (int d = 0; d < dim; d++) { gsize[d] = bbmax[d]-bbmin[d]; sz *= gsize[d]/(gdim[d]-1.0); }
+    int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
+    double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
+    om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
+    long slen = 0;
+    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
+    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
+    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    int *nomask = (int*)xcalloc((size_t)glen, sizeof(int));   /* all inside */
+    /* the image helpers work on complex arrays: real parts carry the fields */
+    double *f = (double*)xcalloc(2*(size_t)m->n, 8);
+    double *ca = (double*)xcalloc(2*(size_t)glen*nm, 8), *ga = (double*)xcalloc(2*(size_t)glen*dim*nm, 8);
+    for (int j = 0; j < nm; j++) {
+        for (int i = 0; i < m->n; i++) { f[2*i] = aphi[(size_t)j*m->n + i]; f[2*i+1] = 0.0; }
+        map_mesh_to_grid(glen, nn, cols, w, f, ca + 2*(size_t)j*glen);
+        om_image_gradient(dim, gdim, gsize, ca + 2*(size_t)j*glen, nomask, ga + 2*(size_t)j*glen*dim);
+    }
+    double *cd = (double*)xcalloc(2*(size_t)glen, 8), *gd = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    for (int q = 0; q < nq; q++) {
+        for (int i = 0; i < m->n; i++) { f[2*i] = dphi[(size_t)q*m->n + i]; f[2*i+1] = 0.0; }
+        map_mesh_to_grid(glen, nn, cols, w, f, cd);
+        om_image_gradient(dim, gdim, gsize, cd, elref, gd);
+        for (int k = lptr[q]; k < lptr[q+1]; k++) {
+            const int j = lidx[k];
+            const double *a = ca + 2*(size_t)j*glen, *gaj = ga + 2*(size_t)j*glen*dim;
+            const double ip = logscale ? 1.0 / proj[k] : 1.0;
+            for (long s = 0; s < slen; s++) {
+                long g = s2b[s];
+                double pm = -cd[2*g] * a[2*g], pk = 0.0;
+                for (int d = 0; d < dim; d++) pk -= gd[2*((size_t)d*glen + g)] * gaj[2*((size_t)d*glen + g)];
+                pm *= sz; pk *= sz;
+                if (logscale) { pm *= ip; pk *= ip; }
+                if (Jmua) Jmua[(size_t)k*slen + s] = pm;
+                if (Jkap) Jkap[(size_t)k*slen + s] = pk;
+            }
+        }
+    }
+    free(cols); free(w); free(s2b); free(nomask); free(f); free(ca); free(ga); free(cd); free(gd);
+}
+
 /* QVec_Point (source.cc:157-196): the element holding p (first in
  * ascending order that contains it, GContains with its padding), else the
  * element of the nearest node pair; shape functions at p, boundary nodes

@@ -138,3 +138,45 @@ def test_grid_jacobian_quadrature_of_nodal_jacobian():
     assert np.max(np.abs(sg_kap / sn_kap - 1)) < 0.07
     # the wrong sign, a swapped Re/Im block or a missing voxel scale fails
     assert not np.allclose(-sg_mua, sn_mua, rtol=0.5)
+
+
+def test_grid_jacobian_cw_oracle():
+    """CW grid Jacobian (GenerateJacobian_cw_grid): its mua block equals the
+    real part of the (pinned) complex grid Jacobian on the same real fields;
+    its kappa block equals it at voxels whose neighbours are all inside the
+    mesh and differs next to the boundary, where the reference takes the
+    adjoint gradients without the mask; voxel sums match the CW nodal
+    Jacobian's node sums (quadrature)."""
+    nodes, el = G.disk(25.0, 16)
+    m = O.Mesh(nodes=nodes, elems=el)
+    n = m.n
+    Q, M = G.ring(4, 25.0), G.ring(4, 25.0, offset=0.5)
+    lp = np.array([0, 2, 4, 6, 8], np.int32)
+    li = np.array([1, 2, 2, 3, 3, 0, 0, 1], np.int32)
+    mua, mus, ref = np.full(n, 0.01), np.full(n, 1.0), np.full(n, 1.4)
+    d, a, mv = _fields(m, Q, M, mua, mus, ref, 0.0)
+    d, a, mv = d.real, a.real, mv.real
+    proj = O.project(d.astype(complex), mv.astype(complex), lp, li, logscale=False).real
+    R = O.Raster(m, [96, 96])
+    nqm, slen = len(li), R.slen
+    Om, Ok = O.jacobian_grid_cw(R, d, a, lp, li, proj, logscale=True)
+    Jc = O.jacobian_grid(R, d.astype(complex), a.astype(complex), lp, li, proj.astype(complex), logscale=True)
+    scale = np.max(np.abs(Om))
+    assert np.max(np.abs(Om - Jc[:nqm, :slen])) <= 1e-14 * scale
+    # interior voxels: both gradients see only inside points
+    g = R.sol2basis
+    nx, ny = R.gdim
+    ix, iy = g % nx, g // nx
+    inside = R.elref >= 0
+    interior = np.ones(slen, bool)
+    for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+        jx, jy = ix + dx, iy + dy
+        ok = (jx >= 0) & (jx < nx) & (jy >= 0) & (jy < ny)
+        interior &= ok & inside[np.clip(jy, 0, ny - 1) * nx + np.clip(jx, 0, nx - 1)]
+    ks = np.max(np.abs(Ok))
+    assert np.max(np.abs(Ok[:, interior] - Jc[:nqm, slen:][:, interior])) <= 1e-13 * ks
+    assert np.max(np.abs(Ok[:, ~interior] - Jc[:nqm, slen:][:, ~interior])) > 1e-6 * ks
+    # quadrature against the CW nodal Jacobian
+    Nm, Nk = O.jacobian_cw_nodal(m, d, a, lp, li, proj, logscale=True)
+    assert np.max(np.abs(Om.sum(1) / Nm.sum(1) - 1)) < 0.005
+    assert np.max(np.abs(Ok.sum(1) / Nk.sum(1) - 1)) < 0.07

@@ -106,6 +106,8 @@ _SIGS = {
     "tmi_solver_time_kernel": [P, C.c_int, C.c_int, DP, DP],
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
+    "tmi_jacobian_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
+    "tmi_jacobian_raster_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
     "tmi_raster_create": [P, I, P, PP],
     "tmi_raster_free": [P],
     "tmi_raster_dims": [P, IP, IP],

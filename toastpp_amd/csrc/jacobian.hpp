@@ -10,8 +10,14 @@ namespace tmi {
 // measurement (link-list order, source-major); d_proj: nqm
 // complex linear projections; d_Pt: scratch of n*nqm*4 doubles; d_J: (2 nqm)
 // x (2 n) row-major.  dmax: longest CSR row of the mesh.
+// cw: CW Jacobian of real fields given with zero imaginary parts; d_proj is
+// then nqm real projections, d_J the nqm x n mua block and d_Jkap the
+// nqm x n kappa block (either may be null).
 void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                            const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
-                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st);
+                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw = false,
+                           double *d_Jkap = nullptr);
+// real [rows][ld] -> complex [rows][cols] (imaginary parts zero)
+void launch_real_to_cplx(long rows, int cols, const double *src, int ld, double *dst, hipStream_t st);
 
 }  // namespace tmi

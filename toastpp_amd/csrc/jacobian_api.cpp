@@ -66,7 +66,8 @@ static void ensure_links(Mesh &m, hipStream_t st)
 }
 
 static void jacobian_nodal_dev(Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi,
-                               int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st)
+                               int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st,
+                               bool cw = false, double *d_Jkap = nullptr)
 {
     const QM &qm = m.qm;
     TMI_CHECK(qm.nq == nq && qm.nm == nm, TMI_ERR_INVALID_ARG,
@@ -81,8 +82,25 @@ static void jacobian_nodal_dev(Mesh &m, int nq, int nm, const double *d_dphi, in
     const size_t ws = (size_t)m.n * qm.nqm * 4;
     if (m.jac_ws.n < ws) m.jac_ws.alloc(ws);
     launch_jacobian_nodal(m, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lq.p, m.d_lm.p, qm.nqm, d_proj,
-                          dscale == TMI_DATA_LOG ? 1 : 0, m.jac_ws.p, d_J, dmax, st);
+                          dscale == TMI_DATA_LOG ? 1 : 0, m.jac_ws.p, d_J, dmax, st, cw, d_Jkap);
 }
+
+// CW inputs on the host: real node-major fields and projections, uploaded and
+// widened to the complex layout of the kernels (imaginary parts zero)
+struct CwInputs {
+    DBuf<double> rd, ra, cd, ca, dp;
+    CwInputs(size_t n, int nq, int nm, const double *dphi, const double *aphi, const double *proj, size_t nqm,
+             hipStream_t st)
+    {
+        rd.upload(dphi, n * nq, st);
+        ra.upload(aphi, n * nm, st);
+        cd.alloc(n * nq * 2);
+        ca.alloc(n * nm * 2);
+        launch_real_to_cplx((long)n, nq, rd.p, nq, cd.p, st);
+        launch_real_to_cplx((long)n, nm, ra.p, nm, ca.p, st);
+        if (proj) dp.upload(proj, nqm, st);
+    }
+};
 
 }  // namespace tmi
 
@@ -114,6 +132,24 @@ int tmi_jacobian_nodal_device(tmi_mesh h, int nq, int nm, const void *d_dphi, in
     hipStream_t st = Context::get().stream;
     jacobian_nodal_dev(h->m, nq, nm, (const double *)d_dphi, ldd, (const double *)d_aphi, lda,
                        (const double *)d_proj, dscale, (double *)d_J, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_cw(tmi_mesh h, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                    int dscale, double *Jmua, double *Jkap)
+{
+    API_BEGIN
+    TMI_CHECK(h && dphi && aphi && (Jmua || Jkap), TMI_ERR_INVALID_ARG, "NULL argument");
+    TMI_CHECK(dscale == TMI_DATA_LIN || proj, TMI_ERR_INVALID_ARG, "log data scaling needs the projections");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    const size_t n = m.n, nqm = m.qm.nqm;
+    CwInputs in(n, nq, nm, dphi, aphi, proj, nqm, st);
+    DBuf<double> jm(Jmua ? nqm * n : 0), jk(Jkap ? nqm * n : 0);
+    jacobian_nodal_dev(m, nq, nm, in.cd.p, nq, in.ca.p, nm, in.dp.p, dscale, jm.p, st, true, jk.p);
+    if (Jmua) jm.download(Jmua, nqm * n, st);
+    if (Jkap) jk.download(Jkap, nqm * n, st);
     TMI_HIP(hipStreamSynchronize(st));
     API_END
 }
@@ -227,7 +263,8 @@ int tmi_raster_map_mesh_to_grid(tmi_raster r, int ncols, const double *mvec, dou
 }
 
 static void jacobian_raster_dev(Raster &R, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi,
-                                int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st)
+                                int lda, const double *d_proj, int dscale, double *d_J, hipStream_t st,
+                                bool cw = false, double *d_Jkap = nullptr)
 {
     Mesh &m = *R.mesh;
     const QM &qm = m.qm;
@@ -239,7 +276,7 @@ static void jacobian_raster_dev(Raster &R, int nq, int nm, const double *d_dphi,
     TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
     ensure_links(m, st);
     launch_jacobian_grid(R, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lm.p, qm.nqm, d_proj,
-                         dscale == TMI_DATA_LOG ? 1 : 0, d_J, st);
+                         dscale == TMI_DATA_LOG ? 1 : 0, d_J, st, cw, d_Jkap);
 }
 
 int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
@@ -256,6 +293,24 @@ int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const 
     dp.upload(proj, nqm * 2, st);
     jacobian_raster_dev(R, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
     dJ.download(J, 4 * nqm * (size_t)R.slen, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_raster_cw(tmi_raster r, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                           int dscale, double *Jmua, double *Jkap)
+{
+    API_BEGIN
+    TMI_CHECK(r && dphi && aphi && (Jmua || Jkap), TMI_ERR_INVALID_ARG, "NULL argument");
+    TMI_CHECK(dscale == TMI_DATA_LIN || proj, TMI_ERR_INVALID_ARG, "log data scaling needs the projections");
+    Raster &R = r->r;
+    hipStream_t st = Context::get().stream;
+    const size_t n = R.mesh->n, nqm = R.mesh->qm.nqm, slen = (size_t)R.slen;
+    CwInputs in(n, nq, nm, dphi, aphi, proj, nqm, st);
+    DBuf<double> jm(Jmua ? nqm * slen : 0), jk(Jkap ? nqm * slen : 0);
+    jacobian_raster_dev(R, nq, nm, in.cd.p, nq, in.ca.p, nm, in.dp.p, dscale, jm.p, st, true, jk.p);
+    if (Jmua) jm.download(Jmua, nqm * slen, st);
+    if (Jkap) jk.download(Jkap, nqm * slen, st);
     TMI_HIP(hipStreamSynchronize(st));
     API_END
 }

@@ -192,9 +192,64 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
     }
 }
 
+// CW (GenerateJacobian_cw_mesh, jacobian.cc:727-790, then the log row scaling
+// of :160-186): from the intermediate of real fields, Jmua[k][i] =
+// -P_mua * (1/proj_k), Jkap[k][i] = -P_kap * (1/proj_k) (without log the
+// factor is dropped); either output may be absent.  Same tiling as above.
+__global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const double2 *__restrict__ Pt,
+                                                          const double *__restrict__ proj, int logscale,
+                                                          double *__restrict__ Jmua, double *__restrict__ Jkap)
+{
+    __shared__ double tm[64][TJK + 1], tk[64][TJK + 1];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int t = threadIdx.x;
+    for (int e = t; e < 64 * TJK; e += 256) {
+        const int r = e / TJK, c = e % TJK;
+        const int i = i0 + r, k = k0 + c;
+        if (i < n && k < nqm) {
+            tm[r][c] = Pt[((size_t)i * nqm + k) * 2].x;
+            tk[r][c] = Pt[((size_t)i * nqm + k) * 2 + 1].x;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * TJK; e += 256) {
+        const int c = e / 64, r = e % 64;
+        const int k = k0 + c, i = i0 + r;
+        if (k < nqm && i < n) {
+            double a = -tm[r][c], b = -tk[r][c];
+            if (logscale) {
+                const double ip = 1.0 / proj[k];
+                a *= ip;
+                b *= ip;
+            }
+            if (Jmua) Jmua[(size_t)k * n + i] = a;
+            if (Jkap) Jkap[(size_t)k * n + i] = b;
+        }
+    }
+}
+
+// real fields (node-major, row strides ldd / lda) into the complex layout of
+// the nodal kernel, imaginary parts zero
+__global__ void k_real_to_cplx(long rows, int cols, const double *__restrict__ src, int ld, double2 *__restrict__ dst)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    const long r = t / cols;
+    const int c = (int)(t % cols);
+    dst[t] = make_double2(src[r * ld + c], 0.0);
+}
+
+void launch_real_to_cplx(long rows, int cols, const double *src, int ld, double *dst, hipStream_t st)
+{
+    const long cnt = rows * cols;
+    hipLaunchKernelGGL(k_real_to_cplx, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, rows, cols, src, ld,
+                       (double2 *)dst);
+    TMI_LAUNCH_CHECK();
+}
+
 void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                            const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
-                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st)
+                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw, double *d_Jkap)
 {
     // source/detector chunk: all of them if the wave's LDS stays within 64 KiB
     // (two waves per CU), else the largest chunk that fits, up to 160 KiB
@@ -217,8 +272,12 @@ void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, 
                            (double2 *)d_Pt);
     TMI_LAUNCH_CHECK();
     dim3 g((m.n + 63) / 64, (nqm + TJK - 1) / TJK);
-    hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
-                       (const double2 *)d_proj, logscale, d_J);
+    if (cw)
+        hipLaunchKernelGGL(k_jac_transpose_cw, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt, d_proj, logscale,
+                           d_J, d_Jkap);
+    else
+        hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
+                           (const double2 *)d_proj, logscale, d_J);
     TMI_LAUNCH_CHECK();
 }
 

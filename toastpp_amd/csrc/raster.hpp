@@ -28,8 +28,11 @@ struct Raster {
 void launch_raster_map(const Raster &r, int ncols, const double *d_f, int ld, double *d_out, hipStream_t st);
 // GenerateJacobian_grid for fields already on the device (node-major
 // [n][ldd] / [n][lda] complex); J: (2 nqm) x (2 slen) row-major
+// cw: CW Jacobian of real fields given with zero imaginary parts; d_proj is
+// then nqm real projections, d_J the nqm x slen mua block and d_Jkap the
+// kappa block (either may be null).
 void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                           const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
-                          double *d_J, hipStream_t st);
+                          double *d_J, hipStream_t st, bool cw = false, double *d_Jkap = nullptr);
 
 }  // namespace tmi

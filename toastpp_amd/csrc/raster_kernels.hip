@@ -209,12 +209,14 @@ void launch_raster_map(const Raster &r, int ncols, const double *d_f, int ld, do
 // ImageGradient (jacobian.cc:944-1060) at the solution voxels: for column
 // c and solution voxel s, fgs[c][s] = (value, d/dx, d/dy, d/dz) with the
 // reference's spacing gsize/n and mask rules (neighbours tested first).
-__device__ inline double2 grad1(const double2 *__restrict__ v, long ldv, const int *__restrict__ mask, long g,
-                                int pos, int len, long s, double ih, double i2h)
+__device__ inline double2 grad1(const double2 *__restrict__ v, long ldv, const int *__restrict__ mask, bool nomask,
+                                long g, int pos, int len, long s, double ih, double i2h)
 {
-    const bool bcnt = mask[g] >= 0;
-    const bool blo = pos > 0 && mask[g - s] >= 0;
-    const bool bhi = pos < len - 1 && mask[g + s] >= 0;
+    // without a mask every grid point counts as inside (ImageGradient with
+    // mask == 0, jacobian.cc:953-958)
+    const bool bcnt = nomask || mask[g] >= 0;
+    const bool blo = pos > 0 && (nomask || mask[g - s] >= 0);
+    const bool bhi = pos < len - 1 && (nomask || mask[g + s] >= 0);
     if (blo && bhi) {
         const double2 a = v[(g + s) * ldv], b = v[(g - s) * ldv];
         return make_double2((a.x - b.x) * i2h, (a.y - b.y) * i2h);
@@ -236,8 +238,9 @@ struct GradDesc {
     double ih[3], i2h[3];
 };
 
+// columns c >= unmasked_from take their gradients without the mask
 __global__ void k_grid_grad(int slen, int ncols, const int *__restrict__ s2b, const int *__restrict__ mask,
-                            const double2 *__restrict__ fgv, GradDesc gd, double2 *__restrict__ fgs)
+                            const double2 *__restrict__ fgv, GradDesc gd, double2 *__restrict__ fgs, int unmasked_from)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)slen * ncols) return;
@@ -251,7 +254,7 @@ __global__ void k_grid_grad(int slen, int ncols, const int *__restrict__ s2b, co
     double2 *o = fgs + ((long)c * slen + s) * 4;
     o[0] = v[g * ldv];
     for (int d = 0; d < 3; d++)
-        o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, g, pos[d], len[d], str[d], gd.ih[d], gd.i2h[d])
+        o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, c >= unmasked_from, g, pos[d], len[d], str[d], gd.ih[d], gd.i2h[d])
                               : make_double2(0.0, 0.0);
 }
 
@@ -298,9 +301,43 @@ __global__ void __launch_bounds__(256) k_jac_grid(int slen, int nq, int dim, con
     }
 }
 
+// CW (GenerateJacobian_cw_grid, jacobian.cc:610-721, then the log row
+// scaling of :160-186) from real fields given with zero imaginary parts:
+// Jmua[k][s] = (-d a sz) (1/proj_k), Jkap[k][s] = (-sum_d grad_d d grad_d a sz)
+// (1/proj_k), the adjoint gradients taken without the mask; either output may
+// be absent.
+__global__ void __launch_bounds__(256) k_jac_grid_cw(int slen, int nq, int dim, const double2 *__restrict__ fgs,
+                                                     const int *__restrict__ lptr, const int *__restrict__ lm,
+                                                     const double *__restrict__ proj, int logscale, double sz,
+                                                     double *__restrict__ Jmua, double *__restrict__ Jkap)
+{
+    const int q = blockIdx.y;
+    const int vs = threadIdx.x & 63, lk = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + vs;
+    if (s >= slen) return;
+    const double2 *D = fgs + ((long)q * slen + s) * 4;
+    const double d0 = D[0].x, d1 = D[1].x, d2 = D[2].x, d3 = D[3].x;
+    for (int k = lptr[q] + lk; k < lptr[q + 1]; k += 4) {
+        const double2 *A = fgs + ((long)(nq + lm[k]) * slen + s) * 4;
+        double pm = -(d0 * A[0].x) * sz;
+        double pk = 0.0;
+        pk -= d1 * A[1].x;
+        pk -= d2 * A[2].x;
+        if (dim > 2) pk -= d3 * A[3].x;
+        pk *= sz;
+        if (logscale) {
+            const double ip = 1.0 / proj[k];
+            pm *= ip;
+            pk *= ip;
+        }
+        if (Jmua) Jmua[(size_t)k * slen + s] = pm;
+        if (Jkap) Jkap[(size_t)k * slen + s] = pk;
+    }
+}
+
 void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                           const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
-                          double *d_J, hipStream_t st)
+                          double *d_J, hipStream_t st, bool cw, double *d_Jkap)
 {
     const int ncols = nq + nm;
     TMI_CHECK(r.slen > 0, TMI_ERR_INVALID_ARG, "the raster has no voxel inside the mesh");
@@ -332,10 +369,14 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
     }
     const long cnt_s = (long)r.slen * ncols;
     hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.slen, ncols,
-                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p);
+                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, cw ? nq : ncols);
     TMI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_jac_grid, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq, r.dim,
-                       (const double2 *)r.fgs.p, d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
+    if (cw)
+        hipLaunchKernelGGL(k_jac_grid_cw, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq,
+                           r.dim, (const double2 *)r.fgs.p, d_lptr, d_lm, d_proj, logscale, sz, d_J, d_Jkap);
+    else
+        hipLaunchKernelGGL(k_jac_grid, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq, r.dim,
+                           (const double2 *)r.fgs.p, d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
     TMI_LAUNCH_CHECK();
 }
 

@@ -228,6 +228,32 @@ class Mesh:
         from .jacobian import jacobian
         return jacobian(self, raster, dphi, aphi, proj)
 
+    def JacobianCW(self, raster, qvec, mvec, mua, mus, ref, solver="bicgstab", tol=1e-12, mode="mixed"):
+        """CW Jacobian of log intensity with respect to mua on the raster
+        basis (toast.JacobianCW, mesh.py:351-363 -> CalcJacobianCW,
+        toastmodule.cc:1630-1680): real fields for qvec and mvec at frequency
+        0, linear projections, GenerateJacobian_cw with DATA_LOG.  Returns
+        (nQM x slen), or (nQM x nlen) with raster None."""
+        from .jacobian import jacobian_cw
+        q = sparse.csc_matrix(qvec)
+        m = sparse.csc_matrix(mvec)
+        for name, v in (("qvec", q), ("mvec", m)):
+            if np.any(np.iscomplex(v.data)):
+                raise ValueError("%s must be real or not have any nonzero complex elements" % name)
+        q, m = q.real, m.real
+        qrows = sparse.csr_matrix((q.data, q.indices, q.indptr), shape=(q.shape[1], q.shape[0]))
+        mrows = sparse.csr_matrix((m.data, m.indices, m.indptr), shape=(m.shape[1], m.shape[0]))
+        fws = FwdSolver(self, solver, tol, complex=False)
+        fws.SetMode(mode)
+        fws.SetPrecon("DIAG")
+        fws.Allocate()
+        fws.Reset(Solution(mua, mus, ref), 0.0)
+        dphi = fws.CalcFields(qrows)
+        aphi = fws.CalcFields(mrows)
+        proj = fws.ProjectAll(mrows, dphi, scale="lin")
+        Jm, _ = jacobian_cw(self, raster, dphi.T, aphi.T, np.real(proj), scale="log")
+        return Jm
+
     def JacobianOptical(self, raster, qvec, mvec, mua, mus, ref, freq=0, solver="bicgstab", tol=1e-12,
                         mode="mixed"):
         """Fields, adjoint fields, projections and Jacobian from optical
