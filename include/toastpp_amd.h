@@ -299,6 +299,17 @@ int tmi_jacobian_raster(tmi_raster r, int nq, int nm, const double *dphi, const 
  * d_J must hold 4 nqm slen doubles. */
 int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
                                int lda, const void *d_proj, int dscale, void *d_J);
+/* AddDataGradient's raster part (toastmodule.cc:1745-1843, used by
+ * toast.Gradient :1884-1941): dphi the direct and aphi the adjoint complex
+ * fields of the nq sources (nlen x nq node-major, column q of aphi the
+ * adjoint field of source q); grad (2 slen: cmua then ckappa, solution
+ * voxels) is decremented by Re(d_q a_q) and Re(grad d_q . grad a_q) for
+ * q = 0, 1, ... in order (gradients with the inside-mesh mask, no voxel
+ * scaling), as the reference accumulates it. */
+int tmi_raster_data_gradient(tmi_raster r, int nq, const double *dphi, const double *aphi, double *grad);
+/* Same on GPU-resident fields (rows of ldd / lda complex) and gradient. */
+int tmi_raster_data_gradient_device(tmi_raster r, int nq, const void *d_dphi, int ldd, const void *d_aphi, int lda,
+                                    void *d_grad);
 /* GenerateJacobian_cw on the raster basis (jacobian.cc:132-186,
  * GenerateJacobian_cw_grid :610-721): real fields as for tmi_jacobian_cw;
  * Jmua / Jkap nqm x slen row-major, PMDFs -d a and -grad d . grad a on the

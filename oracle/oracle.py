@@ -66,6 +66,7 @@ def lib():
             "om_project": (None, [C.c_int, C.c_int, C.c_int, D, D, I, I, C.c_int, D]),
             "om_jacobian_nodal": (None, [P, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
             "om_jacobian_grid_cw": (None, [P, I, D, D, I, C.c_int, C.c_int, D, D, I, I, P, C.c_int, P, P]),
+            "om_data_gradient": (None, [P, I, D, D, I, C.c_int, D, D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -320,6 +321,39 @@ def jacobian_grid_cw(raster, dphi, aphi, lptr, lidx, proj=None, logscale=True):
                               None if pj is None else pj.ctypes.data_as(C.c_void_p), 1 if logscale else 0,
                               Jm.ctypes.data_as(C.c_void_p), Jk.ctypes.data_as(C.c_void_p))
     return Jm, Jk
+
+
+def adjoint_sources(cproj, proj_real, data, sd, mvec, lptr, lidx):
+    """The adjoint source of every source q in AddDataGradient
+    (toastmodule.cc:1772-1808): cproj the nQM linear complex projections,
+    proj_real the log projections [ln amp ; phase] (2 nQM), data / sd the
+    measured data and its standard deviations (2 nQM), mvec (nM x n) complex
+    dense.  Returns (nq x n) complex."""
+    nqm = int(lptr[-1])
+    nq = len(lptr) - 1
+    out = np.zeros((nq, mvec.shape[1]), np.complex128)
+    for q in range(nq):
+        for k in range(lptr[q], lptr[q + 1]):
+            b_mod = (data[k] - proj_real[k]) / sd[k]
+            b_arg = (data[nqm + k] - proj_real[nqm + k]) / sd[nqm + k]
+            rp, ip = cproj[k].real, cproj[k].imag
+            dn = 1.0 / (rp * rp + ip * ip)
+            qs = mvec[lidx[k]]
+            term = -2.0 * b_mod / (1.0 * sd[k])
+            out[q] += qs * complex(term * rp * dn, -term * ip * dn)
+            term = -2.0 * b_arg / (1.0 * sd[nqm + k])
+            out[q] += qs * complex(-term * ip * dn, -term * rp * dn)
+    return out
+
+
+def data_gradient(raster, dphi, aphi, grad=None):
+    """AddDataGradient's raster part: dphi, aphi (nq x n) complex direct and
+    adjoint fields; returns grad (2 slen) decremented as the reference does."""
+    dphi = np.ascontiguousarray(dphi, np.complex128); aphi = np.ascontiguousarray(aphi, np.complex128)
+    g = np.zeros(2 * raster.slen) if grad is None else np.array(grad, np.float64)
+    lib().om_data_gradient(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, dphi.shape[0],
+                           dphi.view(np.float64), aphi.view(np.float64), g)
+    return g
 
 
 def forward(mesh, qm, mua, mus, ref, freq, qwidth=2.0, mwidth=2.0, solver="bicgstab", tol=1e-15,

@@ -1216,6 +1216,48 @@ void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin,
     free(cols); free(w); free(s2b); free(nomask); free(f); free(ca); free(ga); free(cd); free(gd);
 }
 
+/* AddDataGradient's raster part (toastmodule.cc:1767-1843): for each
+ * source q in order, the direct field dphi_q and its adjoint field aphi_q
+ * (nq x n complex each) mapped to the grid, gradients with the element
+ * reference as mask; grad (2 slen, cmua then ckappa) -= Re(d a) and
+ * -= Re(sum_j grad_j d grad_j a) at the solution voxels. */
+void om_data_gradient(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                      int nq, const double *dphi, const double *aphi, double *grad)
+{
+    int dim = m->dim, nn = m->nn;
+    long glen = 1;
+    for (int d = 0; d < dim; d++) glen *= gdim[d];
+    double gsize[3];
+    for (int d = 0; d < dim; d++) gsize[d] = bbmax[d]-bbmin[d];
+    int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
+    double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
+    om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
+    long slen = 0;
+    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
+    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
+    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    double *cd = (double*)xcalloc(2*(size_t)glen, 8), *ca = (double*)xcalloc(2*(size_t)glen, 8);
+    double *gd = (double*)xcalloc(2*(size_t)glen*dim, 8), *ga = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    for (int q = 0; q < nq; q++) {
+        map_mesh_to_grid(glen, nn, cols, w, dphi + 2*(size_t)q*m->n, cd);
+        om_image_gradient(dim, gdim, gsize, cd, elref, gd);
+        map_mesh_to_grid(glen, nn, cols, w, aphi + 2*(size_t)q*m->n, ca);
+        om_image_gradient(dim, gdim, gsize, ca, elref, ga);
+        for (long s = 0; s < slen; s++) {
+            long g = s2b[s];
+            double t[2], gk[2] = {0.0, 0.0};
+            cmul(cd + 2*g, ca + 2*g, t);
+            grad[s] -= t[0];
+            for (int j = 0; j < dim; j++) {
+                cmul(gd + 2*((size_t)j*glen + g), ga + 2*((size_t)j*glen + g), t);
+                gk[0] += t[0]; gk[1] += t[1];
+            }
+            grad[slen + s] -= gk[0];
+        }
+    }
+    free(cols); free(w); free(s2b); free(cd); free(ca); free(gd); free(ga);
+}
+
 /* QVec_Point (source.cc:157-196): the element holding p (first in
  * ascending order that contains it, GContains with its padding), else the
  * element of the nearest node pair; shape functions at p, boundary nodes

@@ -180,3 +180,50 @@ def test_grid_jacobian_cw_oracle():
     Nm, Nk = O.jacobian_cw_nodal(m, d, a, lp, li, proj, logscale=True)
     assert np.max(np.abs(Om.sum(1) / Nm.sum(1) - 1)) < 0.005
     assert np.max(np.abs(Ok.sum(1) / Nk.sum(1) - 1)) < 0.07
+
+
+def test_data_gradient_oracle_matches_jacobian_identity():
+    """AddDataGradient against the (pinned) grid Jacobian: with adjoint
+    fields a_q = sum_m w_qm aphi_m (the adjoint source is linear in the
+    measurement vectors), grad = sum_k (t_mod,k J[k] + t_arg,k J[k + nQM]) / sz
+    with t = -2 (data - proj) / sd^2, J the log-data grid Jacobian and sz the
+    voxel size the Jacobian carries and the gradient does not."""
+    nodes, el = G.disk(25.0, 16)
+    m = O.Mesh(nodes=nodes, elems=el)
+    n = m.n
+    Q, M = G.ring(4, 25.0), G.ring(4, 25.0, offset=0.5)
+    lp = np.array([0, 2, 4, 6, 8], np.int32)
+    li = np.array([1, 2, 2, 3, 3, 0, 0, 1], np.int32)
+    mua, mus, ref = np.full(n, 0.01), np.full(n, 1.0), np.full(n, 1.4)
+    d, a, mv = _fields(m, Q, M, mua, mus, ref, 100.0)
+    nqm = len(li)
+    cproj = O.project(d, mv.astype(complex), lp, li, logscale=False)
+    lproj = np.log(cproj)
+    preal = np.concatenate([lproj.real, lproj.imag])
+    rng = np.random.default_rng(3)
+    data = preal + 0.05 * rng.standard_normal(2 * nqm)
+    sd = 0.5 + rng.random(2 * nqm)
+    wq = O.adjoint_sources(cproj, preal, data, sd, mv.astype(complex), lp, li)
+    # adjoint fields by linearity: the weights of wq applied to aphi
+    W = np.zeros((4, 4), complex)
+    for q in range(4):
+        for k in range(lp[q], lp[q + 1]):
+            rp, ip = cproj[k].real, cproj[k].imag
+            dn = 1.0 / (rp * rp + ip * ip)
+            ta = -2.0 * (data[k] - preal[k]) / sd[k] / sd[k]
+            tp = -2.0 * (data[nqm + k] - preal[nqm + k]) / sd[nqm + k] / sd[nqm + k]
+            W[q, li[k]] += complex(ta * rp * dn, -ta * ip * dn) + complex(-tp * ip * dn, -tp * rp * dn)
+    assert np.allclose(wq, W @ mv, rtol=1e-13, atol=1e-13 * np.abs(wq).max())
+    aq = W @ a
+    R = O.Raster(m, [64, 64])
+    g = O.data_gradient(R, d, aq)
+    J = O.jacobian_grid(R, d, a, lp, li, cproj, logscale=True)
+    sz = np.prod([(R.bbmax[i] - R.bbmin[i]) / (R.gdim[i] - 1) for i in range(2)])
+    t_mod = -2.0 * (data[:nqm] - preal[:nqm]) / sd[:nqm] ** 2
+    t_arg = -2.0 * (data[nqm:] - preal[nqm:]) / sd[nqm:] ** 2
+    ref_g = (t_mod @ J[:nqm] + t_arg @ J[nqm:]) / sz
+    assert g.shape == (2 * R.slen,)
+    scale = np.abs(ref_g).max()
+    assert np.max(np.abs(g - ref_g)) <= 1e-10 * scale
+    # a sign or a block swap fails
+    assert np.max(np.abs(g + ref_g)) > 1e-3 * scale

@@ -107,6 +107,8 @@ _SIGS = {
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
     "tmi_jacobian_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
+    "tmi_raster_data_gradient": [P, C.c_int, D, D, D],
+    "tmi_raster_data_gradient_device": [P, C.c_int, P, C.c_int, P, C.c_int, P],
     "tmi_jacobian_raster_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
     "tmi_raster_create": [P, I, P, PP],
     "tmi_raster_free": [P],

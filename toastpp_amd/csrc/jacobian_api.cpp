@@ -315,6 +315,35 @@ int tmi_jacobian_raster_cw(tmi_raster r, int nq, int nm, const double *dphi, con
     API_END
 }
 
+int tmi_raster_data_gradient(tmi_raster r, int nq, const double *dphi, const double *aphi, double *grad)
+{
+    API_BEGIN
+    TMI_CHECK(r && dphi && aphi && grad && nq > 0, TMI_ERR_INVALID_ARG, "NULL argument");
+    Raster &R = r->r;
+    hipStream_t st = Context::get().stream;
+    const size_t n = R.mesh->n, slen = (size_t)R.slen;
+    DBuf<double> dd, da, dg;
+    dd.upload(dphi, n * nq * 2, st);
+    da.upload(aphi, n * nq * 2, st);
+    dg.upload(grad, 2 * slen, st);
+    launch_data_gradient(R, nq, dd.p, nq, da.p, nq, dg.p, st);
+    dg.download(grad, 2 * slen, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_data_gradient_device(tmi_raster r, int nq, const void *d_dphi, int ldd, const void *d_aphi, int lda,
+                                    void *d_grad)
+{
+    API_BEGIN
+    TMI_CHECK(r && d_dphi && d_aphi && d_grad && nq > 0 && ldd >= nq && lda >= nq, TMI_ERR_INVALID_ARG,
+              "invalid argument");
+    hipStream_t st = Context::get().stream;
+    launch_data_gradient(r->r, nq, (const double *)d_dphi, ldd, (const double *)d_aphi, lda, (double *)d_grad, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
 int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi, int lda,
                                const void *d_proj, int dscale, void *d_J)
 {

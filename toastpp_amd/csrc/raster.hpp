@@ -28,6 +28,11 @@ struct Raster {
 void launch_raster_map(const Raster &r, int ncols, const double *d_f, int ld, double *d_out, hipStream_t st);
 // GenerateJacobian_grid for fields already on the device (node-major
 // [n][ldd] / [n][lda] complex); J: (2 nqm) x (2 slen) row-major
+// AddDataGradient's grid part: d_dphi [n][ldd] direct and d_aphi [n][lda]
+// adjoint complex fields of nq sources; d_grad (2 slen) is decremented by
+// the mua and kappa PMDF sums of every source, in source order.
+void launch_data_gradient(Raster &r, int nq, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                          double *d_grad, hipStream_t st);
 // cw: CW Jacobian of real fields given with zero imaginary parts; d_proj is
 // then nqm real projections, d_J the nqm x slen mua block and d_Jkap the
 // kappa block (either may be null).

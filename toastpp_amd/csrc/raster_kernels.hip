@@ -335,9 +335,12 @@ __global__ void __launch_bounds__(256) k_jac_grid_cw(int slen, int nq, int dim, 
     }
 }
 
-void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
-                          const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
-                          double *d_J, hipStream_t st, bool cw, double *d_Jkap)
+// nq direct then nm adjoint complex columns mapped to the grid (r.fgv) and
+// their values and gradients at the solution voxels (r.fgs [c][s][4]);
+// columns c >= unmasked_from take unmasked gradients.  Returns the voxel
+// size prod gsize/(gdim-1).
+static double grid_fields(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                          int unmasked_from, hipStream_t st)
 {
     const int ncols = nq + nm;
     TMI_CHECK(r.slen > 0, TMI_ERR_INVALID_ARG, "the raster has no voxel inside the mesh");
@@ -369,8 +372,45 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
     }
     const long cnt_s = (long)r.slen * ncols;
     hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.slen, ncols,
-                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, cw ? nq : ncols);
+                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, unmasked_from);
     TMI_LAUNCH_CHECK();
+    return sz;
+}
+
+// AddDataGradient's raster part (toastmodule.cc:1813-1836): for each source q
+// in order, grad_mua -= Re(d_q a_q) and grad_kappa -= Re(sum_d grad_d d_q
+// grad_d a_q) at every solution voxel, a_q the adjoint field of source q;
+// thread per voxel, so each voxel's sums run in the reference's order.
+__global__ void k_data_grad(int slen, int nq, const double2 *__restrict__ fgs, double *__restrict__ grad)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slen) return;
+    double gm = grad[s], gk = grad[slen + s];
+    for (int q = 0; q < nq; q++) {
+        const double2 *D = fgs + ((long)q * slen + s) * 4, *A = fgs + ((long)(nq + q) * slen + s) * 4;
+        gm -= D[0].x * A[0].x - D[0].y * A[0].y;
+        double t = 0.0;   // real part of the complex sum over the dimensions
+        for (int d = 1; d <= 3; d++) t += D[d].x * A[d].x - D[d].y * A[d].y;
+        gk -= t;
+    }
+    grad[s] = gm;
+    grad[slen + s] = gk;
+}
+
+void launch_data_gradient(Raster &r, int nq, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                          double *d_grad, hipStream_t st)
+{
+    grid_fields(r, nq, nq, d_dphi, ldd, d_aphi, lda, 2 * nq, st);
+    hipLaunchKernelGGL(k_data_grad, dim3((unsigned)((r.slen + 255) / 256)), dim3(256), 0, st, r.slen, nq,
+                       (const double2 *)r.fgs.p, d_grad);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                          const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
+                          double *d_J, hipStream_t st, bool cw, double *d_Jkap)
+{
+    const double sz = grid_fields(r, nq, nm, d_dphi, ldd, d_aphi, lda, cw ? nq : nq + nm, st);
     if (cw)
         hipLaunchKernelGGL(k_jac_grid_cw, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq,
                            r.dim, (const double2 *)r.fgs.p, d_lptr, d_lm, d_proj, logscale, sz, d_J, d_Jkap);

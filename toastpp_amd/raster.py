@@ -154,6 +154,20 @@ class Raster:
         raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "matrix %s is not provided" % mapstr)
 
     # ----------------------------------------------------------- Jacobian
+    def DataGradient(self, dphi, aphi, grad=None):
+        """AddDataGradient's raster part (toastmodule.cc:1813-1836): dphi and
+        aphi (nlen x nq) complex, column q of aphi the adjoint field of source
+        q; returns grad (2 slen: cmua, ckappa) decremented by the PMDF sums
+        of every source."""
+        dphi = np.ascontiguousarray(dphi, np.complex128)
+        aphi = np.ascontiguousarray(aphi, np.complex128)
+        if dphi.shape != aphi.shape or dphi.shape[0] != self.mesh.NodeCount():
+            raise ValueError("direct and adjoint fields must both be (nlen x nq)")
+        g = np.zeros(2 * self._slen) if grad is None else np.array(grad, np.float64)
+        L.call("tmi_raster_data_gradient", self.handle_ptr(), dphi.shape[1], dphi.view(np.float64),
+               aphi.view(np.float64), g)
+        return g
+
     def Jacobian(self, dphi, aphi, proj, dscale=L.DATA_LOG):
         """GenerateJacobian_grid (jacobian.cc:293-373): (2 nQM x 2 slen)."""
         nq, nm, nqm = self.mesh.qm_dims()
