@@ -285,6 +285,15 @@ int tmi_raster_bbox(tmi_raster r, double *bbmin, double *bbmax);
 int tmi_raster_get_elref(tmi_raster r, int *elref, int *sol2basis);
 /* rows of B: glen x (dim+1) node indices (-1 outside) and weights */
 int tmi_raster_get_bmap(tmi_raster r, int *cols, double *w);
+/* NodeMapMatrix (mesh.cc:2695-2853), the grid -> mesh map BI: per mesh
+ * node 2^dim grid points (-1: unused slot) and their bi/trilinear weights,
+ * renormalised over the points inside the mesh; built on first use. */
+int tmi_raster_get_nodemap(tmi_raster r, int *cols, double *w);
+/* Map_GridToMesh (raster.cc:212-224, Ax_cplx of BI) for ncols complex
+ * columns: gvec glen x ncols, mvec nlen x ncols (node-major).  With the
+ * basis grid equal to the intermediate grid this is also Map_BasisToMesh,
+ * and Map_SolToMesh after scattering the solution voxels into the grid. */
+int tmi_raster_map_grid_to_mesh(tmi_raster r, int ncols, const double *gvec, double *mvec);
 /* Map_MeshToGrid (raster.cc:188-199, Ax_cplx of B) for ncols complex
  * columns: mvec nlen x ncols (node-major), gvec glen x ncols */
 int tmi_raster_map_mesh_to_grid(tmi_raster r, int ncols, const double *mvec, double *gvec);

@@ -67,6 +67,7 @@ def lib():
             "om_jacobian_nodal": (None, [P, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
             "om_jacobian_grid_cw": (None, [P, I, D, D, I, C.c_int, C.c_int, D, D, I, I, P, C.c_int, P, P]),
             "om_data_gradient": (None, [P, I, D, D, I, C.c_int, D, D, D]),
+            "om_raster_nodemap": (None, [P, I, D, D, I, I, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -286,6 +287,13 @@ class Raster:
 
     def gsize(self):
         return self.bbmax - self.bbmin
+
+    def nodemap(self):
+        nc = 1 << self.dim
+        cols = np.zeros(self.mesh.n * nc, np.int32)
+        w = np.zeros(self.mesh.n * nc)
+        lib().om_raster_nodemap(self.mesh.h, self.gdim, self.bbmin, self.bbmax, self.elref, cols, w)
+        return cols.reshape(self.mesh.n, nc), w.reshape(self.mesh.n, nc)
 
 
 def image_gradient(gdim, gsize, im, mask):

@@ -1029,6 +1029,54 @@ void om_raster_elref(const om_mesh *m, const int *gdim, const double *bbmin, con
     }
 }
 
+/* NodeMapMatrix (mesh.cc:2695-2853): per mesh node the bi/trilinear
+ * weights of the grid points around it that lie inside the mesh (elref >= 0),
+ * corners x fastest, renormalised over the present ones when some are
+ * missing (equal weights if their sum is at most 1e-8); no entries for a
+ * node outside the bounding box.  cols/w: n x 2^dim, unused slots -1 / 0. */
+void om_raster_nodemap(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                       int *cols, double *w)
+{
+    const double EPS = 1e-8;
+    int dim = m->dim, nc = 1 << dim;
+    double dW[3];
+    for (int d = 0; d < dim; d++) dW[d] = (bbmax[d]-bbmin[d])/(double)(gdim[d]-1);
+    for (int i = 0; i < m->n; i++) {
+        int *oc = cols + (size_t)i*nc;
+        double *ow = w + (size_t)i*nc;
+        for (int k = 0; k < nc; k++) { oc[k] = -1; ow[k] = 0.0; }
+        int i0[3] = {0, 0, 0}, ok = 1;
+        double d0[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; d++) {
+            double p = m->x[(size_t)i*dim + d];
+            if (p >= bbmin[d] && p <= bbmax[d]) {
+                i0[d] = (int)((p-bbmin[d])/dW[d]);
+                if (i0[d] < 0) i0[d] = 0;
+                else if (i0[d] >= gdim[d]-1) i0[d] = gdim[d]-2;
+                d0[d] = (p - (i0[d]*dW[d] + bbmin[d]))/dW[d];
+                if (d0[d] < 0.0) d0[d] = 0.0;
+                else if (d0[d] > 1.0) d0[d] = 1.0;
+            } else { ok = 0; break; }
+        }
+        if (!ok) continue;
+        int cnt = 0;
+        for (int k = 0; k < nc; k++) {
+            int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
+            long pix = (i0[0]+cx) + (long)gdim[0]*((i0[1]+cy) + (dim > 2 ? (long)gdim[1]*(i0[2]+cz) : 0));
+            if (elref[pix] < 0) continue;
+            double v = (cx ? d0[0] : (1.0-d0[0])) * (cy ? d0[1] : (1.0-d0[1]));
+            if (dim > 2) v = v * (cz ? d0[2] : (1.0-d0[2]));
+            oc[cnt] = (int)pix; ow[cnt] = v; cnt++;
+        }
+        if (cnt > 0 && cnt < nc) {
+            double weight = 0.0;
+            for (int k = 0; k < cnt; k++) weight += ow[k];
+            if (weight > EPS) for (int k = 0; k < cnt; k++) ow[k] /= weight;
+            else for (int k = 0; k < cnt; k++) ow[k] = 1.0/(double)cnt;
+        }
+    }
+}
+
 /* GridMapMatrix (mesh.cc:2486-2554): row per grid point with the element's
  * global shape functions at the point (tet4.cc:302-313, tri3.cc:231-241);
  * cols/w: glen x nn, col -1 / weight 0 where the point is outside. */

@@ -227,3 +227,44 @@ def test_data_gradient_oracle_matches_jacobian_identity():
     assert np.max(np.abs(g - ref_g)) <= 1e-10 * scale
     # a sign or a block swap fails
     assert np.max(np.abs(g + ref_g)) > 1e-3 * scale
+
+
+def _case_mesh(case):
+    if case == "2d":
+        nodes, el = G.disk(10.0, 6)
+        return O.Mesh(nodes=nodes, elems=el), [17, 15]
+    nodes, el = G.cylinder(8.0, 10.0, 4, 5)
+    return O.Mesh(nodes=nodes, elems=el), [11, 10, 9]
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_nodemap_partition_and_linear_reproduction(case):
+    """NodeMapMatrix: every node inside the box gets weights summing to 1;
+    where all 2^dim surrounding grid points are inside the mesh the weights
+    are the bi/trilinear ones, which reproduce a linear field exactly."""
+    m, gd = _case_mesh(case)
+    R = O.Raster(m, gd)
+    cols, w = R.nodemap()
+    nc = 1 << m.dim
+    has = (cols >= 0).any(1)
+    assert has.all()                                   # the mesh lies in its own bounding box
+    assert np.allclose(w.sum(1), 1.0, rtol=0, atol=1e-13)
+    full = (cols >= 0).sum(1) == nc
+    assert full.mean() > 0.5
+    # grid point coordinates, x fastest
+    dW = (R.bbmax - R.bbmin) / (R.gdim - 1)
+    idx = np.arange(R.glen)
+    pos = []
+    rest = idx.copy()
+    for d in range(m.dim):
+        pos.append(R.bbmin[d] + (rest % R.gdim[d]) * dW[d])
+        rest //= R.gdim[d]
+    a = np.array([0.7, -1.3, 0.4][:m.dim])
+    gval = 2.0 + sum(a[d] * pos[d] for d in range(m.dim))
+    x = m.nodes()
+    f = np.where(cols >= 0, w * gval[np.maximum(cols, 0)], 0.0).sum(1)
+    exact = 2.0 + x @ a
+    assert np.max(np.abs(f[full] - exact[full])) <= 1e-11 * np.max(np.abs(exact))
+    # partial support still reproduces constants (renormalised weights)
+    fc = np.where(cols >= 0, w * 3.0, 0.0).sum(1)
+    assert np.allclose(fc, 3.0, rtol=1e-13)

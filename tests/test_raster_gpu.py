@@ -228,3 +228,22 @@ def test_gradient_end_to_end_2d():
         rel = np.linalg.norm(gg[blk] - go[blk]) / np.linalg.norm(go[blk])
         print("Gradient 2D raster block: relative L2 %.3g" % rel)
         assert rel <= 1e-6
+
+
+def test_nodemap_and_maps_into_the_mesh(pair):
+    """NodeMapMatrix built on the GPU against the oracle (same grid points,
+    weights to 1e-15) and the G/B/S -> M maps built from it."""
+    om, gm, orr, gr = pair
+    cols, w = gr._nodemap()
+    ocols, ow = orr.nodemap()
+    assert np.array_equal(cols, ocols)
+    assert np.max(np.abs(w - ow)) <= 1e-15
+    rng = np.random.default_rng(31)
+    g = rng.standard_normal((gr.GLen(), 2)) + 1j * rng.standard_normal((gr.GLen(), 2))
+    m = gr.GridToMesh(g)
+    ref = np.einsum("nk,nkc->nc", ow, np.where(ocols[:, :, None] >= 0, g[np.maximum(ocols, 0)], 0))
+    assert np.max(np.abs(m - ref)) <= 1e-14 * np.max(np.abs(ref))
+    gv = rng.standard_normal(gr.GLen())
+    assert np.allclose(gr.Map("G->M", gv), gr.GetMatrix("G->M") @ gv, rtol=1e-13, atol=1e-13)
+    s = rng.standard_normal(gr.SLen())
+    assert np.allclose(gr.Map("S->M", s), gr.Map("B->M", gr.Map("S->B", s)), rtol=0, atol=0)

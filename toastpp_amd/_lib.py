@@ -108,6 +108,8 @@ _SIGS = {
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
     "tmi_jacobian_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
     "tmi_raster_data_gradient": [P, C.c_int, D, D, D],
+    "tmi_raster_get_nodemap": [P, P, P],
+    "tmi_raster_map_grid_to_mesh": [P, C.c_int, D, D],
     "tmi_raster_data_gradient_device": [P, C.c_int, P, C.c_int, P, C.c_int, P],
     "tmi_jacobian_raster_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
     "tmi_raster_create": [P, I, P, PP],

@@ -248,6 +248,34 @@ int tmi_raster_get_bmap(tmi_raster r, int *cols, double *w)
     API_END
 }
 
+int tmi_raster_get_nodemap(tmi_raster r, int *cols, double *w)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    hipStream_t st = Context::get().stream;
+    Raster &R = r->r;
+    R.build_nodemap(st);
+    const size_t cnt = (size_t)R.mesh->n * (1 << R.dim);
+    if (cols) R.ncols.download(cols, cnt, st);
+    if (w) R.nw.download(w, cnt, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_map_grid_to_mesh(tmi_raster r, int ncols, const double *gvec, double *mvec)
+{
+    API_BEGIN
+    TMI_CHECK(r && gvec && mvec && ncols > 0, TMI_ERR_INVALID_ARG, "invalid argument");
+    hipStream_t st = Context::get().stream;
+    Raster &R = r->r;
+    DBuf<double> dg, dm((size_t)R.mesh->n * ncols * 2);
+    dg.upload(gvec, (size_t)R.glen * ncols * 2, st);
+    launch_grid_to_mesh(R, ncols, dg.p, dm.p, st);
+    dm.download(mvec, (size_t)R.mesh->n * ncols * 2, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
 int tmi_raster_map_mesh_to_grid(tmi_raster r, int ncols, const double *mvec, double *gvec)
 {
     API_BEGIN

@@ -19,9 +19,16 @@ struct Raster {
     DBuf<int> sol2basis;               // slen: grid points inside the mesh, ascending
     DBuf<double> fgv;                  // Jacobian scratch: fields on the grid
     DBuf<double> fgs;                  // Jacobian scratch: fields + gradients on solution voxels
+    DBuf<int> ncols;                   // nlen x 2^dim: grid points of NodeMapMatrix (-1 unused)
+    DBuf<double> nw;                   // nlen x 2^dim: their interpolation weights
+    int ncols_nodemap = 0;             // 2^dim once built
 
     void build(hipStream_t st);        // elref, B, sol2basis
+    void build_nodemap(hipStream_t st);   // NodeMapMatrix (on first use)
 };
+
+// Map_GridToMesh: ncols complex grid columns [glen][ncols] -> [nlen][ncols]
+void launch_grid_to_mesh(Raster &r, int ncols, const double *d_g, double *d_out, hipStream_t st);
 
 // the grid map B applied to ncols node-major complex columns:
 // out[g][c] = sum_j w[g][j] f[col[g][j]][c]   (f rows of ld entries)

@@ -134,6 +134,111 @@ __global__ void k_bmap(long glen, const int *__restrict__ elref, const int *__re
     }
 }
 
+// NodeMapMatrix (mesh.cc:2695-2853): for every mesh node the bi/trilinear
+// weights of the 2^dim grid points around it that lie inside the mesh, in
+// the reference's corner order (x fastest), renormalised when some of them
+// are outside (equal weights when their sum is below 1e-8); no entries for a
+// node outside the bounding box.  Rows of 2^dim slots, unused ones -1.
+template <int DIM>
+__global__ void k_nodemap(int n, const double *__restrict__ x, GridDesc gd, const int *__restrict__ elref,
+                          int *__restrict__ cols, double *__restrict__ w)
+{
+    constexpr int NC = 1 << DIM;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int *oc = cols + (long)i * NC;
+    double *ow = w + (long)i * NC;
+    for (int k = 0; k < NC; k++) { oc[k] = -1; ow[k] = 0.0; }
+    const double bmin[3] = {gd.bx, gd.by, gd.bz}, bmax[3] = {gd.tx, gd.ty, gd.tz}, dW[3] = {gd.hx, gd.hy, gd.hz};
+    const int g[3] = {gd.nx, gd.ny, gd.nz};
+    int i0[3] = {0, 0, 0};
+    double d0[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < DIM; d++) {
+        const double p = x[(long)i * DIM + d];
+        if (!(p >= bmin[d] && p <= bmax[d])) return;
+        i0[d] = (int)((p - bmin[d]) / dW[d]);
+        if (i0[d] < 0) i0[d] = 0;
+        else if (i0[d] >= g[d] - 1) i0[d] = g[d] - 2;
+        d0[d] = (p - (i0[d] * dW[d] + bmin[d])) / dW[d];
+        if (d0[d] < 0.0) d0[d] = 0.0;
+        else if (d0[d] > 1.0) d0[d] = 1.0;
+    }
+    int cnt = 0;
+    for (int k = 0; k < NC; k++) {
+        const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
+        const long pix = DIM == 3 ? (i0[0] + cx) + (long)g[0] * ((i0[1] + cy) + (long)g[1] * (i0[2] + cz))
+                                  : (i0[0] + cx) + (long)g[0] * (i0[1] + cy);
+        if (elref[pix] < 0) continue;
+        double wt = (cx ? d0[0] : (1.0 - d0[0])) * (cy ? d0[1] : (1.0 - d0[1]));
+        if (DIM == 3) wt = wt * (cz ? d0[2] : (1.0 - d0[2]));
+        oc[cnt] = (int)pix;
+        ow[cnt] = wt;
+        cnt++;
+    }
+    if (cnt > 0 && cnt < NC) {
+        double weight = 0.0;
+        for (int k = 0; k < cnt; k++) weight += ow[k];
+        if (weight > 1e-8)
+            for (int k = 0; k < cnt; k++) ow[k] /= weight;
+        else
+            for (int k = 0; k < cnt; k++) ow[k] = 1.0 / (double)cnt;
+    }
+}
+
+// Map_GridToMesh (raster.cc:212-224, Ax / Ax_cplx of NodeMapMatrix) for
+// ncols complex columns: out[i][c] = sum_k w[i][k] g[col[i][k]][c]
+template <int NC>
+__global__ void k_grid_to_mesh(int n, int ncols, const int *__restrict__ cols, const double *__restrict__ w,
+                               const double2 *__restrict__ g, double2 *__restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * ncols) return;
+    const long i = t / ncols;
+    const int c = (int)(t % ncols);
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < NC; k++) {
+        const int pix = cols[i * NC + k];
+        if (pix < 0) continue;
+        const double2 v = g[(long)pix * ncols + c];
+        re += w[i * NC + k] * v.x;
+        im += w[i * NC + k] * v.y;
+    }
+    out[i * ncols + c] = make_double2(re, im);
+}
+
+void Raster::build_nodemap(hipStream_t st)
+{
+    if (ncols_nodemap) return;
+    Mesh &m = *mesh;
+    const int NC = 1 << dim;
+    GridDesc gd{gdim[0], gdim[1], gdim[2], bbmin[0], bbmin[1], bbmin[2], dW[0], dW[1], dW[2],
+                bbmax[0], bbmax[1], bbmax[2]};
+    ncols.alloc((size_t)m.n * NC);
+    nw.alloc((size_t)m.n * NC);
+    const dim3 g((m.n + 255) / 256);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_nodemap<3>, g, dim3(256), 0, st, m.n, m.nodes.p, gd, elref.p, ncols.p, nw.p);
+    else
+        hipLaunchKernelGGL(k_nodemap<2>, g, dim3(256), 0, st, m.n, m.nodes.p, gd, elref.p, ncols.p, nw.p);
+    TMI_LAUNCH_CHECK();
+    TMI_HIP(hipStreamSynchronize(st));
+    ncols_nodemap = NC;
+}
+
+void launch_grid_to_mesh(Raster &r, int ncols, const double *d_g, double *d_out, hipStream_t st)
+{
+    r.build_nodemap(st);
+    const long cnt = (long)r.mesh->n * ncols;
+    const dim3 g((unsigned)((cnt + 255) / 256));
+    if (r.dim == 3)
+        hipLaunchKernelGGL(k_grid_to_mesh<8>, g, dim3(256), 0, st, r.mesh->n, ncols, r.ncols.p, r.nw.p,
+                           (const double2 *)d_g, (double2 *)d_out);
+    else
+        hipLaunchKernelGGL(k_grid_to_mesh<4>, g, dim3(256), 0, st, r.mesh->n, ncols, r.ncols.p, r.nw.p,
+                           (const double2 *)d_g, (double2 *)d_out);
+    TMI_LAUNCH_CHECK();
+}
+
 void Raster::build(hipStream_t st)
 {
     Mesh &m = *mesh;

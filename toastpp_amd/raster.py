@@ -5,8 +5,8 @@ basis grid equal to the intermediate grid (raster.cc:18-100,
 raster_px.cc:10-54) on the GPU: the element holding each grid point, the
 mesh -> grid map B and the solution mask.  Bases in map strings: M mesh
 nodes, G intermediate grid, B basis grid (= G here), S solution voxels
-(basis voxels inside the mesh).  Maps into the mesh basis need the
-reference's NodeMapMatrix, which this build does not provide.
+(basis voxels inside the mesh).  Maps into the mesh basis use the
+reference's NodeMapMatrix (mesh.cc:2695-2853), built on the GPU on first use.
 """
 import ctypes as C
 
@@ -118,12 +118,40 @@ class Raster:
         g = self.MeshToGrid(mvec)
         return g[self.Sol2Basis()]
 
+    def _nodemap(self):
+        nc = 1 << len(self.grd)
+        n = self.mesh.NodeCount()
+        cols = np.zeros(n * nc, np.int32)
+        w = np.zeros(n * nc)
+        L.call("tmi_raster_get_nodemap", self.handle_ptr(), cols.ctypes.data_as(C.c_void_p),
+               w.ctypes.data_as(C.c_void_p))
+        return cols.reshape(n, nc), w.reshape(n, nc)
+
+    def GridToMesh(self, gvec):
+        """Map_GridToMesh (raster.cc:212-224) of one or more grid columns
+        (glen or glen x k), complex or real."""
+        gvec = np.asarray(gvec)
+        real = not np.iscomplexobj(gvec)
+        one = gvec.ndim == 1
+        g = np.ascontiguousarray(gvec.reshape(gvec.shape[0], -1), np.complex128)
+        out = np.zeros((self.mesh.NodeCount(), g.shape[1]), np.complex128)
+        L.call("tmi_raster_map_grid_to_mesh", self.handle_ptr(), g.shape[1], g.view(np.float64),
+               out.view(np.float64))
+        if real:
+            out = out.real.copy()
+        return out[:, 0] if one else out
+
     def Map(self, mapstr, srcvec):
         """Map a field between bases: mapstr 'S->T' with S, T in M, G, B, S."""
         src, tgt = mapstr.replace(" ", "").split("->")
         v = np.asarray(srcvec).ravel()
         if tgt == "M":
-            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "maps into the mesh basis (NodeMapMatrix) are not provided")
+            if src == "M":
+                return v.copy()
+            if src in ("G", "B"):
+                return self.GridToMesh(v)
+            if src == "S":
+                return self.GridToMesh(self.Map("S->B", v))
         if src == "M":
             g = self.MeshToGrid(v)
             return g[self.Sol2Basis()] if tgt == "S" else g
@@ -147,6 +175,11 @@ class Raster:
             keep = cols >= 0
             rows = np.repeat(np.arange(self._glen), cols.shape[1]).reshape(cols.shape)
             return sparse.csr_matrix((w[keep], (rows[keep], cols[keep])), shape=(self._glen, sizes["M"]))
+        if src in ("G", "B") and tgt == "M":
+            cols, w = self._nodemap()
+            keep = cols >= 0
+            rows = np.repeat(np.arange(sizes["M"]), cols.shape[1]).reshape(cols.shape)
+            return sparse.csr_matrix((w[keep], (rows[keep], cols[keep])), shape=(sizes["M"], self._glen))
         if src in ("G", "B") and tgt == "S":
             s2b = self.Sol2Basis()
             return sparse.csr_matrix((np.ones(self._slen), (np.arange(self._slen), s2b)),
