@@ -14,6 +14,9 @@
 
 namespace tmi {
 
+// row mask of a distinct column in a block of the row-block SpMM (one bit per row)
+typedef unsigned char rb_mask_t;
+
 // A failure carried as a C++ exception inside the library and converted to a
 // status code at the C-ABI boundary (never escapes extern "C").
 struct Error : std::runtime_error {

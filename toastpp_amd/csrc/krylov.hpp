@@ -44,12 +44,22 @@ struct RowBlocks {
 constexpr int DMA_GRID = 256;
 
 // rows per block of the row-block SpMM (one bit each in the column masks)
-constexpr int RB_ROWS = 8;
+#ifndef TMI_RB_ROWS
+#define TMI_RB_ROWS 8
+#endif
+constexpr int RB_ROWS = TMI_RB_ROWS;
+static_assert(RB_ROWS <= 8 * (int)sizeof(rb_mask_t), "row-block masks hold one bit per row");
+// neighbour rows gathered per batch in the row-block SpMM (MI355X, 250k-node
+// tetrahedral mesh: 8 -> 0.165 ms per launch, 16 -> 0.182, 32 -> 0.23, 4 -> 0.175;
+// 4, 6 or 8 rows per block within 1 % of each other at 8)
+#ifndef TMI_RB_BATCH
+#define TMI_RB_BATCH 8
+#endif
 struct RowUnions {
     int nb = 0;                          // blocks (0: not used)
     const int *ofs = nullptr;            // nb+1 bounds into col / mask
     const int *col = nullptr;            // distinct columns of each block, ascending
-    const unsigned char *mask = nullptr; // rows of the block holding each column
+    const rb_mask_t *mask = nullptr;     // rows of the block holding each column
     const void *val = nullptr;           // coefficients in (column, row) order per block
     const void *src = nullptr;           // the operator (CSR values) they were taken from
 };

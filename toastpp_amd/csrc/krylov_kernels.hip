@@ -445,12 +445,12 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
 template <class V, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
 __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int *__restrict__ rowptr,
                                                            const int *__restrict__ bofs, const int *__restrict__ ucol,
-                                                           const unsigned char *__restrict__ umask,
+                                                           const rb_mask_t *__restrict__ umask,
                                                            const V *__restrict__ rbval, const V *__restrict__ X,
                                                            V *__restrict__ Y, const V *__restrict__ W, int want_yy,
                                                            const int *__restrict__ active, double *__restrict__ partial)
 {
-    constexpr int B = 16;
+    constexpr int B = TMI_RB_BATCH;
     const int lane = threadIdx.x & 63, c = lane;
     const bool act = active[c] != 0;
     const int G = gridDim.x, nw = blockDim.x >> 6;

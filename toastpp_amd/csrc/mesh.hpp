@@ -61,7 +61,7 @@ struct Mesh {
         // the kernel's order (distinct column ascending, then row)
         int rb_nb = 0;
         DBuf<int> rb_ofs, rb_col, rb_slot;
-        DBuf<unsigned char> rb_mask;
+        DBuf<rb_mask_t> rb_mask;
     } order[2];
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }

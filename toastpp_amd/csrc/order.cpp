@@ -119,7 +119,7 @@ static bool row_blocks(int n, const std::vector<int> &rp, const std::vector<int>
 // coefficients in (column, row) order.  The block's coefficients occupy the
 // block's own CSR range, so the kernel finds them at rowptr[first row].
 static void row_block_unions(int n, const std::vector<int> &rp, const std::vector<int> &ci, std::vector<int> &ofs,
-                             std::vector<int> &col, std::vector<unsigned char> &mask, std::vector<int> &slot)
+                             std::vector<int> &col, std::vector<rb_mask_t> &mask, std::vector<int> &slot)
 {
     const int nb = (n + RB_ROWS - 1) / RB_ROWS;
     ofs.assign(1, 0);
@@ -139,7 +139,7 @@ static void row_block_unions(int n, const std::vector<int> &rp, const std::vecto
                 col.push_back(e[i][0]);
                 mask.push_back(0);
             }
-            mask.back() |= (unsigned char)(1u << e[i][1]);
+            mask.back() |= (rb_mask_t)(1u << e[i][1]);
             slot[at++] = e[i][2];
         }
         ofs.push_back((int)col.size());
@@ -198,7 +198,7 @@ const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
     }
     {
         std::vector<int> ofs, col, rslot;
-        std::vector<unsigned char> mask;
+        std::vector<rb_mask_t> mask;
         row_block_unions(n, prp, pci, ofs, col, mask, rslot);
         order.rb_nb = (int)ofs.size() - 1;
         order.rb_ofs.upload(ofs.data(), ofs.size(), st);
