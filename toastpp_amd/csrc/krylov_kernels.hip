@@ -88,6 +88,14 @@ __device__ inline float shfl_add(float a, int off) { return a + __shfl_xor(a, of
 __device__ inline double shfl_add(double a, int off) { return a + __shfl_xor(a, off); }
 
 constexpr int NXCD = 8;
+// neighbour-row gathers in flight per wave and batch in the 64-column and
+// narrow-tile SpMM kernels (build-time tuning parameters)
+#ifndef TMI_SPMM64_BATCH
+#define TMI_SPMM64_BATCH 16
+#endif
+#ifndef TMI_NK_BATCH
+#define TMI_NK_BATCH 16
+#endif
 // waves per workgroup of the 64-column SpMM (512 threads: 3-4 % faster than
 // 256 on MI355X at the same 8 waves per SIMD); the other kernels use 4
 constexpr int SPMM64_WAVES = 8;
@@ -145,7 +153,10 @@ __device__ inline void block_reduce2(double d1, double d2, double *partial, int 
 // Gathers are issued in chunks of CH nonzeros (indices and values first,
 // then all CH neighbour loads, then the multiply-adds) so that a wave keeps
 // CH independent memory requests in flight instead of one per nonzero.
-constexpr int CH = 16;
+#ifndef TMI_DOT_CH
+#define TMI_DOT_CH 16
+#endif
+constexpr int CH = TMI_DOT_CH;
 
 template <class V, int K, bool PRE>
 __device__ inline V row_gather(int kb, int ke, int c, const int *__restrict__ colidx, const V *__restrict__ val,
@@ -236,7 +247,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *
                                                 V *__restrict__ Y, const V *__restrict__ W, int want_yy,
                                                 const int *__restrict__ active, double *__restrict__ partial, int R)
 {
-    constexpr int B = 16;
+    constexpr int B = TMI_SPMM64_BATCH;
     const int lane = threadIdx.x & 63;
     const int c = lane;
     const bool act = active[c] != 0;
@@ -354,7 +365,7 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
                                                  V *__restrict__ Y, const V *__restrict__ W, int want_yy,
                                                  const int *__restrict__ active, double *__restrict__ partial)
 {
-    constexpr int S = 64 / K, RS = nk_rows(K), RW = S * RS, B = K < 16 ? K : 16;
+    constexpr int S = 64 / K, RS = nk_rows(K), RW = S * RS, B = K < TMI_NK_BATCH ? K : TMI_NK_BATCH;
     static_assert(K >= 2 && K <= 32 && RW <= 63, "narrow SpMM: 2 <= K <= 32");
     const int lane = threadIdx.x & 63, c = lane % K, s = lane / K, sb = s * K;
     const bool act = active[c] != 0;
