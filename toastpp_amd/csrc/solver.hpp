@@ -17,7 +17,10 @@ struct Solver {
     int mode = TMI_MODE_MIXED_BLOCK;
     int tile = 0;
     double inner_tol = 1e-5;
-    int reliable_every = 16;   // inner iterations between FP64 residual replacements (mixed mode)
+#ifndef TMI_RELIABLE_EVERY
+#define TMI_RELIABLE_EVERY 16
+#endif
+    int reliable_every = TMI_RELIABLE_EVERY;   // inner iterations between FP64 residual replacements (mixed mode)
     int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
     int spmm_variant = 3;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
