@@ -351,6 +351,28 @@ int tmi_server_wait(tmi_server s, long long id, double timeout_s, tmi_colstat *s
 int tmi_server_counts(tmi_server s, int *running, int *queued, double *hbm_bytes);
 double tmi_server_clock(tmi_server s);      /* seconds since creation */
 
+/* ------------------------------------------------------------- data files */
+/* Nodal image (NIM) files: "NIM" header, "ImageSize = n", "EndHeader", then
+ * blocks "Image k" of n values.  Host only, no device needed. */
+#define TMI_NIM_FWDFEM    0   /* values "%.10g " (WriteNIM, fwdfem.cc:316-323) */
+#define TMI_NIM_TOASTMOD  1   /* values "%.12e " (toast_write_nim, toastmodule.cc:942-972) */
+/* toast.ReadNim / toast_read_nim (toastmodule.cc:905-940, mesh.py:206-209,
+ * 338-339) and ReadNim (fwdfem.cc:327-347): image idx (0-based; idx < 0 or
+ * past the end: the last image).  size / nimg (may be NULL) return the
+ * image length and the number of images; img (may be NULL) receives the
+ * image. */
+int tmi_read_nim(const char *path, int idx, int *size, int *nimg, double *img);
+/* toast.WriteNim (toastmodule.cc:942-972) for nimg == 1, OpenNIM + WriteNIM
+ * (fwdfem.cc:306-323) for a series: imgs is nimg x size, image k written
+ * as "Image k". */
+int tmi_write_nim(const char *path, const char *meshname, int size, int nimg, const double *imgs, int style);
+/* Vector text "[v0 v1 ...]" of the .fem data files: TVector operator>>
+ * (vector_imp.hpp:1370-1420).  Call with v == NULL for the length; else *n
+ * is the buffer length on input and the vector length on output. */
+int tmi_read_vector(const char *path, int *n, double *v);
+/* WriteData (fwdfem.cc:457-462): "[" values "%.10e" separated by blanks "]" */
+int tmi_write_vector(const char *path, int n, const double *v);
+
 #ifdef __cplusplus
 }
 #endif

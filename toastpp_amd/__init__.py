@@ -7,10 +7,13 @@ C-ABI in include/toastpp_amd.h); this package is its Python surface:
   FwdSolver  -- libstoast TFwdSolver<T> (src/libstoast/fwdsolver.h)
   Solution   -- optical -> FEM coefficients (cmua, ckappa, c2a)
   Raster     -- toast.Raster with a Raster_Pixel basis (bdim == gdim)
+  ReadNim, WriteNim -- nodal image files (toast.ReadNim / WriteNim)
+The command-line forward model (fwdfem, fwdfem_cw of src/supertoast) is
+built as bin/fwdfem and bin/fwdfem_cw on the same library.
 """
 from ._lib import TmiError, NotConverged  # noqa: F401
 from .fwdsolver import FwdSolver, Solution  # noqa: F401
-from .mesh import Mesh  # noqa: F401
+from .mesh import Mesh, ReadNim, WriteNim  # noqa: F401
 from .raster import Raster  # noqa: F401
 
-__all__ = ["Mesh", "FwdSolver", "Solution", "Raster", "TmiError", "NotConverged"]
+__all__ = ["Mesh", "FwdSolver", "Solution", "Raster", "ReadNim", "WriteNim", "TmiError", "NotConverged"]

@@ -126,7 +126,14 @@ _SIGS = {
     "tmi_server_submit": [P, P, C.c_int, P, P, C.c_int, C.POINTER(C.c_longlong)],
     "tmi_server_wait": [P, C.c_longlong, C.c_double, P, DP, IP],
     "tmi_server_counts": [P, IP, IP, DP],
+    "tmi_read_nim": [C.c_char_p, C.c_int, IP, IP, P],
+    "tmi_write_nim": [C.c_char_p, C.c_char_p, C.c_int, C.c_int, D, C.c_int],
+    "tmi_read_vector": [C.c_char_p, IP, P],
+    "tmi_write_vector": [C.c_char_p, C.c_int, D],
 }
+
+TMI_NIM_FWDFEM = 0
+TMI_NIM_TOASTMOD = 1
 
 _lib = None
 

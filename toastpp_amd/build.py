@@ -21,7 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-I" + os.path
 # files whose FP64 arithmetic must follow the reference operation by operation
 NO_CONTRACT = {"mesh_kernels.hip", "mesh_io.cpp", "jacobian_kernels.hip", "raster.cpp", "raster_kernels.hip"}
 SOURCES = ["mesh_kernels.hip", "krylov_kernels.hip", "jacobian_kernels.hip", "raster_kernels.hip", "mesh_io.cpp",
-           "order.cpp", "solver.cpp", "jacobian_api.cpp", "serve.cpp", "api.cpp"]
+           "order.cpp", "solver.cpp", "jacobian_api.cpp", "serve.cpp", "data_io.cpp", "api.cpp"]
 
 
 def _compile(src):
@@ -36,6 +36,24 @@ def _compile(src):
     return obj
 
 
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+# command-line programs on the C-ABI (host code only): name -> extra flags
+TOOLS = {"fwdfem": [], "fwdfem_cw": ["-DTMI_FWDFEM_CW"]}
+
+
+def _build_tool(name, flags):
+    out = os.path.join(BIN, name)
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall"] + flags + [os.path.join(CSRC, "fwdfem.cpp"), "-o", out + ".tmp",
+                                                         "-L" + HERE, "-l:libtoastpp_amd.so",
+                                                         "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("tool build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -47,8 +65,10 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    tools = [_build_tool(name, flags) for name, flags in TOOLS.items()]
     if verbose:
-        print("built", LIB)
+        print("built", LIB, *tools)
     return LIB
 
 

@@ -113,6 +113,14 @@ class Mesh:
         L.call("tmi_mesh_sparse_structure", self.handle_ptr(), L.ptr(rp), L.ptr(ci), C.byref(nnz))
         return rp, ci
 
+    def ReadNim(self, name, idx=-1):
+        """Nodal image `idx` of a NIM file (mesh.py:206-209): its length must
+        be the node count."""
+        nim = ReadNim(name, idx)
+        if nim.size != self.NodeCount():
+            raise ValueError("NIM image has %d values, the mesh %d nodes" % (nim.size, self.NodeCount()))
+        return nim
+
     # ---------------------------------------------------------------- QM
     def ReadQM(self, name):
         L.call("tmi_mesh_read_qm", self.handle_ptr(), name.encode())
@@ -325,3 +333,21 @@ class Mesh:
         aphi = fws.CalcFields(mrows)
         proj = fws.ProjectAll(mrows, dphi, scale="lin")
         return jacobian(self, raster, dphi.T, aphi.T, proj)
+
+
+def ReadNim(nimname, idx=-1):
+    """Image `idx` (0-based; -1 or past the end: the last) of a NIM file
+    (toast.ReadNim, mesh.py:338-339; toastmodule.cc:905-940)."""
+    size, nimg = C.c_int(), C.c_int()
+    name = str(nimname).encode()
+    L.call("tmi_read_nim", name, int(idx), C.byref(size), C.byref(nimg), None)
+    img = np.zeros(size.value)
+    L.call("tmi_read_nim", name, int(idx), None, None, L.ptr(img))
+    return img
+
+
+def WriteNim(nimname, meshname, nim):
+    """One-image NIM file, values "%.12e" (toast.WriteNim, mesh.py:341-342;
+    toastmodule.cc:942-972)."""
+    v = np.ascontiguousarray(np.asarray(nim, np.float64).ravel())
+    L.call("tmi_write_nim", str(nimname).encode(), str(meshname).encode(), v.size, 1, v, L.TMI_NIM_TOASTMOD)
