@@ -275,13 +275,36 @@ int tmi_jacobian_cw(tmi_mesh m, int nq, int nm, const double *dphi, const double
  * bb: NULL for the mesh bounding box (Mesh::BoundingBox, pad 0), else
  * [bbmin(dim), bbmax(dim)].  Built on the GPU. */
 int tmi_raster_create(tmi_mesh m, const int *gdim, const double *bb, tmi_raster *out);
+/* Raster_Pixel(bdim, gdim) (raster_px.cc:10-54; toast.Raster(mesh, grd,
+ * intgrd) with grd = bdim, intgrd = gdim): fields live on the intermediate
+ * grid gdim, the solution on the basis grid bdim.  With bdim != gdim the
+ * grid -> basis map G (Grid2LinPixMatrix, mesh.cc:3138-3288) is built, the
+ * solution voxels are the basis voxels with support (SubsampleLinPixel of
+ * the inside mask, raster.cc:54-73), and the Jacobians and the data
+ * gradient map their grid PMDFs through G (Map_GridToSol).  Built on the
+ * GPU. */
+int tmi_raster_create_basis(tmi_mesh m, const int *bdim, const int *gdim, const double *bb, tmi_raster *out);
 int tmi_raster_free(tmi_raster r);
-/* glen = prod(gdim) (= blen), slen = grid points inside the mesh */
+/* glen = prod(gdim), slen = solution voxels (grid points inside the mesh
+ * when bdim == gdim) */
 int tmi_raster_dims(tmi_raster r, int *glen, int *slen);
+/* blen = prod(bdim); has_g = 1 if bdim != gdim (either may be NULL) */
+int tmi_raster_basis_len(tmi_raster r, int *blen, int *has_g);
+/* G = Grid2LinPixMatrix (blen x glen CSR, columns grid indices) and
+ * GI = LinPix2GridMatrix (mesh.cc:3540-3718, glen x blen CSR); rowptr
+ * always, colidx / val when not NULL, nnz the entry count.
+ * TMI_ERR_UNSUPPORTED when bdim == gdim (the reference has no G then). */
+int tmi_raster_get_gmap(tmi_raster r, int *rowptr, int *colidx, double *val, int *nnz);
+int tmi_raster_get_gimap(tmi_raster r, int *rowptr, int *colidx, double *val, int *nnz);
+/* Map_GridToBasis / Map_BasisToGrid (raster_px.cc:102-130, Ax_cplx of G /
+ * GI) for ncols complex columns: gvec glen x ncols, bvec blen x ncols */
+int tmi_raster_map_grid_to_basis(tmi_raster r, int ncols, const double *gvec, double *bvec);
+int tmi_raster_map_basis_to_grid(tmi_raster r, int ncols, const double *bvec, double *gvec);
 /* bounding box: bbmin(dim), bbmax(dim) */
 int tmi_raster_bbox(tmi_raster r, double *bbmin, double *bbmax);
 /* element per grid point (-1 outside), glen; sol2basis (Raster::sol2basis,
- * raster.cc:64-77), slen; either may be NULL */
+ * raster.cc:64-77: basis index of each solution voxel), slen; either may
+ * be NULL */
 int tmi_raster_get_elref(tmi_raster r, int *elref, int *sol2basis);
 /* rows of B: glen x (dim+1) node indices (-1 outside) and weights */
 int tmi_raster_get_bmap(tmi_raster r, int *cols, double *w);
@@ -292,7 +315,8 @@ int tmi_raster_get_nodemap(tmi_raster r, int *cols, double *w);
 /* Map_GridToMesh (raster.cc:212-224, Ax_cplx of BI) for ncols complex
  * columns: gvec glen x ncols, mvec nlen x ncols (node-major).  With the
  * basis grid equal to the intermediate grid this is also Map_BasisToMesh,
- * and Map_SolToMesh after scattering the solution voxels into the grid. */
+ * and Map_SolToMesh after scattering the solution voxels into the grid;
+ * else Map_BasisToMesh is tmi_raster_map_basis_to_grid followed by this. */
 int tmi_raster_map_grid_to_mesh(tmi_raster r, int ncols, const double *gvec, double *mvec);
 /* Map_MeshToGrid (raster.cc:188-199, Ax_cplx of B) for ncols complex
  * columns: mvec nlen x ncols (node-major), gvec glen x ncols */

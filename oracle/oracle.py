@@ -68,6 +68,13 @@ def lib():
             "om_jacobian_grid_cw": (None, [P, I, D, D, I, C.c_int, C.c_int, D, D, I, I, P, C.c_int, P, P]),
             "om_data_gradient": (None, [P, I, D, D, I, C.c_int, D, D, D]),
             "om_raster_nodemap": (None, [P, I, D, D, I, I, D]),
+            "om_subsample_linpixel": (None, [C.c_int, I, I, P, D, D]),
+            "om_grid2linpix": (None, [C.c_int, I, I, P, I, P, P]),
+            "om_linpix2grid": (None, [C.c_int, I, I, P, I, P, P]),
+            "om_raster_slen": (C.c_long, [C.c_int, I, I, D, D, I, P]),
+            "om_jacobian_grid_b": (None, [P, I, I, D, D, I, C.c_int, C.c_int, D, D, I, I, D, C.c_int, D]),
+            "om_jacobian_grid_cw_b": (None, [P, I, I, D, D, I, C.c_int, C.c_int, D, D, I, I, P, C.c_int, P, P]),
+            "om_data_gradient_b": (None, [P, I, I, D, D, I, C.c_int, D, D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -258,14 +265,18 @@ def qvec_point(mesh, pts, neumann=True):
 
 
 class Raster:
-    """Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,
-    raster_px.cc:10-54) restated: bounding box of the mesh, element of every
-    grid point, the grid map B and the solution mask."""
+    """Raster_Pixel (raster.cc:18-100, raster_px.cc:10-54) restated:
+    bounding box of the mesh, element of every grid point, the grid map B,
+    and the solution voxels: grid points inside the mesh when the basis grid
+    bdim equals the intermediate grid gdim, else basis voxels whose linear
+    pixel support holds a grid point inside the mesh (SubsampleLinPixel of
+    the inside mask), with G = Grid2LinPixMatrix mapping grid to basis."""
 
-    def __init__(self, mesh, gdim, bb=None):
+    def __init__(self, mesh, gdim, bb=None, bdim=None):
         self.mesh = mesh
         self.dim = mesh.dim
         self.gdim = np.ascontiguousarray(gdim, np.int32)
+        self.bdim = self.gdim.copy() if bdim is None else np.ascontiguousarray(bdim, np.int32)
         if bb is None:
             lo, hi = np.zeros(self.dim), np.zeros(self.dim)
             lib().om_bbox(mesh.h, lo, hi)
@@ -273,10 +284,38 @@ class Raster:
             lo, hi = np.ascontiguousarray(bb[0], np.float64), np.ascontiguousarray(bb[1], np.float64)
         self.bbmin, self.bbmax = lo, hi
         self.glen = int(np.prod(self.gdim))
+        self.blen = int(np.prod(self.bdim))
         self.elref = np.zeros(self.glen, np.int32)
         lib().om_raster_elref(mesh.h, self.gdim, lo, hi, self.elref)
-        self.sol2basis = np.nonzero(self.elref >= 0)[0].astype(np.int32)
-        self.slen = len(self.sol2basis)
+        n = lib().om_raster_slen(self.dim, self.gdim, self.bdim, lo, hi, self.elref, None)
+        s2b = np.zeros(max(n, 1), np.int64)
+        lib().om_raster_slen(self.dim, self.gdim, self.bdim, lo, hi, self.elref, s2b.ctypes.data_as(C.c_void_p))
+        self.sol2basis = s2b[:n].astype(np.int32)
+        self.slen = n
+
+    def _csr(self, fn, nrows, ncols, masked=True):
+        rp = np.zeros(nrows + 1, np.int32)
+        el = self.elref.ctypes.data_as(C.c_void_p) if masked else None
+        fn(self.dim, self.gdim, self.bdim, el, rp, None, None)
+        ci = np.zeros(max(rp[-1], 1), np.int32)
+        val = np.zeros(max(rp[-1], 1))
+        fn(self.dim, self.gdim, self.bdim, el, rp, ci.ctypes.data_as(C.c_void_p), val.ctypes.data_as(C.c_void_p))
+        return rp, ci[:rp[-1]], val[:rp[-1]]
+
+    def gmap(self):
+        """G = Grid2LinPixMatrix (blen x glen) as (rowptr, colidx, val)."""
+        return self._csr(lib().om_grid2linpix, self.blen, self.glen)
+
+    def gimap(self):
+        """GI = LinPix2GridMatrix (glen x blen) as (rowptr, colidx, val)."""
+        return self._csr(lib().om_linpix2grid, self.glen, self.blen)
+
+    def subsample(self, gf, masked=False):
+        """SubsampleLinPixel of a grid image (elref mask optional)."""
+        bf = np.zeros(self.blen)
+        el = self.elref.ctypes.data_as(C.c_void_p) if masked else None
+        lib().om_subsample_linpixel(self.dim, self.gdim, self.bdim, el, _c(gf, np.float64), bf)
+        return bf
 
     def bmap(self):
         nn = self.dim + 1
@@ -310,9 +349,9 @@ def jacobian_grid(raster, dphi, aphi, lptr, lidx, proj, logscale=True):
     nq, nm = dphi.shape[0], aphi.shape[0]
     nqm = int(lptr[-1])
     J = np.zeros((2 * nqm, 2 * raster.slen))
-    lib().om_jacobian_grid(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, nq, nm,
-                           dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32), _c(lidx, np.int32),
-                           _cplx(proj), 1 if logscale else 0, J)
+    lib().om_jacobian_grid_b(raster.mesh.h, raster.gdim, raster.bdim, raster.bbmin, raster.bbmax, raster.elref, nq,
+                             nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
+                             _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, J)
     return J
 
 
@@ -324,8 +363,8 @@ def jacobian_grid_cw(raster, dphi, aphi, lptr, lidx, proj=None, logscale=True):
     nqm = int(lptr[-1])
     Jm = np.zeros((nqm, raster.slen)); Jk = np.zeros((nqm, raster.slen))
     pj = None if proj is None else np.ascontiguousarray(proj, np.float64)
-    lib().om_jacobian_grid_cw(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, nq, nm, dphi, aphi,
-                              _c(lptr, np.int32), _c(lidx, np.int32),
+    lib().om_jacobian_grid_cw_b(raster.mesh.h, raster.gdim, raster.bdim, raster.bbmin, raster.bbmax, raster.elref, nq,
+                                nm, dphi, aphi, _c(lptr, np.int32), _c(lidx, np.int32),
                               None if pj is None else pj.ctypes.data_as(C.c_void_p), 1 if logscale else 0,
                               Jm.ctypes.data_as(C.c_void_p), Jk.ctypes.data_as(C.c_void_p))
     return Jm, Jk
@@ -359,8 +398,8 @@ def data_gradient(raster, dphi, aphi, grad=None):
     adjoint fields; returns grad (2 slen) decremented as the reference does."""
     dphi = np.ascontiguousarray(dphi, np.complex128); aphi = np.ascontiguousarray(aphi, np.complex128)
     g = np.zeros(2 * raster.slen) if grad is None else np.array(grad, np.float64)
-    lib().om_data_gradient(raster.mesh.h, raster.gdim, raster.bbmin, raster.bbmax, raster.elref, dphi.shape[0],
-                           dphi.view(np.float64), aphi.view(np.float64), g)
+    lib().om_data_gradient_b(raster.mesh.h, raster.gdim, raster.bdim, raster.bbmin, raster.bbmax, raster.elref,
+                             dphi.shape[0], dphi.view(np.float64), aphi.view(np.float64), g)
     return g
 
 

@@ -1151,33 +1151,258 @@ static void map_mesh_to_grid(long glen, int nn, const int *cols, const double *w
     }
 }
 
-/* GenerateJacobian_grid (jacobian.cc:293-373) for Raster_Pixel with
- * bdim == gdim: PMDF_mua = -d a, PMDF_kap = -sum_d grad_d d grad_d a
- * (jacobian.cc:57-79), divided by the projection for log data, gathered
- * into the solution basis (grid points inside the mesh, ascending), scaled
- * by prod gsize/(bdim-1).  dphi: nq x n, aphi: nm x n complex;
- * J: (2 nqm) x (2 slen). */
-void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
-                      int nq, int nm, const double *dphi, const double *aphi, const int *lptr, const int *lidx,
-                      const double *proj, int logscale, double *J)
+/* ---------------------------------------------------------------------
+ * Linear pixel basis coarser than the intermediate grid (Raster_Pixel with
+ * bdim != gdim, raster_px.cc:10-35).  Basis voxel ix (of bx per axis, at
+ * x = ix/(bx-1) in unit box coordinates) covers the grid points xx of
+ * [floor(x0 (gx-1)), ceil(x1 (gx-1))], x0/x1 its neighbours (0 and 1 at the
+ * ends), with d = |xx/(gx-1) - x| (bx-1) < 1 per axis; the tent weight is
+ * prod (1 - d).  Loops, operation order and the elref mask follow
+ * SubsampleLinPixel (mesh.cc:3047-3136), Grid2LinPixMatrix (:3138-3288,
+ * rows normalised by their weight sum) and LinPix2GridMatrix (:3540-3718,
+ * unnormalised weights, one row per grid point). */
+
+typedef struct { double x, nb, dg; int g0, g1; } lp_axis;
+
+static lp_axis lp_axis_of(int b, int nbv, int ng)
+{
+    lp_axis a;
+    double db = 1.0/(nbv-1.0);
+    a.nb = nbv-1.0;
+    a.dg = 1.0/(ng-1.0);
+    a.x = (double)b*db;
+    double x0 = (b ? (b-1)*db : 0.0);
+    double x1 = (b < nbv-1 ? (b+1)*db : 1.0);
+    a.g0 = (int)floor(x0*(ng-1));
+    a.g1 = (int)ceil(x1*(ng-1));
+    return a;
+}
+
+/* grid points and tent weights of basis voxel b in the reference's loop
+ * order; returns their count (gi, t may be NULL to count only) */
+static int lp_row(int dim, const int *gdim, const int *bdim, const int *elref, long b, long *gi, double *t)
+{
+    int gx = gdim[0], gy = gdim[1];
+    int bx = bdim[0], by = bdim[1];
+    int ix = (int)(b % bx), iy = (int)((b / bx) % by), iz = dim > 2 ? (int)(b / ((long)bx*by)) : 0;
+    lp_axis ax = lp_axis_of(ix, bx, gx), ay = lp_axis_of(iy, by, gy);
+    int cnt = 0;
+    if (dim == 2) {
+        for (int yy = ay.g0; yy <= ay.g1; yy++) {
+            double dy = fabs(yy*ay.dg - ay.x)*ay.nb;
+            if (dy >= 1.0) continue;
+            double idy = 1.0-dy;
+            for (int xx = ax.g0; xx <= ax.g1; xx++) {
+                long gidx = xx + (long)yy*gx;
+                if (elref && elref[gidx] < 0) continue;
+                double dx = fabs(xx*ax.dg - ax.x)*ax.nb;
+                if (dx >= 1.0) continue;
+                if (gi) { gi[cnt] = gidx; t[cnt] = (1.0-dx)*idy; }
+                cnt++;
+            }
+        }
+    } else {
+        lp_axis az = lp_axis_of(iz, bdim[2], gdim[2]);
+        for (int zz = az.g0; zz <= az.g1; zz++) {
+            double dz = fabs(zz*az.dg - az.x)*az.nb;
+            if (dz >= 1.0) continue;
+            double idz = 1.0-dz;
+            for (int yy = ay.g0; yy <= ay.g1; yy++) {
+                double dy = fabs(yy*ay.dg - ay.x)*ay.nb;
+                if (dy >= 1.0) continue;
+                double idyz = (1.0-dy)*idz;
+                long gidx = ax.g0 + ((long)yy + (long)zz*gy)*gx;
+                for (int xx = ax.g0; xx <= ax.g1; xx++, gidx++) {
+                    if (elref && elref[gidx] < 0) continue;
+                    double dx = fabs(xx*ax.dg - ax.x)*ax.nb;
+                    if (dx >= 1.0) continue;
+                    if (gi) { gi[cnt] = gidx; t[cnt] = (1.0-dx)*idyz; }
+                    cnt++;
+                }
+            }
+        }
+    }
+    return cnt;
+}
+
+static long lp_len(int dim, const int *d) { long n = 1; for (int i = 0; i < dim; i++) n *= d[i]; return n; }
+
+static int lp_maxrow(int dim, const int *gdim, const int *bdim)
+{
+    int m = 1;
+    for (int d = 0; d < dim; d++) m *= 2*((gdim[d]-1)/(bdim[d]-1) + 2) + 1;
+    return m;
+}
+
+/* SubsampleLinPixel (mesh.cc:3047-3136): bf[b] = sum t gf / sum t over the
+ * grid points of basis voxel b (0 where there are none); elref may be NULL */
+void om_subsample_linpixel(int dim, const int *gdim, const int *bdim, const int *elref, const double *gf, double *bf)
+{
+    long blen = lp_len(dim, bdim);
+    int mx = lp_maxrow(dim, gdim, bdim);
+    long *gi = (long*)xcalloc(mx, sizeof(long));
+    double *t = (double*)xcalloc(mx, sizeof(double));
+    for (long b = 0; b < blen; b++) {
+        int c = lp_row(dim, gdim, bdim, elref, b, gi, t);
+        double sum = 0.0, wgt = 0.0;
+        for (int k = 0; k < c; k++) { wgt += t[k]; sum += t[k]*gf[gi[k]]; }
+        bf[b] = wgt ? sum/wgt : 0.0;
+    }
+    free(gi); free(t);
+}
+
+/* Grid2LinPixMatrix (mesh.cc:3138-3288): blen x glen CSR, row b the
+ * normalised tent weights of basis voxel b.  With ci == NULL only rp
+ * (blen+1) is filled. */
+void om_grid2linpix(int dim, const int *gdim, const int *bdim, const int *elref, int *rp, int *ci, double *val)
+{
+    long blen = lp_len(dim, bdim);
+    int mx = lp_maxrow(dim, gdim, bdim);
+    long *gi = (long*)xcalloc(mx, sizeof(long));
+    double *t = (double*)xcalloc(mx, sizeof(double));
+    rp[0] = 0;
+    for (long b = 0; b < blen; b++) {
+        int c = lp_row(dim, gdim, bdim, elref, b, gi, t);
+        rp[b+1] = rp[b] + c;
+        if (!ci) continue;
+        double wgt = 0.0;
+        for (int k = 0; k < c; k++) { ci[rp[b]+k] = (int)gi[k]; val[rp[b]+k] = t[k]; wgt += t[k]; }
+        for (int k = 0; k < c; k++) val[rp[b]+k] = wgt ? val[rp[b]+k]/wgt : 0.0;
+    }
+    free(gi); free(t);
+}
+
+/* LinPix2GridMatrix (mesh.cc:3540-3718): glen x blen CSR, row g the
+ * (unnormalised) tent weights of the basis voxels covering grid point g,
+ * basis voxels ascending.  With ci == NULL only rp (glen+1) is filled. */
+void om_linpix2grid(int dim, const int *gdim, const int *bdim, const int *elref, int *rp, int *ci, double *val)
+{
+    long blen = lp_len(dim, bdim), glen = lp_len(dim, gdim);
+    int mx = lp_maxrow(dim, gdim, bdim);
+    long *gi = (long*)xcalloc(mx, sizeof(long));
+    double *t = (double*)xcalloc(mx, sizeof(double));
+    int *len = (int*)xcalloc(glen, sizeof(int));
+    for (long b = 0; b < blen; b++) {
+        int c = lp_row(dim, gdim, bdim, elref, b, gi, t);
+        for (int k = 0; k < c; k++) len[gi[k]]++;
+    }
+    rp[0] = 0;
+    for (long g = 0; g < glen; g++) rp[g+1] = rp[g] + len[g];
+    if (ci) {
+        for (long g = 0; g < glen; g++) len[g] = 0;
+        for (long b = 0; b < blen; b++) {
+            int c = lp_row(dim, gdim, bdim, elref, b, gi, t);
+            for (int k = 0; k < c; k++) {
+                long g = gi[k];
+                int idx = rp[g] + len[g]++;
+                ci[idx] = (int)b;
+                val[idx] = t[k];
+            }
+        }
+    }
+    free(gi); free(t); free(len);
+}
+
+/* Map_GridToSol of Raster_Pixel (raster_px.cc:102-114,148-151): with
+ * bdim == gdim a gather of the grid points inside the mesh (G = NULL); else
+ * Grid2LinPixMatrix then the basis voxels with support (SubsampleLinPixel
+ * of the inside mask without elref, raster.cc:54-73).  sz: prod
+ * gsize/(bdim-1), the voxel size of the Jacobians (jacobian.cc:368-372). */
+typedef struct {
+    int has_g;
+    long slen, gslen;
+    long *s2b;            /* slen solution voxels (grid points, or basis voxels with G) */
+    long *g2s;            /* gslen grid points inside the mesh */
+    int *rp, *ci;
+    double *val, sz;
+} om_gmap;
+
+static void gmap_init(om_gmap *G, int dim, const int *gdim, const int *bdim, const double *bbmin,
+                      const double *bbmax, const int *elref)
+{
+    memset(G, 0, sizeof *G);
+    long glen = lp_len(dim, gdim);
+    G->sz = 1.0;
+    for (int d = 0; d < dim; d++) {
+        G->sz *= (bbmax[d]-bbmin[d])/(bdim[d]-1.0);
+        if (bdim[d] != gdim[d]) G->has_g = 1;
+    }
+    for (long i = 0; i < glen; i++) G->gslen += elref[i] >= 0;
+    G->g2s = (long*)xcalloc((size_t)G->gslen, sizeof(long));
+    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) G->g2s[s++] = i;
+    if (!G->has_g) {
+        G->slen = G->gslen;
+        G->s2b = (long*)xcalloc((size_t)G->slen, sizeof(long));
+        memcpy(G->s2b, G->g2s, sizeof(long)*G->slen);
+        return;
+    }
+    long blen = lp_len(dim, bdim);
+    double *mask = (double*)xcalloc(glen, 8), *bs = (double*)xcalloc(blen, 8);
+    for (long i = 0; i < glen; i++) if (elref[i] >= 0) mask[i] = 1.0;
+    om_subsample_linpixel(dim, gdim, bdim, NULL, mask, bs);
+    for (long b = 0; b < blen; b++) G->slen += bs[b] > 0.0;
+    G->s2b = (long*)xcalloc((size_t)G->slen, sizeof(long));
+    for (long b = 0, s = 0; b < blen; b++) if (bs[b] > 0.0) G->s2b[s++] = b;
+    G->rp = (int*)xcalloc(blen+1, sizeof(int));
+    om_grid2linpix(dim, gdim, bdim, elref, G->rp, NULL, NULL);
+    G->ci = (int*)xcalloc(G->rp[blen], sizeof(int));
+    G->val = (double*)xcalloc(G->rp[blen], sizeof(double));
+    om_grid2linpix(dim, gdim, bdim, elref, G->rp, G->ci, G->val);
+    free(mask); free(bs);
+}
+
+static void gmap_free(om_gmap *G)
+{
+    free(G->s2b); free(G->g2s); free(G->rp); free(G->ci); free(G->val);
+}
+
+/* solution voxel s of a complex grid image f (2 glen doubles) */
+static void gmap_sol(const om_gmap *G, const double *f, long s, double *re, double *im)
+{
+    if (!G->has_g) { long g = G->s2b[s]; *re = f[2*g]; *im = f[2*g+1]; return; }
+    long b = G->s2b[s];
+    double br = 0.0, bi = 0.0;
+    for (int k = G->rp[b]; k < G->rp[b+1]; k++) { br += f[2*(size_t)G->ci[k]]*G->val[k]; bi += f[2*(size_t)G->ci[k]+1]*G->val[k]; }
+    *re = br; *im = bi;
+}
+
+long om_raster_slen(int dim, const int *gdim, const int *bdim, const double *bbmin, const double *bbmax,
+                    const int *elref, long *s2b)
+{
+    om_gmap G;
+    gmap_init(&G, dim, gdim, bdim, bbmin, bbmax, elref);
+    long n = G.slen;
+    if (s2b) memcpy(s2b, G.s2b, sizeof(long)*n);
+    gmap_free(&G);
+    return n;
+}
+
+/* GenerateJacobian_grid (jacobian.cc:293-373) for Raster_Pixel: PMDF_mua =
+ * -d a, PMDF_kap = -sum_d grad_d d grad_d a (jacobian.cc:57-79) on the
+ * grid, divided by the projection for log data, mapped into the solution
+ * basis (Map_GridToSol), scaled by prod gsize/(bdim-1).  dphi: nq x n,
+ * aphi: nm x n complex; J: (2 nqm) x (2 slen). */
+void om_jacobian_grid_b(const om_mesh *m, const int *gdim, const int *bdim, const double *bbmin,
+                        const double *bbmax, const int *elref, int nq, int nm, const double *dphi,
+                        const double *aphi, const int *lptr, const int *lidx, const double *proj, int logscale,
+                        double *J)
 {
     int dim = m->dim, nn = m->nn;
-    long glen = 1;
-    for (int d = 0; d < dim; d++) glen *= gdim[d];
-    double gsize[3], sz = 1.0;
-    for (int d = 0; d < dim; d++) { gsize[d] = bbmax[d]-bbmin[d]; sz *= gsize[d]/(gdim[d]-1.0); }
+    long glen = lp_len(dim, gdim);
+    double gsize[3];
+    for (int d = 0; d < dim; d++) gsize[d] = bbmax[d]-bbmin[d];
+    om_gmap G;
+    gmap_init(&G, dim, gdim, bdim, bbmin, bbmax, elref);
     int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
     double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
     om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
-    long slen = 0;
-    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
-    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
-    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    long slen = G.slen;
     int nqm = lptr[nq];
     double *ca = (double*)xcalloc(2*(size_t)glen*nm, 8);
     for (int j = 0; j < nm; j++) map_mesh_to_grid(glen, nn, cols, w, aphi + 2*(size_t)j*m->n, ca + 2*(size_t)j*glen);
     double *cd = (double*)xcalloc(2*(size_t)glen, 8);
     double *gd = (double*)xcalloc(2*(size_t)glen*dim, 8), *ga = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    double *pmg = (double*)xcalloc(2*(size_t)glen, 8), *pkg = (double*)xcalloc(2*(size_t)glen, 8);
     size_t rowlen = 2*(size_t)slen;
     for (int q = 0; q < nq; q++) {
         map_mesh_to_grid(glen, nn, cols, w, dphi + 2*(size_t)q*m->n, cd);
@@ -1186,9 +1411,10 @@ void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, co
             const double *a = ca + 2*(size_t)lidx[k]*glen;
             om_image_gradient(dim, gdim, gsize, a, elref, ga);
             double pr = proj[2*k], pi = proj[2*k+1], den = pr*pr + pi*pi;
-            double *R0 = J + (size_t)k*rowlen, *R1 = J + (size_t)(k+nqm)*rowlen;
-            for (long s = 0; s < slen; s++) {
-                long g = s2b[s];
+            /* PMDFs at the grid points inside the mesh (the only ones the
+             * basis map reads) */
+            for (long i = 0; i < G.gslen; i++) {
+                long g = G.g2s[i];
                 double pm[2], pk[2] = {0.0, 0.0}, t[2];
                 cmul(cd + 2*g, a + 2*g, t);
                 pm[0] = -t[0]; pm[1] = -t[1];
@@ -1201,37 +1427,51 @@ void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, co
                     double br = (pk[0]*pr + pk[1]*pi)/den, bi = (pk[1]*pr - pk[0]*pi)/den;
                     pm[0] = ar; pm[1] = ai; pk[0] = br; pk[1] = bi;
                 }
-                R0[s] = pm[0]*sz; R1[s] = pm[1]*sz; R0[s+slen] = pk[0]*sz; R1[s+slen] = pk[1]*sz;
+                pmg[2*g] = pm[0]; pmg[2*g+1] = pm[1]; pkg[2*g] = pk[0]; pkg[2*g+1] = pk[1];
+            }
+            double *R0 = J + (size_t)k*rowlen, *R1 = J + (size_t)(k+nqm)*rowlen;
+            for (long s = 0; s < slen; s++) {
+                double mr, mi, kr, ki;
+                gmap_sol(&G, pmg, s, &mr, &mi);
+                gmap_sol(&G, pkg, s, &kr, &ki);
+                R0[s] = mr*G.sz; R1[s] = mi*G.sz; R0[s+slen] = kr*G.sz; R1[s+slen] = ki*G.sz;
             }
         }
     }
-    free(cols); free(w); free(s2b); free(ca); free(cd); free(gd); free(ga);
+    free(cols); free(w); free(ca); free(cd); free(gd); free(ga); free(pmg); free(pkg);
+    gmap_free(&G);
 }
 
-/* GenerateJacobian_cw_grid (jacobian.cc:610-721) for Raster_Pixel with
- * bdim == gdim, followed by the log row scaling of GenerateJacobian_cw
- * (:160-186): real fields dphi (nq x n), aphi (nm x n); PMDF_mua = -d a,
- * PMDF_kap = -sum_d grad_d d grad_d a with the direct gradients masked by
- * the element reference and the adjoint gradients unmasked (ImageGradient
- * with mask 0, as the reference calls it there), gathered into the solution
- * basis, times prod gsize/(bdim-1), then each row times 1/proj for log data.
+void om_jacobian_grid(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                      int nq, int nm, const double *dphi, const double *aphi, const int *lptr, const int *lidx,
+                      const double *proj, int logscale, double *J)
+{
+    om_jacobian_grid_b(m, gdim, gdim, bbmin, bbmax, elref, nq, nm, dphi, aphi, lptr, lidx, proj, logscale, J);
+}
+
+/* GenerateJacobian_cw_grid (jacobian.cc:610-721) for Raster_Pixel,
+ * followed by the log row scaling of GenerateJacobian_cw (:160-186): real
+ * fields dphi (nq x n), aphi (nm x n); PMDF_mua = -d a, PMDF_kap = -sum_d
+ * grad_d d grad_d a with the direct gradients masked by the element
+ * reference and the adjoint gradients unmasked (ImageGradient with mask 0,
+ * as the reference calls it there), mapped into the solution basis, times
+ * prod gsize/(bdim-1), then each row times 1/proj for log data.
  * Jmua, Jkap: nqm x slen (either may be NULL). */
-void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax,
-                         const int *elref, int nq, int nm, const double *dphi, const double *aphi, const int *lptr,
-                         const int *lidx, const double *proj, int logscale, double *Jmua, double *Jkap)
+void om_jacobian_grid_cw_b(const om_mesh *m, const int *gdim, const int *bdim, const double *bbmin,
+                           const double *bbmax, const int *elref, int nq, int nm, const double *dphi,
+                           const double *aphi, const int *lptr, const int *lidx, const double *proj, int logscale,
+                           double *Jmua, double *Jkap)
 {
     int dim = m->dim, nn = m->nn;
-    long glen = 1;
-    for (int d = 0; d < dim; d++) glen *= gdim[d];
-    double gsize[3], sz = 1.0;
-    for (int d = 0; d < dim; d++) { gsize[d] = bbmax[d]-bbmin[d]; sz *= gsize[d]/(gdim[d]-1.0); }
+    long glen = lp_len(dim, gdim);
+    double gsize[3];
+    for (int d = 0; d < dim; d++) gsize[d] = bbmax[d]-bbmin[d];
+    om_gmap G;
+    gmap_init(&G, dim, gdim, bdim, bbmin, bbmax, elref);
     int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
     double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
     om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
-    long slen = 0;
-    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
-    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
-    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    long slen = G.slen;
     int *nomask = (int*)xcalloc((size_t)glen, sizeof(int));   /* all inside */
     /* the image helpers work on complex arrays: real parts carry the fields */
     double *f = (double*)xcalloc(2*(size_t)m->n, 8);
@@ -1242,6 +1482,7 @@ void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin,
         om_image_gradient(dim, gdim, gsize, ca + 2*(size_t)j*glen, nomask, ga + 2*(size_t)j*glen*dim);
     }
     double *cd = (double*)xcalloc(2*(size_t)glen, 8), *gd = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    double *pmg = (double*)xcalloc(2*(size_t)glen, 8), *pkg = (double*)xcalloc(2*(size_t)glen, 8);
     for (int q = 0; q < nq; q++) {
         for (int i = 0; i < m->n; i++) { f[2*i] = dphi[(size_t)q*m->n + i]; f[2*i+1] = 0.0; }
         map_mesh_to_grid(glen, nn, cols, w, f, cd);
@@ -1250,60 +1491,89 @@ void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin,
             const int j = lidx[k];
             const double *a = ca + 2*(size_t)j*glen, *gaj = ga + 2*(size_t)j*glen*dim;
             const double ip = logscale ? 1.0 / proj[k] : 1.0;
-            for (long s = 0; s < slen; s++) {
-                long g = s2b[s];
-                double pm = -cd[2*g] * a[2*g], pk = 0.0;
+            for (long i = 0; i < G.gslen; i++) {
+                long g = G.g2s[i];
+                double pk = 0.0;
                 for (int d = 0; d < dim; d++) pk -= gd[2*((size_t)d*glen + g)] * gaj[2*((size_t)d*glen + g)];
-                pm *= sz; pk *= sz;
+                pmg[2*g] = -cd[2*g] * a[2*g];
+                pkg[2*g] = pk;
+            }
+            for (long s = 0; s < slen; s++) {
+                double pm, pk, im;
+                gmap_sol(&G, pmg, s, &pm, &im);
+                gmap_sol(&G, pkg, s, &pk, &im);
+                pm *= G.sz; pk *= G.sz;
                 if (logscale) { pm *= ip; pk *= ip; }
                 if (Jmua) Jmua[(size_t)k*slen + s] = pm;
                 if (Jkap) Jkap[(size_t)k*slen + s] = pk;
             }
         }
     }
-    free(cols); free(w); free(s2b); free(nomask); free(f); free(ca); free(ga); free(cd); free(gd);
+    free(cols); free(w); free(nomask); free(f); free(ca); free(ga); free(cd); free(gd); free(pmg); free(pkg);
+    gmap_free(&G);
+}
+
+void om_jacobian_grid_cw(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax,
+                         const int *elref, int nq, int nm, const double *dphi, const double *aphi, const int *lptr,
+                         const int *lidx, const double *proj, int logscale, double *Jmua, double *Jkap)
+{
+    om_jacobian_grid_cw_b(m, gdim, gdim, bbmin, bbmax, elref, nq, nm, dphi, aphi, lptr, lidx, proj, logscale,
+                          Jmua, Jkap);
 }
 
 /* AddDataGradient's raster part (toastmodule.cc:1767-1843): for each
  * source q in order, the direct field dphi_q and its adjoint field aphi_q
  * (nq x n complex each) mapped to the grid, gradients with the element
- * reference as mask; grad (2 slen, cmua then ckappa) -= Re(d a) and
- * -= Re(sum_j grad_j d grad_j a) at the solution voxels. */
-void om_data_gradient(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
-                      int nq, const double *dphi, const double *aphi, double *grad)
+ * reference as mask; grad (2 slen, cmua then ckappa) -= Re(Map_GridToSol(d a))
+ * and -= Re(Map_GridToSol(sum_j grad_j d grad_j a)). */
+void om_data_gradient_b(const om_mesh *m, const int *gdim, const int *bdim, const double *bbmin,
+                        const double *bbmax, const int *elref, int nq, const double *dphi, const double *aphi,
+                        double *grad)
 {
     int dim = m->dim, nn = m->nn;
-    long glen = 1;
-    for (int d = 0; d < dim; d++) glen *= gdim[d];
+    long glen = lp_len(dim, gdim);
     double gsize[3];
     for (int d = 0; d < dim; d++) gsize[d] = bbmax[d]-bbmin[d];
+    om_gmap G;
+    gmap_init(&G, dim, gdim, bdim, bbmin, bbmax, elref);
     int *cols = (int*)xcalloc((size_t)glen*nn, sizeof(int));
     double *w = (double*)xcalloc((size_t)glen*nn, sizeof(double));
     om_raster_bmap(m, gdim, bbmin, bbmax, elref, cols, w);
-    long slen = 0;
-    for (long i = 0; i < glen; i++) slen += elref[i] >= 0;
-    long *s2b = (long*)xcalloc((size_t)slen, sizeof(long));
-    for (long i = 0, s = 0; i < glen; i++) if (elref[i] >= 0) s2b[s++] = i;
+    long slen = G.slen;
     double *cd = (double*)xcalloc(2*(size_t)glen, 8), *ca = (double*)xcalloc(2*(size_t)glen, 8);
     double *gd = (double*)xcalloc(2*(size_t)glen*dim, 8), *ga = (double*)xcalloc(2*(size_t)glen*dim, 8);
+    double *pm = (double*)xcalloc(2*(size_t)glen, 8), *pk = (double*)xcalloc(2*(size_t)glen, 8);
     for (int q = 0; q < nq; q++) {
         map_mesh_to_grid(glen, nn, cols, w, dphi + 2*(size_t)q*m->n, cd);
         om_image_gradient(dim, gdim, gsize, cd, elref, gd);
         map_mesh_to_grid(glen, nn, cols, w, aphi + 2*(size_t)q*m->n, ca);
         om_image_gradient(dim, gdim, gsize, ca, elref, ga);
-        for (long s = 0; s < slen; s++) {
-            long g = s2b[s];
+        for (long i = 0; i < G.gslen; i++) {
+            long g = G.g2s[i];
             double t[2], gk[2] = {0.0, 0.0};
-            cmul(cd + 2*g, ca + 2*g, t);
-            grad[s] -= t[0];
+            cmul(cd + 2*g, ca + 2*g, pm + 2*g);
             for (int j = 0; j < dim; j++) {
                 cmul(gd + 2*((size_t)j*glen + g), ga + 2*((size_t)j*glen + g), t);
                 gk[0] += t[0]; gk[1] += t[1];
             }
-            grad[slen + s] -= gk[0];
+            pk[2*g] = gk[0]; pk[2*g+1] = gk[1];
+        }
+        for (long s = 0; s < slen; s++) {
+            double re, im;
+            gmap_sol(&G, pm, s, &re, &im);
+            grad[s] -= re;
+            gmap_sol(&G, pk, s, &re, &im);
+            grad[slen + s] -= re;
         }
     }
-    free(cols); free(w); free(s2b); free(cd); free(ca); free(gd); free(ga);
+    free(cols); free(w); free(cd); free(ca); free(gd); free(ga); free(pm); free(pk);
+    gmap_free(&G);
+}
+
+void om_data_gradient(const om_mesh *m, const int *gdim, const double *bbmin, const double *bbmax, const int *elref,
+                      int nq, const double *dphi, const double *aphi, double *grad)
+{
+    om_data_gradient_b(m, gdim, gdim, bbmin, bbmax, elref, nq, dphi, aphi, grad);
 }
 
 /* QVec_Point (source.cc:157-196): the element holding p (first in

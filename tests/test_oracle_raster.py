@@ -268,3 +268,148 @@ def test_nodemap_partition_and_linear_reproduction(case):
     # partial support still reproduces constants (renormalised weights)
     fc = np.where(cols >= 0, w * 3.0, 0.0).sum(1)
     assert np.allclose(fc, 3.0, rtol=1e-13)
+
+
+# ------------------------------------------------ basis grid coarser than the grid
+
+def _tent_dense(R, masked=True):
+    """Brute force of the linear pixel tents: w[b, g] = prod_d max(0, 1 -
+    |u_g - u_b| (bdim_d - 1)) in unit-box coordinates, zero at grid points
+    outside the mesh (elref < 0) when masked."""
+    W = np.ones((R.blen, R.glen))
+    for d in range(R.dim):
+        sb = int(np.prod(R.bdim[:d])) if d else 1
+        sg = int(np.prod(R.gdim[:d])) if d else 1
+        ub = (np.arange(R.blen) // sb % R.bdim[d]) / (R.bdim[d] - 1.0)
+        ug = (np.arange(R.glen) // sg % R.gdim[d]) / (R.gdim[d] - 1.0)
+        W *= np.maximum(0.0, 1.0 - np.abs(ug[None, :] - ub[:, None]) * (R.bdim[d] - 1.0))
+    if masked:
+        W[:, R.elref < 0] = 0.0
+    return W
+
+
+def _dense(rp, ci, val, shape):
+    A = np.zeros(shape)
+    for r in range(shape[0]):
+        A[r, ci[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+    return A
+
+
+def _coarse_case(case, aligned=False):
+    """aligned: every basis point is a grid point ((g-1) a multiple of (b-1))"""
+    if case == "2d":
+        nodes, el = G.disk(10.0, 6)
+        return O.Mesh(nodes=nodes, elems=el), [17, 15], ([9, 8] if aligned else [9, 6])
+    nodes, el = G.cylinder(8.0, 10.0, 4, 5)
+    return O.Mesh(nodes=nodes, elems=el), [13, 11, 9], ([5, 6, 5] if aligned else [5, 6, 4])
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_linpix_maps_brute_force(case):
+    """Grid2LinPixMatrix / LinPix2GridMatrix / SubsampleLinPixel against the
+    brute-force tents: G rows are the masked tents normalised to sum 1, GI
+    the unnormalised masked tents transposed, and the solution voxels are
+    exactly the basis voxels whose tent holds a grid point inside the mesh."""
+    m, gd, bd = _coarse_case(case)
+    R = O.Raster(m, gd, bdim=bd)
+    W = _tent_dense(R)
+    Gd = _dense(*R.gmap(), (R.blen, R.glen))
+    rs = W.sum(1)
+    ref = np.divide(W, rs[:, None], out=np.zeros_like(W), where=rs[:, None] > 0)
+    assert np.max(np.abs(Gd - ref)) <= 1e-14
+    # the structures differ only where a grid point sits on a neighbouring
+    # basis point (d = 1 exactly, a weight of rounding size either side)
+    diff = (Gd != 0) != (ref != 0)
+    assert np.all(np.maximum(np.abs(Gd), np.abs(ref))[diff] <= 1e-14)
+    GI = _dense(*R.gimap(), (R.glen, R.blen))
+    assert np.max(np.abs(GI - W.T)) <= 1e-14
+    # support: SubsampleLinPixel of the inside mask without elref
+    bs = R.subsample((R.elref >= 0).astype(float), masked=False)
+    assert np.array_equal(R.sol2basis, np.nonzero(bs > 0)[0])
+    assert np.array_equal(R.sol2basis, np.nonzero(np.diff(R.gmap()[0]) > 0)[0])
+    assert set(np.nonzero(rs > 1e-12)[0]) <= set(R.sol2basis)
+    # masked subsampling is G applied to the image
+    img = np.random.default_rng(1).standard_normal(R.glen)
+    assert np.allclose(R.subsample(img, masked=True), Gd @ img, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_linpix_maps_linear_reproduction(case):
+    """GI interpolates a linear field given at the basis points exactly at
+    every grid point inside the mesh (the tents of a cell sum to 1); G
+    averages a linear grid field to its value at interior basis points whose
+    whole tent lies inside the mesh (weights symmetric about the basis
+    point, which holds when basis points are grid points)."""
+    m, gd, bd = _coarse_case(case, aligned=True)
+    R = O.Raster(m, gd, bdim=bd)
+    c = np.array([0.7, -1.3, 0.4])[:R.dim]
+
+    def lin(dims, npts):
+        out = np.full(npts, 0.25)
+        for d in range(R.dim):
+            s = int(np.prod(dims[:d])) if d else 1
+            out += c[d] * (np.arange(npts) // s % dims[d]) / (dims[d] - 1.0)
+        return out
+
+    fb, fg = lin(R.bdim, R.blen), lin(R.gdim, R.glen)
+    GI = _dense(*R.gimap(), (R.glen, R.blen))
+    inside = R.elref >= 0
+    assert np.allclose(GI[inside].sum(1), 1.0, atol=1e-13)
+    assert np.allclose(GI[inside] @ fb, fg[inside], atol=1e-13)
+    assert np.all(GI[~inside] == 0)
+    Gd = _dense(*R.gmap(), (R.blen, R.glen))
+    W = _tent_dense(R, masked=False)
+    full = np.array([np.all(inside[W[b] > 0]) for b in range(R.blen)])
+    interior = np.ones(R.blen, bool)
+    for d in range(R.dim):
+        s = int(np.prod(R.bdim[:d])) if d else 1
+        i = np.arange(R.blen) // s % R.bdim[d]
+        interior &= (i > 0) & (i < R.bdim[d] - 1)
+    sel = full & interior
+    assert sel.sum() >= 2
+    assert np.allclose(Gd[sel] @ fg, fb[sel], atol=1e-13)
+
+
+def test_grid_jacobian_basis_map_identity():
+    """Jacobians and the data gradient in a coarser basis against the
+    (pinned) grid-basis ones: every column of the basis version is the G row
+    of its voxel applied to the grid version, with the voxel size of the
+    basis (prod gsize/(bdim-1)) in place of the grid's."""
+    nodes, el = G.disk(25.0, 12)
+    m = O.Mesh(nodes=nodes, elems=el)
+    n = m.n
+    Q, M = G.ring(3, 25.0), G.ring(3, 25.0, offset=0.5)
+    lp = np.array([0, 2, 4, 6], np.int32)
+    li = np.array([1, 2, 2, 0, 0, 1], np.int32)
+    mua, mus, ref = np.full(n, 0.01), np.full(n, 1.0), np.full(n, 1.4)
+    d, a, mv = _fields(m, Q, M, mua, mus, ref, 100.0)
+    proj = O.project(d, mv.astype(complex), lp, li, logscale=False)
+    Rg = O.Raster(m, [41, 37])
+    Rb = O.Raster(m, [41, 37], bdim=[15, 12])
+    assert Rb.slen < Rg.slen
+    Gd = _dense(*Rb.gmap(), (Rb.blen, Rb.glen))
+    # grid columns -> basis solution columns
+    P = Gd[Rb.sol2basis][:, Rg.sol2basis]
+    szg = np.prod((Rg.bbmax - Rg.bbmin) / (Rg.gdim - 1.0))
+    szb = np.prod((Rb.bbmax - Rb.bbmin) / (Rb.bdim - 1.0))
+    Jg = O.jacobian_grid(Rg, d, a, lp, li, proj, logscale=True)
+    Jb = O.jacobian_grid(Rb, d, a, lp, li, proj, logscale=True)
+    ng, nb = Rg.slen, Rb.slen
+    for blk in range(2):
+        ref_b = (Jg[:, blk * ng:(blk + 1) * ng] / szg) @ P.T * szb
+        got = Jb[:, blk * nb:(blk + 1) * nb]
+        assert np.max(np.abs(got - ref_b)) <= 1e-12 * np.abs(ref_b).max()
+    # CW on the real parts of the fields
+    dr, ar = d.real.copy(), a.real.copy()
+    pr = O.project(dr.astype(complex), mv.astype(complex), lp, li, logscale=False).real
+    Jgm, Jgk = O.jacobian_grid_cw(Rg, dr, ar, lp, li, pr, logscale=True)
+    Jbm, Jbk = O.jacobian_grid_cw(Rb, dr, ar, lp, li, pr, logscale=True)
+    for Jgx, Jbx in ((Jgm, Jbm), (Jgk, Jbk)):
+        ref_b = (Jgx / szg) @ P.T * szb
+        assert np.max(np.abs(Jbx - ref_b)) <= 1e-12 * np.abs(ref_b).max()
+    # data gradient (no voxel size)
+    gg = O.data_gradient(Rg, d, a[:3])
+    gb = O.data_gradient(Rb, d, a[:3])
+    for blk in range(2):
+        ref_b = P @ gg[blk * ng:(blk + 1) * ng]
+        assert np.max(np.abs(gb[blk * nb:(blk + 1) * nb] - ref_b)) <= 1e-12 * np.abs(ref_b).max()

@@ -126,6 +126,12 @@ _SIGS = {
     "tmi_server_submit": [P, P, C.c_int, P, P, C.c_int, C.POINTER(C.c_longlong)],
     "tmi_server_wait": [P, C.c_longlong, C.c_double, P, DP, IP],
     "tmi_server_counts": [P, IP, IP, DP],
+    "tmi_raster_create_basis": [P, I, I, P, PP],
+    "tmi_raster_basis_len": [P, IP, IP],
+    "tmi_raster_get_gmap": [P, P, P, P, IP],
+    "tmi_raster_get_gimap": [P, P, P, P, IP],
+    "tmi_raster_map_grid_to_basis": [P, C.c_int, D, D],
+    "tmi_raster_map_basis_to_grid": [P, C.c_int, D, D],
     "tmi_read_nim": [C.c_char_p, C.c_int, IP, IP, P],
     "tmi_write_nim": [C.c_char_p, C.c_char_p, C.c_int, C.c_int, D, C.c_int],
     "tmi_read_vector": [C.c_char_p, IP, P],

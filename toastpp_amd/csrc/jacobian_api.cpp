@@ -1,5 +1,6 @@
 // Jacobian entry points of the C-ABI: GenerateJacobian for the nodal basis
 // (jacobian.cc:109-126 with RASTER_NDBASIS) and the Raster_Pixel basis.
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -154,10 +155,9 @@ int tmi_jacobian_cw(tmi_mesh h, int nq, int nm, const double *dphi, const double
     API_END
 }
 
-int tmi_raster_create(tmi_mesh h, const int *gdim, const double *bb, tmi_raster *out)
+static void raster_create(tmi_mesh h, const int *bdim, const int *gdim, const double *bb, tmi_raster *out)
 {
-    API_BEGIN
-    TMI_CHECK(h && gdim && out, TMI_ERR_INVALID_ARG, "NULL argument");
+    TMI_CHECK(h && bdim && gdim && out, TMI_ERR_INVALID_ARG, "NULL argument");
     Mesh &m = h->m;
     hipStream_t st = Context::get().stream;
     tmi_raster r = new tmi_raster_s;
@@ -166,11 +166,17 @@ int tmi_raster_create(tmi_mesh h, const int *gdim, const double *bb, tmi_raster 
         R.mesh = &m;
         R.dim = m.dim;
         R.glen = 1;
+        R.blen = 1;
         for (int d = 0; d < m.dim; d++) {
-            TMI_CHECK(gdim[d] >= 2, TMI_ERR_INVALID_ARG, "grid dimension %d must be >= 2", gdim[d]);
+            TMI_CHECK(gdim[d] >= 2 && bdim[d] >= 2, TMI_ERR_INVALID_ARG, "grid dimensions must be >= 2 (got %d, %d)",
+                      gdim[d], bdim[d]);
             R.gdim[d] = gdim[d];
+            R.bdim[d] = bdim[d];
             R.glen *= gdim[d];
+            R.blen *= bdim[d];
+            if (bdim[d] != gdim[d]) R.has_g = true;
         }
+        TMI_CHECK(R.glen < INT_MAX && R.blen < INT_MAX, TMI_ERR_UNSUPPORTED, "grid too large");
         if (bb) {
             for (int d = 0; d < m.dim; d++) { R.bbmin[d] = bb[d]; R.bbmax[d] = bb[m.dim + d]; }
         } else {   // Mesh::BoundingBox with pad 0
@@ -192,6 +198,92 @@ int tmi_raster_create(tmi_mesh h, const int *gdim, const double *bb, tmi_raster 
         throw;
     }
     *out = r;
+}
+
+int tmi_raster_create(tmi_mesh h, const int *gdim, const double *bb, tmi_raster *out)
+{
+    API_BEGIN
+    raster_create(h, gdim, gdim, bb, out);
+    API_END
+}
+
+int tmi_raster_create_basis(tmi_mesh h, const int *bdim, const int *gdim, const double *bb, tmi_raster *out)
+{
+    API_BEGIN
+    raster_create(h, bdim, gdim, bb, out);
+    API_END
+}
+
+int tmi_raster_basis_len(tmi_raster r, int *blen, int *has_g)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    if (blen) *blen = (int)r->r.blen;
+    if (has_g) *has_g = r->r.has_g ? 1 : 0;
+    API_END
+}
+
+// CSR export: rowptr (rows+1) always; colidx / val when given
+static void export_csr(const DBuf<int> &rp, const DBuf<int> &ci, const DBuf<double> &val, long rows, int *rowptr,
+                       int *colidx, double *v, int *nnz, hipStream_t st)
+{
+    int nz = 0;
+    TMI_HIP(hipMemcpyAsync(&nz, rp.p + rows, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    if (nnz) *nnz = nz;
+    if (rowptr) rp.download(rowptr, rows + 1, st);
+    if (colidx) ci.download(colidx, nz, st);
+    if (v) val.download(v, nz, st);
+    TMI_HIP(hipStreamSynchronize(st));
+}
+
+int tmi_raster_get_gmap(tmi_raster r, int *rowptr, int *colidx, double *val, int *nnz)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    Raster &R = r->r;
+    TMI_CHECK(R.has_g, TMI_ERR_UNSUPPORTED, "basis grid equals the intermediate grid: G is the identity");
+    export_csr(R.g_rp, R.g_ci, R.g_val, R.blen, rowptr, colidx, val, nnz, Context::get().stream);
+    API_END
+}
+
+int tmi_raster_get_gimap(tmi_raster r, int *rowptr, int *colidx, double *val, int *nnz)
+{
+    API_BEGIN
+    TMI_CHECK(r, TMI_ERR_INVALID_ARG, "invalid raster handle");
+    Raster &R = r->r;
+    TMI_CHECK(R.has_g, TMI_ERR_UNSUPPORTED, "basis grid equals the intermediate grid: GI is the identity");
+    hipStream_t st = Context::get().stream;
+    R.build_gimap(st);
+    export_csr(R.gi_rp, R.gi_ci, R.gi_val, R.glen, rowptr, colidx, val, nnz, st);
+    API_END
+}
+
+int tmi_raster_map_grid_to_basis(tmi_raster r, int ncols, const double *gvec, double *bvec)
+{
+    API_BEGIN
+    TMI_CHECK(r && gvec && bvec && ncols > 0, TMI_ERR_INVALID_ARG, "invalid argument");
+    hipStream_t st = Context::get().stream;
+    Raster &R = r->r;
+    DBuf<double> dg, db((size_t)R.blen * ncols * 2);
+    dg.upload(gvec, (size_t)R.glen * ncols * 2, st);
+    launch_grid_to_basis(R, ncols, dg.p, db.p, st);
+    db.download(bvec, (size_t)R.blen * ncols * 2, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_raster_map_basis_to_grid(tmi_raster r, int ncols, const double *bvec, double *gvec)
+{
+    API_BEGIN
+    TMI_CHECK(r && gvec && bvec && ncols > 0, TMI_ERR_INVALID_ARG, "invalid argument");
+    hipStream_t st = Context::get().stream;
+    Raster &R = r->r;
+    DBuf<double> db, dg((size_t)R.glen * ncols * 2);
+    db.upload(bvec, (size_t)R.blen * ncols * 2, st);
+    launch_basis_to_grid(R, ncols, db.p, dg.p, st);
+    dg.download(gvec, (size_t)R.glen * ncols * 2, st);
+    TMI_HIP(hipStreamSynchronize(st));
     API_END
 }
 

@@ -239,6 +239,298 @@ void launch_grid_to_mesh(Raster &r, int ncols, const double *d_g, double *d_out,
     TMI_LAUNCH_CHECK();
 }
 
+
+// ------------------------------------------------------------ linear pixel basis
+// Basis voxel ix of bx per axis sits at x = ix/(bx-1) of the unit box and
+// covers the grid points xx of [floor(x0 (gx-1)), ceil(x1 (gx-1))] (x0, x1
+// its neighbours, 0 and 1 at the ends) with d = |xx/(gx-1) - x| (bx-1) < 1;
+// its tent weight is prod (1 - d).  Same loops and operation order as
+// Grid2LinPixMatrix / LinPix2GridMatrix / SubsampleLinPixel
+// (mesh.cc:3047-3718), so the weights are bit-identical to the reference's.
+
+struct LpDims {
+    int g[3], b[3];
+};
+
+struct LpAxis {
+    double x, nb, dg;
+    int g0, g1;
+};
+
+__device__ inline LpAxis lp_axis(int b, int nbv, int ng)
+{
+    LpAxis a;
+    const double db = 1.0 / (nbv - 1.0);
+    a.nb = nbv - 1.0;
+    a.dg = 1.0 / (ng - 1.0);
+    a.x = (double)b * db;
+    const double x0 = (b ? (b - 1) * db : 0.0);
+    const double x1 = (b < nbv - 1 ? (b + 1) * db : 1.0);
+    a.g0 = (int)floor(x0 * (ng - 1));
+    a.g1 = (int)ceil(x1 * (ng - 1));
+    // the reference's windows lie inside [0, ng-1]; the clamp only guards memory
+    a.g0 = max(a.g0, 0);
+    a.g1 = min(a.g1, ng - 1);
+    return a;
+}
+
+// visit the grid points of basis voxel b in the reference's order: f(g, t)
+template <int DIM, class F>
+__device__ inline void lp_row(const LpDims &L, const int *__restrict__ elref, long b, F &&f)
+{
+    const int ix = (int)(b % L.b[0]), iy = (int)((b / L.b[0]) % L.b[1]);
+    const LpAxis ax = lp_axis(ix, L.b[0], L.g[0]), ay = lp_axis(iy, L.b[1], L.g[1]);
+    if (DIM == 2) {
+        for (int yy = ay.g0; yy <= ay.g1; yy++) {
+            const double dy = fabs(yy * ay.dg - ay.x) * ay.nb;
+            if (dy >= 1.0) continue;
+            const double idy = 1.0 - dy;
+            for (int xx = ax.g0; xx <= ax.g1; xx++) {
+                const long g = xx + (long)yy * L.g[0];
+                if (elref[g] < 0) continue;
+                const double dx = fabs(xx * ax.dg - ax.x) * ax.nb;
+                if (dx >= 1.0) continue;
+                f(g, (1.0 - dx) * idy);
+            }
+        }
+    } else {
+        const int iz = (int)(b / ((long)L.b[0] * L.b[1]));
+        const LpAxis az = lp_axis(iz, L.b[2], L.g[2]);
+        for (int zz = az.g0; zz <= az.g1; zz++) {
+            const double dz = fabs(zz * az.dg - az.x) * az.nb;
+            if (dz >= 1.0) continue;
+            const double idz = 1.0 - dz;
+            for (int yy = ay.g0; yy <= ay.g1; yy++) {
+                const double dy = fabs(yy * ay.dg - ay.x) * ay.nb;
+                if (dy >= 1.0) continue;
+                const double idyz = (1.0 - dy) * idz;
+                for (int xx = ax.g0; xx <= ax.g1; xx++) {
+                    const long g = xx + ((long)yy + (long)zz * L.g[1]) * L.g[0];
+                    if (elref[g] < 0) continue;
+                    const double dx = fabs(xx * ax.dg - ax.x) * ax.nb;
+                    if (dx >= 1.0) continue;
+                    f(g, (1.0 - dx) * idyz);
+                }
+            }
+        }
+    }
+}
+
+template <int DIM>
+__global__ void k_g_count(long blen, LpDims L, const int *__restrict__ elref, int *__restrict__ cnt)
+{
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= blen) return;
+    int c = 0;
+    lp_row<DIM>(L, elref, b, [&](long, double) { c++; });
+    cnt[b] = c;
+}
+
+// G row b: tent weights divided by their sum (summed in visiting order),
+// columns as grid indices and as positions among the grid points inside
+// the mesh; flag[b] = 1 if the row is not empty (the basis support test of
+// raster.cc:54-73: SubsampleLinPixel of the inside mask is > 0 exactly where
+// an inside grid point has a positive weight)
+template <int DIM>
+__global__ void k_g_fill(long blen, LpDims L, const int *__restrict__ elref, const int *__restrict__ gpos,
+                         const int *__restrict__ rp, int *__restrict__ ci, int *__restrict__ cs,
+                         double *__restrict__ val, int *__restrict__ flag)
+{
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= blen) return;
+    int k = rp[b];
+    double wgt = 0.0;
+    lp_row<DIM>(L, elref, b, [&](long g, double t) {
+        ci[k] = (int)g;
+        cs[k] = gpos[g];
+        val[k] = t;
+        wgt += t;
+        k++;
+    });
+    for (int i = rp[b]; i < k; i++) val[i] = wgt ? val[i] / wgt : 0.0;
+    flag[b] = k > rp[b] ? 1 : 0;
+}
+
+// basis indices along one axis whose window holds grid index xx with
+// d < 1, ascending, and their 1 - d
+__device__ inline int lp_cover(int xx, int nbv, int ng, int *ib, double *w)
+{
+    const double u = xx * (1.0 / (ng - 1.0)) * (nbv - 1.0);
+    const int lo = max((int)floor(u) - 1, 0), hi = min((int)floor(u) + 2, nbv - 1);
+    int c = 0;
+    for (int b = lo; b <= hi; b++) {
+        const LpAxis a = lp_axis(b, nbv, ng);
+        if (xx < a.g0 || xx > a.g1) continue;
+        const double d = fabs(xx * a.dg - a.x) * a.nb;
+        if (d >= 1.0) continue;
+        ib[c] = b;
+        w[c] = 1.0 - d;
+        c++;
+    }
+    return c;
+}
+
+// GI row g (LinPix2GridMatrix): the unnormalised tent weights of the basis
+// voxels covering grid point g, basis index ascending; with ci == nullptr
+// only the count
+template <int DIM>
+__global__ void k_gi_row(long glen, LpDims L, const int *__restrict__ elref, const int *__restrict__ rp,
+                         int *__restrict__ cnt, int *__restrict__ ci, double *__restrict__ val)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= glen) return;
+    int n = 0;
+    if (elref[g] >= 0) {
+        const int xx = (int)(g % L.g[0]), yy = (int)((g / L.g[0]) % L.g[1]);
+        const int zz = DIM > 2 ? (int)(g / ((long)L.g[0] * L.g[1])) : 0;
+        int bx[4], by[4], bz[4] = {0, 0, 0, 0};
+        double wx[4], wy[4], wz[4] = {1.0, 1.0, 1.0, 1.0};
+        const int nx = lp_cover(xx, L.b[0], L.g[0], bx, wx);
+        const int ny = lp_cover(yy, L.b[1], L.g[1], by, wy);
+        const int nz = DIM > 2 ? lp_cover(zz, L.b[2], L.g[2], bz, wz) : 1;
+        int k = ci ? rp[g] : 0;
+        for (int a = 0; a < nz; a++)
+            for (int j = 0; j < ny; j++) {
+                const double wyz = DIM > 2 ? wy[j] * wz[a] : wy[j];
+                for (int i = 0; i < nx; i++) {
+                    if (ci) {
+                        ci[k] = bx[i] + (by[j] + bz[a] * L.b[1]) * L.b[0];
+                        val[k] = wx[i] * wyz;
+                    }
+                    k++;
+                    n++;
+                }
+            }
+    }
+    if (!ci) cnt[g] = n;
+}
+
+static LpDims lp_dims(const Raster &r)
+{
+    LpDims L;
+    for (int d = 0; d < 3; d++) {
+        L.g[d] = r.gdim[d];
+        L.b[d] = r.bdim[d];
+    }
+    return L;
+}
+
+// G and the solution voxels; gpos[g]: position of grid point g among the
+// grid points inside the mesh
+static void build_gmap(Raster &r, const int *gpos, hipStream_t st)
+{
+    const LpDims L = lp_dims(r);
+    TMI_CHECK(r.blen < INT_MAX, TMI_ERR_UNSUPPORTED, "basis of %ld voxels is too large", r.blen);
+    const int blen = (int)r.blen;
+    const dim3 gb((unsigned)((blen + 255) / 256));
+    DBuf<int> cnt(blen), flag(blen), spos(blen + 1);
+    r.g_rp.alloc(blen + 1);
+    if (r.dim == 3) hipLaunchKernelGGL(k_g_count<3>, gb, dim3(256), 0, st, r.blen, L, r.elref.p, cnt.p);
+    else hipLaunchKernelGGL(k_g_count<2>, gb, dim3(256), 0, st, r.blen, L, r.elref.p, cnt.p);
+    TMI_LAUNCH_CHECK();
+    launch_exclusive_scan(cnt.p, r.g_rp.p, blen, st);
+    int nnz = 0;
+    TMI_HIP(hipMemcpyAsync(&nnz, r.g_rp.p + blen, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    r.g_ci.alloc(nnz > 0 ? nnz : 1);
+    r.g_cs.alloc(nnz > 0 ? nnz : 1);
+    r.g_val.alloc(nnz > 0 ? nnz : 1);
+    if (r.dim == 3)
+        hipLaunchKernelGGL(k_g_fill<3>, gb, dim3(256), 0, st, r.blen, L, r.elref.p, gpos, r.g_rp.p, r.g_ci.p,
+                           r.g_cs.p, r.g_val.p, flag.p);
+    else
+        hipLaunchKernelGGL(k_g_fill<2>, gb, dim3(256), 0, st, r.blen, L, r.elref.p, gpos, r.g_rp.p, r.g_ci.p,
+                           r.g_cs.p, r.g_val.p, flag.p);
+    TMI_LAUNCH_CHECK();
+    launch_exclusive_scan(flag.p, spos.p, blen, st);
+    TMI_HIP(hipMemcpyAsync(&r.slen, spos.p + blen, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    r.sol2basis.alloc(r.slen > 0 ? r.slen : 1);
+    hipLaunchKernelGGL(k_compact, gb, dim3(256), 0, st, flag.p, spos.p, r.blen, r.sol2basis.p);
+    TMI_LAUNCH_CHECK();
+    TMI_HIP(hipStreamSynchronize(st));
+}
+
+void Raster::build_gimap(hipStream_t st)
+{
+    if (gi_built || !has_g) return;
+    const LpDims L = lp_dims(*this);
+    const dim3 gg((unsigned)((glen + 255) / 256));
+    DBuf<int> cnt(glen);
+    gi_rp.alloc(glen + 1);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_gi_row<3>, gg, dim3(256), 0, st, glen, L, elref.p, (const int *)nullptr, cnt.p,
+                           (int *)nullptr, (double *)nullptr);
+    else
+        hipLaunchKernelGGL(k_gi_row<2>, gg, dim3(256), 0, st, glen, L, elref.p, (const int *)nullptr, cnt.p,
+                           (int *)nullptr, (double *)nullptr);
+    TMI_LAUNCH_CHECK();
+    launch_exclusive_scan(cnt.p, gi_rp.p, (int)glen, st);
+    int nnz = 0;
+    TMI_HIP(hipMemcpyAsync(&nnz, gi_rp.p + glen, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipStreamSynchronize(st));
+    gi_ci.alloc(nnz > 0 ? nnz : 1);
+    gi_val.alloc(nnz > 0 ? nnz : 1);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_gi_row<3>, gg, dim3(256), 0, st, glen, L, elref.p, gi_rp.p, cnt.p, gi_ci.p, gi_val.p);
+    else
+        hipLaunchKernelGGL(k_gi_row<2>, gg, dim3(256), 0, st, glen, L, elref.p, gi_rp.p, cnt.p, gi_ci.p, gi_val.p);
+    TMI_LAUNCH_CHECK();
+    TMI_HIP(hipStreamSynchronize(st));
+    gi_built = true;
+}
+
+double Raster::voxel_size() const
+{
+    double sz = 1.0;
+    for (int d = 0; d < dim; d++) sz *= (bbmax[d] - bbmin[d]) / (bdim[d] - 1.0);
+    return sz;
+}
+
+// out[r][c] = sum_e val[e] in[col[e]][c] over CSR row r in order, complex
+// columns (Ax_cplx, crmatrix_imp.hpp:1302-1327); thread per (r, c)
+__global__ void k_csr_cols(long rows, int ncols, const int *__restrict__ rp, const int *__restrict__ ci,
+                           const double *__restrict__ val, const double2 *__restrict__ in, double2 *__restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * ncols) return;
+    const long r = t / ncols;
+    const int c = (int)(t % ncols);
+    double br = 0.0, bi = 0.0;
+    for (int e = rp[r]; e < rp[r + 1]; e++) {
+        const double v = val[e];
+        const double2 z = in[(long)ci[e] * ncols + c];
+        br += z.x * v;
+        bi += z.y * v;
+    }
+    out[t] = make_double2(br, bi);
+}
+
+void launch_grid_to_basis(Raster &r, int ncols, const double *d_g, double *d_b, hipStream_t st)
+{
+    if (!r.has_g) {
+        TMI_HIP(hipMemcpyAsync(d_b, d_g, sizeof(double) * 2 * (size_t)r.glen * ncols, hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    const long cnt = r.blen * ncols;
+    hipLaunchKernelGGL(k_csr_cols, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, r.blen, ncols, r.g_rp.p,
+                       r.g_ci.p, r.g_val.p, (const double2 *)d_g, (double2 *)d_b);
+    TMI_LAUNCH_CHECK();
+}
+
+void launch_basis_to_grid(Raster &r, int ncols, const double *d_b, double *d_g, hipStream_t st)
+{
+    if (!r.has_g) {
+        TMI_HIP(hipMemcpyAsync(d_g, d_b, sizeof(double) * 2 * (size_t)r.glen * ncols, hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    r.build_gimap(st);
+    const long cnt = r.glen * ncols;
+    hipLaunchKernelGGL(k_csr_cols, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, r.glen, ncols, r.gi_rp.p,
+                       r.gi_ci.p, r.gi_val.p, (const double2 *)d_b, (double2 *)d_g);
+    TMI_LAUNCH_CHECK();
+}
+
 void Raster::build(hipStream_t st)
 {
     Mesh &m = *mesh;
@@ -261,11 +553,18 @@ void Raster::build(hipStream_t st)
     TMI_LAUNCH_CHECK();
     TMI_CHECK(glen < INT_MAX, TMI_ERR_UNSUPPORTED, "grid of %ld points is too large", glen);
     launch_exclusive_scan(flag.p, pos.p, (int)glen, st);
-    TMI_HIP(hipMemcpyAsync(&slen, pos.p + glen, sizeof(int), hipMemcpyDeviceToHost, st));
+    TMI_HIP(hipMemcpyAsync(&gslen, pos.p + glen, sizeof(int), hipMemcpyDeviceToHost, st));
     TMI_HIP(hipStreamSynchronize(st));
-    sol2basis.alloc(slen > 0 ? slen : 1);
-    hipLaunchKernelGGL(k_compact, gg, dim3(256), 0, st, flag.p, pos.p, glen, sol2basis.p);
+    gsol.alloc(gslen > 0 ? gslen : 1);
+    hipLaunchKernelGGL(k_compact, gg, dim3(256), 0, st, flag.p, pos.p, glen, gsol.p);
     TMI_LAUNCH_CHECK();
+    if (has_g) {
+        build_gmap(*this, pos.p, st);
+    } else {   // solution voxels = grid points inside the mesh
+        slen = gslen;
+        sol2basis.alloc(slen > 0 ? slen : 1);
+        TMI_HIP(hipMemcpyAsync(sol2basis.p, gsol.p, sizeof(int) * (size_t)slen, hipMemcpyDeviceToDevice, st));
+    }
     bcols.alloc((size_t)glen * nn);
     bw.alloc((size_t)glen * nn);
     if (dim == 3)
@@ -441,16 +740,16 @@ __global__ void __launch_bounds__(256) k_jac_grid_cw(int slen, int nq, int dim, 
 }
 
 // nq direct then nm adjoint complex columns mapped to the grid (r.fgv) and
-// their values and gradients at the solution voxels (r.fgs [c][s][4]);
-// columns c >= unmasked_from take unmasked gradients.  Returns the voxel
-// size prod gsize/(gdim-1).
+// their values and gradients at the grid points inside the mesh (r.fgs
+// [c][gslen][4]); columns c >= unmasked_from take unmasked gradients.
+// Returns the voxel size prod gsize/(bdim-1).
 static double grid_fields(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                           int unmasked_from, hipStream_t st)
 {
     const int ncols = nq + nm;
-    TMI_CHECK(r.slen > 0, TMI_ERR_INVALID_ARG, "the raster has no voxel inside the mesh");
+    TMI_CHECK(r.gslen > 0 && r.slen > 0, TMI_ERR_INVALID_ARG, "the raster has no voxel inside the mesh");
     r.fgv.alloc((size_t)r.glen * ncols * 2);
-    r.fgs.alloc((size_t)r.slen * ncols * 8);
+    r.fgs.alloc((size_t)r.gslen * ncols * 8);
     // fields on the grid: direct columns 0..nq-1, adjoint nq..nq+nm-1
     const long cnt_q = r.glen * nq, cnt_m = r.glen * nm;
     const int NN = r.dim + 1;
@@ -467,19 +766,48 @@ static double grid_fields(Raster &r, int nq, int nm, const double *d_dphi, int l
     }
     TMI_LAUNCH_CHECK();
     GradDesc gd{r.gdim[0], r.gdim[1], r.gdim[2], r.dim, {0, 0, 0}, {0, 0, 0}};
-    double sz = 1.0;
     for (int d = 0; d < r.dim; d++) {
         const double gsize = r.bbmax[d] - r.bbmin[d];
         const double h = gsize / r.gdim[d];
         gd.ih[d] = 1.0 / h;
         gd.i2h[d] = 0.5 * gd.ih[d];
-        sz *= gsize / (r.gdim[d] - 1.0);
     }
-    const long cnt_s = (long)r.slen * ncols;
-    hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.slen, ncols,
-                       r.sol2basis.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, unmasked_from);
+    const long cnt_s = (long)r.gslen * ncols;
+    hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.gslen, ncols,
+                       r.gsol.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, unmasked_from);
     TMI_LAUNCH_CHECK();
-    return sz;
+    return r.voxel_size();
+}
+
+// Map_GridToSol (raster_px.cc:102-114,148-151) of Jacobian rows computed at
+// the grid points inside the mesh: out[row][s] = (sum_e G[b_s][e]
+// in[row][e]) * sz, times 1/proj[row] when proj is given (the log row
+// scaling of GenerateJacobian_cw, jacobian.cc:160-186); thread per (row, s)
+__global__ void k_jac_to_basis(int rows, int slen, const int *__restrict__ s2b, const int *__restrict__ rp,
+                               const int *__restrict__ cs, const double *__restrict__ val,
+                               const double *__restrict__ in, long ldin, double *__restrict__ out, long ldout,
+                               double sz, const double *__restrict__ proj)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)rows * slen) return;
+    const int row = (int)(t / slen), s = (int)(t % slen);
+    const int b = s2b[s];
+    const double *x = in + (size_t)row * ldin;
+    double sum = 0.0;
+    for (int e = rp[b]; e < rp[b + 1]; e++) sum += x[cs[e]] * val[e];
+    double v = sum * sz;
+    if (proj) v *= 1.0 / proj[row];
+    out[(size_t)row * ldout + s] = v;
+}
+
+static void jac_to_basis(const Raster &r, int rows, const double *in, long ldin, double *out, long ldout, double sz,
+                         const double *proj, hipStream_t st)
+{
+    const long cnt = (long)rows * r.slen;
+    if (cnt == 0) return;
+    hipLaunchKernelGGL(k_jac_to_basis, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, rows, r.slen,
+                       r.sol2basis.p, r.g_rp.p, r.g_cs.p, r.g_val.p, in, ldin, out, ldout, sz, proj);
+    TMI_LAUNCH_CHECK();
 }
 
 // AddDataGradient's raster part (toastmodule.cc:1813-1836): for each source q
@@ -502,12 +830,44 @@ __global__ void k_data_grad(int slen, int nq, const double2 *__restrict__ fgs, d
     grad[slen + s] = gk;
 }
 
+// the same through G: per solution voxel and source in order, the G row
+// applied to Re(d a) and to Re(sum_d grad_d d grad_d a) at the grid points
+// inside the mesh (Map_GridToSol then Re, toastmodule.cc:1821-1832)
+__global__ void k_data_grad_g(int slen, int nq, int gslen, const int *__restrict__ s2b, const int *__restrict__ rp,
+                              const int *__restrict__ cs, const double *__restrict__ val,
+                              const double2 *__restrict__ fgs, double *__restrict__ grad)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slen) return;
+    const int b = s2b[s];
+    double gm = grad[s], gk = grad[slen + s];
+    for (int q = 0; q < nq; q++) {
+        double sm = 0.0, sk = 0.0;
+        for (int e = rp[b]; e < rp[b + 1]; e++) {
+            const double v = val[e];
+            const double2 *D = fgs + ((long)q * gslen + cs[e]) * 4, *A = fgs + ((long)(nq + q) * gslen + cs[e]) * 4;
+            sm += (D[0].x * A[0].x - D[0].y * A[0].y) * v;
+            double t = 0.0;
+            for (int d = 1; d <= 3; d++) t += D[d].x * A[d].x - D[d].y * A[d].y;
+            sk += t * v;
+        }
+        gm -= sm;
+        gk -= sk;
+    }
+    grad[s] = gm;
+    grad[slen + s] = gk;
+}
+
 void launch_data_gradient(Raster &r, int nq, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                           double *d_grad, hipStream_t st)
 {
     grid_fields(r, nq, nq, d_dphi, ldd, d_aphi, lda, 2 * nq, st);
-    hipLaunchKernelGGL(k_data_grad, dim3((unsigned)((r.slen + 255) / 256)), dim3(256), 0, st, r.slen, nq,
-                       (const double2 *)r.fgs.p, d_grad);
+    if (r.has_g)
+        hipLaunchKernelGGL(k_data_grad_g, dim3((unsigned)((r.slen + 255) / 256)), dim3(256), 0, st, r.slen, nq,
+                           r.gslen, r.sol2basis.p, r.g_rp.p, r.g_cs.p, r.g_val.p, (const double2 *)r.fgs.p, d_grad);
+    else
+        hipLaunchKernelGGL(k_data_grad, dim3((unsigned)((r.slen + 255) / 256)), dim3(256), 0, st, r.slen, nq,
+                           (const double2 *)r.fgs.p, d_grad);
     TMI_LAUNCH_CHECK();
 }
 
@@ -516,13 +876,40 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
                           double *d_J, hipStream_t st, bool cw, double *d_Jkap)
 {
     const double sz = grid_fields(r, nq, nm, d_dphi, ldd, d_aphi, lda, cw ? nq : nq + nm, st);
-    if (cw)
-        hipLaunchKernelGGL(k_jac_grid_cw, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq,
-                           r.dim, (const double2 *)r.fgs.p, d_lptr, d_lm, d_proj, logscale, sz, d_J, d_Jkap);
-    else
-        hipLaunchKernelGGL(k_jac_grid, dim3((unsigned)((r.slen + 63) / 64), nq), dim3(256), 0, st, r.slen, nq, r.dim,
-                           (const double2 *)r.fgs.p, d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
+    const dim3 grid((unsigned)((r.gslen + 63) / 64), nq);
+    if (!r.has_g) {   // solution voxels = grid points inside the mesh
+        if (cw)
+            hipLaunchKernelGGL(k_jac_grid_cw, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
+                               d_lptr, d_lm, d_proj, logscale, sz, d_J, d_Jkap);
+        else
+            hipLaunchKernelGGL(k_jac_grid, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
+                               d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
+        TMI_LAUNCH_CHECK();
+        return;
+    }
+    // PMDFs at the grid points inside the mesh (unit scale, log division of
+    // the complex rows included), then G, the voxel size and for CW the
+    // 1/proj row scaling
+    const size_t gs = (size_t)r.gslen;
+    if (cw) {
+        DBuf<double> jm(d_J ? (size_t)nqm * gs : 0), jk(d_Jkap ? (size_t)nqm * gs : 0);
+        hipLaunchKernelGGL(k_jac_grid_cw, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
+                           d_lptr, d_lm, d_proj, 0, 1.0, jm.p, jk.p);
+        TMI_LAUNCH_CHECK();
+        const double *rs = logscale ? d_proj : nullptr;
+        if (d_J) jac_to_basis(r, nqm, jm.p, (long)gs, d_J, r.slen, sz, rs, st);
+        if (d_Jkap) jac_to_basis(r, nqm, jk.p, (long)gs, d_Jkap, r.slen, sz, rs, st);
+        TMI_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    DBuf<double> jg((size_t)4 * nqm * gs);
+    hipLaunchKernelGGL(k_jac_grid, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p, d_lptr,
+                       d_lm, (const double2 *)d_proj, logscale, 1.0, nqm, jg.p);
     TMI_LAUNCH_CHECK();
+    const long ldo = 2 * (long)r.slen;
+    jac_to_basis(r, 2 * nqm, jg.p, 2 * (long)gs, d_J, ldo, sz, nullptr, st);                  // mua block
+    jac_to_basis(r, 2 * nqm, jg.p + gs, 2 * (long)gs, d_J + r.slen, ldo, sz, nullptr, st);     // kappa block
+    TMI_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace tmi

@@ -1,12 +1,14 @@
 """toast.Raster surface (src/python/toast/raster.py) on the GPU library.
 
-Raster(mesh, grd) builds the regular-grid basis of Raster_Pixel with the
-basis grid equal to the intermediate grid (raster.cc:18-100,
-raster_px.cc:10-54) on the GPU: the element holding each grid point, the
-mesh -> grid map B and the solution mask.  Bases in map strings: M mesh
-nodes, G intermediate grid, B basis grid (= G here), S solution voxels
-(basis voxels inside the mesh).  Maps into the mesh basis use the
-reference's NodeMapMatrix (mesh.cc:2695-2853), built on the GPU on first use.
+Raster(mesh, grd, intgrd=None) builds Raster_Pixel (raster.cc:18-100,
+raster_px.cc:10-54) on the GPU: basis grid grd, intermediate grid intgrd
+(default grd), the element holding each grid point, the mesh -> grid map B,
+the grid -> basis map G (Grid2LinPixMatrix) and its converse GI
+(LinPix2GridMatrix) when the two grids differ, and the solution voxels.
+Bases in map strings: M mesh nodes, G intermediate grid, B basis grid, S
+solution voxels (basis voxels with mesh support).  Maps into the mesh basis
+use the reference's NodeMapMatrix (mesh.cc:2695-2853), built on the GPU on
+first use.
 """
 import ctypes as C
 
@@ -24,17 +26,19 @@ class Raster:
     # ------------------------------------------------------------ handle
     def Make(self, mesh, grd, intgrd=None):
         grd = np.ascontiguousarray(grd, np.int32)
-        if intgrd is not None and not np.array_equal(np.asarray(intgrd, np.int32), grd):
-            raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "only an intermediate grid equal to the basis grid is supported")
-        if len(grd) != mesh.Dim():
-            raise ValueError("grid dimension %d does not match the mesh dimension %d" % (len(grd), mesh.Dim()))
+        intgrd = grd.copy() if intgrd is None else np.ascontiguousarray(intgrd, np.int32)
+        if len(grd) != mesh.Dim() or len(intgrd) != mesh.Dim():
+            raise ValueError("grid dimensions %d/%d do not match the mesh dimension %d"
+                             % (len(grd), len(intgrd), mesh.Dim()))
         self.Clear()
         self.mesh = mesh
         self.grd = grd
-        L.call("tmi_raster_create", mesh.handle_ptr(), grd, None, C.byref(self._h))
-        g, s = C.c_int(), C.c_int()
+        self.intgrd = intgrd
+        L.call("tmi_raster_create_basis", mesh.handle_ptr(), grd, intgrd, None, C.byref(self._h))
+        g, s, b, hg = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         L.call("tmi_raster_dims", self._h, C.byref(g), C.byref(s))
-        self._glen, self._slen = g.value, s.value
+        L.call("tmi_raster_basis_len", self._h, C.byref(b), C.byref(hg))
+        self._glen, self._slen, self._blen, self._has_g = g.value, s.value, b.value, bool(hg.value)
         dim = len(grd)
         self.bbmin, self.bbmax = np.zeros(dim), np.zeros(dim)
         L.call("tmi_raster_bbox", self._h, self.bbmin, self.bbmax)
@@ -60,7 +64,7 @@ class Raster:
         return self._glen
 
     def BLen(self):
-        return self._glen
+        return self._blen
 
     def SLen(self):
         return self._slen
@@ -76,12 +80,12 @@ class Raster:
         return s
 
     def Basis2Sol(self):
-        b = np.full(self._glen, -1, np.int32)
+        b = np.full(self._blen, -1, np.int32)
         b[self.Sol2Basis()] = np.arange(self._slen, dtype=np.int32)
         return b
 
     def BasisPoints(self):
-        """Grid point coordinates (blen x dim), x fastest (GenerateVoxelPositions)."""
+        """Basis voxel coordinates (blen x dim), x fastest (BasisVoxelPositions)."""
         dim = len(self.grd)
         axes = [self.bbmin[d] + np.arange(self.grd[d]) * ((self.bbmax[d] - self.bbmin[d]) / (self.grd[d] - 1))
                 for d in range(dim)]
@@ -114,9 +118,40 @@ class Raster:
             out = out.real.copy()
         return out[:, 0] if one else out
 
+    def _cols(self, vec, rows):
+        vec = np.asarray(vec)
+        real = not np.iscomplexobj(vec)
+        one = vec.ndim == 1
+        if vec.shape[0] != rows:
+            raise ValueError("expected %d rows, got %d" % (rows, vec.shape[0]))
+        return np.ascontiguousarray(vec.reshape(rows, -1), np.complex128), real, one
+
+    def GridToBasis(self, gvec):
+        """Map_GridToBasis (raster_px.cc:102-114): G applied on the GPU (a
+        copy when the grids coincide)."""
+        g, real, one = self._cols(gvec, self._glen)
+        out = np.zeros((self._blen, g.shape[1]), np.complex128)
+        L.call("tmi_raster_map_grid_to_basis", self.handle_ptr(), g.shape[1], g.view(np.float64),
+               out.view(np.float64))
+        if real:
+            out = out.real.copy()
+        return out[:, 0] if one else out
+
+    def BasisToGrid(self, bvec):
+        """Map_BasisToGrid (raster_px.cc:118-130): GI applied on the GPU."""
+        b, real, one = self._cols(bvec, self._blen)
+        out = np.zeros((self._glen, b.shape[1]), np.complex128)
+        L.call("tmi_raster_map_basis_to_grid", self.handle_ptr(), b.shape[1], b.view(np.float64),
+               out.view(np.float64))
+        if real:
+            out = out.real.copy()
+        return out[:, 0] if one else out
+
+    def MeshToBasis(self, mvec):
+        return self.GridToBasis(self.MeshToGrid(mvec))
+
     def MeshToSol(self, mvec):
-        g = self.MeshToGrid(mvec)
-        return g[self.Sol2Basis()]
+        return self.MeshToBasis(mvec)[self.Sol2Basis()]
 
     def _nodemap(self):
         nc = 1 << len(self.grd)
@@ -141,50 +176,84 @@ class Raster:
             out = out.real.copy()
         return out[:, 0] if one else out
 
+    # the bases in the order the maps chain them: M <-> G <-> B <-> S
+    _CHAIN = "MGBS"
+
+    def _step(self, src, tgt, v):
+        if src == "M" and tgt == "G":
+            return self.MeshToGrid(v)
+        if src == "G" and tgt == "M":
+            return self.GridToMesh(v)
+        if src == "G" and tgt == "B":
+            return self.GridToBasis(v)
+        if src == "B" and tgt == "G":
+            return self.BasisToGrid(v)
+        if src == "B" and tgt == "S":
+            return v[self.Sol2Basis()]
+        out = np.zeros((self._blen,) + v.shape[1:], v.dtype)   # S -> B
+        out[self.Sol2Basis()] = v
+        return out
+
     def Map(self, mapstr, srcvec):
-        """Map a field between bases: mapstr 'S->T' with S, T in M, G, B, S."""
+        """Map a field between bases: mapstr 'S->T' with S, T in M, G, B, S
+        (Raster::Map_*, raster.cc:188-388, raster_px.cc:102-176)."""
         src, tgt = mapstr.replace(" ", "").split("->")
+        if src not in self._CHAIN or tgt not in self._CHAIN:
+            raise ValueError("invalid map string %r" % mapstr)
         v = np.asarray(srcvec).ravel()
-        if tgt == "M":
-            if src == "M":
-                return v.copy()
-            if src in ("G", "B"):
-                return self.GridToMesh(v)
-            if src == "S":
-                return self.GridToMesh(self.Map("S->B", v))
-        if src == "M":
-            g = self.MeshToGrid(v)
-            return g[self.Sol2Basis()] if tgt == "S" else g
-        if src in ("G", "B"):
-            return v.copy() if tgt in ("G", "B") else v[self.Sol2Basis()]
-        if src == "S":
-            if tgt == "S":
-                return v.copy()
-            out = np.zeros(self._glen, v.dtype)
-            out[self.Sol2Basis()] = v
-            return out
-        raise ValueError("invalid map string %r" % mapstr)
+        i, j = self._CHAIN.index(src), self._CHAIN.index(tgt)
+        if i == j:
+            return v.copy()
+        step = 1 if j > i else -1
+        for k in range(i, j, step):
+            v = self._step(self._CHAIN[k], self._CHAIN[k + step], v)
+        return v
 
     def GetMatrix(self, mapstr):
+        """Sparse map matrices (toast_raster_matrix, toastmodule.cc:416-507):
+        M->G (B), G->M (BI), G->B (G), B->G (GI), M->B (C = G B), B->M
+        (CI = BI GI), B->S (D); identity for G<->B when the grids coincide."""
         src, tgt = mapstr.replace(" ", "").split("->")
-        sizes = {"S": self._slen, "G": self._glen, "B": self._glen, "M": self.mesh.NodeCount()}
-        if src in ("G", "B") and tgt in ("G", "B"):
-            return sparse.identity(self._glen)
-        if src == "M" and tgt in ("G", "B"):
+        sizes = {"S": self._slen, "G": self._glen, "B": self._blen, "M": self.mesh.NodeCount()}
+        if src == tgt:
+            return sparse.identity(sizes[src], format="csr")
+        if src in ("G", "B") and tgt in ("G", "B") and not self._has_g:
+            return sparse.identity(self._glen, format="csr")
+        if src == "M" and tgt == "G":
             cols, w = self._bmap()
             keep = cols >= 0
             rows = np.repeat(np.arange(self._glen), cols.shape[1]).reshape(cols.shape)
             return sparse.csr_matrix((w[keep], (rows[keep], cols[keep])), shape=(self._glen, sizes["M"]))
-        if src in ("G", "B") and tgt == "M":
+        if src == "G" and tgt == "M":
             cols, w = self._nodemap()
             keep = cols >= 0
             rows = np.repeat(np.arange(sizes["M"]), cols.shape[1]).reshape(cols.shape)
             return sparse.csr_matrix((w[keep], (rows[keep], cols[keep])), shape=(sizes["M"], self._glen))
-        if src in ("G", "B") and tgt == "S":
+        if src == "G" and tgt == "B":
+            return self._csr("tmi_raster_get_gmap", self._blen, self._glen)
+        if src == "B" and tgt == "G":
+            return self._csr("tmi_raster_get_gimap", self._glen, self._blen)
+        if src == "M" and tgt == "B":
+            B = self.GetMatrix("M->G")
+            return B if not self._has_g else sparse.csr_matrix(self.GetMatrix("G->B") @ B)
+        if src == "B" and tgt == "M":
+            BI = self.GetMatrix("G->M")
+            return BI if not self._has_g else sparse.csr_matrix(BI @ self.GetMatrix("B->G"))
+        if src == "B" and tgt == "S":
             s2b = self.Sol2Basis()
             return sparse.csr_matrix((np.ones(self._slen), (np.arange(self._slen), s2b)),
-                                     shape=(self._slen, self._glen))
+                                     shape=(self._slen, self._blen))
         raise L.TmiError(L.TMI_ERR_UNSUPPORTED, "matrix %s is not provided" % mapstr)
+
+    def _csr(self, fn, rows, cols):
+        nnz = C.c_int()
+        rp = np.zeros(rows + 1, np.int32)
+        L.call(fn, self.handle_ptr(), L.ptr(rp), None, None, C.byref(nnz))
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1))
+        L.call(fn, self.handle_ptr(), L.ptr(rp), L.ptr(ci), L.ptr(val), C.byref(nnz))
+        n = nnz.value
+        return sparse.csr_matrix((val[:n], ci[:n], rp), shape=(rows, cols))
 
     # ----------------------------------------------------------- Jacobian
     def DataGradient(self, dphi, aphi, grad=None):
