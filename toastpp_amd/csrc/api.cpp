@@ -696,19 +696,19 @@ static void project_all(Solver &s, int nm, const int *mrp, const int *mci, const
     TMI_CHECK(nm == qm.nm, TMI_ERR_INVALID_ARG, "mvec has %d rows, the QM description %d detectors", nm, qm.nm);
     int n = m.n, cw = s.cplx ? 2 : 1, nqm = qm.nqm;
     if (nqm == 0) return;
-    std::vector<int> lq(nqm), lm(nqm);
+    // link index of every (detector, source) pair, -1 where not linked
+    std::vector<int> lk((size_t)nm * qm.nq, -1);
     for (int q = 0; q < qm.nq; q++)
-        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) { lq[k] = q; lm[k] = qm.lidx[k]; }
+        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) lk[(size_t)qm.lidx[k] * qm.nq + q] = k;
     int mnnz = mrp[nm];
     for (int k = 0; k < mnnz; k++) TMI_CHECK(mci[k] >= 0 && mci[k] < n, TMI_ERR_INVALID_ARG, "mvec column out of range");
-    DBuf<int> dlq, dlm, drp, dci;
+    DBuf<int> dlk, drp, dci;
     DBuf<double> dval, dproj((size_t)nqm * cw);
-    dlq.upload(lq.data(), nqm, s.st);
-    dlm.upload(lm.data(), nqm, s.st);
+    dlk.upload(lk.data(), lk.size(), s.st);
     drp.upload(mrp, nm + 1, s.st);
     dci.upload(mci, mnnz, s.st);
     dval.upload(mval, (size_t)mnnz * cw, s.st);
-    launch_project(nqm, dlq.p, dlm.p, drp.p, dci.p, dval.p, d_phi, sq, sj, s.cplx ? 1 : 0, dproj.p, s.st);
+    launch_project(qm.nq, nm, dlk.p, drp.p, dci.p, dval.p, d_phi, sq, sj, s.cplx ? 1 : 0, dproj.p, s.st);
     dproj.download(proj, (size_t)nqm * cw, s.st);
     TMI_HIP(hipStreamSynchronize(s.st));
     if (dscale == TMI_DATA_LOG) {   // complex log: Re = ln|y|, Im = arg y

@@ -7,7 +7,7 @@
 // where A_i, B_i (|N(i)| x |N(i)|) collect the element integrals of the
 // elements holding i.  Over all source/detector pairs they are two small
 // GEMMs per node: P_i = F_i (A_i G_i^T), F_i = dphi on N(i) (nq x d),
-// G_i = aphi on N(i) (nm x d).  One wavefront per node: it builds A_i, B_i
+// G_i = aphi on N(i) (nm x d).  One workgroup per node: it builds A_i, B_i
 // in LDS from the element integrals, then for each chunk of detectors
 // gathers G into LDS and forms H = G A, K = G B, and for each chunk of
 // sources gathers F and forms P for the linked pairs of the two chunks,
@@ -27,6 +27,14 @@
 
 namespace tmi {
 
+// threads per node (four waves: every SIMD of a CU gets work even at the
+// two or three nodes per CU that the LDS allows)
+constexpr int JAC_NT = 256;
+#ifndef TMI_JAC_LDS_KIB
+#define TMI_JAC_LDS_KIB 40
+#endif
+constexpr int JAC_LDS_KIB = TMI_JAC_LDS_KIB;
+
 struct JacDims {
     int n, nq, nm, nqm, dmax, chunk, ldd, lda;
 };
@@ -38,16 +46,24 @@ __device__ inline int find_pos(const int *cols, int d, int v)
     return -1;
 }
 
-// LDS layout per wave (doubles): A[dmax^2] B[dmax^2] W[C dmax] (complex: the
+// LDS layout per node (doubles): A[dmax^2] B[dmax^2] W[C dmax] (complex: the
 // detector chunk G, then the source chunk F) H[C dmax] Kk[C dmax] cols[dmax]
 // (ints), C = source/detector chunk
+// the W/H/Kk region also holds the element-integral scratch (2 doubles and
+// an int per thread) before the GEMMs start
+__host__ __device__ inline size_t jac_region_doubles(int dmax, int chunk)
+{
+    const size_t g = (size_t)6 * chunk * dmax, e = (size_t)3 * JAC_NT;
+    return g > e ? g : e;
+}
+
 static size_t jac_lds_doubles(int dmax, int chunk)
 {
-    return (size_t)2 * dmax * dmax + (size_t)6 * chunk * dmax + (dmax + 1) / 2 + 1;
+    return (size_t)2 * dmax * dmax + jac_region_doubles(dmax, chunk) + (dmax + 1) / 2 + 1;
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restrict__ rowptr,
+__global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__restrict__ rowptr,
                                                   const int *__restrict__ colidx, const int *__restrict__ n2e_ptr,
                                                   const int *__restrict__ n2e, const int *__restrict__ el,
                                                   const double *__restrict__ geo,
@@ -61,42 +77,61 @@ __global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restr
     constexpr int GS = DIM == 3 ? GEO3 : GEO2;
     extern __shared__ double lds[];
     const int i = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const int dmax = jd.dmax, nq = jd.nq, nm = jd.nm, C = jd.chunk;
     double *A = lds, *B = A + dmax * dmax;
     double2 *W = (double2 *)(B + dmax * dmax);
     double2 *H = W + C * dmax, *Kk = H + C * dmax;
-    int *cols = (int *)(Kk + C * dmax);
+    int *cols = (int *)((double *)W + jac_region_doubles(dmax, C));
     const int rb = rowptr[i], d = rowptr[i + 1] - rb;
-    for (int t = lane; t < d; t += 64) cols[t] = colidx[rb + t];
-    for (int t = lane; t < dmax * dmax; t += 64) { A[t] = 0.0; B[t] = 0.0; }
+    for (int t = tid; t < d; t += JAC_NT) cols[t] = colidx[rb + t];
+    for (int t = tid; t < dmax * dmax; t += JAC_NT) { A[t] = 0.0; B[t] = 0.0; }
     __syncthreads();
-    // element integrals, one element after another (deterministic sums):
-    // lane (a, b) adds IntFFF(li, a, b) and IntFDD(li, a, b) of the element
-    for (int q = n2e_ptr[i]; q < n2e_ptr[i + 1]; q++) {
-        const int e = n2e[q];
-        const int *nd = el + (long)NN * e;
-        const double *g = geo + (long)GS * e;
-        if (lane < NN * NN) {
-            const int a = lane / NN, b = lane % NN;
-            int li = 0;
-            for (int k = 0; k < NN; k++)
-                if (nd[k] == i) li = k;
-            const int pa = find_pos(cols, d, nd[a]), pb = find_pos(cols, d, nd[b]);
-            double fff = DIM == 3 ? tet_intfff(g, li, a, b) : tri_intfff(g, li, a, b);
-            double fdd = DIM == 3 ? tet_intfdd(g, a, b) : tri_intfdd(g, a, b);
-            if (pa >= 0 && pb >= 0) {   // always: the row of i holds every node of its elements
-                A[pa * dmax + pb] += fff;
-                B[pa * dmax + pb] += fdd;
+    // element integrals: the elements of node i in rounds of R, thread
+    // (element, a, b) evaluating IntFFF(li, a, b) and IntFDD(a, b) into a
+    // scratch in the W/H/Kk region (not in use yet), then threads (a, b)
+    // adding them into A, B one element after another (deterministic sums,
+    // the same order as a serial loop over the elements)
+    {
+        double *ef = (double *)W, *ed = ef + (size_t)JAC_NT;
+        int *ep = (int *)(ed + (size_t)JAC_NT);
+        constexpr int R = JAC_NT / (NN * NN);
+        const int q0 = n2e_ptr[i], q1 = n2e_ptr[i + 1];
+        for (int qb = q0; qb < q1; qb += R) {
+            const int ne = min(R, q1 - qb);
+            if (tid < ne * NN * NN) {
+                const int le = tid / (NN * NN), ab = tid % (NN * NN);
+                const int e = n2e[qb + le];
+                const int *nd = el + (long)NN * e;
+                const double *g = geo + (long)GS * e;
+                const int a = ab / NN, b = ab % NN;
+                int li = 0;
+                for (int k = 0; k < NN; k++)
+                    if (nd[k] == i) li = k;
+                const int pa = find_pos(cols, d, nd[a]), pb = find_pos(cols, d, nd[b]);
+                ef[tid] = DIM == 3 ? tet_intfff(g, li, a, b) : tri_intfff(g, li, a, b);
+                ed[tid] = DIM == 3 ? tet_intfdd(g, a, b) : tri_intfdd(g, a, b);
+                // the row of i holds every node of its elements: pa, pb >= 0
+                ep[tid] = (pa >= 0 && pb >= 0) ? pa * dmax + pb : -1;
             }
+            __syncthreads();
+            if (tid < NN * NN) {   // one wave: the (a, b) of an element hit distinct entries
+                for (int le = 0; le < ne; le++) {
+                    const int at = le * NN * NN + tid, pos = ep[at];
+                    if (pos >= 0) {
+                        A[pos] += ef[at];
+                        B[pos] += ed[at];
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     double2 *out = Pt + (size_t)i * jd.nqm * 2;
     for (int m0 = 0; m0 < nm; m0 += C) {
         const int mc = min(C, nm - m0);
         // detector chunk G on the neighbour set
-        for (int t = lane; t < mc * d; t += 64) {
+        for (int t = tid; t < mc * d; t += JAC_NT) {
             const int mm = t / d, j = t % d;
             W[mm * dmax + j] = aphi[(size_t)cols[j] * jd.lda + m0 + mm];
         }
@@ -104,7 +139,7 @@ __global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restr
         {
 #pragma clang fp contract(fast)
             // H = G A^T, Kk = G B^T  (A, B symmetric)
-            for (int t = lane; t < mc * d; t += 64) {
+            for (int t = tid; t < mc * d; t += JAC_NT) {
                 const int mm = t / d, j = t % d;
                 double hr = 0.0, hi = 0.0, kr = 0.0, ki = 0.0;
                 for (int k = 0; k < d; k++) {
@@ -120,7 +155,7 @@ __global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restr
         __syncthreads();
         for (int q0 = 0; q0 < nq; q0 += C) {
             const int qc = min(C, nq - q0);
-            for (int t = lane; t < qc * d; t += 64) {
+            for (int t = tid; t < qc * d; t += JAC_NT) {
                 const int qq = t / d, j = t % d;
                 W[qq * dmax + j] = dphi[(size_t)cols[j] * jd.ldd + q0 + qq];
             }
@@ -128,7 +163,7 @@ __global__ void __launch_bounds__(64) k_jac_nodes(JacDims jd, const int *__restr
             {
 #pragma clang fp contract(fast)
                 // P(q,m) = F_q . H_m for the linked pairs of this chunk pair
-                for (int k = lptr[q0] + lane; k < lptr[q0 + qc]; k += 64) {
+                for (int k = lptr[q0] + tid; k < lptr[q0 + qc]; k += JAC_NT) {
                     const int mrel = lm[k] - m0;
                     if (mrel < 0 || mrel >= mc) continue;
                     const double2 *f = W + (lq[k] - q0) * dmax;
@@ -251,23 +286,27 @@ void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, 
                            const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
                            int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw, double *d_Jkap)
 {
-    // source/detector chunk: all of them if the wave's LDS stays within 64 KiB
-    // (two waves per CU), else the largest chunk that fits, up to 160 KiB
+    // LDS row stride: the neighbour count rounded up to odd, so that the
+    // rows of A, B, W, H read by consecutive lanes start in different banks
+    dmax |= 1;
+    // source/detector chunk: all of them if a node's LDS stays within
+    // JAC_LDS_KIB (several nodes per CU), else the largest chunk that fits
+    // (at least 8 when the LDS of one CU allows)
     const int want = std::max(nq, nm);
     int chunk = want;
-    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > 64 * 1024) chunk--;
+    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > JAC_LDS_KIB * 1024) chunk--;
     if (chunk < std::min(want, 8))
-        while (chunk < want && jac_lds_doubles(dmax, chunk + 1) * sizeof(double) <= 160 * 1024) chunk++;
+        while (chunk < want && chunk < 8 && jac_lds_doubles(dmax, chunk + 1) * sizeof(double) <= 160 * 1024) chunk++;
     const size_t lds = jac_lds_doubles(dmax, chunk) * sizeof(double);
     TMI_CHECK(lds <= 160 * 1024, TMI_ERR_UNSUPPORTED,
               "nodal Jacobian: %d neighbours per node exceed the LDS of one CU", dmax);
     JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda};
     if (m.dim == 3)
-        hipLaunchKernelGGL(k_jac_nodes<3>, dim3(m.n), dim3(64), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
+        hipLaunchKernelGGL(k_jac_nodes<3>, dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
                            m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
                            (double2 *)d_Pt);
     else
-        hipLaunchKernelGGL(k_jac_nodes<2>, dim3(m.n), dim3(64), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
+        hipLaunchKernelGGL(k_jac_nodes<2>, dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
                            m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
                            (double2 *)d_Pt);
     TMI_LAUNCH_CHECK();

@@ -445,34 +445,76 @@ namespace tmi {
 // mvec row m in ascending j -- the order of the reference's dense dot
 // (vector_imp.hpp:1045-1052), whose other terms are exact zeros.
 // phi element (q, j) is at phi[q*sq + j*sj] (complex: interleaved).
-__global__ void k_project(int nqm, const int *__restrict__ lq, const int *__restrict__ lm,
-                          const int *__restrict__ mrp, const int *__restrict__ mci, const double *__restrict__ mval,
-                          const double *__restrict__ phi, long sq, long sj, int cplx, double *__restrict__ proj)
+// Thread per (detector m, source q), the sources of one detector on
+// consecutive lanes: a wave walks one mvec row (entries broadcast to the
+// lanes) and, with node-major fields (sq = 1), reads consecutive phi
+// entries; lanes whose pair is not linked (lk = -1) stay idle.
+template <bool CPLX>
+__global__ void k_project(int nq, int nm, const int *__restrict__ lk, const int *__restrict__ mrp,
+                          const int *__restrict__ mci, const double *__restrict__ mval, const double *__restrict__ phi,
+                          long sq, long sj, double *__restrict__ proj)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nqm) return;
-    int q = lq[k], m = lm[k];
+    const int qt = (nq + 63) / 64 * 64;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // one detector per wave (qt is a multiple of 64): its row index is uniform
+    const int m = __builtin_amdgcn_readfirstlane((int)(t / qt)), q = (int)(t % qt);
+    if (m >= nm || q >= nq) return;
+    const int k = lk[(long)m * nq + q];
+    if (k < 0) return;
+    // the row's terms are loaded PB at a time (the row entries first, then
+    // all field entries of the batch in flight together) and summed one
+    // after another in ascending order
+    constexpr int PB = 32;
+    constexpr int CW = CPLX ? 2 : 1;
     double sr = 0.0, si = 0.0;
-    for (int t = mrp[m]; t < mrp[m + 1]; t++) {
-        long j = mci[t];
-        long at = (long)q * sq + j * sj;
-        if (cplx) {
-            double a = phi[2 * at], b = phi[2 * at + 1], c = mval[2 * (long)t], d = mval[2 * (long)t + 1];
-            sr += a * c - b * d;
-            si += a * d + b * c;
-        } else {
-            sr += phi[at] * mval[t];
+    const int e0 = mrp[m], e1 = mrp[m + 1];
+    for (int e = e0; e < e1; e += PB) {
+        long jj[PB];
+        double c[PB], d[PB], a[PB], b[PB];
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const int eu = min(e + u, e1 - 1);
+            jj[u] = mci[eu];
+            c[u] = mval[CW * (long)eu];
+            d[u] = CPLX ? mval[CW * (long)eu + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const long at = (long)q * sq + jj[u] * sj;
+            a[u] = phi[CW * at];
+            b[u] = CPLX ? phi[CW * at + 1] : 0.0;
+        }
+        // keep the batch's loads ahead of the sums (the scheduler would
+        // otherwise sink each load to its use in the serial chain)
+        __builtin_amdgcn_sched_barrier(0);
+        const int nu = min(PB, e1 - e);
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const bool on = u < nu;   // terms past the row end are not added
+            if (CPLX) {
+                const double tr = a[u] * c[u] - b[u] * d[u], ti = a[u] * d[u] + b[u] * c[u];
+                sr = on ? sr + tr : sr;
+                si = on ? si + ti : si;
+            } else {
+                const double tr = a[u] * c[u];
+                sr = on ? sr + tr : sr;
+            }
         }
     }
-    if (cplx) { proj[2 * (long)k] = sr; proj[2 * (long)k + 1] = si; }
+    if (CPLX) { proj[2 * (long)k] = sr; proj[2 * (long)k + 1] = si; }
     else proj[k] = sr;
 }
 
-void launch_project(int nqm, const int *d_lq, const int *d_lm, const int *mrp, const int *mci, const double *mval,
+void launch_project(int nq, int nm, const int *d_lk, const int *mrp, const int *mci, const double *mval,
                     const double *phi, long sq, long sj, int cplx, double *proj, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_project, dim3(grid_for(nqm, 128)), dim3(128), 0, st, nqm, d_lq, d_lm, mrp, mci, mval, phi, sq,
-                       sj, cplx, proj);
+    const long threads = (long)nm * ((nq + 63) / 64 * 64);
+    if (cplx)
+        hipLaunchKernelGGL(k_project<true>, dim3(grid_for(threads, 256)), dim3(256), 0, st, nq, nm, d_lk, mrp, mci,
+                           mval, phi, sq, sj, proj);
+    else
+        hipLaunchKernelGGL(k_project<false>, dim3(grid_for(threads, 256)), dim3(256), 0, st, nq, nm, d_lk, mrp, mci,
+                           mval, phi, sq, sj, proj);
     TMI_LAUNCH_CHECK();
 }
 

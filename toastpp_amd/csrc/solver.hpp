@@ -107,8 +107,9 @@ struct Solver {
     int count_active(Exec &e, const int *d_flags, int K);
 };
 
-// projections (mesh_kernels.hip): proj[k] = sum_j phi[q_k][j] * mvec[m_k][j]
-void launch_project(int nqm, const int *d_lq, const int *d_lm, const int *mrp, const int *mci, const double *mval,
+// projections (mesh_kernels.hip): proj[k] = sum_j phi[q][j] * mvec[m][j] for
+// the link k = lk[m * nq + q] of every linked pair (lk = -1: not linked)
+void launch_project(int nq, int nm, const int *d_lk, const int *mrp, const int *mci, const double *mval,
                     const double *phi, long sq, long sj, int cplx, double *proj, hipStream_t st);
 
 }  // namespace tmi
