@@ -56,12 +56,17 @@ static void ensure_links(Mesh &m, hipStream_t st)
 {
     if (m.links_valid) return;
     const QM &qm = m.qm;
-    std::vector<int> lq(qm.nqm), lm(qm.nqm);
+    std::vector<int> lq(qm.nqm), lm(qm.nqm), lk((size_t)qm.nq * qm.nm, -1);
     for (int q = 0; q < qm.nq; q++)
-        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) { lq[k] = q; lm[k] = qm.lidx[k]; }
+        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) {
+            lq[k] = q;
+            lm[k] = qm.lidx[k];
+            lk[(size_t)q * qm.nm + qm.lidx[k]] = k;
+        }
     m.d_lptr.upload(qm.lptr.data(), qm.lptr.size(), st);
     m.d_lq.upload(lq.data(), lq.size(), st);
     m.d_lm.upload(lm.data(), lm.size(), st);
+    m.d_lk.upload(lk.data(), lk.size(), st);
     TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
     m.links_valid = true;
 }
@@ -395,7 +400,7 @@ static void jacobian_raster_dev(Raster &R, int nq, int nm, const double *d_dphi,
     TMI_CHECK(ldd >= nq && lda >= nm, TMI_ERR_INVALID_ARG, "field row strides below the field counts");
     TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
     ensure_links(m, st);
-    launch_jacobian_grid(R, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lm.p, qm.nqm, d_proj,
+    launch_jacobian_grid(R, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lk.p, qm.nqm, d_proj,
                          dscale == TMI_DATA_LOG ? 1 : 0, d_J, st, cw, d_Jkap);
 }
 

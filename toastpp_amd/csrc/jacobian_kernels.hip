@@ -35,6 +35,19 @@ constexpr int JAC_NT = 256;
 #endif
 constexpr int JAC_LDS_KIB = TMI_JAC_LDS_KIB;
 
+// J and the node-major intermediate are written once and read at most once:
+// streaming (nontemporal) accesses keep them out of the caches
+typedef double jac_dv2 __attribute__((ext_vector_type(2)));
+__device__ inline void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ inline void st_stream(double2 *p, double2 v)
+{
+    __builtin_nontemporal_store(__builtin_bit_cast(jac_dv2, v), (jac_dv2 *)p);
+}
+__device__ inline double2 ld_stream(const double2 *p)
+{
+    return __builtin_bit_cast(double2, __builtin_nontemporal_load((const jac_dv2 *)p));
+}
+
 struct JacDims {
     int n, nq, nm, nqm, dmax, chunk, ldd, lda;
 };
@@ -176,8 +189,8 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
                         pr += fv.x * kv.x - fv.y * kv.y;
                         pi += fv.x * kv.y + fv.y * kv.x;
                     }
-                    out[2 * k] = make_double2(mr, mi);
-                    out[2 * k + 1] = make_double2(pr, pi);
+                    st_stream(out + 2 * k, make_double2(mr, mi));
+                    st_stream(out + 2 * k + 1, make_double2(pr, pi));
                 }
             }
             __syncthreads();
@@ -202,8 +215,8 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
         if (i < n && k < nqm) {
-            tm[r][c] = Pt[((size_t)i * nqm + k) * 2];
-            tk[r][c] = Pt[((size_t)i * nqm + k) * 2 + 1];
+            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2);
+            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1);
         }
     }
     __syncthreads();
@@ -219,10 +232,10 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
                 a = make_double2((a.x * p.x + a.y * p.y) / den, (a.y * p.x - a.x * p.y) / den);
                 b = make_double2((b.x * p.x + b.y * p.y) / den, (b.y * p.x - b.x * p.y) / den);
             }
-            J[(size_t)k * ld + i] = -a.x;
-            J[(size_t)(k + nqm) * ld + i] = -a.y;
-            J[(size_t)k * ld + n + i] = -b.x;
-            J[(size_t)(k + nqm) * ld + n + i] = -b.y;
+            st_stream(J + (size_t)k * ld + i, -a.x);
+            st_stream(J + (size_t)(k + nqm) * ld + i, -a.y);
+            st_stream(J + (size_t)k * ld + n + i, -b.x);
+            st_stream(J + (size_t)(k + nqm) * ld + n + i, -b.y);
         }
     }
 }
@@ -242,8 +255,8 @@ __global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const 
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
         if (i < n && k < nqm) {
-            tm[r][c] = Pt[((size_t)i * nqm + k) * 2].x;
-            tk[r][c] = Pt[((size_t)i * nqm + k) * 2 + 1].x;
+            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2).x;
+            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1).x;
         }
     }
     __syncthreads();
@@ -257,8 +270,8 @@ __global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const 
                 a *= ip;
                 b *= ip;
             }
-            if (Jmua) Jmua[(size_t)k * n + i] = a;
-            if (Jkap) Jkap[(size_t)k * n + i] = b;
+            if (Jmua) st_stream(Jmua + (size_t)k * n + i, a);
+            if (Jkap) st_stream(Jkap + (size_t)k * n + i, b);
         }
     }
 }

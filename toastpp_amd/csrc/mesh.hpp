@@ -41,7 +41,7 @@ struct Mesh {
     // and detector per link; rebuilt when the QM changes) and the Jacobian's
     // node-major intermediate, kept between calls
     bool links_valid = false;
-    DBuf<int> d_lptr, d_lq, d_lm;
+    DBuf<int> d_lptr, d_lq, d_lm, d_lk;   // link list; d_lk[q * nm + m] = link of (q, m) or -1
     DBuf<double> jac_ws;
     // node orders of the solvers' private copies of the system (order.cpp),
     // one per kind (ORDER_NATURAL, ORDER_MORTON): row r of the relabelled

@@ -64,8 +64,9 @@ void launch_data_gradient(Raster &r, int nq, const double *d_dphi, int ldd, cons
 // cw: CW Jacobian of real fields given with zero imaginary parts; d_proj is
 // then nqm real projections, d_J the nqm x slen mua block and d_Jkap the
 // kappa block (either may be null).
+// d_lk: nq x nm link index of each (source, detector) pair, -1 if not linked
 void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
-                          const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
-                          double *d_J, hipStream_t st, bool cw = false, double *d_Jkap = nullptr);
+                          const int *d_lk, int nqm, const double *d_proj, int logscale, double *d_J, hipStream_t st,
+                          bool cw = false, double *d_Jkap = nullptr);
 
 }  // namespace tmi

@@ -642,45 +642,62 @@ struct GradDesc {
     double ih[3], i2h[3];
 };
 
-// columns c >= unmasked_from take their gradients without the mask
-__global__ void k_grid_grad(int slen, int ncols, const int *__restrict__ s2b, const int *__restrict__ mask,
-                            const double2 *__restrict__ fgv, GradDesc gd, double2 *__restrict__ fgs, int unmasked_from)
+// columns c >= unmasked_from take their gradients without the mask.  Block
+// of 64 column lanes x 4 voxel lanes over 16 consecutive voxels: the reads
+// of a grid point's columns are contiguous across the lanes, and the 16
+// voxels of a column are written as one contiguous run.
+constexpr int GG_V = 16;
+
+__global__ void __launch_bounds__(256) k_grid_grad(int slen, int ncols, const int *__restrict__ s2b,
+                                                   const int *__restrict__ mask, const double2 *__restrict__ fgv,
+                                                   GradDesc gd, double2 *__restrict__ fgs, int unmasked_from)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)slen * ncols) return;
-    const int s = (int)(t % slen), c = (int)(t / slen);
-    const long g = s2b[s];
-    const double2 *v = fgv + c;    // column c of [g][ncols]
+    const int c0 = threadIdx.x & 63, vl = threadIdx.x >> 6;
     const long ldv = ncols;
-    const int pos[3] = {(int)(g % gd.nx), (int)((g / gd.nx) % gd.ny), (int)(g / ((long)gd.nx * gd.ny))};
     const int len[3] = {gd.nx, gd.ny, gd.nz};
     const long str[3] = {1, gd.nx, (long)gd.nx * gd.ny};
-    double2 *o = fgs + ((long)c * slen + s) * 4;
-    o[0] = v[g * ldv];
-    for (int d = 0; d < 3; d++)
-        o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, c >= unmasked_from, g, pos[d], len[d], str[d], gd.ih[d], gd.i2h[d])
-                              : make_double2(0.0, 0.0);
+    for (int i = vl; i < GG_V; i += 4) {
+        const int s = blockIdx.x * GG_V + i;
+        if (s >= slen) break;
+        const long g = s2b[s];
+        const int pos[3] = {(int)(g % gd.nx), (int)((g / gd.nx) % gd.ny), (int)(g / ((long)gd.nx * gd.ny))};
+        for (int c = c0; c < ncols; c += 64) {
+            const double2 *v = fgv + c;    // column c of [g][ncols]
+            double2 *o = fgs + ((long)c * slen + s) * 4;
+            o[0] = v[g * ldv];
+            for (int d = 0; d < 3; d++)
+                o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, c >= unmasked_from, g, pos[d], len[d], str[d], gd.ih[d],
+                                              gd.i2h[d])
+                                      : make_double2(0.0, 0.0);
+        }
+    }
 }
 
-// J rows of one source: block (voxel tile of 64, source q); thread
-// (voxel, link lane of 4) walks the links of q.  PMDF_mua = -d a,
-// PMDF_kap = -sum_d grad_d d grad_d a (jacobian.cc:57-79), divided by the
-// projection for log data, times the voxel size sz.
-__global__ void __launch_bounds__(256) k_jac_grid(int slen, int nq, int dim, const double2 *__restrict__ fgs,
-                                                  const int *__restrict__ lptr, const int *__restrict__ lm,
-                                                  const double2 *__restrict__ proj, int logscale, double sz,
-                                                  int nqm, double *__restrict__ J)
+// J rows through the detectors: block (voxel tile of 64, group of JG_M
+// detectors), thread (voxel, detector m) holds the adjoint values and
+// gradients of m at its voxel and walks the sources q linked to m, reading
+// the direct ones (shared by the block's detectors, so each adjoint value
+// is read once and each direct one once per detector group).  PMDF_mua =
+// -d a, PMDF_kap = -sum_d grad_d d grad_d a (jacobian.cc:57-79), divided by
+// the projection for log data, times the voxel size sz.
+constexpr int JG_M = 8;
+
+__global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm, int dim,
+                                                        const double2 *__restrict__ fgs, const int *__restrict__ lk,
+                                                        const double2 *__restrict__ proj, int logscale, double sz,
+                                                        int nqm, double *__restrict__ J)
 {
-    const int q = blockIdx.y;
-    const int vs = threadIdx.x & 63, lk = threadIdx.x >> 6;
-    const int s = blockIdx.x * 64 + vs;
-    if (s >= slen) return;
-    const double2 *D = fgs + ((long)q * slen + s) * 4;
-    const double2 d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
+    const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int m = blockIdx.y * JG_M + (threadIdx.x >> 6);
+    if (s >= slen || m >= nm) return;
+    const double2 *A = fgs + ((long)(nq + m) * slen + s) * 4;
+    const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3];
     const size_t ld = 2 * (size_t)slen;
-    for (int k = lptr[q] + lk; k < lptr[q + 1]; k += 4) {
-        const double2 *A = fgs + ((long)(nq + lm[k]) * slen + s) * 4;
-        const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3];
+    for (int q = 0; q < nq; q++) {
+        const int k = lk[(long)q * nm + m];
+        if (k < 0) continue;
+        const double2 *D = fgs + ((long)q * slen + s) * 4;
+        const double2 d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
         double pm_r = -(d0.x * a0.x - d0.y * a0.y), pm_i = -(d0.x * a0.y + d0.y * a0.x);
         double pk_r = 0.0, pk_i = 0.0;
         pk_r -= d1.x * a1.x - d1.y * a1.y;
@@ -698,10 +715,11 @@ __global__ void __launch_bounds__(256) k_jac_grid(int slen, int nq, int dim, con
             const double br = (pk_r * p.x + pk_i * p.y) / den, bi = (pk_i * p.x - pk_r * p.y) / den;
             pm_r = ar; pm_i = ai; pk_r = br; pk_i = bi;
         }
-        J[(size_t)k * ld + s] = pm_r * sz;
-        J[(size_t)(k + nqm) * ld + s] = pm_i * sz;
-        J[(size_t)k * ld + slen + s] = pk_r * sz;
-        J[(size_t)(k + nqm) * ld + slen + s] = pk_i * sz;
+        // J is written once: streaming stores keep it out of the caches
+        __builtin_nontemporal_store(pm_r * sz, J + (size_t)k * ld + s);
+        __builtin_nontemporal_store(pm_i * sz, J + (size_t)(k + nqm) * ld + s);
+        __builtin_nontemporal_store(pk_r * sz, J + (size_t)k * ld + slen + s);
+        __builtin_nontemporal_store(pk_i * sz, J + (size_t)(k + nqm) * ld + slen + s);
     }
 }
 
@@ -709,33 +727,35 @@ __global__ void __launch_bounds__(256) k_jac_grid(int slen, int nq, int dim, con
 // scaling of :160-186) from real fields given with zero imaginary parts:
 // Jmua[k][s] = (-d a sz) (1/proj_k), Jkap[k][s] = (-sum_d grad_d d grad_d a sz)
 // (1/proj_k), the adjoint gradients taken without the mask; either output may
-// be absent.
-__global__ void __launch_bounds__(256) k_jac_grid_cw(int slen, int nq, int dim, const double2 *__restrict__ fgs,
-                                                     const int *__restrict__ lptr, const int *__restrict__ lm,
-                                                     const double *__restrict__ proj, int logscale, double sz,
-                                                     double *__restrict__ Jmua, double *__restrict__ Jkap)
+// be absent.  Same thread mapping as k_jac_grid.
+__global__ void __launch_bounds__(64 * JG_M) k_jac_grid_cw(int slen, int nq, int nm, int dim,
+                                                           const double2 *__restrict__ fgs,
+                                                           const int *__restrict__ lk, const double *__restrict__ proj,
+                                                           int logscale, double sz, double *__restrict__ Jmua,
+                                                           double *__restrict__ Jkap)
 {
-    const int q = blockIdx.y;
-    const int vs = threadIdx.x & 63, lk = threadIdx.x >> 6;
-    const int s = blockIdx.x * 64 + vs;
-    if (s >= slen) return;
-    const double2 *D = fgs + ((long)q * slen + s) * 4;
-    const double d0 = D[0].x, d1 = D[1].x, d2 = D[2].x, d3 = D[3].x;
-    for (int k = lptr[q] + lk; k < lptr[q + 1]; k += 4) {
-        const double2 *A = fgs + ((long)(nq + lm[k]) * slen + s) * 4;
-        double pm = -(d0 * A[0].x) * sz;
+    const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int m = blockIdx.y * JG_M + (threadIdx.x >> 6);
+    if (s >= slen || m >= nm) return;
+    const double2 *A = fgs + ((long)(nq + m) * slen + s) * 4;
+    const double a0 = A[0].x, a1 = A[1].x, a2 = A[2].x, a3 = A[3].x;
+    for (int q = 0; q < nq; q++) {
+        const int k = lk[(long)q * nm + m];
+        if (k < 0) continue;
+        const double2 *D = fgs + ((long)q * slen + s) * 4;
+        double pm = -(D[0].x * a0) * sz;
         double pk = 0.0;
-        pk -= d1 * A[1].x;
-        pk -= d2 * A[2].x;
-        if (dim > 2) pk -= d3 * A[3].x;
+        pk -= D[1].x * a1;
+        pk -= D[2].x * a2;
+        if (dim > 2) pk -= D[3].x * a3;
         pk *= sz;
         if (logscale) {
             const double ip = 1.0 / proj[k];
             pm *= ip;
             pk *= ip;
         }
-        if (Jmua) Jmua[(size_t)k * slen + s] = pm;
-        if (Jkap) Jkap[(size_t)k * slen + s] = pk;
+        if (Jmua) __builtin_nontemporal_store(pm, Jmua + (size_t)k * slen + s);
+        if (Jkap) __builtin_nontemporal_store(pk, Jkap + (size_t)k * slen + s);
     }
 }
 
@@ -772,8 +792,7 @@ static double grid_fields(Raster &r, int nq, int nm, const double *d_dphi, int l
         gd.ih[d] = 1.0 / h;
         gd.i2h[d] = 0.5 * gd.ih[d];
     }
-    const long cnt_s = (long)r.gslen * ncols;
-    hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((cnt_s + 255) / 256)), dim3(256), 0, st, r.gslen, ncols,
+    hipLaunchKernelGGL(k_grid_grad, dim3((unsigned)((r.gslen + GG_V - 1) / GG_V)), dim3(256), 0, st, r.gslen, ncols,
                        r.gsol.p, r.elref.p, (const double2 *)r.fgv.p, gd, (double2 *)r.fgs.p, unmasked_from);
     TMI_LAUNCH_CHECK();
     return r.voxel_size();
@@ -872,18 +891,19 @@ void launch_data_gradient(Raster &r, int nq, const double *d_dphi, int ldd, cons
 }
 
 void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
-                          const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale,
-                          double *d_J, hipStream_t st, bool cw, double *d_Jkap)
+                          const int *d_lk, int nqm, const double *d_proj, int logscale, double *d_J, hipStream_t st,
+                          bool cw, double *d_Jkap)
 {
     const double sz = grid_fields(r, nq, nm, d_dphi, ldd, d_aphi, lda, cw ? nq : nq + nm, st);
-    const dim3 grid((unsigned)((r.gslen + 63) / 64), nq);
+    const dim3 grid((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + JG_M - 1) / JG_M));
+    const dim3 blk(64 * JG_M);
     if (!r.has_g) {   // solution voxels = grid points inside the mesh
         if (cw)
-            hipLaunchKernelGGL(k_jac_grid_cw, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
-                               d_lptr, d_lm, d_proj, logscale, sz, d_J, d_Jkap);
+            hipLaunchKernelGGL(k_jac_grid_cw, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
+                               d_lk, d_proj, logscale, sz, d_J, d_Jkap);
         else
-            hipLaunchKernelGGL(k_jac_grid, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
-                               d_lptr, d_lm, (const double2 *)d_proj, logscale, sz, nqm, d_J);
+            hipLaunchKernelGGL(k_jac_grid, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
+                               d_lk, (const double2 *)d_proj, logscale, sz, nqm, d_J);
         TMI_LAUNCH_CHECK();
         return;
     }
@@ -893,8 +913,8 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
     const size_t gs = (size_t)r.gslen;
     if (cw) {
         DBuf<double> jm(d_J ? (size_t)nqm * gs : 0), jk(d_Jkap ? (size_t)nqm * gs : 0);
-        hipLaunchKernelGGL(k_jac_grid_cw, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p,
-                           d_lptr, d_lm, d_proj, 0, 1.0, jm.p, jk.p);
+        hipLaunchKernelGGL(k_jac_grid_cw, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
+                           d_lk, d_proj, 0, 1.0, jm.p, jk.p);
         TMI_LAUNCH_CHECK();
         const double *rs = logscale ? d_proj : nullptr;
         if (d_J) jac_to_basis(r, nqm, jm.p, (long)gs, d_J, r.slen, sz, rs, st);
@@ -903,8 +923,8 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
         return;
     }
     DBuf<double> jg((size_t)4 * nqm * gs);
-    hipLaunchKernelGGL(k_jac_grid, grid, dim3(256), 0, st, r.gslen, nq, r.dim, (const double2 *)r.fgs.p, d_lptr,
-                       d_lm, (const double2 *)d_proj, logscale, 1.0, nqm, jg.p);
+    hipLaunchKernelGGL(k_jac_grid, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p, d_lk,
+                       (const double2 *)d_proj, logscale, 1.0, nqm, jg.p);
     TMI_LAUNCH_CHECK();
     const long ldo = 2 * (long)r.slen;
     jac_to_basis(r, 2 * nqm, jg.p, 2 * (long)gs, d_J, ldo, sz, nullptr, st);                  // mua block
