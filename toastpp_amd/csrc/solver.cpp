@@ -138,6 +138,12 @@ void Solver::reset_precon()
     precon_valid = true;
 }
 
+// workgroups of a tile's block kernels and reduction partials, at most
+// (2048 measured best for the 250k-node, 64-column solve)
+#ifndef TMI_WORK_G
+#define TMI_WORK_G 2048
+#endif
+
 Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md)
 {
     auto &w = e.wk[log2i(K)];
@@ -153,7 +159,7 @@ Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md)
     // reduction workgroups: a multiple of 8 (one slice per XCD), enough to
     // fill the GPU, few enough that the per-block partials stay small
     long rows_work = ((long)n * K + 1023) / 1024;
-    int G = (int)std::min<long>(2048, std::max<long>(8, rows_work));
+    int G = (int)std::min<long>(TMI_WORK_G, std::max<long>(8, rows_work));
     G = (G + 7) / 8 * 8;
     w.G = G;
     size_t nb64 = (size_t)n * K * vbytes(vk64()), nbi = (size_t)n * K * vbytes(vin);
