@@ -40,10 +40,13 @@ Solver::Exec::~Exec()
 size_t Solver::Exec::bytes() const
 {
     size_t b = 0;
-    for (const Work &w : wk)
-        b += w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
-             w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
-             w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+    for (const Work *ws : {wk, wk_alt})
+        for (int i = 0; i < 7; i++) {
+            const Work &w = ws[i];
+            b += w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
+                 w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
+                 w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+        }
     return b;
 }
 
@@ -68,7 +71,8 @@ int Solver::tile_for(int nq, int md) const
 size_t Solver::work_bytes(int nq) const
 {
     // the tile and, with compaction, its narrower copies (K/2 + K/4 + ... < K)
-    const size_t K = tile_for(nq), n = mesh->n, Kall = compact_min > 0 ? 2 * K : K;
+    // and the second workspaces of the repacked widths (K and K/2)
+    const size_t K = tile_for(nq), n = mesh->n, Kall = compact_min > 0 ? 2 * K + K + K / 2 : K;
     const size_t s64 = vbytes(vk64()), sin = vbytes(mode == TMI_MODE_MIXED_BLOCK ? vk32() : vk64());
     return n * Kall * (3 * s64 + 7 * sin) + 2 * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
 }
@@ -144,9 +148,9 @@ void Solver::reset_precon()
 #define TMI_WORK_G 2048
 #endif
 
-Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md)
+Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md, bool alt)
 {
-    auto &w = e.wk[log2i(K)];
+    auto &w = alt ? e.wk_alt[log2i(K)] : e.wk[log2i(K)];
     int n = mesh->n;
     VKind vin = (md == TMI_MODE_MIXED_BLOCK) ? vk32() : vk64();
     if (!e.h_flags) TMI_HIP(hipHostMalloc((void **)&e.h_flags, sizeof(int) * I_NSLOT * 64 * 2));
@@ -181,6 +185,24 @@ int Solver::count_active(Exec &e, const int *d_flags, int K)
 }
 
 static constexpr int CHECK_EVERY = 8;
+
+// Repacking the running columns to the front of the tile pays when it
+// frees a whole 128-byte line (16 complex FP32 columns) of every row, or
+// when three or more lines are only partly active (each costs partial
+// stores); flags: the running flag of each of the K columns.
+static bool repack_pays(const int *flags, int K, int nact)
+{
+    const int LINE = 16, nl = K / LINE;
+    int touched = 0, partial = 0;
+    for (int l = 0; l < nl; l++) {
+        int a = 0;
+        for (int j = l * LINE; j < (l + 1) * LINE; j++) a += flags[j] != 0;
+        touched += a > 0;
+        partial += a > 0 && a < LINE;
+    }
+    const int packed = (nact + LINE - 1) / LINE;
+    return touched > packed || partial >= 3;
+}
 
 // One tile: BiCGSTAB (complex) or PCG (real) on K columns with reliable
 // updates (Sleijpen & van der Vorst): the inner iteration runs in the inner
@@ -302,8 +324,13 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     };
     // narrow the tile to the columns still running: finished columns are
     // recorded and written out, the state of the others moves to the
-    // workspace of the smallest power-of-two width that holds them
-    auto compact = [&]() -> bool {
+    // workspace of the smallest power-of-two width that holds them.  With
+    // repack, the width stays and the running columns move to the front of
+    // the second workspace of that width: converged columns scattered over
+    // the tile leave every 128-byte line of a row partly active (partial,
+    // read-modify-write stores and no line skipped); packed, the lines past
+    // the running columns are not touched at all.
+    auto compact = [&](bool repack) -> bool {
         TMI_HIP(hipMemcpyAsync(h_oflags, oact, sizeof(int) * K, hipMemcpyDeviceToHost, st));
         TMI_HIP(hipStreamSynchronize(st));
         std::vector<int> done(K, -1), sel;
@@ -315,9 +342,11 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         int K2 = 1;
         while (K2 < (int)sel.size()) K2 *= 2;
         K2 = std::max(K2, compact_min);
-        if (K2 >= K) return false;
+        if (repack) K2 = K;
+        else if (K2 >= K) return false;
         finish(done);
-        Exec::Work &w2 = ensure_work(e, K2, md);
+        const bool alt = repack && wp == &e.wk[log2i(K)];
+        Exec::Work &w2 = ensure_work(e, K2, md, alt);
         const int ns = (int)sel.size();
         TMI_HIP(hipMemcpyAsync(e.cols.p + 64, sel.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
         const int *d_sel = e.cols.p + 64;
@@ -331,9 +360,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         mv(sin, n, wp->ri.p, w2.ri.p);
         mv(sin, n, wp->rdi.p, w2.rdi.p);
         mv(sin, n, wp->p.p, w2.p.p);
-        mv(sin, n, wp->v.p, w2.v.p);
-        mv(sin, n, wp->s.p, w2.s.p);
-        mv(sin, n, wp->t.p, w2.t.p);
+        mv(sin, n, wp->v.p, w2.v.p);   // s and t are rewritten before they are read
         mv(8, S_NSLOT, sc, w2.sc.p);
         mv(8, S_NSLOT, osc, w2.osc.p);
         mv(4, I_NSLOT, ic, w2.ic.p);
@@ -349,7 +376,8 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     for (;;) {
         const int nact = count_active(e, oact, K);
         if (nact == 0) break;
-        if (compact_min > 0 && nact <= K / 2 && K / 2 >= compact_min) compact();
+        if (compact_min > 0 && nact <= K / 2 && K / 2 >= compact_min) compact(false);
+        else if (compact_min > 0 && K >= 32 && repack_pays(e.h_flags, K, nact)) compact(true);
         if (count_active(e, ic + I_ACTIVE * K, K) == 0) {
             // every column's recursive residual met the tolerance: FP64 modes
             // stop here (as the reference does); the mixed mode verifies in FP64

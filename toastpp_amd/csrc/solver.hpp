@@ -58,6 +58,7 @@ struct Solver {
             DBuf<int> ic, oic;
         };
         Work wk[7];                    // one workspace per tile width 1, 2, 4, ..., 64
+        Work wk_alt[7];                // second workspace of a width, for repacking in place
         Work &w = wk[6];               // the 64-column workspace
         DBuf<int> cols;                // column maps of compaction (2 x 64)
         int *h_flags = nullptr;        // pinned host mirror of per-column ints (inner, outer)
@@ -102,7 +103,7 @@ struct Solver {
 
    private:
     BlockDesc block(const Exec::Work &w, int K) const;
-    Exec::Work &ensure_work(Exec &e, int K, int md);
+    Exec::Work &ensure_work(Exec &e, int K, int md, bool alt = false);
     int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md);
     int count_active(Exec &e, const int *d_flags, int K);
 };
