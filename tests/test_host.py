@@ -102,3 +102,21 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_raster_and_data_file_entry_points_reject_bad_arguments():
+    """Argument checks of the raster and data-file entry points run before
+    any GPU work and come back as status codes with a message."""
+    import ctypes as C
+    from toastpp_amd import _lib
+    L = _lib.lib()
+    out = C.c_void_p()
+    g = np.array([8, 8], np.int32)
+    assert L.tmi_raster_create_basis(None, g, g, None, C.byref(out)) == _lib.TMI_ERR_INVALID_ARG
+    assert b"NULL" in L.tmi_last_error()
+    n = C.c_int()
+    assert L.tmi_raster_basis_len(None, C.byref(n), None) == _lib.TMI_ERR_INVALID_ARG
+    assert L.tmi_read_nim(b"/nonexistent/x.nim", 0, C.byref(n), None, None) == _lib.TMI_ERR_IO
+    assert L.tmi_read_vector(b"/nonexistent/x.fem", C.byref(n), None) == _lib.TMI_ERR_IO
+    v = np.zeros(3)
+    assert L.tmi_write_nim(b"/tmp/x.nim", b"m", 3, 1, v, 7) == _lib.TMI_ERR_INVALID_ARG
