@@ -25,6 +25,18 @@
  *     raster Jacobian vs nodal Jacobian contracted with the grid map).
  *     The reference's EPS and edge rules are restated but not pinned by a
  *     reference output: parity unpinned beyond those invariants.
+ *   - NodeMapMatrix: weights summing to 1 at every node, exact
+ *     reproduction of linear fields where all grid corners are inside.
+ *   - linear pixel maps (Grid2LinPixMatrix, LinPix2GridMatrix,
+ *     SubsampleLinPixel): brute-force tent weights, exact interpolation of
+ *     linear fields, the solution voxels as the support of the inside mask;
+ *     the Jacobians and the data gradient in a coarser basis equal the
+ *     grid-basis ones mapped through G.
+ *   - CW Jacobians and the data gradient: identities with the pinned
+ *     complex Jacobians (real parts, the adjoint-source linearity of
+ *     AddDataGradient).
+ *   - QVec_Point: a unit vector at a node, partition of unity and linear
+ *     reproduction inside an element, boundary nodes only for Neumann.
  */
 #include <complex.h>
 #include <math.h>
