@@ -82,8 +82,12 @@ inline size_t vbytes(VKind k) { return k == VK_C64 ? 16 : (k == VK_R32 ? 4 : 8);
 // ---- launchers (krylov_kernels.hip) ----
 // Y = A * D^-1 X (D^-1 = idiag, or identity if null); partials
 // red[0] = <W, Y>_R and, when want_yy, red[1] = <Y, Y>_R, per column.
+// Wq (optional): W given as the packed shadow residual of a 64-column
+// complex FP32 tile (launch_pack_shadow); used by the row-block kernel,
+// the other kernels read W, which must hold the same values.
 void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
-                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st);
+                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st,
+                     const unsigned char *Wq = nullptr);
 // R = B - A X (FP64), red[0] = ||R||^2
 void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
                   const int *iflags, double *partial, hipStream_t st);
@@ -95,7 +99,10 @@ void launch_bicg_s(VKind vk, const BlockDesc &b, const void *r, const void *v, v
                    const int *ic, hipStream_t st);
 void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *p, const void *s,
                     const void *t, void *r, const void *rd, const double *sc, const int *ic, double *partial,
-                    hipStream_t st);
+                    hipStream_t st, const unsigned char *rdq = nullptr);
+// packed copy (4 bits per entry) of the sign-pattern shadow residual rd of
+// a 64-column complex FP32 tile: q is n x 32 bytes
+void launch_pack_shadow(int n, const void *rd, unsigned char *q, hipStream_t st);
 // PCG element-wise steps
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
                   const double *sc, const int *ic, double *partial, hipStream_t st);
@@ -132,7 +139,8 @@ void launch_inner_reset(const BlockDesc &b, const double *osc, const int *oic, d
 // mixed-precision refinement helpers
 // r32 = r64 * scale[c]; rd32 = r32; x32 = 0 ; partial [<r32,r32>, <rd32,r32>]
 void launch_to_inner(VKind vk32, const BlockDesc &b, const void *r64, void *r32, void *rd32, void *x32,
-                     const double *scale, const int *oflags, double *partial, hipStream_t st);
+                     const double *scale, const int *oflags, double *partial, hipStream_t st,
+                     bool sign_shadow = false);
 // reliable update of the inner residual: r_in = r64 * scale, x_in = 0,
 // partials [<r_in, r_in>, <rd, r_in>] (rd given) or [<r_in,r_in>, <r_in, D^-1 r_in>]
 void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,

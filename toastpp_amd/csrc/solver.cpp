@@ -45,7 +45,7 @@ size_t Solver::Exec::bytes() const
             const Work &w = ws[i];
             b += w.x64.bytes() + w.b64.bytes() + w.r64.bytes() + w.xi.bytes() + w.ri.bytes() + w.rdi.bytes() +
                  w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
-                 w.osc.bytes() + w.ic.bytes() + w.oic.bytes();
+                 w.osc.bytes() + w.ic.bytes() + w.oic.bytes() + w.rdq.bytes();
         }
     return b;
 }
@@ -170,6 +170,7 @@ Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md, bool alt)
     w.x64.alloc(nb64); w.b64.alloc(nb64); w.r64.alloc(nb64);
     w.xi.alloc(nbi); w.ri.alloc(nbi); w.rdi.alloc(nbi); w.p.alloc(nbi); w.v.alloc(nbi); w.s.alloc(nbi); w.t.alloc(nbi);
     w.partial.alloc((size_t)2 * G * K);
+    if (K == 64 && vin == VK_C32) w.rdq.alloc((size_t)n * 32);
     w.sc.alloc((size_t)S_NSLOT * K); w.osc.alloc((size_t)S_NSLOT * K);
     w.ic.alloc((size_t)I_NSLOT * K); w.oic.alloc((size_t)I_NSLOT * K);
     return w;
@@ -249,9 +250,13 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     double *sc, *osc, *part, *scale;
     int *ic, *oic;
     const int *oact;
+    // packed copy of the (sign-pattern) shadow residual, read by the
+    // 64-column mixed BiCGSTAB kernels that stream it every iteration
+    unsigned char *rdq = nullptr;
     auto bind = [&]() {
         K = wp->K;
         bd = block(*wp, K);
+        rdq = (K == 64 && mixed && !use_cg && vk == VK_C32) ? wp->rdq.p : nullptr;
         sc = wp->sc.p;
         osc = wp->osc.p;
         part = wp->partial.p;
@@ -278,7 +283,11 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     // inner state
     launch_inner_reset(bd, osc, oic, sc, ic, st);
     TMI_HIP(hipMemcpyAsync(sc + S_TOL2 * K, itol2.data(), K * sizeof(double), hipMemcpyHostToDevice, st));
-    launch_to_inner(vk, bd, wp->r64.p, wp->ri.p, use_cg ? nullptr : wp->rdi.p, wp->xi.p, scale, oact, part, st);
+    // the mixed BiCGSTAB takes the sign pattern of r0 as its shadow residual
+    // (fewer iterations); the FP64 modes keep the reference's rd = r0
+    launch_to_inner(vk, bd, wp->r64.p, wp->ri.p, use_cg ? nullptr : wp->rdi.p, wp->xi.p, scale, oact, part, st,
+                    mixed);
+    if (rdq) launch_pack_shadow(n, wp->rdi.p, rdq, st);
     if (use_cg) {
         launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st);
         launch_finalize(FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
@@ -370,6 +379,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         cmap.swap(cm2);
         wp = &w2;
         bind();
+        if (rdq) launch_pack_shadow(n, wp->rdi.p, rdq, st);
         ncompact++;
         return true;
     };
@@ -400,13 +410,13 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
             } else {
                 launch_bicg_p(vk, bd, wp->ri.p, wp->p.p, wp->v.p, sc, ic, st);
                 launch_spmm_dot(vk, bd, op_in, nullptr, wp->p.p, wp->v.p, wp->rdi.p, false, ic + I_ACTIVE * K, part,
-                                st);
+                                st, rdq);
                 launch_finalize(FIN_BICG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
                 launch_bicg_s(vk, bd, wp->ri.p, wp->v.p, wp->s.p, sc, ic, st);
                 launch_spmm_dot(vk, bd, op_in, nullptr, wp->s.p, wp->t.p, wp->s.p, true, ic + I_ACTIVE * K, part, st);
                 launch_finalize(FIN_BICG_OMEGA, bd, part, sc, ic, 0, 0, 0, st);
                 launch_bicg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->s.p, wp->t.p, wp->ri.p, wp->rdi.p, sc, ic,
-                               part, st);
+                               part, st, rdq);
                 launch_finalize(FIN_BICG_RHO, bd, part, sc, ic, maxit_total, 0, 0, st);
             }
         }

@@ -54,6 +54,7 @@ struct Solver {
             int K = 0, n = 0, G = 0;
             VKind vk_in = VK_C32;
             DBuf<char> x64, b64, r64, xi, ri, rdi, p, v, s, t;
+            DBuf<unsigned char> rdq;   // packed shadow residual (64-column complex FP32 tiles)
             DBuf<double> partial, sc, osc;
             DBuf<int> ic, oic;
         };
