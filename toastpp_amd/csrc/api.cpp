@@ -641,27 +641,29 @@ int tmi_solver_calc_fields(tmi_solver h, int nq, const int *qrp, const int *qci,
     if (nq == 0) return TMI_OK;
     int n = s.mesh->n, cw = s.cplx ? 2 : 1;
     TMI_CHECK(qrp[0] == 0, TMI_ERR_INVALID_ARG, "qvec row pointer must start at 0");
-    // dense node-major right-hand sides and initial guesses
-    std::vector<double> B((size_t)n * nq * cw, 0.0), X((size_t)n * nq * cw);
     for (int q = 0; q < nq; q++) {
         TMI_CHECK(qrp[q + 1] >= qrp[q], TMI_ERR_INVALID_ARG, "qvec row pointer not ascending at %d", q);
-        for (int k = qrp[q]; k < qrp[q + 1]; k++) {
-            int j = qci[k];
-            TMI_CHECK(j >= 0 && j < n, TMI_ERR_INVALID_ARG, "qvec column %d out of range", j);
-            for (int c = 0; c < cw; c++) B[((size_t)j * nq + q) * cw + c] += qval[(size_t)k * cw + c];
-        }
-        for (int j = 0; j < n; j++)
-            for (int c = 0; c < cw; c++) X[((size_t)j * nq + q) * cw + c] = phi[((size_t)q * n + j) * cw + c];
+        for (int k = qrp[q]; k < qrp[q + 1]; k++)
+            TMI_CHECK(qci[k] >= 0 && qci[k] < n, TMI_ERR_INVALID_ARG, "qvec column %d out of range", qci[k]);
     }
-    DBuf<double> dB, dX;
-    dB.upload(B.data(), B.size(), s.st);
-    dX.upload(X.data(), X.size(), s.st);
+    // the caller's arrays go over as they are (qvec CSR, phi source-major);
+    // the dense node-major right-hand sides and initial guesses are formed,
+    // and the fields transposed back, on the GPU
+    const size_t tot = (size_t)n * nq * cw, qnnz = (size_t)qrp[nq];
+    const int es = 8 * cw;
+    DBuf<double> dB(tot), dX(tot), dT(tot), dqv;
+    DBuf<int> dqr, dqc;
+    dqr.upload(qrp, nq + 1, s.st);
+    dqc.upload(qci, qnnz, s.st);
+    dqv.upload(qval, qnnz * cw, s.st);
+    TMI_HIP(hipMemsetAsync(dB.p, 0, tot * sizeof(double), s.st));
+    launch_qvec_dense(nq, cw, dqr.p, dqc.p, dqv.p, dB.p, s.st);
+    TMI_HIP(hipMemcpyAsync(dT.p, phi, tot * sizeof(double), hipMemcpyHostToDevice, s.st));
+    launch_transpose(es, nq, n, dT.p, dX.p, s.st);   // [nq][n] -> [n][nq]
     int notconv = s.solve(nq, dB.p, dX.p, nq, stats);
-    dX.download(X.data(), X.size(), s.st);
+    launch_transpose(es, n, nq, dX.p, dT.p, s.st);   // [n][nq] -> [nq][n]
+    TMI_HIP(hipMemcpyAsync(phi, dT.p, tot * sizeof(double), hipMemcpyDeviceToHost, s.st));
     TMI_HIP(hipStreamSynchronize(s.st));
-    for (int q = 0; q < nq; q++)
-        for (int j = 0; j < n; j++)
-            for (int c = 0; c < cw; c++) phi[((size_t)q * n + j) * cw + c] = X[((size_t)j * nq + q) * cw + c];
     if (notconv) {
         set_last_error(std::to_string(notconv) + " right-hand side(s) did not reach the tolerance");
         return TMI_ERR_NOT_CONVERGED;

@@ -175,6 +175,13 @@ void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const vo
 // is column sel[j] of src for j < nsel, zero beyond (elem_bytes 4, 8, 16)
 void launch_select_cols(int elem_bytes, long rows, int ks, int kd, int nsel, const int *sel, const void *src,
                         void *dst, hipStream_t st);
+// out[c][r] = in[r][c] for a rows x cols array of elem_bytes (8 or 16)
+// entries (tiled through LDS)
+void launch_transpose(int elem_bytes, long rows, int cols, const void *in, void *out, hipStream_t st);
+// dense node-major right-hand sides from a CSR qvec (row q = source q):
+// B[j][q] += val[k] for the entries k of row q, in order (B zeroed first);
+// cw = 2 for complex values
+void launch_qvec_dense(int nq, int cw, const int *rp, const int *ci, const double *val, double *B, hipStream_t st);
 // scatter with a column map: dst[perm[r]][q0 + cmap[j]] = src[r][j] where cmap[j] >= 0
 void launch_scatter_map(int es, int n, int K, const int *perm, const int *cmap, const void *src, void *dst, long ld,
                         int q0, hipStream_t st);

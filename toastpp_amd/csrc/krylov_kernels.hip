@@ -1897,6 +1897,68 @@ void launch_scatter_tile(int es, int n, int K, int kc, const int *perm, const vo
     TMI_LAUNCH_CHECK();
 }
 
+// 32 x 32 tiles: reads along a row of `in`, writes along a row of `out`
+template <class T>
+__global__ void __launch_bounds__(256) k_transpose(long rows, int cols, const T *__restrict__ in, T *__restrict__ out)
+{
+    __shared__ T tile[32][33];
+    const long nct = (cols + 31) / 32;   // column tiles; blockIdx.x = row tile * nct + column tile
+    const long r0 = (long)(blockIdx.x / nct) * 32;
+    const int c0 = (int)(blockIdx.x % nct) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const long r = r0 + k;
+        const int c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k;
+        const long r = r0 + tx;
+        if (r < rows && c < cols) out[(long)c * rows + r] = tile[tx][k];
+    }
+}
+
+void launch_transpose(int elem_bytes, long rows, int cols, const void *in, void *out, hipStream_t st)
+{
+    TMI_CHECK(rows >= 0 && cols >= 0, TMI_ERR_INVALID_ARG, "transpose of %ld x %d", rows, cols);
+    if (rows == 0 || cols == 0) return;
+    const long tiles = ((rows + 31) / 32) * (long)((cols + 31) / 32);
+    TMI_CHECK(tiles < (1L << 31), TMI_ERR_UNSUPPORTED, "transpose of %ld x %d is too large", rows, cols);
+    const dim3 g((unsigned)tiles);
+    switch (elem_bytes) {
+    case 16:
+        hipLaunchKernelGGL(k_transpose<double2>, g, dim3(256), 0, st, rows, cols, (const double2 *)in, (double2 *)out);
+        break;
+    case 8:
+        hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, st, rows, cols, (const double *)in, (double *)out);
+        break;
+    default:
+        TMI_CHECK(false, TMI_ERR_INVALID_ARG, "transpose of %d-byte entries", elem_bytes);
+    }
+    TMI_LAUNCH_CHECK();
+}
+
+// thread per source: its entries added in order, so duplicate columns sum
+// as the host loop over the CSR row would
+__global__ void k_qvec_dense(int nq, int cw, const int *__restrict__ rp, const int *__restrict__ ci,
+                             const double *__restrict__ val, double *__restrict__ B)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    for (int k = rp[q]; k < rp[q + 1]; k++) {
+        const size_t at = ((size_t)ci[k] * nq + q) * cw;
+        for (int c = 0; c < cw; c++) B[at + c] += val[(size_t)k * cw + c];
+    }
+}
+
+void launch_qvec_dense(int nq, int cw, const int *rp, const int *ci, const double *val, double *B, hipStream_t st)
+{
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_qvec_dense, dim3((unsigned)((nq + 63) / 64)), dim3(64), 0, st, nq, cw, rp, ci, val, B);
+    TMI_LAUNCH_CHECK();
+}
+
 void launch_select_cols(int elem_bytes, long rows, int ks, int kd, int nsel, const int *sel, const void *src,
                         void *dst, hipStream_t st)
 {
