@@ -41,6 +41,7 @@ extern "C" {
 #define TMI_LSOLVER_DIRECT    0   /* FP64-accurate solve: BiCGSTAB/PCG to tol 1e-14 on the GPU */
 #define TMI_LSOLVER_CG        1
 #define TMI_LSOLVER_BICGSTAB  2
+#define TMI_LSOLVER_GMRES     3   /* GMRES(20), FP64 in every mode (gmres_imp.hpp:41-230) */
 
 /* preconditioners (PreconType, precon.h) */
 #define TMI_PRECON_NONE  0
@@ -63,9 +64,11 @@ typedef struct tmi_server_s *tmi_server;
 /* per right-hand-side convergence record (IterativeSolverResult, matrix.h:34-37,
  * extended with refinement sweeps and a converged flag) */
 typedef struct {
-    int    iters;      /* Krylov iterations spent on this column (all sweeps) */
-    int    sweeps;     /* FP64 refinement sweeps (1 for FP64 modes) */
-    double relres;     /* final FP64 ||q - K phi|| / ||q|| */
+    int    iters;      /* Krylov iterations spent on this column (all sweeps);
+                          GMRES: restart cycles (gmres() returns them) */
+    int    sweeps;     /* FP64 refinement sweeps (1 for FP64 modes); GMRES: Arnoldi steps */
+    double relres;     /* final FP64 ||q - K phi|| / ||q||; GMRES: of the left-
+                          preconditioned system, ||M^-1 (q - K phi)|| / ||M^-1 q|| */
     int    converged;  /* relres <= tol */
 } tmi_colstat;
 
@@ -151,7 +154,8 @@ int tmi_solver_create(tmi_mesh m, int complex_valued, int lsolver, double tol, t
 int tmi_solver_free(tmi_solver s);
 /* SetLinSolver("DIRECT"|"CG"|"BICGSTAB", tol) (fwdsolver.cc:1279-1299).
  * CG on a complex solver is rejected (the reference's complex CG is a stub
- * that returns the initial guess, gsmatrix_imp.hpp:289-296). */
+ * that returns the initial guess, gsmatrix_imp.hpp:289-296).  Names: DIRECT,
+ * CG, BICGSTAB, GMRES (restart 20, as IterativeSolve, gsmatrix_imp.hpp:477). */
 int tmi_solver_set_linsolver(tmi_solver s, const char *name, double tol);
 int tmi_solver_set_precon(tmi_solver s, int precon);            /* SetPrecon */
 int tmi_solver_set_maxit(tmi_solver s, int maxit);              /* SetLinSolverMaxit; 0 = n+1 */

@@ -215,6 +215,73 @@ def bicgstab(rp, ci, val, b, tol, maxit=0, diag=False, x0=None):
     return x, it, err.value
 
 
+def gmres(rp, ci, val, b, tol, maxit=0, diag=False, x0=None, restart=20):
+    """GMRES(restart) of gmres_imp.hpp:41-230, one right-hand side, as
+    IterativeSolve runs it (gsmatrix_imp.hpp:477-478, 590-593): left
+    diagonal preconditioning (Mb = D^-1 b), modified Gram-Schmidt with
+    H(j,k) = sum_n w_n conj(v_k,n), the reference's rotation (for complex H
+    r = sqrt(H(j,j)^2 + H(j,j+1)^2) without conjugation), back substitution
+    and x += y_l v_l for l = j..0; the cycle limit is maxit (n+1 if 0).
+    Returns (x, cycles, ||M^-1 (b - A x)|| / ||M^-1 b||)."""
+    from scipy import sparse
+    n = len(rp) - 1
+    cplx = np.iscomplexobj(val) or np.iscomplexobj(b)
+    dt = np.complex128 if cplx else np.float64
+    A = sparse.csr_matrix((np.asarray(val, dt), np.asarray(ci), np.asarray(rp)), shape=(n, n))
+    idg = 1.0 / A.diagonal() if diag else None
+    pre = (lambda v: v * idg) if diag else (lambda v: v)
+    b = np.asarray(b, dt)
+    x = np.zeros(n, dt) if x0 is None else np.array(x0, dt)
+    max_cycle = maxit if maxit else n + 1
+    Mb = pre(b)
+    nb = np.sqrt(np.sum(np.abs(Mb) ** 2))
+    v0 = Mb.copy() if np.sqrt(np.sum(np.abs(x) ** 2)) < tol * n else Mb - pre(A @ x)
+    cycle = 0
+    R = restart
+    while True:
+        nv = np.sqrt(np.sum(np.abs(v0) ** 2))
+        if not (nv > tol * nb and cycle < max_cycle):
+            break
+        s = np.zeros(R + 1, dt)
+        s[0] = nv
+        V = [v0 / nv]
+        H = np.zeros((R, R + 1), dt)
+        c1 = np.zeros(R, dt)
+        c2 = np.zeros(R, dt)
+        j = -1
+        cycle += 1
+        while abs(s[j + 1]) > tol * nb and j < R - 1 and cycle < max_cycle:
+            j += 1
+            w = pre(A @ V[j])
+            for k in range(j + 1):
+                h = np.sum(w * np.conj(V[k]))
+                H[j, k] = h
+                w = w - h * V[k]
+            t = np.sqrt(np.sum(np.abs(w) ** 2))
+            H[j, j + 1] = t
+            V.append(w * (1.0 / t))
+            for k in range(j):
+                h1 = c1[k] * H[j, k] + c2[k] * H[j, k + 1]
+                h2 = -c2[k] * H[j, k] + c1[k] * H[j, k + 1]
+                H[j, k], H[j, k + 1] = h1, h2
+            r = np.sqrt(H[j, j] * H[j, j] + H[j, j + 1] * H[j, j + 1])
+            c1[j] = H[j, j] / r
+            c2[j] = H[j, j + 1] / r
+            H[j, j] = r
+            H[j, j + 1] = 0.0
+            s[j + 1] = -c2[j] * s[j]
+            s[j] = s[j] * c1[j]
+        y = np.zeros(R, dt)
+        for l in range(j, -1, -1):
+            y[l] = s[l] / H[l, l]
+            for k in range(l - 1, -1, -1):
+                s[k] -= H[l, k] * y[l]
+        for l in range(j, -1, -1):
+            x = x + y[l] * V[l]
+        v0 = Mb - pre(A @ x)
+    return x, cycle, (nv / nb if nb > 0 else 0.0)
+
+
 def relres(rp, ci, val, b, x):
     return lib().om_relres_c(len(rp) - 1, _c(rp, np.int32), _c(ci, np.int32), _cplx(val), _cplx(b), _cplx(x))
 
@@ -418,6 +485,8 @@ def forward(mesh, qm, mua, mus, ref, freq, qwidth=2.0, mwidth=2.0, solver="bicgs
         if solver == "cg":
             x, _, _ = pcg(rp, ci, val.real.copy(), qv[i], tol, maxit, diag)
             phi[i] = x
+        elif solver == "gmres":
+            phi[i] = gmres(rp, ci, val if freq else val.real.copy(), qv[i], tol, maxit, diag)[0]
         else:
             x, _, _ = bicgstab(rp, ci, val, qv[i].astype(np.complex128), tol, maxit, diag)
             phi[i] = x

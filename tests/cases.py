@@ -33,6 +33,15 @@ CASES = {
                                   zip=os.path.join(G2, "testdata.zip"), mua=0.025, mus=2.0, n=1.4, freq=100.0,
                                   solver="bicgstab", precon="diag",
                                   files={"fmod_cplx_bicgstab_diag.fem": 1e-7, "farg_cplx_bicgstab_diag.fem": 1e-6}),
+    # test/2D/fwdfem/Makefile:128-148
+    "2d_cplx_gmres": dict(mesh=os.path.join(G2, "circle25_32.msh"), qm=os.path.join(G2, "circle25_32x32.qm"),
+                          zip=os.path.join(G2, "testdata.zip"), mua=0.025, mus=2.0, n=1.4, freq=100.0,
+                          solver="gmres", precon=None,
+                          files={"fmod_cplx_gmres.fem": 1e-7, "farg_cplx_gmres.fem": 1e-7}),
+    "2d_cplx_gmres_diag": dict(mesh=os.path.join(G2, "circle25_32.msh"), qm=os.path.join(G2, "circle25_32x32.qm"),
+                               zip=os.path.join(G2, "testdata.zip"), mua=0.025, mus=2.0, n=1.4, freq=100.0,
+                               solver="gmres", precon="diag",
+                               files={"fmod_cplx_gmres_diag.fem": 1e-7, "farg_cplx_gmres_diag.fem": 1e-7}),
     # test/3D/fwdfem/Makefile:10-74 (cyl2 Tet4, mua 0.01, mus 1, 100 MHz, tol 1e-8)
     "3d_real_direct": dict(mesh=os.path.join(G3, "cyl2.msh"), qm=os.path.join(G3, "cyl_5ring.qm"),
                            zip=os.path.join(G3, "testdata.zip"), mua=0.01, mus=1.0, n=1.4, freq=0.0, solver="cg",

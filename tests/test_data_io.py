@@ -116,8 +116,8 @@ def fwd_dir(tmp_path):
 
 @pytest.mark.parametrize("answers,msg", [
     # test/2D/fwdfem/Makefile targets the library does not provide
-    ("../meshes/circle25_32.msh ../meshes/circle25_32x32.qm 2 5 1e-15 10000 0 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
-     "GMRES is not provided"),
+    ("../meshes/circle25_32.msh ../meshes/circle25_32x32.qm 2 3 1e-15 10000 0 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
+     "BICG is not provided"),
     ("../meshes/circle25_32.msh ../meshes/circle25_32x32.qm 2 4 1e-15 10000 2 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
      "ICH is not provided"),
     ("../meshes/circle25_32.msh ../meshes/circle25_32x32.qm 2 4 1e-15 10000 3 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
@@ -139,10 +139,10 @@ def test_fwdfem_param_file_and_log(fwd_dir):
     records every choice in the same syntax up to the refusal."""
     prm = os.path.join(fwd_dir, "run.prm")
     with open(prm, "w") as f:
-        f.write("LINSOLVER = GMRES ; not provided\nMESHFILE = ../meshes3/cyl2.msh\nQMFILE = ../meshes3/cyl_5ring.qm\n"
+        f.write("LINSOLVER = BICG ; not provided\nMESHFILE = ../meshes3/cyl2.msh\nQMFILE = ../meshes3/cyl_5ring.qm\n"
                 "DATASCALE = LOG\nLINSOLVER_TOL = 1e-12\nLINSOLVER_MAXIT = 0\nLINSOLVER_PRECON = DIAG\n")
     rc, out, err = _run("fwdfem", "", fwd_dir, [prm, "run.log"])
-    assert rc == 1 and "GMRES is not provided" in err
+    assert rc == 1 and "BICG is not provided" in err
     log = open(os.path.join(fwd_dir, "run.log")).read().splitlines()
     assert log[2:5] == ["MESHFILE = ../meshes3/cyl2.msh", "QMFILE = ../meshes3/cyl_5ring.qm", "DATASCALE = LOG"]
 

@@ -91,7 +91,8 @@ def _run_case(name, mode, tol=1e-15):
     n = gm.NodeCount()
     mua, mus, ref = np.full(n, case["mua"]), np.full(n, case["mus"]), np.full(n, case["n"])
     cplx = case["freq"] != 0.0
-    fws = TMI.FwdSolver(gm, "BICGSTAB" if cplx else "CG", tol, complex=cplx)
+    ls = "GMRES" if case["solver"] == "gmres" else "BICGSTAB" if cplx else "CG"
+    fws = TMI.FwdSolver(gm, ls, tol, complex=cplx)
     fws.SetMode(mode)
     if case["precon"] == "diag":
         fws.SetPrecon("DIAG")

@@ -4,8 +4,8 @@ test/2D/fwdfem/Makefile and test/3D/fwdfem/Makefile pipe a fixed answer
 line into the program, which writes fmod.fem / farg.fem, and compare those
 with the vectors in testdata.zip at the Makefile's `cmpvec -r` tolerances.
 Every target of those Makefiles whose mesh, solver and preconditioner the
-library provides is run here (the ICH, DILU and GMRES targets and the Tri10
-mesh are refused, see test_data_io.py).  A parameter-file run with nodal
+library provides is run here (the ICH and DILU targets and the Tri10 mesh are
+refused, see test_data_io.py).  A parameter-file run with nodal
 parameters from the reference's NIM files is checked against the oracle."""
 import os
 import subprocess
@@ -41,6 +41,12 @@ TARGETS = {
     "2d_fwd_cplx_bicgstab_diag": (G2, "fwdfem", M2 + "2 4 1e-15 10000 1 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
                                   {"fmod.fem": ("fmod_cplx_bicgstab_diag.fem", 1e-7),
                                    "farg.fem": ("farg_cplx_bicgstab_diag.fem", 1e-6)}),
+    # test/2D/fwdfem/Makefile:128-148
+    "2d_fwd_cplx_gmres": (G2, "fwdfem", M2 + "2 5 1e-15 10000 0 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
+                          {"fmod.fem": ("fmod_cplx_gmres.fem", 1e-7), "farg.fem": ("farg_cplx_gmres.fem", 1e-7)}),
+    "2d_fwd_cplx_gmres_diag": (G2, "fwdfem", M2 + "2 5 1e-15 10000 1 1 2 2 2 2 100 2 0.025 2 2 2 1.4 0 1",
+                               {"fmod.fem": ("fmod_cplx_gmres_diag.fem", 1e-7),
+                                "farg.fem": ("farg_cplx_gmres_diag.fem", 1e-7)}),
     # test/3D/fwdfem/Makefile:10-30
     "3d_fwd_real_direct": (G3, "fwdfem_cw", M3 + "2 1 1 2 2 2 2 1 0.01 1 1 1 1.4 0 1",
                            {"fmod.fem": ("fmod_real_direct.fem", 1e-8)}),

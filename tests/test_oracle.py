@@ -33,7 +33,8 @@ def cyl():
 
 
 # ------------------------------------------------------------ golden vectors
-@pytest.mark.parametrize("name", ["2d_real_cg", "2d_cplx_bicgstab", "3d_real_direct", "3d_cplx_direct"])
+@pytest.mark.parametrize("name", ["2d_real_cg", "2d_cplx_bicgstab", "2d_cplx_gmres", "2d_cplx_gmres_diag",
+                                  "3d_real_direct", "3d_cplx_direct"])
 def test_oracle_golden_vectors(name):
     case = CASES[name]
     m = O.Mesh(case["mesh"])

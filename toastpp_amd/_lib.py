@@ -21,7 +21,7 @@ TMI_ERR_STATE = 6
 TMI_ERR_UNSUPPORTED = 7
 TMI_ERR_TIMEOUT = 8
 
-LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB = 0, 1, 2
+LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB, LSOLVER_GMRES = 0, 1, 2, 3
 PRECON_NONE, PRECON_DIAG = 0, 1
 MODE_FP64_SINGLE, MODE_MIXED_BLOCK, MODE_FP64_BLOCK = 0, 1, 2
 DATA_LIN, DATA_LOG = 0, 1

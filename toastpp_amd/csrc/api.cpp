@@ -419,7 +419,7 @@ int tmi_solver_create(tmi_mesh mh, int complex_valued, int lsolver, double tol, 
     API_BEGIN
     Mesh &m = M(mh);
     NEED(out);
-    TMI_CHECK(lsolver >= TMI_LSOLVER_DIRECT && lsolver <= TMI_LSOLVER_BICGSTAB, TMI_ERR_INVALID_ARG,
+    TMI_CHECK(lsolver >= TMI_LSOLVER_DIRECT && lsolver <= TMI_LSOLVER_GMRES, TMI_ERR_INVALID_ARG,
               "unknown linear solver %d", lsolver);
     tmi_solver h = new tmi_solver_s;
     Solver &s = h->s;
@@ -466,7 +466,8 @@ int tmi_solver_set_linsolver(tmi_solver h, const char *name, double tol)
     int ls;
     if (!strcasecmp(name, "CG")) ls = TMI_LSOLVER_CG;
     else if (!strcasecmp(name, "BICGSTAB")) ls = TMI_LSOLVER_BICGSTAB;
-    else fail(TMI_ERR_UNSUPPORTED, "linear solver '%s' is not supported (DIRECT, CG, BICGSTAB)", name);
+    else if (!strcasecmp(name, "GMRES")) ls = TMI_LSOLVER_GMRES;
+    else fail(TMI_ERR_UNSUPPORTED, "linear solver '%s' is not supported (DIRECT, CG, BICGSTAB, GMRES)", name);
     TMI_CHECK(!(s.cplx && ls == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG is a stub)");
     TMI_CHECK(tol > 0.0, TMI_ERR_INVALID_ARG, "tolerance must be positive");

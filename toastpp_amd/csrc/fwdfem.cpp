@@ -15,7 +15,7 @@
 //
 // What the library does not provide is refused with a message and exit
 // status 1 before any GPU work: Tri3 and Tet4 meshes only, linear solvers
-// DIRECT / CG / BICGSTAB (no BiCG, GMRES, Gauss-Seidel; CG only for the
+// DIRECT / CG / BICGSTAB / GMRES (no BiCG, Gauss-Seidel; CG only for the
 // real CW system), preconditioners NONE / DIAG (no ICH, DILU), source and
 // measurement profiles POINT / GAUSSIAN (no COSINE, no trigonometric
 // sources).  DIRECT is the library's FP64-accurate iterative solve
@@ -238,8 +238,8 @@ void select_solver(Params &pp, Config &c)
         pp.put_int("LINSOLVER_MAXIT", c.maxit);
         pp.put("LINSOLVER_PRECON", PRECON_NAMES[c.precon]);
     }
-    if (c.solver == S_BICG || c.solver == S_GMRES || c.solver == S_GS)
-        die("linear solver %s is not provided (DIRECT, CG, BICGSTAB)", SOLVER_NAMES[c.solver]);
+    if (c.solver == S_BICG || c.solver == S_GS)
+        die("linear solver %s is not provided (DIRECT, CG, BICGSTAB, GMRES)", SOLVER_NAMES[c.solver]);
     if (CPLX && c.solver == S_CG)
         die("CG is not defined for the complex system (the reference's complex CG returns without solving); "
             "use BICGSTAB");
@@ -474,7 +474,10 @@ int main(int argc, char **argv)
     qd.clear();
     md.clear();
 
-    int ls = c.solver == S_DIRECT ? TMI_LSOLVER_DIRECT : c.solver == S_CG ? TMI_LSOLVER_CG : TMI_LSOLVER_BICGSTAB;
+    int ls = c.solver == S_DIRECT ? TMI_LSOLVER_DIRECT
+             : c.solver == S_CG   ? TMI_LSOLVER_CG
+             : c.solver == S_GMRES ? TMI_LSOLVER_GMRES
+                                   : TMI_LSOLVER_BICGSTAB;
     tmi_solver s = nullptr;
     check(tmi_solver_create(mesh, CPLX ? 1 : 0, ls, c.tol, &s), "solver setup");
     if (c.solver != S_DIRECT) {

@@ -34,6 +34,7 @@ Solver::Exec::~Exec()
     if (st) (void)hipStreamSynchronize(st);
     if (h_flags) (void)hipHostFree(h_flags);
     if (h_dbl) (void)hipHostFree(h_dbl);
+    free_gmres_work(gm);
     if (own_stream) (void)hipStreamDestroy(st);
 }
 
@@ -47,7 +48,7 @@ size_t Solver::Exec::bytes() const
                  w.p.bytes() + w.v.bytes() + w.s.bytes() + w.t.bytes() + w.partial.bytes() + w.sc.bytes() +
                  w.osc.bytes() + w.ic.bytes() + w.oic.bytes() + w.rdq.bytes();
         }
-    return b;
+    return b + gmres_work_bytes(gm);
 }
 
 static int log2i(int K)
@@ -70,6 +71,7 @@ int Solver::tile_for(int nq, int md) const
 
 size_t Solver::work_bytes(int nq) const
 {
+    if (lsolver == TMI_LSOLVER_GMRES) return gmres_tile_bytes(mesh->n, tile_for(nq, TMI_MODE_FP64_BLOCK), vbytes(vk64()));
     // the tile and, with compaction, its narrower copies (K/2 + K/4 + ... < K)
     // and the second workspaces of the repacked widths (K and K/2)
     const size_t K = tile_for(nq), n = mesh->n, Kall = compact_min > 0 ? 2 * K + K + K / 2 : K;
@@ -451,6 +453,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
     TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
+    if (lsolver == TMI_LSOLVER_GMRES) return gmres_on(e, nq, d_b, d_x, ld, stats);
     if (md < 0) md = mode;
     if (precon == TMI_PRECON_DIAG) {
         std::lock_guard<std::mutex> lk(prep_mu);

@@ -7,6 +7,12 @@
 
 namespace tmi {
 
+// workspace of the GMRES tiles (gmres.hip)
+struct GmresWork;
+void free_gmres_work(GmresWork *w);
+size_t gmres_work_bytes(const GmresWork *w);
+size_t gmres_tile_bytes(size_t n, size_t K, size_t es);
+
 struct Solver {
     Mesh *mesh = nullptr;
     bool cplx = true;
@@ -64,6 +70,7 @@ struct Solver {
         DBuf<int> cols;                // column maps of compaction (2 x 64)
         int *h_flags = nullptr;        // pinned host mirror of per-column ints (inner, outer)
         double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
+        GmresWork *gm = nullptr;       // GMRES workspace (allocated by the first GMRES solve)
         double last_solve_ms = 0.0, last_iters = 0.0;
         Exec() = default;
         explicit Exec(bool new_stream);
@@ -106,6 +113,8 @@ struct Solver {
     BlockDesc block(const Exec::Work &w, int K) const;
     Exec::Work &ensure_work(Exec &e, int K, int md, bool alt = false);
     int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md);
+    // GMRES(20) in FP64 on tiles of the columns (gmres.hip); every mode
+    int gmres_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
     int count_active(Exec &e, const int *d_flags, int K);
 };
 

@@ -39,9 +39,10 @@ class FwdSolver:
         self.complex = bool(complex)
         h = C.c_void_p()
         name = solver.upper()
-        ls = {"DIRECT": L.LSOLVER_DIRECT, "CG": L.LSOLVER_CG, "BICGSTAB": L.LSOLVER_BICGSTAB}.get(name)
+        ls = {"DIRECT": L.LSOLVER_DIRECT, "CG": L.LSOLVER_CG, "BICGSTAB": L.LSOLVER_BICGSTAB,
+              "GMRES": L.LSOLVER_GMRES}.get(name)
         if ls is None:
-            raise ValueError("unsupported linear solver %r (DIRECT, CG, BICGSTAB)" % solver)
+            raise ValueError("unsupported linear solver %r (DIRECT, CG, BICGSTAB, GMRES)" % solver)
         L.call("tmi_solver_create", mesh.handle_ptr(), 1 if self.complex else 0, ls, float(tol), C.byref(h))
         self._h = h
         self.stats = None
