@@ -98,3 +98,30 @@ def test_gmres_cycle_limit_and_zero_rhs(disk):
     assert np.linalg.norm(phi[0] - xo) / np.linalg.norm(xo) <= 1e-10
     assert abs(rr0 - rr) <= 1e-6 * rr
     assert fws.stats[1][0] == 0 and fws.stats[1][3] and np.all(phi[1] == 0)
+
+
+def test_gmres_device_entry_autotune_and_work_bytes(disk):
+    """GPU-resident CalcFields with GMRES ([n][nq] node-major device
+    buffers) gives the host entry point's fields bit for bit; the autotuner
+    runs a GMRES solve without kernel timings; the workspace estimate covers
+    the restart basis."""
+    case, om, gm = disk
+    from toastpp_amd import _lib as L
+    n = om.n
+    fws = TMI.FwdSolver(gm, "GMRES", 1e-12, complex=True)
+    fws.SetPrecon("DIAG")
+    fws.Allocate()
+    fws.Reset(TMI.Solution(np.full(n, 0.025), np.full(n, 2.0), np.full(n, 1.4)), 100.0 * 2 * np.pi * 1e-6)
+    qv = sparse.csr_matrix(gm.Qvec(width=2.0).T)[:12]
+    phi_h = fws.CalcFields(qv)
+    nq = qv.shape[0]
+    rhs = L.DeviceArray.from_host(np.ascontiguousarray(qv.toarray().T.astype(np.complex128)))
+    phi = L.DeviceArray.from_host(np.zeros((n, nq), np.complex128))
+    stats = fws.CalcFieldsDevice(nq, rhs.ptr, phi.ptr)
+    assert all(s[3] for s in stats)
+    assert np.array_equal(phi.to_host().T, phi_h)
+    phi2 = L.DeviceArray.from_host(np.zeros((n, nq), np.complex128))
+    tuned = fws.Autotune(nq, rhs.ptr, phi2.ptr)
+    print("GMRES autotune:", tuned)
+    assert tuned["solve_ms"] > 0
+    assert fws.WorkBytes(16) >= 16 * n * 16 * 21

@@ -538,6 +538,7 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
 {
     auto &w = ex.wk[log2i(tile_for(64, mode))];
     hipStream_t st = ex.st ? ex.st : this->st;
+    TMI_CHECK(lsolver != TMI_LSOLVER_GMRES, TMI_ERR_UNSUPPORTED, "kernel timing covers the BiCGSTAB/PCG kernels");
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
     int K = w.K, n = mesh->n;
@@ -582,6 +583,16 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
                       double *ms_out)
 {
     TMI_CHECK(assembled, TMI_ERR_STATE, "autotune before Reset");
+    if (lsolver == TMI_LSOLVER_GMRES) {
+        // GMRES has no SpMM variant or tile choice to tune: report the
+        // current settings and the time of one solve
+        solve(nq, d_b, d_x, ld, nullptr);
+        if (tile_out) *tile_out = tile;
+        if (rpw_out) *rpw_out = rpw;
+        if (variant_out) *variant_out = spmm_variant;
+        if (ms_out) *ms_out = ex.last_solve_ms;
+        return;
+    }
     const size_t es = vbytes(vk64());
     auto zero_x = [&]() { TMI_HIP(hipMemset2DAsync(d_x, (size_t)ld * es, 0, (size_t)nq * es, mesh->n, st)); };
     // 1. row schedule and SpMM variant of the 64-column kernel (mixed mode)
