@@ -168,3 +168,25 @@ def test_project_all_real_phase_unwrap():
     assert np.any(shift != 0.0) or np.all(wrapped[nqm:] <= 0.0)
     assert np.array_equal(fws.ProjectAllReal(mv, phi, scale="lin"), np.concatenate(
         [fws.ProjectAll(mv, phi, "lin").real, fws.ProjectAll(mv, phi, "lin").imag]))
+
+
+def test_module_level_functions(pair):
+    """toast's module-level Sysmat_CW / Jacobian (mesh.py:344-349) on Mesh
+    and Raster objects give the methods' results."""
+    case, om, gm = pair
+    n = om.n
+    rng = np.random.default_rng(3)
+    mua, mus, ref = 0.01 + 0.01 * rng.random(n), 1.0 + rng.random(n), np.full(n, 1.4)
+    A = TMI.Sysmat_CW(gm, mua, mus, ref, 0)
+    B = gm.Sysmat_CW(mua, mus, ref)
+    assert (A != B).nnz == 0
+    gm.ReadQM(case["qm"])
+    nq = gm.QMPositions()[0].shape[0]
+    nm = gm.QMPositions()[1].shape[0]
+    dphi = rng.standard_normal((n, nq)) + 1j * rng.standard_normal((n, nq))
+    aphi = rng.standard_normal((n, nm)) + 1j * rng.standard_normal((n, nm))
+    nqm = int(gm.QMPositions()[2][-1])
+    proj = rng.standard_normal(nqm) + 1j * rng.standard_normal(nqm) + 3.0
+    grid = np.array([16] * om.dim)
+    r = TMI.Raster(gm, grid)
+    assert np.array_equal(TMI.Jacobian(gm, r, dphi, aphi, proj), gm.Jacobian(r, dphi, aphi, proj))

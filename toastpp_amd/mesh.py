@@ -351,3 +351,28 @@ def WriteNim(nimname, meshname, nim):
     toastmodule.cc:942-972)."""
     v = np.ascontiguousarray(np.asarray(nim, np.float64).ravel())
     L.call("tmi_write_nim", str(nimname).encode(), str(meshname).encode(), v.size, 1, v, L.TMI_NIM_TOASTMOD)
+
+
+# module-level forms of toast's functions (src/python/toast/mesh.py:344-376),
+# taking a Mesh (and Raster) where the reference takes integer handles
+
+
+def Sysmat_CW(mesh, mua, mus, ref, freq=0):
+    """Real CW system matrix as scipy CSR (toast.Sysmat_CW, mesh.py:344-346;
+    the reference ignores freq)."""
+    return mesh.Sysmat_CW(mua, mus, ref)
+
+
+def Jacobian(mesh, raster, dphi, aphi, proj):
+    """Jacobian from given fields and projections (toast.Jacobian, mesh.py:348-349)."""
+    return mesh.Jacobian(raster, dphi, aphi, proj)
+
+
+def JacobianCW(mesh, raster, qvec, mvec, mua, mus, ref, solver="bicgstab", tol=1e-12):
+    """CW Jacobian (toast.JacobianCW, mesh.py:352-366)."""
+    return mesh.JacobianCW(raster, qvec, mvec, mua, mus, ref, solver=solver, tol=tol)
+
+
+def Gradient(mesh, raster, qvec, mvec, mua, mus, ref, freq, data, sd):
+    """Data misfit gradient (toast.Gradient, mesh.py:369-376)."""
+    return mesh.Gradient(raster, qvec, mvec, mua, mus, ref, freq, data, sd)
