@@ -101,8 +101,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sources shard by column blocks; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(sources would shard by column blocks, one block of sources per GPU). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import toastpp_amd as T
@@ -227,7 +228,7 @@ def main():
         batch = multi_mesh_leg(args)
         serve = serving_leg(args)
 
-    # CPU baseline: FP64 oracle, one column, one thread
+    # CPU baseline: FP64 oracle, one source per host thread
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nodes, el, Q, mua, mus, ref, omega, args.tol, args.cpu_seconds,
