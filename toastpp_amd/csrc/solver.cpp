@@ -550,9 +550,13 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
     const void *idg = nullptr;
     const void *op = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     if (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
+    // the 64-column complex FP32 BiCGSTAB SpMM reads the packed shadow
+    // residual in the solve, and so here
+    const unsigned char *wq =
+        (mixed && cplx && K == 64 && lsolver != TMI_LSOLVER_CG && w.vk_in == VK_C32 && w.rdq.p) ? w.rdq.p : nullptr;
     auto launch = [&]() {
         if (which == 0)
-            launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, w.rdi.p, false, act.p, w.partial.p, st);
+            launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, w.rdi.p, false, act.p, w.partial.p, st, wq);
         else
             launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, act.p, w.partial.p, st);
     };
