@@ -37,6 +37,8 @@
  *     AddDataGradient).
  *   - QVec_Point: a unit vector at a node, partition of unity and linear
  *     reproduction inside an element, boundary nodes only for Neumann.
+ *   - GMRES(20) (numpy, oracle.py gmres): the golden vectors of the 2D
+ *     GMRES targets without and with DIAG (fmod/farg_cplx_gmres[_diag].fem).
  */
 #include <complex.h>
 #include <math.h>

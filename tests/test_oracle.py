@@ -1,7 +1,8 @@
 """The FP64 CPU oracle, pinned before it is trusted as the checker.
 
 Pins (none of them restates the oracle's own formulas):
-  * the reference's golden vectors (test/2D/fwdfem, test/3D/fwdfem);
+  * the reference's golden vectors (test/2D/fwdfem, test/3D/fwdfem), for
+    PCG, BiCGSTAB, GMRES(20) and the direct targets;
   * closed forms of linear-element integrals (barycentric monomials);
   * structural identities of the assembled operators (row sums, volume,
     perimeter/surface, symmetry);
