@@ -198,7 +198,7 @@ int tmi_solver_autotune(tmi_solver s, int nq, const void *d_rhs, void *d_phi, in
  * most half of a tile's columns are still iterating, their state moves to
  * a tile of the smallest power-of-two width >= max(active, min_width) and
  * the finished columns are written out.  min_width 0 turns it off; default
- * 4.  Results are unchanged up to the rounding of narrower kernels. */
+ * 2.  Results are unchanged up to the rounding of narrower kernels. */
 int tmi_solver_set_compaction(tmi_solver s, int min_width);
 /* Device bytes of the workspace a CalcFields of nq columns uses (no
  * reference counterpart; the serving loop's admission unit). */

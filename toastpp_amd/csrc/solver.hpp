@@ -31,7 +31,7 @@ struct Solver {
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
     int spmm_variant = 3;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
                                      // 3 row blocks with shared gathers (default)
-    int compact_min = 4;             // narrowest tile the column compaction goes down to (0: off)
+    int compact_min = 2;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
     void prepare_row_blocks();   // rbval32 for the current operator (caller holds prep_mu)
     double omega = 0.0;

@@ -155,7 +155,7 @@ class FwdSolver:
                 "solve_ms": ms.value}
 
     def SetCompaction(self, min_width):
-        """narrowest tile of the column compaction (0: off, default 4)"""
+        """narrowest tile of the column compaction (0: off, default 2)"""
         L.call("tmi_solver_set_compaction", self._h, int(min_width))
 
     def WorkBytes(self, nq):
