@@ -78,7 +78,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ellipsoid3d")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--tile", type=int, default=64)
     ap.add_argument("--fp64-cols", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -131,8 +130,6 @@ def main():
         f.SetPrecon("DIAG")
         if tile:
             f.SetTile(tile)
-        if args.inner_tol > 0:
-            f.SetInnerTol(args.inner_tol)
         f.Allocate()
         t0 = time.perf_counter()
         f.Reset(sol, omega)

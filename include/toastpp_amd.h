@@ -162,8 +162,6 @@ int tmi_solver_set_maxit(tmi_solver s, int maxit);              /* SetLinSolverM
 int tmi_solver_set_mode(tmi_solver s, int mode);                /* TMI_MODE_* */
 /* Column tile of the block modes (RHS columns per Krylov block; 0 = auto) */
 int tmi_solver_set_tile(tmi_solver s, int tile);
-/* Inner relative tolerance of the FP32 Krylov solve of the mixed mode */
-int tmi_solver_set_inner_tol(tmi_solver s, double inner_tol);
 /* Node order of the solver's private copy of the system (no reference
  * counterpart): TMI_ORDER_MORTON (default) relabels the nodes along a
  * Z-order curve of their coordinates so that the sparse kernels' neighbour

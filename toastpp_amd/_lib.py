@@ -86,7 +86,6 @@ _SIGS = {
     "tmi_solver_set_maxit": [P, C.c_int],
     "tmi_solver_set_mode": [P, C.c_int],
     "tmi_solver_set_tile": [P, C.c_int],
-    "tmi_solver_set_inner_tol": [P, C.c_double],
     "tmi_solver_set_order": [P, C.c_int],
     "tmi_solver_set_rows_per_wave": [P, C.c_int],
     "tmi_solver_set_spmm_variant": [P, C.c_int],

@@ -573,14 +573,6 @@ int tmi_solver_work_bytes(tmi_solver h, int nq, double *bytes)
     API_END
 }
 
-int tmi_solver_set_inner_tol(tmi_solver h, double t)
-{
-    API_BEGIN
-    TMI_CHECK(t > 0.0 && t < 1.0, TMI_ERR_INVALID_ARG, "inner tolerance must lie in (0,1)");
-    S(h).inner_tol = t;
-    API_END
-}
-
 int tmi_solver_allocate(tmi_solver h)
 {
     API_BEGIN

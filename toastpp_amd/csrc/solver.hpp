@@ -22,7 +22,6 @@ struct Solver {
     int maxit = 0;
     int mode = TMI_MODE_MIXED_BLOCK;
     int tile = 0;
-    double inner_tol = 1e-5;
 #ifndef TMI_RELIABLE_EVERY
 #define TMI_RELIABLE_EVERY 16
 #endif

@@ -87,9 +87,6 @@ class FwdSolver:
         blocks of 8 rows gathering each shared neighbour row once (default)."""
         L.call("tmi_solver_set_spmm_variant", self._h, SPMM_VARIANTS.index(variant))
 
-    def SetInnerTol(self, tol):
-        L.call("tmi_solver_set_inner_tol", self._h, float(tol))
-
     # --- lifecycle ---
     def Allocate(self):
         L.call("tmi_solver_allocate", self._h)
