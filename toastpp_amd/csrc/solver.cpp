@@ -187,7 +187,10 @@ int Solver::count_active(Exec &e, const int *d_flags, int K)
     return c;
 }
 
-static constexpr int CHECK_EVERY = 8;
+#ifndef TMI_CHECK_EVERY
+#define TMI_CHECK_EVERY 8
+#endif
+static constexpr int CHECK_EVERY = TMI_CHECK_EVERY;
 
 // Repacking the running columns to the front of the tile pays when it
 // frees a whole 128-byte line (16 complex FP32 columns) of every row, or
