@@ -198,8 +198,17 @@ int tmi_solver_autotune(tmi_solver s, int nq, const void *d_rhs, void *d_phi, in
  * the finished columns are written out.  min_width 0 turns it off; default
  * 2.  Results are unchanged up to the rounding of narrower kernels. */
 int tmi_solver_set_compaction(tmi_solver s, int min_width);
-/* Device bytes of the workspace a CalcFields of nq columns uses (no
- * reference counterpart; the serving loop's admission unit). */
+/* FP64 fallback of the mixed mode (no reference counterpart): columns the
+ * FP32 iteration leaves above the tolerance (stagnation, breakdown or its
+ * iteration cap) are solved again by the FP64 block iteration, starting
+ * from where they stopped; their records add the FP64 iterations and one
+ * sweep.  enable: 1 (default) or 0 (such columns come back unconverged);
+ * mixed_maxit: iteration cap of the FP32 iteration (0, the default: the
+ * solver's maxit). */
+int tmi_solver_set_fallback(tmi_solver s, int enable, int mixed_maxit);
+/* Device bytes a CalcFields of nq columns may use for its workspace,
+ * including the FP64 fallback's (no reference counterpart; the serving
+ * loop's admission unit). */
 int tmi_solver_work_bytes(tmi_solver s, int nq, double *bytes);
 /* Allocate (fwdsolver.cc:193-261): CSR pattern of the system matrix on the GPU */
 int tmi_solver_allocate(tmi_solver s);
@@ -208,6 +217,16 @@ int tmi_solver_allocate(tmi_solver s);
 int tmi_solver_assemble(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a, double omega);
 /* Reset (fwdsolver.cc:494-505): assemble + preconditioner setup */
 int tmi_solver_reset(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a, double omega);
+/* AssembleSystemMatrix / Reset with the basis argument (fwdsolver.cc:338-357,
+ * :405-424, `bool elbasis`).  elbasis = 0: nodal coefficients (nlen each),
+ * as tmi_solver_assemble / tmi_solver_reset.  elbasis = 1: one coefficient
+ * per element (elen each; ASSEMBLE_PFF_EL / PDD_EL / BNDPFF_EL, AddToElMatrix
+ * mesh.cc:1985-1993, :2112-2120): K = sum_el cmua_el IntFF + ckappa_el IntDD
+ * + c2a_el BndIntFF [+ i omega FF]. */
+int tmi_solver_assemble_basis(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a,
+                              double omega, int elbasis);
+int tmi_solver_reset_basis(tmi_solver s, const double *cmua, const double *ckappa, const double *c2a,
+                           double omega, int elbasis);
 int tmi_solver_nnz(tmi_solver s, int *nnz);
 /* Copy the FP64 system matrix back (public member F, toastmodule.cc:1069-1077).
  * val: nnz complex (interleaved) for complex solvers, nnz real otherwise. */
@@ -267,6 +286,53 @@ int tmi_jacobian_nodal_device(tmi_mesh m, int nq, int nm, const void *d_dphi, in
  * output may be NULL. */
 int tmi_jacobian_cw(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
                     int dscale, double *Jmua, double *Jkap);
+
+/* GenerateJacobian, element basis (jacobian.cc:95-96,117-119 with
+ * RASTER_ELBASIS -> GenerateJacobian_mesh :441-497, IntFG_el :879-906,
+ * IntGradFGradG_el :911-939): fields and proj as tmi_jacobian_nodal; J:
+ * (2 nqm) x (2 elen) row-major, per element -Re/-Im of
+ * sum_ij (f_i g_j) IntFF(i,j) and sum_ij (f_i g_j) IntDD(i,j), each over
+ * proj for TMI_DATA_LOG.  Columns are derivatives with respect to the
+ * per-element cmua and ckappa of tmi_solver_assemble_basis(elbasis = 1). */
+int tmi_jacobian_element(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi,
+                         const double *proj, int dscale, double *J);
+/* Same on GPU-resident arrays (as tmi_jacobian_nodal_device); d_J must hold
+ * 4 nqm elen doubles. */
+int tmi_jacobian_element_device(tmi_mesh m, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi,
+                                int lda, const void *d_proj, int dscale, void *d_J);
+/* GenerateJacobian_cw, element basis (jacobian.cc:141-142 ->
+ * GenerateJacobian_cw_mesh :727-790 with elbasis = true): real fields as for
+ * tmi_jacobian_cw; Jmua = -IntFG_el, Jkap = -IntGradFGradG_el (nqm x elen),
+ * each row times 1/proj for TMI_DATA_LOG; either output may be NULL. */
+int tmi_jacobian_cw_element(tmi_mesh m, int nq, int nm, const double *dphi, const double *aphi,
+                            const double *proj, int dscale, double *Jmua, double *Jkap);
+
+/* basis argument of the mvec overloads (the raster argument of
+ * GenerateJacobian: RASTER_NDBASIS / RASTER_ELBASIS, jacobian.h:33-35, or a
+ * Raster) */
+#define TMI_BASIS_NODAL    0
+#define TMI_BASIS_ELEMENT  1
+#define TMI_BASIS_RASTER   2   /* the raster handle's solution basis */
+/* GenerateJacobian with the measurement vectors instead of the projections
+ * (jacobian.cc:86-101 -> GenerateJacobian_mesh :380-435 /
+ * GenerateJacobian_grid :202-286): the linear projections are computed
+ * from mvec (nm x nlen CSR, complex interleaved values; ProjectSingle with
+ * DATA_LIN) and the Jacobian follows as with given projections.  J:
+ * (2 nqm) x (2 len), len = nlen, elen or the raster's slen.  raster: used
+ * with TMI_BASIS_RASTER only.  For TMI_BASIS_ELEMENT the result is laid out
+ * as the reference's overload lays it out: its kappa PMDF is a view at
+ * offset nlen instead of elen (jacobian.cc:397), so mua columns
+ * nlen .. elen-1 hold the kappa PMDF of elements 0 .. elen-nlen-1 (the
+ * given-projections overload, tmi_jacobian_element, has no such overlap). */
+int tmi_jacobian_mvec(tmi_mesh m, tmi_raster raster, int basis, int nq, int nm, const int *mrowptr,
+                      const int *mcolidx, const double *mval, const double *dphi, const double *aphi, int dscale,
+                      double *J);
+/* GenerateJacobian_cw with mvec (jacobian.cc:160-177): real fields and real
+ * mvec values; the rows are scaled by 1/ProjectAll(mvec, dphi) for
+ * TMI_DATA_LOG.  Jmua / Jkap: nqm x len, either may be NULL. */
+int tmi_jacobian_cw_mvec(tmi_mesh m, tmi_raster raster, int basis, int nq, int nm, const int *mrowptr,
+                         const int *mcolidx, const double *mval, const double *dphi, const double *aphi,
+                         int dscale, double *Jmua, double *Jkap);
 
 /* ----------------------------------------------------------------- raster */
 /* Raster_Pixel with basis grid == intermediate grid (raster.cc:18-100,

@@ -43,6 +43,8 @@ def lib():
             "om_csr_nnz": (C.c_int, [P]), "om_csr": (None, [P, I, I]),
             "om_assemble": (None, [P, I, I, D, D, D, C.c_double, D]),
             "om_assemble_ff": (None, [P, I, I, D]),
+            "om_assemble_el": (None, [P, I, I, D, D, D, C.c_double, D]),
+            "om_jacobian_el": (None, [P, C.c_int, C.c_int, D, D, I, I, D, C.c_int, C.c_int, D]),
             "om_el_intfff": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
             "om_el_intfdd": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
             "om_el_intpff": (C.c_double, [P, C.c_int, C.c_int, C.c_int, D]),
@@ -154,6 +156,14 @@ class Mesh:
         val = np.zeros(2 * len(ci))
         lib().om_assemble(self.h, rp, ci, _c(cmua, np.float64), _c(ckap, np.float64), _c(c2a, np.float64),
                           float(omega), val)
+        return rp, ci, val.view(np.complex128)
+
+    def sysmat_el(self, cmua, ckap, c2a, omega):
+        """AssembleSystemMatrix with elbasis = true: elen coefficients each."""
+        rp, ci = self.csr()
+        val = np.zeros(2 * len(ci))
+        lib().om_assemble_el(self.h, rp, ci, _c(cmua, np.float64), _c(ckap, np.float64), _c(c2a, np.float64),
+                             float(omega), val)
         return rp, ci, val.view(np.complex128)
 
     def massmat(self):
@@ -304,6 +314,30 @@ def jacobian_nodal(mesh, dphi, aphi, lptr, lidx, proj, logscale=True):
     lib().om_jacobian_nodal(mesh.h, nq, nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
                             _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, J)
     return J
+
+
+def jacobian_el(mesh, dphi, aphi, lptr, lidx, proj, logscale=True, mvec_offset=False):
+    """Element-basis Jacobian (2 nQM x 2 elen); mvec_offset reproduces the
+    mvec overload's kappa view at offset n (jacobian.cc:397)."""
+    dphi = np.ascontiguousarray(dphi, np.complex128); aphi = np.ascontiguousarray(aphi, np.complex128)
+    nq, nm = dphi.shape[0], aphi.shape[0]
+    nqm = int(lptr[-1])
+    J = np.zeros((2 * nqm, 2 * mesh.elen))
+    lib().om_jacobian_el(mesh.h, nq, nm, dphi.view(np.float64), aphi.view(np.float64), _c(lptr, np.int32),
+                         _c(lidx, np.int32), _cplx(proj), 1 if logscale else 0, 1 if mvec_offset else 0, J)
+    return J
+
+
+def jacobian_cw_el(mesh, dphi, aphi, lptr, lidx, proj=None, logscale=True):
+    """GenerateJacobian_cw_mesh with elbasis = true (jacobian.cc:727-790) +
+    the log row scaling: -IntFG_el / proj and -IntGradFGradG_el / proj on
+    real fields.  Returns (Jmua, Jkap), each nqm x elen."""
+    dphi = np.asarray(dphi, np.float64).astype(np.complex128)
+    aphi = np.asarray(aphi, np.float64).astype(np.complex128)
+    nqm = int(lptr[-1])
+    p = np.ones(nqm) if proj is None else np.asarray(proj, np.float64)
+    J = jacobian_el(mesh, dphi, aphi, lptr, lidx, p.astype(np.complex128), logscale)
+    return J[:nqm, :mesh.elen].copy(), J[:nqm, mesh.elen:].copy()
 
 
 def jacobian_cw_nodal(mesh, dphi, aphi, lptr, lidx, proj=None, logscale=True):

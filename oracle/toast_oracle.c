@@ -20,11 +20,15 @@
  *     row sums of the mass matrix, zero row sums of the stiffness matrix).
  *   - nodal Jacobian: finite differences of the projections with respect to
  *     nodal cmua / ckappa (the check of test/2D/matlab/jacobian_test.m).
- *   - raster (pixel) basis: closed-form checks only (linear-field
- *     reproduction by the grid map, exact gradients of linear fields,
- *     raster Jacobian vs nodal Jacobian contracted with the grid map).
- *     The reference's EPS and edge rules are restated but not pinned by a
- *     reference output: parity unpinned beyond those invariants.
+ *   - raster (pixel) basis: the element of every grid point against a
+ *     brute-force barycentric search (EPS and shared-face rules included),
+ *     the grid map B by exact reproduction of linear fields, and every
+ *     entry of the grid Jacobian (mua and kappa blocks, complex with the
+ *     masked adjoint gradient and CW with the unmasked one) against its
+ *     closed form on linear fields; ImageGradient's central, forward,
+ *     backward and zero branches by value on a quadratic image under a
+ *     random mask (tests/test_oracle_raster.py).  The reference's tests
+ *     hold no raster output, so these closed forms are the pin.
  *   - NodeMapMatrix: weights summing to 1 at every node, exact
  *     reproduction of linear fields where all grid corners are inside.
  *   - linear pixel maps (Grid2LinPixMatrix, LinPix2GridMatrix,
@@ -497,6 +501,88 @@ void om_assemble(const om_mesh *m, const int *rp, const int *ci, const double *c
                 case 3: val[2*k+1] += el_intff(m, e, i, j) * omega; break;
                 }
             }
+        }
+}
+
+/* ------------------------------------------------------------------------
+ * Element-basis assembly: AssembleSystemMatrix(sol, omega, elbasis = true)
+ * (fwdsolver.cc:338-357, :405-424) with ASSEMBLE_PFF_EL / PDD_EL /
+ * BNDPFF_EL (AddToElMatrix, mesh.cc:1985-1993; complex twin :2112-2120):
+ * the element's coefficient times IntFF, IntDD (the stored intdd,
+ * tet4.cc:1016-1031, tri3.cc:490-506) and BndIntFF (the stored intbff,
+ * tet4.cc:1109-1122, tri3.cc:561-582; zero off the boundary), pass by pass
+ * over ascending elements, then iCFF(omega) as in the nodal assembly.
+ * cmua, ckappa, c2a: elen values each.  val: complex interleaved.
+ * ------------------------------------------------------------------------ */
+static void cmul(const double *a, const double *b, double *o);
+static double el_intdd(const om_mesh *m, int e, int i, int j) { return m->intdd[(size_t)e*m->nn*m->nn + i*m->nn + j]; }
+static double el_bndintff(const om_mesh *m, int e, int i, int j) { return m->intbff[(size_t)e*m->nn*m->nn + i*m->nn + j]; }
+
+void om_assemble_el(const om_mesh *m, const int *rp, const int *ci, const double *cmua, const double *ckappa,
+                    const double *c2a, double omega, double *val)
+{
+    int nz = rp[m->n], nn = m->nn;
+    memset(val, 0, sizeof(double)*2*(size_t)nz);
+    for (int pass = 0; pass < 4; pass++)
+        for (int e = 0; e < m->elen; e++) {
+            const int *nd = m->el + e*nn;
+            for (int i = 0; i < nn; i++) for (int j = 0; j < nn; j++) {
+                int k = slot(rp, ci, nd[i], nd[j]);
+                switch (pass) {
+                case 0: val[2*k] += el_intff(m, e, i, j) * cmua[e]; break;
+                case 1: val[2*k] += el_intdd(m, e, i, j) * ckappa[e]; break;
+                case 2: val[2*k] += el_bndintff(m, e, i, j) * c2a[e]; break;
+                case 3: val[2*k+1] += el_intff(m, e, i, j) * omega; break;
+                }
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------
+ * Element-basis Jacobian (GenerateJacobian with RASTER_ELBASIS,
+ * jacobian.cc:95-96,117-119 -> GenerateJacobian_mesh :441-497 with
+ * elbasis = true): per element
+ *   pmdf_mua[el] = sum_i sum_j (f[n_i] g[n_j]) IntFF(i,j)   (IntFG_el, :879-906)
+ *   pmdf_kap[el] = sum_i sum_j (f[n_i] g[n_j]) IntDD(i,j)   (IntGradFGradG_el, :911-939)
+ * then /proj for logscale (PMDF_log, :74-79) and J = -Re / -Im in rows k and
+ * k + nQM, mua block in columns [0, elen), kappa block in [elen, 2 elen).
+ * dphi: nq x n, aphi: nm x n complex; proj: nQM complex (link order).
+ * mvec_offset != 0 reproduces the mvec overload's view of the kappa PMDF
+ * at offset n instead of elen (:397): mua columns n..elen-1 then hold the
+ * kappa PMDF of elements 0..elen-n-1.
+ * ------------------------------------------------------------------------ */
+void om_jacobian_el(const om_mesh *m, int nq, int nm, const double *dphi, const double *aphi,
+                    const int *lptr, const int *lidx, const double *proj, int logscale, int mvec_offset, double *J)
+{
+    (void)nm;
+    int n = m->n, nn = m->nn, E = m->elen, nqm = lptr[nq];
+    size_t rowlen = 2*(size_t)E;
+    for (int q = 0; q < nq; q++)
+        for (int k = lptr[q]; k < lptr[q+1]; k++) {
+            const double *f = dphi + 2*(size_t)q*n, *g = aphi + 2*(size_t)lidx[k]*n;
+            double pr = proj[2*k], pi = proj[2*k+1], den = pr*pr + pi*pi;
+            double *R0 = J + (size_t)k*rowlen, *R1 = J + (size_t)(k+nqm)*rowlen;
+            for (int e = 0; e < E; e++) {
+                const int *nd = m->el + e*nn;
+                double sm[2] = {0.0, 0.0}, sk[2] = {0.0, 0.0};
+                for (int i = 0; i < nn; i++)
+                    for (int j = 0; j < nn; j++) {
+                        double fg[2];
+                        cmul(f + 2*nd[i], g + 2*nd[j], fg);
+                        double w = el_intff(m, e, i, j), wd = el_intdd(m, e, i, j);
+                        sm[0] += fg[0]*w; sm[1] += fg[1]*w;
+                        sk[0] += fg[0]*wd; sk[1] += fg[1]*wd;
+                    }
+                double a[2] = {sm[0], sm[1]}, b[2] = {sk[0], sk[1]};
+                if (logscale) {   /* complex division by proj */
+                    double ar = (a[0]*pr + a[1]*pi)/den, ai = (a[1]*pr - a[0]*pi)/den;
+                    double br = (b[0]*pr + b[1]*pi)/den, bi = (b[1]*pr - b[0]*pi)/den;
+                    a[0] = ar; a[1] = ai; b[0] = br; b[1] = bi;
+                }
+                R0[e] = -a[0]; R1[e] = -a[1]; R0[e+E] = -b[0]; R1[e+E] = -b[1];
+            }
+            if (mvec_offset)
+                for (int e = n; e < E; e++) { R0[e] = R0[E + e - n]; R1[e] = R1[E + e - n]; }
         }
 }
 

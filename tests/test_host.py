@@ -120,3 +120,49 @@ def test_raster_and_data_file_entry_points_reject_bad_arguments():
     assert L.tmi_read_vector(b"/nonexistent/x.fem", C.byref(n), None) == _lib.TMI_ERR_IO
     v = np.zeros(3)
     assert L.tmi_write_nim(b"/tmp/x.nim", b"m", 3, 1, v, 7) == _lib.TMI_ERR_INVALID_ARG
+
+
+def test_optical_to_coeffs_index_matched_and_nonfinite():
+    """A_Keijzer(1) = 1 (param.cc:24-32: theta_c = pi/2, R0 = 0), so n = 1
+    gives c2a = c0 / 2; n < 1 and non-finite parameters are refused."""
+    import toastpp_amd as T
+    from toastpp_amd import _lib
+    s = T.Solution(np.full(3, 0.01), np.full(3, 1.0), np.ones(3))
+    assert np.allclose(s.c2a, 0.3 / 2.0, rtol=1e-15, atol=0)
+    assert np.allclose(s.cmua, 0.3 * 0.01, rtol=1e-15, atol=0)
+    assert np.allclose(s.ckappa, 0.3 / (3.0 * 1.01), rtol=1e-15, atol=0)
+    for mua, mus, ref in ((0.01, 1.0, 0.9), (np.nan, 1.0, 1.4), (0.01, np.inf, 1.4), (0.01, 1.0, np.nan)):
+        with pytest.raises(_lib.TmiError) as ei:
+            T.Solution(np.full(2, mua), np.full(2, mus), np.full(2, ref))
+        assert ei.value.code == _lib.TMI_ERR_INVALID_ARG
+
+
+class _FakeMesh:
+    """Stands in for a device mesh: only the sizes the host checks read."""
+
+    def __init__(self, n, nq, nm):
+        self.n, self.dims = n, (nq, nm, nq * nm)
+
+    def NodeCount(self):
+        return self.n
+
+    def qm_dims(self):
+        return self.dims
+
+
+def test_fwdsolver_rejects_mismatched_shapes_on_the_host():
+    """CalcFields / ProjectAll check qvec, mvec and phi against the mesh
+    before any C call (the C entry points copy nq x n entries)."""
+    from toastpp_amd.fwdsolver import FwdSolver
+    f = object.__new__(FwdSolver)   # no device handle: the checks run first
+    f.mesh, f.complex, f._h = _FakeMesh(10, 2, 3), True, None
+    with pytest.raises(ValueError):
+        f.CalcFields(np.ones((2, 9)))                       # qvec narrower than the mesh
+    with pytest.raises(ValueError):
+        f.CalcFields(np.ones((2, 10)), phi=np.zeros((2, 9)))   # short phi
+    with pytest.raises(ValueError):
+        f.ProjectAll(np.ones((3, 10)), np.zeros((1, 10)))      # phi of too few sources
+    with pytest.raises(ValueError):
+        f.ProjectAll(np.ones((2, 10)), np.zeros((2, 10)))      # mvec of too few detectors
+    with pytest.raises(ValueError):
+        f.ProjectAllDevice(np.ones((3, 10)), 0, ld=1)          # row stride below the source count

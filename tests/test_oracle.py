@@ -15,6 +15,7 @@ import math
 
 import numpy as np
 import pytest
+from scipy import sparse
 
 from oracle import oracle as O
 from tests.cases import CASES, compare
@@ -254,3 +255,115 @@ def test_qvec_point_closed_forms(disk, cyl):
         qn = O.qvec_point(m, p, neumann=True)[0]
         bnd = m.bnd().astype(bool)
         assert np.all(qn[~bnd] == 0) and np.allclose(qn[bnd], q[bnd])
+
+
+# --------------------------------------------------------- element basis
+def _el_system(m, rp, ci, cm, ck, c2, omega):
+    return m.sysmat_el(cm, ck, c2, omega)
+
+
+@pytest.mark.parametrize("case", ["disk", "cyl"])
+def test_element_basis_assembly_closed_forms(case):
+    """AssembleSystemMatrix(elbasis = true): constant element coefficients
+    give the nodal assembly with the same constants (IntPFF / IntPDD /
+    BndIntPFF of a constant P are P IntFF / P IntDD / P BndIntFF); a unit
+    coefficient on one element gives that element's local matrices: mass
+    V/20 (1 + delta_ij) (Tet4) or A/12 (1 + delta_ij) (Tri3), a stiffness
+    with zero row sums and u^T K u = |a|^2 V for u = a.x, and a boundary
+    mass whose entries sum to the element's boundary measure."""
+    nodes, el = (G.disk(10.0, 5) if case == "disk" else G.cylinder(6.0, 6.0, 3, 3))
+    m = O.Mesh(nodes=nodes, elems=el)
+    n, E = m.n, m.elen
+    omega = 2 * np.pi * 100e-6
+    rp, ci, kn = m.sysmat(np.full(n, 0.02), np.full(n, 0.3), np.full(n, 0.15), omega)
+    _, _, ke = m.sysmat_el(np.full(E, 0.02), np.full(E, 0.3), np.full(E, 0.15), omega)
+    assert np.max(np.abs(ke - kn)) <= 1e-14 * np.max(np.abs(kn))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    dim = nodes.shape[1]
+    size = m.size()
+    bnd_el = np.flatnonzero(m.bndside().reshape(E, -1).any(axis=1))
+    for e in (0, E // 2, int(bnd_el[0])):
+        nd = el[e]
+        unit = np.zeros(E); unit[e] = 1.0
+        _, _, kf = m.sysmat_el(unit, 0 * unit, 0 * unit, 0.0)
+        _, _, kd = m.sysmat_el(0 * unit, unit, 0 * unit, 0.0)
+        _, _, kb = m.sysmat_el(0 * unit, 0 * unit, unit, 0.0)
+        inel = np.isin(rows, nd) & np.isin(ci, nd)
+        assert np.all(kf[~inel] == 0) and np.all(kd[~inel] == 0) and np.all(kb[~inel] == 0)
+        c = 1.0 / 20.0 if dim == 3 else 1.0 / 12.0
+        for k in np.flatnonzero(inel):
+            assert kf[k].real == pytest.approx(c * size[e] * (2.0 if rows[k] == ci[k] else 1.0), rel=1e-14)
+        Kd = sparse.csr_matrix((kd.real, ci, rp), shape=(n, n))
+        assert np.max(np.abs(Kd @ np.ones(n))) <= 1e-13 * np.max(np.abs(kd))
+        a = np.array([0.3, -1.2, 0.7][:dim])
+        u = nodes @ a
+        assert u @ (Kd @ u) == pytest.approx(a @ a * size[e], rel=1e-12)
+        # boundary measure of the element's boundary sides (Heron areas / edge lengths)
+        side_nodes = [[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]] if dim == 3 else [[0, 1], [1, 2], [2, 0]]
+        meas = 0.0
+        for s_, on in enumerate(m.bndside()[e]):
+            if not on:
+                continue
+            P = nodes[nd[side_nodes[s_]]]
+            meas += (np.linalg.norm(np.cross(P[1] - P[0], P[2] - P[0])) / 2 if dim == 3
+                     else np.linalg.norm(P[1] - P[0]))
+        assert kb.real.sum() == pytest.approx(meas, rel=1e-12, abs=1e-300)
+
+
+def test_element_basis_jacobian_finite_differences():
+    """GenerateJacobian(RASTER_ELBASIS) against central finite differences
+    of the log projections in per-element cmua / ckappa (the per-element
+    check of test/2D/matlab/jacobian_test.m), with the fields of the
+    element-basis system matrix."""
+    nodes, el = G.cylinder(6.0, 6.0, 3, 3)
+    m = O.Mesh(nodes=nodes, elems=el)
+    E = m.elen
+    Q = np.array([[6.0, 0.0, 1.0], [0.0, 6.0, -1.0]])
+    M = np.array([[-6.0, 0.0, 0.0], [0.0, -6.0, 1.0], [4.2, -4.2, 0.0]])
+    lp = np.array([0, 3, 6], np.int32)
+    li = np.array([0, 1, 2, 0, 1, 2], np.int32)
+    rng = np.random.default_rng(4)
+    cm = 0.01 + 0.01 * rng.random(E)
+    ck = 0.1 + 0.05 * rng.random(E)
+    c2 = np.full(E, 0.2)
+    omega = 2 * np.pi * 100e-6
+    qv = m.qvec_gaussian(Q, 2.0)
+    mv = m.qvec_gaussian(M, 2.0) * 0.2
+
+    def fields(cm_, ck_):
+        rp, ci, val = m.sysmat_el(cm_, ck_, c2, omega)
+        d = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-15, 5000)[0] for q in qv])
+        a = np.array([O.bicgstab(rp, ci, val, q.astype(complex), 1e-15, 5000)[0] for q in mv])
+        return d, a
+
+    d, a = fields(cm, ck)
+    proj = O.project(d, mv.astype(complex), lp, li, logscale=False)
+    J = O.jacobian_el(m, d, a, lp, li, proj, logscale=True)
+    nqm = len(li)
+    assert J.shape == (2 * nqm, 2 * E)
+    for e in rng.choice(E, 4, replace=False):
+        for blk, base in ((0, cm), (1, ck)):
+            h = 1e-6 * base[e]
+            cp, cn, cq, cr = cm.copy(), ck.copy(), cm.copy(), ck.copy()
+            if blk == 0:
+                cp[e] += h; cq[e] -= h
+            else:
+                cn[e] += h; cr[e] -= h
+            yp = O.project(fields(cp, cn)[0], mv.astype(complex), lp, li, logscale=True)
+            ym = O.project(fields(cq, cr)[0], mv.astype(complex), lp, li, logscale=True)
+            fd = (yp - ym) / (2 * h)
+            col = e + blk * E
+            jc = J[:nqm, col] + 1j * J[nqm:, col]
+            assert np.max(np.abs(jc - fd)) <= 1e-5 * np.max(np.abs(jc)), (e, blk)
+    # the element sums of the mua block equal the node sums of the nodal
+    # Jacobian on the same fields (sum_i IntFFF(i,j,k) = IntFF(j,k)); for
+    # Tet4 so do the kappa blocks (sum_i IntFDD(i,j,k) = IntDD(j,k))
+    Jn = O.jacobian_nodal(m, d, a, lp, li, proj, logscale=True)
+    for blk in (0, 1):
+        se = J[:, blk * E:(blk + 1) * E].sum(1)
+        sn = Jn[:, blk * m.n:(blk + 1) * m.n].sum(1)
+        assert np.max(np.abs(se - sn)) <= 1e-12 * np.max(np.abs(sn))
+    # the mvec overload's kappa view at offset n: mua columns n.. hold kappa
+    Jo = O.jacobian_el(m, d, a, lp, li, proj, logscale=True, mvec_offset=True)
+    assert np.array_equal(Jo[:, :m.n], J[:, :m.n]) and np.array_equal(Jo[:, E:], J[:, E:])
+    assert np.array_equal(Jo[:, m.n:E], J[:, E:2 * E - m.n])

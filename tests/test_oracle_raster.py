@@ -413,3 +413,201 @@ def test_grid_jacobian_basis_map_identity():
     for blk in range(2):
         ref_b = P @ gg[blk * ng:(blk + 1) * ng]
         assert np.max(np.abs(gb[blk * nb:(blk + 1) * nb] - ref_b)) <= 1e-12 * np.abs(ref_b).max()
+
+
+# ---------------------------------------------------------------------------
+# Closed forms of every grid-Jacobian entry on linear fields
+#
+# With nodal fields that are linear functions of position, the grid map B
+# reproduces them exactly at the grid points inside the mesh (pinned above),
+# and ImageGradient's central and one-sided differences are both exact up to
+# the reference's step choice: grid spacing h = gsize/(n-1) over the step
+# dx = gsize/n (jacobian.cc:961,1000,1039), i.e. a factor n/(n-1).  A point
+# with no usable neighbour along an axis gets 0 there.  So every solution
+# voxel's entries follow from the field coefficients alone:
+#   mua:   -f(x) g(x) / proj * sz
+#   kappa: -sum_d Df_d Dg_d / proj * sz,   Df_d = a_d n_d/(n_d-1) [neighbour]
+# (PMDF_mua / PMDF_kap / PMDF_log, jacobian.cc:57-79; sz = prod gsize/(n-1),
+# :368-372; rows link-ordered, Re in row k and Im in row k + nQM).
+
+def _grid_points(R):
+    gdim = R.gdim
+    h = (R.bbmax - R.bbmin) / (gdim - 1)
+    idx = np.arange(R.glen)
+    coords, stride = [], 1
+    for d in range(R.dim):
+        i_d = (idx // stride) % gdim[d]
+        coords.append(R.bbmin[d] + i_d * h[d])
+        stride *= gdim[d]
+    return np.stack(coords, axis=1), h
+
+
+def _neighbour_flags(R):
+    """per grid point and axis: (left exists and inside, right exists and
+    inside, left exists, right exists)"""
+    gdim, inside = R.gdim, R.elref >= 0
+    idx = np.arange(R.glen)
+    out, stride = [], 1
+    for d in range(R.dim):
+        i_d = (idx // stride) % gdim[d]
+        has_l, has_r = i_d > 0, i_d < gdim[d] - 1
+        il = np.where(has_l, idx - stride, idx)
+        ir = np.where(has_r, idx + stride, idx)
+        out.append((has_l & inside[il], has_r & inside[ir], has_l, has_r, il, ir))
+        stride *= gdim[d]
+    return out
+
+
+def _linear_case(case):
+    rng = np.random.default_rng(5)
+    if case == "disk":
+        nodes, el = G.disk(10.0, 6)
+        gdim = [23, 19]
+    else:
+        nodes, el = G.cylinder(6.0, 8.0, 3, 4)
+        gdim = [11, 9, 13]
+    dim = nodes.shape[1]
+    nq, nm = 3, 2
+    A = rng.normal(size=(nq, dim + 1)) + 1j * rng.normal(size=(nq, dim + 1))
+    Bc = rng.normal(size=(nm, dim + 1)) + 1j * rng.normal(size=(nm, dim + 1))
+    lp = np.array([0, 2, 3, 5], np.int32)
+    li = np.array([0, 1, 1, 0, 1], np.int32)
+    proj = rng.normal(size=5) + 1j * rng.normal(size=5)
+    return nodes, el, gdim, A, Bc, lp, li, proj
+
+
+def _lin(C_, X):
+    return C_[:, :1] + C_[:, 1:] @ X.T   # rows: one field per coefficient row
+
+
+@pytest.mark.parametrize("case", ["disk", "cyl"])
+def test_grid_jacobian_closed_form_on_linear_fields(case):
+    nodes, el, gdim, A, Bc, lp, li, proj = _linear_case(case)
+    m = O.Mesh(nodes=nodes, elems=el)
+    R = O.Raster(m, gdim)
+    dim = nodes.shape[1]
+    dphi, aphi = _lin(A, nodes), _lin(Bc, nodes)
+    J = O.jacobian_grid(R, dphi, aphi, lp, li, proj, logscale=True)
+    X, h = _grid_points(R)
+    sol = np.flatnonzero(R.elref >= 0)
+    assert np.array_equal(sol, R.sol2basis)
+    Xs = X[sol]
+    r = np.asarray(gdim, float) / (np.asarray(gdim, float) - 1.0)
+    has_nb = np.stack([(fl[0] | fl[1])[sol] for fl in _neighbour_flags(R)], axis=1)   # slen x dim
+    sz = np.prod(h)
+    nqm = len(li)
+    exp = np.zeros_like(J)
+    k = 0
+    for q in range(len(lp) - 1):
+        fq = _lin(A[q:q + 1], Xs)[0]
+        Df = A[q, 1:][None, :] * r[None, :] * has_nb
+        for t in range(lp[q], lp[q + 1]):
+            mi = li[t]
+            gm = _lin(Bc[mi:mi + 1], Xs)[0]
+            Dg = Bc[mi, 1:][None, :] * r[None, :] * has_nb
+            pm = -fq * gm / proj[k] * sz
+            pk = -(Df * Dg).sum(1) / proj[k] * sz
+            exp[k, :R.slen], exp[k + nqm, :R.slen] = pm.real, pm.imag
+            exp[k, R.slen:], exp[k + nqm, R.slen:] = pk.real, pk.imag
+            k += 1
+    for blk in (slice(0, R.slen), slice(R.slen, 2 * R.slen)):
+        scale = np.max(np.abs(exp[:, blk]))
+        assert np.max(np.abs(J[:, blk] - exp[:, blk])) <= 1e-12 * scale
+    # some voxels sit next to the boundary, with a one-sided difference or none
+    assert np.any(~has_nb) or any(np.any(~(fl[0] & fl[1])[sol]) for fl in _neighbour_flags(R))
+    # the checks fail a dropped n/(n-1), a lost sign or swapped Re/Im rows
+    assert np.max(np.abs(exp[:, R.slen:])) > 0
+    bad = exp.copy()
+    bad[:, R.slen:] /= r[0] ** 2
+    assert np.max(np.abs(J - bad)) > 1e-6 * np.max(np.abs(exp))
+
+
+@pytest.mark.parametrize("case", ["disk", "cyl"])
+def test_grid_jacobian_cw_closed_form_on_linear_fields(case):
+    """CW (GenerateJacobian_cw_grid, jacobian.cc:610-721): the direct-field
+    gradient is masked, the adjoint gradient is not (ImageGradient without
+    a mask, :656), so next to the boundary it differences against the zero
+    the grid map gives outside the mesh.  Closed form: g-tilde = g inside,
+    0 outside; central where both grid neighbours exist, one-sided at the
+    grid's own edges."""
+    nodes, el, gdim, A, Bc, lp, li, proj = _linear_case(case)
+    A, Bc, proj = A.real.copy(), Bc.real.copy(), np.abs(proj.real) + 0.5
+    m = O.Mesh(nodes=nodes, elems=el)
+    R = O.Raster(m, gdim)
+    dphi, aphi = _lin(A, nodes), _lin(Bc, nodes)
+    Jm, Jk = O.jacobian_grid_cw(R, dphi, aphi, lp, li, proj, logscale=True)
+    X, h = _grid_points(R)
+    inside = R.elref >= 0
+    sol = np.flatnonzero(inside)
+    r = np.asarray(gdim, float) / (np.asarray(gdim, float) - 1.0)
+    flags = _neighbour_flags(R)
+    dx = (R.bbmax - R.bbmin) / np.asarray(gdim, float)
+    sz = np.prod(h)
+    k = 0
+    for q in range(len(lp) - 1):
+        f = _lin(A[q:q + 1], X)[0]
+        Df = np.stack([A[q, 1 + d] * r[d] * (flags[d][0] | flags[d][1]) for d in range(R.dim)], axis=1)[sol]
+        for t in range(lp[q], lp[q + 1]):
+            mi = li[t]
+            gt = np.where(inside, _lin(Bc[mi:mi + 1], X)[0], 0.0)
+            Dg = []
+            for d in range(R.dim):
+                _, _, hl, hr, il, ir = flags[d]
+                c = np.where(hl & hr, (gt[ir] - gt[il]) / (2 * dx[d]),
+                             np.where(hr, (gt[ir] - gt) / dx[d], np.where(hl, (gt - gt[il]) / dx[d], 0.0)))
+                Dg.append(c)
+            Dg = np.stack(Dg, axis=1)[sol]
+            em = -f[sol] * gt[sol] / proj[k] * sz
+            ek = -(Df * Dg).sum(1) / proj[k] * sz
+            assert np.max(np.abs(Jm[k] - em)) <= 1e-12 * np.max(np.abs(em))
+            assert np.max(np.abs(Jk[k] - ek)) <= 1e-12 * np.max(np.abs(ek))
+            k += 1
+
+
+def test_image_gradient_branches_on_quadratic_image():
+    """Each branch of ImageGradient (jacobian.cc:962-985 and its y/z
+    twins) checked by value: on f = sum_d c_d x_d^2 + l.x + e x y the
+    central difference gives f'(x) h/dx, the forward one (f'(x) + c h) h/dx,
+    the backward one (f'(x) - c h) h/dx, with h the grid spacing
+    gsize/(n-1) and dx = gsize/n the reference's step.  The branch follows
+    the mask: both neighbours inside -> central (even at a masked centre);
+    else a masked centre -> 0; else the one inside neighbour; else 0."""
+    gdim = np.array([7, 5, 4])
+    gsize = np.array([6.0, 8.0, 3.0])
+    h = gsize / (gdim - 1)
+    dx = gsize / gdim
+    c = np.array([0.7, -1.1, 0.4])
+    lcoef = np.array([0.3, 0.2, -0.5])
+    e = 0.6
+    glen = int(np.prod(gdim))
+    idx = np.arange(glen)
+    I = np.stack([idx % 7, (idx // 7) % 5, idx // 35], axis=1)
+    X = I * h
+    im = (c * X ** 2).sum(1) + X @ lcoef + e * X[:, 0] * X[:, 1]
+    rng = np.random.default_rng(9)
+    mask = np.where(rng.random(glen) < 0.3, -1, 0).astype(np.int32)
+    g = O.image_gradient(gdim, gsize, im + 0j, mask)
+    fprime = [2 * c[0] * X[:, 0] + lcoef[0] + e * X[:, 1],
+              2 * c[1] * X[:, 1] + lcoef[1] + e * X[:, 0],
+              2 * c[2] * X[:, 2] + lcoef[2]]
+    strides = [1, 7, 35]
+    counts = {"central": 0, "forward": 0, "backward": 0, "zero": 0}
+    for d in range(3):
+        for p in range(glen):
+            i = I[p, d]
+            bl = i > 0 and mask[p - strides[d]] >= 0
+            br = i < gdim[d] - 1 and mask[p + strides[d]] >= 0
+            if bl and br:
+                exp, kind = fprime[d][p] * h[d] / dx[d], "central"
+            elif mask[p] < 0:
+                exp, kind = 0.0, "zero"
+            elif bl:
+                exp, kind = (fprime[d][p] - c[d] * h[d]) * h[d] / dx[d], "backward"
+            elif br:
+                exp, kind = (fprime[d][p] + c[d] * h[d]) * h[d] / dx[d], "forward"
+            else:
+                exp, kind = 0.0, "zero"
+            counts[kind] += 1
+            assert abs(g[d][p].real - exp) <= 1e-12 * (1 + abs(exp)), (d, p, kind)
+            assert g[d][p].imag == 0.0
+    assert min(counts.values()) > 10, counts

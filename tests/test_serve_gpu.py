@@ -95,3 +95,38 @@ def test_request_larger_than_budget_is_refused(pool):
     with pytest.raises(L.TmiError):
         srv.submit(f, 8, dB.ptr, dB.ptr, shape[1])
     srv.close()
+
+
+def test_retained_workspaces_counted_within_budget(pool):
+    """Slots keep their workspaces between requests; the server counts them
+    (held) beside the running requests' bounds (reserved) and stays within
+    the budget: with room for one 32-column workspace only, the second
+    large request makes the idle slot give its workspace back first.  The
+    results equal those of a server with room for everything."""
+    mesh, f, dB, shape = pool[2]
+    big = f.WorkBytes(32)
+    budget = 1.05 * big
+    srv = Server(slots=2, hbm_budget_gib=budget / 2.0 ** 30)
+    outs = []
+    peak = 0.0
+    for nq in (32, 32, 4, 32):
+        dX = L.DeviceArray(shape, np.complex128)
+        L.call("tmi_memcpy_h2d", dX._p, np.zeros(shape, np.complex128).ctypes.data_as(L.C.c_void_p), dX.nbytes)
+        rid = srv.submit(f, nq, dB.ptr, dX.ptr, shape[1])
+        while True:
+            running, queued, held = srv.counts()
+            peak = max(peak, held)
+            try:
+                st, _, status = srv.wait(rid, timeout=0.002)
+                break
+            except L.TmiError as e:
+                if e.code != L.TMI_ERR_TIMEOUT:
+                    raise
+        assert status == L.TMI_OK and all(s[3] for s in st)
+        peak = max(peak, srv.counts()[2])
+        outs.append((dX.to_host()[:, :nq], nq))
+    srv.close()
+    assert peak <= budget, "counted device bytes %.0f above the budget %.0f" % (peak, budget)
+    ref = _run(pool, [(2, nq) for _, nq in outs], slots=1, concurrent=False, budget=64.0)
+    for (a, _), b in zip(outs, ref):
+        assert np.array_equal(a, b)

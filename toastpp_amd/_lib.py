@@ -27,6 +27,7 @@ MODE_FP64_SINGLE, MODE_MIXED_BLOCK, MODE_FP64_BLOCK = 0, 1, 2
 DATA_LIN, DATA_LOG = 0, 1
 ORDER_NATURAL, ORDER_MORTON = 0, 1
 ELID_TET4, ELID_TRI3 = 3, 15
+BASIS_NODAL, BASIS_ELEMENT, BASIS_RASTER = 0, 1, 2
 
 _ERRNAMES = {1: "invalid argument", 2: "not converged", 3: "out of memory", 4: "device error", 5: "I/O error",
              6: "invalid state", 7: "unsupported", 8: "timeout"}
@@ -91,10 +92,13 @@ _SIGS = {
     "tmi_solver_set_spmm_variant": [P, C.c_int],
     "tmi_solver_work_bytes": [P, C.c_int, DP],
     "tmi_solver_set_compaction": [P, C.c_int],
+    "tmi_solver_set_fallback": [P, C.c_int, C.c_int],
     "tmi_solver_autotune": [P, C.c_int, P, P, C.c_int, IP, IP, IP, DP],
     "tmi_solver_allocate": [P],
     "tmi_solver_assemble": [P, D, D, D, C.c_double],
     "tmi_solver_reset": [P, D, D, D, C.c_double],
+    "tmi_solver_assemble_basis": [P, D, D, D, C.c_double, C.c_int],
+    "tmi_solver_reset_basis": [P, D, D, D, C.c_double, C.c_int],
     "tmi_solver_nnz": [P, IP],
     "tmi_solver_get_sysmat": [P, P, P, P],
     "tmi_solver_calc_fields": [P, C.c_int, I, I, D, D, P],
@@ -106,6 +110,11 @@ _SIGS = {
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
     "tmi_jacobian_cw": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
+    "tmi_jacobian_element": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
+    "tmi_jacobian_element_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],
+    "tmi_jacobian_cw_element": [P, C.c_int, C.c_int, D, D, P, C.c_int, P, P],
+    "tmi_jacobian_mvec": [P, P, C.c_int, C.c_int, C.c_int, I, I, D, D, D, C.c_int, D],
+    "tmi_jacobian_cw_mvec": [P, P, C.c_int, C.c_int, C.c_int, I, I, D, D, D, C.c_int, P, P],
     "tmi_raster_data_gradient": [P, C.c_int, D, D, D],
     "tmi_raster_get_nodemap": [P, P, P],
     "tmi_raster_map_grid_to_mesh": [P, C.c_int, D, D],

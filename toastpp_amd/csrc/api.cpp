@@ -405,7 +405,10 @@ int tmi_optical_to_coeffs(int nlen, const double *mua, const double *mus, const 
     NEED(mua); NEED(mus); NEED(ref);
     const double c0 = 0.3;
     for (int i = 0; i < nlen; i++) {
-        TMI_CHECK(ref[i] > 1.0, TMI_ERR_INVALID_ARG, "refractive index at node %d must exceed 1 (got %g)", i, ref[i]);
+        // A_Keijzer is defined for n >= 1 (n = 1, the index-matched boundary, gives A = 1)
+        TMI_CHECK(std::isfinite(mua[i]) && std::isfinite(mus[i]) && std::isfinite(ref[i]), TMI_ERR_INVALID_ARG,
+                  "non-finite optical parameter at node %d (mua %g, mus %g, ref %g)", i, mua[i], mus[i], ref[i]);
+        TMI_CHECK(ref[i] >= 1.0, TMI_ERR_INVALID_ARG, "refractive index at node %d must be >= 1 (got %g)", i, ref[i]);
         if (cmua) cmua[i] = mua[i] * c0 / ref[i];
         if (ckappa) ckappa[i] = c0 / (3.0 * ref[i] * (mua[i] + mus[i]));
         if (c2a) c2a[i] = c0 / (2.0 * ref[i] * a_keijzer(ref[i]));
@@ -563,6 +566,16 @@ int tmi_solver_set_compaction(tmi_solver h, int min_width)
     API_END
 }
 
+int tmi_solver_set_fallback(tmi_solver h, int enable, int mixed_maxit)
+{
+    API_BEGIN
+    TMI_CHECK(mixed_maxit >= 0, TMI_ERR_INVALID_ARG, "mixed_maxit must be >= 0");
+    Solver &s = S(h);
+    s.fp64_fallback = enable != 0;
+    s.mixed_maxit = mixed_maxit;
+    API_END
+}
+
 int tmi_solver_work_bytes(tmi_solver h, int nq, double *bytes)
 {
     API_BEGIN
@@ -580,11 +593,50 @@ int tmi_solver_allocate(tmi_solver h)
     API_END
 }
 
+// FEM coefficients and omega must be finite: a NaN or Inf would reach every
+// entry of the rows it touches and every iterate of the solve
+static void check_coeffs(const Solver &s, const double *cmua, const double *ckappa, const double *c2a, double omega,
+                         bool elbasis = false)
+{
+    TMI_CHECK(std::isfinite(omega), TMI_ERR_INVALID_ARG, "non-finite omega");
+    const int n = elbasis ? s.mesh->elen : s.mesh->n;
+    for (int i = 0; i < n; i++)
+        TMI_CHECK(std::isfinite(cmua[i]) && std::isfinite(ckappa[i]) && std::isfinite(c2a[i]), TMI_ERR_INVALID_ARG,
+                  "non-finite coefficient at %s %d (cmua %g, ckappa %g, c2a %g)", elbasis ? "element" : "node", i,
+                  cmua[i], ckappa[i], c2a[i]);
+}
+
+int tmi_solver_assemble_basis(tmi_solver h, const double *cmua, const double *ckappa, const double *c2a,
+                              double omega, int elbasis)
+{
+    API_BEGIN
+    NEED(cmua); NEED(ckappa); NEED(c2a);
+    Solver &s = S(h);
+    check_coeffs(s, cmua, ckappa, c2a, omega, elbasis != 0);
+    s.assemble(cmua, ckappa, c2a, omega, true, elbasis != 0);
+    TMI_HIP(hipStreamSynchronize(s.st));
+    API_END
+}
+
+int tmi_solver_reset_basis(tmi_solver h, const double *cmua, const double *ckappa, const double *c2a, double omega,
+                           int elbasis)
+{
+    API_BEGIN
+    NEED(cmua); NEED(ckappa); NEED(c2a);
+    Solver &s = S(h);
+    check_coeffs(s, cmua, ckappa, c2a, omega, elbasis != 0);
+    s.assemble(cmua, ckappa, c2a, omega, true, elbasis != 0);
+    s.reset_precon();
+    TMI_HIP(hipStreamSynchronize(s.st));
+    API_END
+}
+
 int tmi_solver_assemble(tmi_solver h, const double *cmua, const double *ckappa, const double *c2a, double omega)
 {
     API_BEGIN
     NEED(cmua); NEED(ckappa); NEED(c2a);
     Solver &s = S(h);
+    check_coeffs(s, cmua, ckappa, c2a, omega);
     s.assemble(cmua, ckappa, c2a, omega, true);
     TMI_HIP(hipStreamSynchronize(s.st));
     API_END
@@ -595,6 +647,7 @@ int tmi_solver_reset(tmi_solver h, const double *cmua, const double *ckappa, con
     API_BEGIN
     NEED(cmua); NEED(ckappa); NEED(c2a);
     Solver &s = S(h);
+    check_coeffs(s, cmua, ckappa, c2a, omega);
     s.assemble(cmua, ckappa, c2a, omega, true);
     s.reset_precon();
     TMI_HIP(hipStreamSynchronize(s.st));

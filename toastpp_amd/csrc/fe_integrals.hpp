@@ -135,6 +135,15 @@ TMI_HD double tet_intfdd(const double *g, int j, int k)   // IntFDD(.,j,k): inde
     return (b[j] * b[k] + c[j] * c[k] + d[j] * d[k]) * scale;
 }
 
+// IntDD (the stored intdd, tet4.cc:1016-1031): (b_i b_j + c_i c_j + d_i d_j) / (36 V)
+TMI_HD double tet_intdd(const double *g, int i, int j)
+{
+    if (i < j) { int t = i; i = j; j = t; }
+    const double *b = g + 4, *c = g + 8, *d = g + 12;
+    double scale = 1.0 / (36.0 * g[16]);
+    return scale * (b[j] * b[i] + c[j] * c[i] + d[j] * d[i]);
+}
+
 TMI_HD double tet_intpff(const double *g, int i, int j, double P0, double P1, double P2, double P3)
 {
     return ((double)mult3(0, i, j) / 120.0 * P0 + (double)mult3(1, i, j) / 120.0 * P1 +

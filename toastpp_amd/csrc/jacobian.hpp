@@ -17,6 +17,14 @@ void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, 
                            const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
                            int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw = false,
                            double *d_Jkap = nullptr);
+// Element-basis Jacobian: d_J (2 nqm) x (2 elen) row-major, or with cw the
+// nqm x elen blocks d_J (mua) and d_Jkap (kappa, either may be null) from real
+// fields with zero imaginary parts and nqm real projections.  mvec_n > 0: the
+// mua columns n .. elen-1 repeat the kappa PMDF of elements 0 .. elen-n-1, as
+// the reference's mvec overload lays it out.
+void launch_jacobian_el(const Mesh &m, int nq, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                        const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale, int mvec_n,
+                        double *d_J, hipStream_t st, bool cw = false, double *d_Jkap = nullptr);
 // real [rows][ld] -> complex [rows][cols] (imaginary parts zero)
 void launch_real_to_cplx(long rows, int cols, const double *src, int ld, double *dst, hipStream_t st);
 

@@ -481,4 +481,158 @@ int tmi_jacobian_raster_device(tmi_raster r, int nq, int nm, const void *d_dphi,
     API_END
 }
 
+// ------------------------------------------------------------ element basis
+static void jacobian_el_dev(Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                            const double *d_proj, int dscale, double *d_J, hipStream_t st, bool cw = false,
+                            double *d_Jkap = nullptr, int mvec_n = 0)
+{
+    const QM &qm = m.qm;
+    TMI_CHECK(qm.nq == nq && qm.nm == nm, TMI_ERR_INVALID_ARG,
+              "field counts (%d sources, %d detectors) do not match the QM description (%d, %d)", nq, nm, qm.nq,
+              qm.nm);
+    TMI_CHECK(qm.nqm > 0, TMI_ERR_INVALID_ARG, "the QM description links no measurements");
+    TMI_CHECK(ldd >= nq && lda >= nm, TMI_ERR_INVALID_ARG, "field row strides (%d, %d) below the field counts (%d, %d)",
+              ldd, lda, nq, nm);
+    TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
+    TMI_CHECK(nq <= 65535, TMI_ERR_UNSUPPORTED, "element-basis Jacobian: at most 65535 sources");
+    ensure_links(m, st);
+    launch_jacobian_el(m, nq, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lm.p, qm.nqm, d_proj,
+                       dscale == TMI_DATA_LOG ? 1 : 0, mvec_n, d_J, st, cw, d_Jkap);
+}
+
+// linear projections from the measurement vectors (ProjectSingle / ProjectAll
+// with DATA_LIN, fwdsolver.cc:1415-1447): d_phi [n][nq] node-major (complex
+// or real), mvec nm x n CSR on the host; d_proj gets nqm values in link order
+static void project_from_mvec(Mesh &m, int nm, const int *mrp, const int *mci, const double *mval,
+                              const double *d_phi, int cplx, DBuf<double> &d_proj, hipStream_t st)
+{
+    const QM &qm = m.qm;
+    TMI_CHECK(nm == qm.nm, TMI_ERR_INVALID_ARG, "mvec has %d rows, the QM description %d detectors", nm, qm.nm);
+    TMI_CHECK(mrp[0] == 0, TMI_ERR_INVALID_ARG, "mvec row pointer must start at 0");
+    for (int i = 0; i < nm; i++) TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
+    const int mnnz = mrp[nm];
+    for (int k = 0; k < mnnz; k++)
+        TMI_CHECK(mci[k] >= 0 && mci[k] < m.n, TMI_ERR_INVALID_ARG, "mvec column out of range");
+    std::vector<int> lk((size_t)nm * qm.nq, -1);   // link of (detector, source), -1 if not linked
+    for (int q = 0; q < qm.nq; q++)
+        for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) lk[(size_t)qm.lidx[k] * qm.nq + q] = k;
+    const int cw = cplx ? 2 : 1;
+    DBuf<int> dlk, drp, dci;
+    DBuf<double> dval;
+    dlk.upload(lk.data(), lk.size(), st);
+    drp.upload(mrp, nm + 1, st);
+    dci.upload(mci, mnnz, st);
+    dval.upload(mval, (size_t)mnnz * cw, st);
+    d_proj.alloc((size_t)qm.nqm * cw);
+    launch_project(qm.nq, nm, dlk.p, drp.p, dci.p, dval.p, d_phi, 1, qm.nq, cplx, d_proj.p, st);
+    TMI_HIP(hipStreamSynchronize(st));   // the uploads above go out of scope
+}
+
+static size_t basis_len(Mesh &m, tmi_raster r, int basis)
+{
+    switch (basis) {
+    case TMI_BASIS_NODAL: return m.n;
+    case TMI_BASIS_ELEMENT: return m.elen;
+    case TMI_BASIS_RASTER:
+        TMI_CHECK(r && r->r.mesh == &m, TMI_ERR_INVALID_ARG, "the raster basis needs a raster of this mesh");
+        return (size_t)r->r.slen;
+    }
+    fail(TMI_ERR_INVALID_ARG, "unknown basis %d", basis);
+}
+
+int tmi_jacobian_element(tmi_mesh h, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                         int dscale, double *J)
+{
+    API_BEGIN
+    TMI_CHECK(h && dphi && aphi && proj && J, TMI_ERR_INVALID_ARG, "NULL argument");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    size_t n = m.n, nqm = m.qm.nqm, E = m.elen;
+    DBuf<double> dd, da, dp, dJ(4 * nqm * E);
+    dd.upload(dphi, n * nq * 2, st);
+    da.upload(aphi, n * nm * 2, st);
+    dp.upload(proj, nqm * 2, st);
+    jacobian_el_dev(m, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
+    dJ.download(J, 4 * nqm * E, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_element_device(tmi_mesh h, int nq, int nm, const void *d_dphi, int ldd, const void *d_aphi, int lda,
+                                const void *d_proj, int dscale, void *d_J)
+{
+    API_BEGIN
+    TMI_CHECK(h && d_dphi && d_aphi && d_proj && d_J, TMI_ERR_INVALID_ARG, "NULL argument");
+    hipStream_t st = Context::get().stream;
+    jacobian_el_dev(h->m, nq, nm, (const double *)d_dphi, ldd, (const double *)d_aphi, lda, (const double *)d_proj,
+                    dscale, (double *)d_J, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_cw_element(tmi_mesh h, int nq, int nm, const double *dphi, const double *aphi, const double *proj,
+                            int dscale, double *Jmua, double *Jkap)
+{
+    API_BEGIN
+    TMI_CHECK(h && dphi && aphi && (Jmua || Jkap), TMI_ERR_INVALID_ARG, "NULL argument");
+    TMI_CHECK(dscale == TMI_DATA_LIN || proj, TMI_ERR_INVALID_ARG, "log data scaling needs the projections");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    const size_t n = m.n, nqm = m.qm.nqm, E = m.elen;
+    CwInputs in(n, nq, nm, dphi, aphi, proj, nqm, st);
+    DBuf<double> jm(Jmua ? nqm * E : 0), jk(Jkap ? nqm * E : 0);
+    jacobian_el_dev(m, nq, nm, in.cd.p, nq, in.ca.p, nm, in.dp.p, dscale, jm.p, st, true, jk.p);
+    if (Jmua) jm.download(Jmua, nqm * E, st);
+    if (Jkap) jk.download(Jkap, nqm * E, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_mvec(tmi_mesh h, tmi_raster r, int basis, int nq, int nm, const int *mrp, const int *mci,
+                      const double *mval, const double *dphi, const double *aphi, int dscale, double *J)
+{
+    API_BEGIN
+    TMI_CHECK(h && mrp && mci && mval && dphi && aphi && J, TMI_ERR_INVALID_ARG, "NULL argument");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    const size_t n = m.n, nqm = m.qm.nqm, len = basis_len(m, r, basis);
+    TMI_CHECK(m.qm.nq == nq, TMI_ERR_INVALID_ARG, "%d sources, the QM description %d", nq, m.qm.nq);
+    DBuf<double> dd, da, dp, dJ(4 * nqm * len);
+    dd.upload(dphi, n * nq * 2, st);
+    da.upload(aphi, n * nm * 2, st);
+    project_from_mvec(m, nm, mrp, mci, mval, dd.p, 1, dp, st);
+    if (basis == TMI_BASIS_NODAL) jacobian_nodal_dev(m, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
+    else if (basis == TMI_BASIS_ELEMENT)
+        jacobian_el_dev(m, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st, false, nullptr, m.elen > m.n ? m.n : 0);
+    else jacobian_raster_dev(r->r, nq, nm, dd.p, nq, da.p, nm, dp.p, dscale, dJ.p, st);
+    dJ.download(J, 4 * nqm * len, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int tmi_jacobian_cw_mvec(tmi_mesh h, tmi_raster r, int basis, int nq, int nm, const int *mrp, const int *mci,
+                         const double *mval, const double *dphi, const double *aphi, int dscale, double *Jmua,
+                         double *Jkap)
+{
+    API_BEGIN
+    TMI_CHECK(h && mrp && mci && mval && dphi && aphi && (Jmua || Jkap), TMI_ERR_INVALID_ARG, "NULL argument");
+    Mesh &m = h->m;
+    hipStream_t st = Context::get().stream;
+    const size_t n = m.n, nqm = m.qm.nqm, len = basis_len(m, r, basis);
+    TMI_CHECK(m.qm.nq == nq, TMI_ERR_INVALID_ARG, "%d sources, the QM description %d", nq, m.qm.nq);
+    CwInputs in(n, nq, nm, dphi, aphi, nullptr, nqm, st);
+    DBuf<double> dp;
+    project_from_mvec(m, nm, mrp, mci, mval, in.rd.p, 0, dp, st);
+    DBuf<double> jm(Jmua ? nqm * len : 0), jk(Jkap ? nqm * len : 0);
+    if (basis == TMI_BASIS_NODAL)
+        jacobian_nodal_dev(m, nq, nm, in.cd.p, nq, in.ca.p, nm, dp.p, dscale, jm.p, st, true, jk.p);
+    else if (basis == TMI_BASIS_ELEMENT)
+        jacobian_el_dev(m, nq, nm, in.cd.p, nq, in.ca.p, nm, dp.p, dscale, jm.p, st, true, jk.p);
+    else jacobian_raster_dev(r->r, nq, nm, in.cd.p, nq, in.ca.p, nm, dp.p, dscale, jm.p, st, true, jk.p);
+    if (Jmua) jm.download(Jmua, nqm * len, st);
+    if (Jkap) jk.download(Jkap, nqm * len, st);
+    TMI_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
 }  // extern "C"

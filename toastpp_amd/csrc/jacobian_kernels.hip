@@ -276,6 +276,123 @@ __global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const 
     }
 }
 
+// Element-basis Jacobian (GenerateJacobian_mesh with elbasis = true,
+// jacobian.cc:441-497; IntFG_el :879-906, IntGradFGradG_el :911-939): one
+// thread per (element, source).  The element's IntFF / IntDD entries and the
+// direct field at its nodes stay in registers; for every detector linked to
+// the source the thread gathers the adjoint field at the nodes, forms
+//   P_mua = sum_i sum_j (f_i g_j) IntFF(i,j),  P_kap = sum_i sum_j (f_i g_j) IntDD(i,j)
+// in the reference's order (i outer, j inner, complex product first), scales
+// by 1/proj (log data) and writes the link's four J entries -- rows k and
+// k + nqm, columns e and elen + e -- coalesced over the workgroup's
+// consecutive elements.
+// mvec_n > 0 reproduces the mvec overload of GenerateJacobian_mesh, whose
+// kappa PMDF is a view at offset n instead of elen (jacobian.cc:397): mua
+// columns n .. elen-1 then hold the kappa PMDF of elements 0 .. elen-n-1.
+// cw: real fields given with zero imaginary parts; Jmua / Jkap (nqm x elen)
+// take -P * (1/proj) (GenerateJacobian_cw_mesh :727-790 + the row scaling
+// of :160-186).
+template <int DIM, bool CW>
+__global__ void __launch_bounds__(256) k_jac_el(int elen, int nqm, const int *__restrict__ el,
+                                                const double *__restrict__ geo, const double2 *__restrict__ dphi,
+                                                int ldd, const double2 *__restrict__ aphi, int lda,
+                                                const int *__restrict__ lptr, const int *__restrict__ lm,
+                                                const double2 *__restrict__ proj, const double *__restrict__ projr,
+                                                int logscale, int mvec_n, double *__restrict__ J,
+                                                double *__restrict__ Jkap)
+{
+    constexpr int NN = DIM + 1;
+    constexpr int GS = DIM == 3 ? GEO3 : GEO2;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (e >= elen) return;
+    int nd[NN];
+    for (int i = 0; i < NN; i++) nd[i] = el[(long)NN * e + i];
+    const double *g = geo + (long)GS * e;
+    double ff[NN][NN], dd[NN][NN];
+    for (int i = 0; i < NN; i++)
+        for (int j = 0; j < NN; j++) {
+            ff[i][j] = DIM == 3 ? tet_intff(g, i, j) : tri_intff(g, i, j);
+            dd[i][j] = DIM == 3 ? tet_intdd(g, i, j) : tri_intdd(g, i, j);
+        }
+    double2 f[NN];
+    for (int i = 0; i < NN; i++) f[i] = dphi[(long)nd[i] * ldd + q];
+    const size_t ld = CW ? (size_t)elen : 2 * (size_t)elen;
+    for (int k = lptr[q]; k < lptr[q + 1]; k++) {
+        const int mi = lm[k];
+        double2 a[NN];
+        for (int j = 0; j < NN; j++) a[j] = aphi[(long)nd[j] * lda + mi];
+        double mr = 0.0, mim = 0.0, kr = 0.0, kim = 0.0;
+        for (int i = 0; i < NN; i++)
+            for (int j = 0; j < NN; j++) {
+                const double pr = f[i].x * a[j].x - f[i].y * a[j].y;
+                const double pi = f[i].x * a[j].y + f[i].y * a[j].x;
+                mr += pr * ff[i][j];
+                mim += pi * ff[i][j];
+                kr += pr * dd[i][j];
+                kim += pi * dd[i][j];
+            }
+        if (CW) {
+            double am = -mr, ak = -kr;
+            if (logscale) {
+                const double ip = 1.0 / projr[k];
+                am *= ip;
+                ak *= ip;
+            }
+            if (J) st_stream(J + (size_t)k * ld + e, am);
+            if (Jkap) st_stream(Jkap + (size_t)k * ld + e, ak);
+            continue;
+        }
+        if (logscale) {
+            const double2 p = proj[k];
+            const double den = p.x * p.x + p.y * p.y;
+            const double ar = (mr * p.x + mim * p.y) / den, ai = (mim * p.x - mr * p.y) / den;
+            const double br = (kr * p.x + kim * p.y) / den, bi = (kim * p.x - kr * p.y) / den;
+            mr = ar; mim = ai; kr = br; kim = bi;
+        }
+        double *R0 = J + (size_t)k * ld, *R1 = J + (size_t)(k + nqm) * ld;
+        if (mvec_n <= 0 || e < mvec_n) {
+            st_stream(R0 + e, -mr);
+            st_stream(R1 + e, -mim);
+        }
+        st_stream(R0 + elen + e, -kr);
+        st_stream(R1 + elen + e, -kim);
+        if (mvec_n > 0 && e + mvec_n < elen) {
+            st_stream(R0 + e + mvec_n, -kr);
+            st_stream(R1 + e + mvec_n, -kim);
+        }
+    }
+}
+
+void launch_jacobian_el(const Mesh &m, int nq, const double *d_dphi, int ldd, const double *d_aphi, int lda,
+                        const int *d_lptr, const int *d_lm, int nqm, const double *d_proj, int logscale, int mvec_n,
+                        double *d_J, hipStream_t st, bool cw, double *d_Jkap)
+{
+    dim3 g((m.elen + 255) / 256, nq);
+    const double2 *pc = cw ? nullptr : (const double2 *)d_proj;
+    const double *pr = cw ? d_proj : nullptr;
+    if (m.dim == 3) {
+        if (cw)
+            hipLaunchKernelGGL((k_jac_el<3, true>), g, dim3(256), 0, st, m.elen, nqm, m.elist.p, m.geo.p,
+                               (const double2 *)d_dphi, ldd, (const double2 *)d_aphi, lda, d_lptr, d_lm, pc, pr,
+                               logscale, 0, d_J, d_Jkap);
+        else
+            hipLaunchKernelGGL((k_jac_el<3, false>), g, dim3(256), 0, st, m.elen, nqm, m.elist.p, m.geo.p,
+                               (const double2 *)d_dphi, ldd, (const double2 *)d_aphi, lda, d_lptr, d_lm, pc, pr,
+                               logscale, mvec_n, d_J, nullptr);
+    } else {
+        if (cw)
+            hipLaunchKernelGGL((k_jac_el<2, true>), g, dim3(256), 0, st, m.elen, nqm, m.elist.p, m.geo.p,
+                               (const double2 *)d_dphi, ldd, (const double2 *)d_aphi, lda, d_lptr, d_lm, pc, pr,
+                               logscale, 0, d_J, d_Jkap);
+        else
+            hipLaunchKernelGGL((k_jac_el<2, false>), g, dim3(256), 0, st, m.elen, nqm, m.elist.p, m.geo.p,
+                               (const double2 *)d_dphi, ldd, (const double2 *)d_aphi, lda, d_lptr, d_lm, pc, pr,
+                               logscale, mvec_n, d_J, nullptr);
+    }
+    TMI_LAUNCH_CHECK();
+}
+
 // real fields (node-major, row strides ldd / lda) into the complex layout of
 // the nodal kernel, imaginary parts zero
 __global__ void k_real_to_cplx(long rows, int cols, const double *__restrict__ src, int ld, double2 *__restrict__ dst)

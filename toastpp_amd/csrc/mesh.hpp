@@ -85,7 +85,8 @@ void qvec_point(const Mesh &m, const double *p, bool neumann, double *out);
 void launch_exclusive_scan(const int *in, int *out, int n, hipStream_t st);  // out has n+1 entries
 void launch_qvec_gaussian(const Mesh &m, int npts, const double *d_centres, double width, int neumann,
                           const double *d_scale, double *d_out, hipStream_t st);
+// elbasis = 0: nodal coefficients (n each); 1: per-element coefficients (elen each)
 void launch_assemble(const Mesh &m, const double *d_cmua, const double *d_ckap, const double *d_c2a, double omega,
-                     int complex_valued, double *d_val64, void *d_val32, hipStream_t st);
+                     int complex_valued, double *d_val64, void *d_val32, hipStream_t st, int elbasis = 0);
 
 }  // namespace tmi

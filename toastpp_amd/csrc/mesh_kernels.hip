@@ -292,6 +292,10 @@ void Mesh::build_csr(hipStream_t st)
 // One thread per nonzero (i, j): walk the elements of row i in ascending
 // order and add each element's (i, j) contribution, pass by pass, exactly as
 // the reference adds them into the CSR slot (AddToElMatrix -> Add).
+// Nodal coefficients (elbasis = 0): IntPFF / IntPDD / BndIntPFF of the
+// nodal values; per-element coefficients (elbasis = 1, ASSEMBLE_*_EL,
+// mesh.cc:1985-1993, :2112-2120): the element's value times IntFF / IntDD /
+// BndIntFF.
 __device__ inline int local_index(const int *nd, int nn, int v)
 {
     for (int k = 0; k < nn; k++)
@@ -304,7 +308,7 @@ __global__ void k_assemble(const int *__restrict__ el, const double *__restrict_
                            const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                            const int *__restrict__ rowidx, const int *__restrict__ colidx, int nnz,
                            const double *__restrict__ cmua, const double *__restrict__ ckap,
-                           const double *__restrict__ c2a, double omega, int complex_valued,
+                           const double *__restrict__ c2a, double omega, int complex_valued, int elbasis,
                            double *__restrict__ val64, void *__restrict__ val32)
 {
     constexpr int NN = DIM + 1;
@@ -324,6 +328,22 @@ __global__ void k_assemble(const int *__restrict__ el, const double *__restrict_
             if (lj < 0) continue;
             int li = local_index(nd, NN, i);
             const double *g = geo + (long)GS * elx;
+            if (elbasis && pass < 3) {
+                const double pe = P[elx];
+                if (DIM == 3) {
+                    if (pass == 0) re += tet_intff(g, li, lj) * pe;
+                    else if (pass == 1) re += tet_intdd(g, li, lj) * pe;
+                    else {
+                        int fl = eflags[elx];
+                        if (fl & BNDEL_BIT) re += tet_bndff(g, fl, li, lj) * pe;
+                    }
+                } else {
+                    if (pass == 0) re += tri_intff(g, li, lj) * pe;
+                    else if (pass == 1) re += tri_intdd(g, li, lj) * pe;
+                    else re += tri_bndff(g, li, lj) * pe;
+                }
+                continue;
+            }
             if (DIM == 3) {
                 if (pass == 3) { im += tet_intff(g, li, lj) * omega; continue; }
                 double p0 = P[nd[0]], p1 = P[nd[1]], p2 = P[nd[2]], p3 = P[nd[3]];
@@ -356,17 +376,17 @@ __global__ void k_assemble(const int *__restrict__ el, const double *__restrict_
 }
 
 void launch_assemble(const Mesh &m, const double *d_cmua, const double *d_ckap, const double *d_c2a, double omega,
-                     int complex_valued, double *d_val64, void *d_val32, hipStream_t st)
+                     int complex_valued, double *d_val64, void *d_val32, hipStream_t st, int elbasis)
 {
     dim3 g(grid_for(m.nnz, 256)), b(256);
     if (m.dim == 3)
         hipLaunchKernelGGL(k_assemble<3>, g, b, 0, st, m.elist.p, m.geo.p, m.eflags.p, m.n2e_ptr.p, m.n2e.p,
-                           m.rowidx.p, m.colidx.p, m.nnz, d_cmua, d_ckap, d_c2a, omega, complex_valued, d_val64,
-                           d_val32);
+                           m.rowidx.p, m.colidx.p, m.nnz, d_cmua, d_ckap, d_c2a, omega, complex_valued, elbasis,
+                           d_val64, d_val32);
     else
         hipLaunchKernelGGL(k_assemble<2>, g, b, 0, st, m.elist.p, m.geo.p, m.eflags.p, m.n2e_ptr.p, m.n2e.p,
-                           m.rowidx.p, m.colidx.p, m.nnz, d_cmua, d_ckap, d_c2a, omega, complex_valued, d_val64,
-                           d_val32);
+                           m.rowidx.p, m.colidx.p, m.nnz, d_cmua, d_ckap, d_c2a, omega, complex_valued, elbasis,
+                           d_val64, d_val32);
     TMI_LAUNCH_CHECK();
 }
 

@@ -4,10 +4,18 @@
 // device) and a block of right-hand sides in device memory.  The server owns
 // `slots` worker threads, each with its own HIP stream and solve workspace
 // (Solver::Exec).  Requests are admitted in arrival order as soon as a slot
-// is free and the workspaces of the running requests plus the new one fit
-// the HBM budget; admitted requests run concurrently, their kernels
+// is free and the device memory the server's workspaces may hold fits the
+// HBM budget; admitted requests run concurrently, their kernels
 // interleaving on the GPU.  All reductions are fixed-order, so a request's
 // result does not depend on what else is running.
+//
+// Accounting: a slot keeps the workspaces of its earlier requests (reused
+// when the next request has the same mesh size and tile widths).  Each slot
+// counts held (the bytes its workspace holds between requests) plus
+// reserved (the bound Solver::work_bytes gives for its running request,
+// FP64 fallback included), and a request is admitted on a slot only when
+// the sum over all slots stays within the budget.  An idle slot whose held
+// memory keeps the queue's head out frees its workspace.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -37,7 +45,8 @@ struct ServeRequest {
 
 class Server {
    public:
-    Server(int slots, double budget) : budget_(budget), t0_(std::chrono::steady_clock::now())
+    Server(int slots, double budget)
+        : budget_(budget), held_(slots, 0.0), resv_(slots, 0.0), t0_(std::chrono::steady_clock::now())
     {
         for (int i = 0; i < slots; i++) execs_.emplace_back(new Solver::Exec(true));
         for (int i = 0; i < slots; i++) workers_.emplace_back(&Server::work, this, i);
@@ -92,30 +101,58 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         if (running) *running = running_;
         if (queued) *queued = (int)queue_.size();
-        if (bytes) *bytes = inuse_;
+        if (bytes) *bytes = accounted();
     }
 
    private:
+    // device bytes counted for all slots (caller holds mu_)
+    double accounted() const
+    {
+        double b = 0;
+        for (size_t j = 0; j < held_.size(); j++) b += held_[j] + resv_[j];
+        return b;
+    }
+    // whether a request of `need` bytes fits on idle slot `slot` (caller holds mu_)
+    bool fits(int slot, double need) const { return accounted() + need <= budget_; }
+
     void work(int slot)
     {
         Solver::Exec &e = *execs_[slot];
         for (;;) {
             ServeRequest *r = nullptr;
+            bool trim = false;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 cv_work_.wait(lk, [&] {
                     if (stop_) return true;
                     if (queue_.empty()) return false;
-                    return inuse_ + reqs_[queue_.front()]->bytes <= budget_;
+                    // admit, or give back this idle slot's workspace if it keeps the head out
+                    return fits(slot, reqs_[queue_.front()]->bytes) || held_[slot] > 0;
                 });
                 if (stop_ && queue_.empty()) return;
                 if (queue_.empty()) continue;
                 long long id = queue_.front();
-                queue_.pop_front();
-                r = reqs_[id].get();
-                inuse_ += r->bytes;
-                running_++;
-                r->t_started = now();
+                if (!fits(slot, reqs_[id]->bytes)) {
+                    trim = true;
+                } else {
+                    queue_.pop_front();
+                    r = reqs_[id].get();
+                    resv_[slot] = (double)r->bytes;
+                    running_++;
+                    r->t_started = now();
+                }
+            }
+            if (trim) {
+                try {
+                    e.release_work();
+                } catch (...) {
+                }
+                {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    held_[slot] = (double)e.bytes();
+                }
+                cv_work_.notify_all();
+                continue;
             }
             r->stats.assign(r->nq, tmi_colstat{});
             try {
@@ -128,11 +165,13 @@ class Server {
                 r->status = TMI_ERR_DEVICE;
                 r->error = ex.what();
             }
+            const double held = (double)e.bytes();
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 r->t_finished = now();
                 r->done = true;
-                inuse_ -= r->bytes;
+                held_[slot] = held;
+                resv_[slot] = 0;
                 running_--;
             }
             cv_done_.notify_all();
@@ -141,7 +180,7 @@ class Server {
     }
 
     double budget_;
-    double inuse_ = 0;
+    std::vector<double> held_, resv_;   // per slot: workspace kept between requests, bound of the running request
     int running_ = 0;
     bool stop_ = false;
     long long next_id_ = 1;

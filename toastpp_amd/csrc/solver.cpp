@@ -1,14 +1,16 @@
 // Forward-solver driver: Allocate / Assemble / Reset / CalcFields on the GPU.
 //
-// CalcFields runs every mode as the same two-level loop over a tile of K
-// right-hand-side columns:
-//   outer (FP64):  r = q - K x  (FP64 CSR SpMM), per-column convergence test
-//                  ||r|| <= tol ||q|| (the reference criterion,
-//                  matrix_imp.hpp:350-413), then x += delta
-//   inner:         block BiCGSTAB (complex) or PCG (real) on K delta = r/||r||
-//                  -- complex FP32 in TMI_MODE_MIXED_BLOCK (to a loose inner
-//                  tolerance), FP64 in the FP64 modes (to the outer
-//                  tolerance, so one sweep normally suffices).
+// CalcFields runs every mode on tiles of K right-hand-side columns:
+//   FP64:   r = q - K x (FP64 CSR SpMM) and the per-column convergence test
+//           ||r|| <= tol ||q|| (the reference criterion, matrix_imp.hpp:350-413)
+//   Krylov: block BiCGSTAB (complex) or PCG (real) -- complex/real FP32 on the
+//           system scaled by 1/||q|| in TMI_MODE_MIXED_BLOCK, FP64 in the FP64
+//           modes.  The mixed mode makes reliable updates: every few
+//           iterations, and whenever every column's recursive residual meets
+//           the tolerance, the FP32 correction is folded into the FP64
+//           solution and the FP64 true residual replaces the FP32 recursive
+//           one (search directions kept).  The FP64 modes stop on the
+//           recursive residual, as the reference does.
 // Columns leave the iteration individually (device-side active flags); the
 // host reads the flags every few iterations only.
 #include "solver.hpp"
@@ -75,8 +77,37 @@ size_t Solver::work_bytes(int nq) const
     // the tile and, with compaction, its narrower copies (K/2 + K/4 + ... < K)
     // and the second workspaces of the repacked widths (K and K/2)
     const size_t K = tile_for(nq), n = mesh->n, Kall = compact_min > 0 ? 2 * K + K + K / 2 : K;
-    const size_t s64 = vbytes(vk64()), sin = vbytes(mode == TMI_MODE_MIXED_BLOCK ? vk32() : vk64());
-    return n * Kall * (3 * s64 + 7 * sin) + 2 * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
+    const bool mixed = mode == TMI_MODE_MIXED_BLOCK;
+    const size_t s64 = vbytes(vk64()), sin = vbytes(mixed ? vk32() : vk64());
+    auto tiles = [&](size_t s_inner) {
+        return n * Kall * (3 * s64 + 7 * s_inner) + 2 * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
+    };
+    size_t b = tiles(sin);
+    if (mixed && K == 64 && cplx) b += 2 * n * 32;   // packed shadow residuals (both 64-column workspaces)
+    // the FP64 fallback of the mixed mode re-solves unconverged columns in
+    // FP64: the workspaces it uses are reallocated with FP64 inner vectors,
+    // and their right-hand sides and solutions are copied (at most nq each)
+    if (mixed && fp64_fallback) b = std::max(b, tiles(s64)) + 2 * n * (size_t)nq * s64;
+    return b;
+}
+
+void Solver::Exec::release_work()
+{
+    if (st) TMI_HIP(hipStreamSynchronize(st));
+    for (Work *ws : {wk, wk_alt})
+        for (int i = 0; i < 7; i++) {
+            Work &w = ws[i];
+            for (DBuf<char> *b : {&w.x64, &w.b64, &w.r64, &w.xi, &w.ri, &w.rdi, &w.p, &w.v, &w.s, &w.t}) b->release();
+            w.rdq.release();
+            w.partial.release();
+            w.sc.release();
+            w.osc.release();
+            w.ic.release();
+            w.oic.release();
+            w.K = w.n = w.G = 0;
+        }
+    free_gmres_work(gm);
+    gm = nullptr;
 }
 
 void Solver::allocate()
@@ -104,11 +135,13 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
     return b;
 }
 
-void Solver::assemble(const double *cmua, const double *ckap, const double *c2a, double om, bool host_ptrs)
+void Solver::assemble(const double *cmua, const double *ckap, const double *c2a, double om, bool host_ptrs,
+                      bool elbasis)
 {
     TMI_CHECK(allocated, TMI_ERR_STATE, "AssembleSystemMatrix called before Allocate");
     TMI_CHECK(cplx || om == 0.0, TMI_ERR_INVALID_ARG, "nonzero omega on a real (CW) solver");
-    int n = mesh->n;
+    // nodal coefficients, or one per element in the element basis
+    int n = elbasis ? mesh->elen : mesh->n;
     coef.alloc(3 * (size_t)n);
     if (host_ptrs) {
         TMI_HIP(hipMemcpyAsync(coef.p, cmua, n * sizeof(double), hipMemcpyHostToDevice, st));
@@ -122,7 +155,8 @@ void Solver::assemble(const double *cmua, const double *ckap, const double *c2a,
     omega = om;
     precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
     rb_src = nullptr;       // ... and so do the row-block coefficients
-    launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st);
+    launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st,
+                    elbasis ? 1 : 0);
     // the solver's copy in its own node order
     launch_permute_vals(cplx, mesh->nnz, ord().slot.p, val64.p, sval64.p, sval32.p, st);
     assembled = true;
@@ -219,7 +253,7 @@ static bool repack_pays(const int *flags, int K, int nact)
 // search directions are kept, so the iteration continues instead of
 // restarting.  A column is done when its FP64 true residual meets the
 // tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
-// improving (no 4x drop over 4 true-residual evaluations).
+// improving (no 4x drop of ||r||^2 over 8 true-residual evaluations).
 int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md)
 {
     hipStream_t st = e.st;
@@ -236,7 +270,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         idiag_in = mixed ? (const void *)idiag32.p : (const void *)idiag64.p;
         if (!use_cg) op_in = mixed ? (const void *)pval32.p : (const void *)pval64.p;
     }
-    const int maxit_total = maxit > 0 ? maxit : n + 1;
+    const int maxit_total = (mixed && mixed_maxit > 0) ? mixed_maxit : (maxit > 0 ? maxit : n + 1);
     // reliable updates only in the mixed mode: in FP64 the recursive
     // residual is the reference's own stopping test, and replacing it by the
     // computed true residual would inject eps*||K||*||x|| noise into the
@@ -544,12 +578,18 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
     TMI_CHECK(lsolver != TMI_LSOLVER_GMRES, TMI_ERR_UNSUPPORTED, "kernel timing covers the BiCGSTAB/PCG kernels");
     TMI_CHECK(w.K > 0 && assembled, TMI_ERR_STATE, "time_kernel needs a previous CalcFields");
     TMI_CHECK(reps > 0 && which >= 0 && which <= 1, TMI_ERR_INVALID_ARG, "bad kernel timing request");
+    bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
+    // the workspace must hold vectors of the precision the current mode's
+    // operator reads: an FP64 fallback or a mode change since the last
+    // CalcFields can leave FP64 vectors where FP32 ones are expected
+    TMI_CHECK(w.vk_in == (mixed ? vk32() : vk64()), TMI_ERR_STATE,
+              "time_kernel: the last CalcFields left this tile's workspace in another precision than the "
+              "current mode; run CalcFields in this mode first");
     int K = w.K, n = mesh->n;
     BlockDesc bd = block(w, K);
     std::vector<int> ones(K, 1);
     DBuf<int> act;
     act.upload(ones.data(), K, st);
-    bool mixed = (mode == TMI_MODE_MIXED_BLOCK);
     const void *idg = nullptr;
     const void *op = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     if (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
@@ -602,12 +642,24 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
     }
     const size_t es = vbytes(vk64());
     auto zero_x = [&]() { TMI_HIP(hipMemset2DAsync(d_x, (size_t)ld * es, 0, (size_t)nq * es, mesh->n, st)); };
-    // 1. row schedule and SpMM variant of the 64-column kernel (mixed mode)
-    if (mode == TMI_MODE_MIXED_BLOCK && cplx) {
+    // 1. row schedule and SpMM variant of the 64-column kernel (mixed mode;
+    // fewer than 64 right-hand sides never run a 64-column tile)
+    if (mode == TMI_MODE_MIXED_BLOCK && cplx && nq >= 64) {
         const int tile0 = tile;
+        const bool fb0 = fp64_fallback;
         tile = 64;
         zero_x();
-        solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
+        // the trial solve only fills the 64-column workspace the kernel
+        // timings read: without the FP64 fallback it keeps FP32 vectors
+        fp64_fallback = false;
+        try {
+            solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
+        } catch (...) {
+            fp64_fallback = fb0;
+            tile = tile0;
+            throw;
+        }
+        fp64_fallback = fb0;
         struct Cand { int variant; int rpw; };
         const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}};
         double best = 1e30;

@@ -77,6 +77,7 @@ struct Solver {
         Exec &operator=(const Exec &) = delete;
         ~Exec();
         size_t bytes() const;          // device bytes held by the workspace
+        void release_work();           // free every tile workspace (waits for the stream first)
     };
     Exec ex;                           // default context (library stream)
     std::mutex prep_mu;                // lazy preconditioner setup
@@ -87,7 +88,9 @@ struct Solver {
     int nval() const { return cplx ? 2 * mesh->nnz : mesh->nnz; }
     const Mesh::NodeOrder &ord() const { return mesh->order[order_kind]; }
     void allocate();
-    void assemble(const double *cmua, const double *ckap, const double *c2a, double omega, bool host_ptrs);
+    // elbasis: per-element coefficients (elen each, ASSEMBLE_*_EL) instead of nodal ones
+    void assemble(const double *cmua, const double *ckap, const double *c2a, double omega, bool host_ptrs,
+                  bool elbasis = false);
     void reset_precon();
     // Solve A X = B for nq columns.  d_b, d_x: [n][ld] node-major device
     // arrays (FP64, complex or real); d_x holds the initial guess.
@@ -98,6 +101,7 @@ struct Solver {
     size_t work_bytes(int nq) const;
     int tile_for(int nq, int md = -1) const;
     bool fp64_fallback = true;       // mixed mode: columns that do not converge are re-solved in FP64
+    int mixed_maxit = 0;             // mixed mode: iteration cap of the FP32 iteration (0: maxit)
     // Time `reps` launches of one kernel of the last tile (which: 0 = inner
     // SpMM, 1 = FP64 residual SpMM) with HIP events on the solver stream.
     // Returns the average ms and the algorithmic bytes of one launch.

@@ -91,11 +91,21 @@ class FwdSolver:
     def Allocate(self):
         L.call("tmi_solver_allocate", self._h)
 
-    def AssembleSystemMatrix(self, sol, omega=0.0):
-        L.call("tmi_solver_assemble", self._h, _f64(sol.cmua), _f64(sol.ckappa), _f64(sol.c2a), float(omega))
+    def _coeffs(self, sol, elbasis):
+        want = self.mesh.ElementCount() if elbasis else self.mesh.NodeCount()
+        c = [_f64(sol.cmua), _f64(sol.ckappa), _f64(sol.c2a)]
+        if any(len(a) != want for a in c):
+            raise ValueError("the solution holds %d values per parameter, the %s basis needs %d"
+                             % (len(c[0]), "element" if elbasis else "nodal", want))
+        return c
 
-    def Reset(self, sol, omega=0.0):
-        L.call("tmi_solver_reset", self._h, _f64(sol.cmua), _f64(sol.ckappa), _f64(sol.c2a), float(omega))
+    def AssembleSystemMatrix(self, sol, omega=0.0, elbasis=False):
+        """AssembleSystemMatrix(sol, omega, elbasis) (fwdsolver.cc:338-424):
+        nodal coefficients, or one per element with elbasis=True."""
+        L.call("tmi_solver_assemble_basis", self._h, *self._coeffs(sol, elbasis), float(omega), 1 if elbasis else 0)
+
+    def Reset(self, sol, omega=0.0, elbasis=False):
+        L.call("tmi_solver_reset_basis", self._h, *self._coeffs(sol, elbasis), float(omega), 1 if elbasis else 0)
 
     def nnz(self):
         v = C.c_int()
@@ -119,11 +129,15 @@ class FwdSolver:
 
         q = sparse.csr_matrix(qvec)
         nq, n = q.shape
+        if n != self.mesh.NodeCount():
+            raise ValueError("qvec has %d columns, the mesh %d nodes" % (n, self.mesh.NodeCount()))
         dt = np.complex128 if self.complex else np.float64
         vals = np.ascontiguousarray(q.data, dtype=dt)
         if phi is None:
             phi = np.zeros((nq, n), dt)
         phi = np.ascontiguousarray(phi, dtype=dt)
+        if phi.shape != (nq, n):
+            raise ValueError("phi has shape %s, expected (%d, %d) (sources x nodes)" % (phi.shape, nq, n))
         stats = (L.ColStat * nq)()
         st = L.call("tmi_solver_calc_fields", self._h, nq, np.ascontiguousarray(q.indptr, np.int32),
                     np.ascontiguousarray(q.indices, np.int32), vals.view(np.float64), phi.view(np.float64),
@@ -155,6 +169,12 @@ class FwdSolver:
         """narrowest tile of the column compaction (0: off, default 2)"""
         L.call("tmi_solver_set_compaction", self._h, int(min_width))
 
+    def SetFallback(self, enable=True, mixed_maxit=0):
+        """mixed mode: re-solve the columns the FP32 iteration leaves above the
+        tolerance with the FP64 block iteration (default on); mixed_maxit caps
+        the FP32 iteration (0: the solver's maxit)"""
+        L.call("tmi_solver_set_fallback", self._h, 1 if enable else 0, int(mixed_maxit))
+
     def WorkBytes(self, nq):
         """device bytes of the workspace a solve of nq columns uses"""
         b = C.c_double()
@@ -179,7 +199,12 @@ class FwdSolver:
 
         m = sparse.csr_matrix(mvec)
         dt = np.complex128 if self.complex else np.float64
-        nqm = self.mesh.qm_dims()[2]
+        nq, nm, nqm = self.mesh.qm_dims()
+        if m.shape != (nm, self.mesh.NodeCount()):
+            raise ValueError("mvec has shape %s, expected (%d, %d) (detectors x nodes)"
+                             % (m.shape, nm, self.mesh.NodeCount()))
+        if int(ld) < nq:
+            raise ValueError("field row stride %d below the source count %d" % (ld, nq))
         proj = np.zeros(nqm, dt)
         vals = np.ascontiguousarray(m.data, dtype=dt)
         L.call("tmi_solver_project_all_device", self._h, m.shape[0], np.ascontiguousarray(m.indptr, np.int32),
@@ -211,10 +236,15 @@ class FwdSolver:
 
         m = sparse.csr_matrix(mvec)
         dt = np.complex128 if self.complex else np.float64
-        nqm = self.mesh.qm_dims()[2]
+        nq, nm, nqm = self.mesh.qm_dims()
+        n = self.mesh.NodeCount()
+        if m.shape != (nm, n):
+            raise ValueError("mvec has shape %s, expected (%d, %d) (detectors x nodes)" % (m.shape, nm, n))
         proj = np.zeros(nqm, dt)
         vals = np.ascontiguousarray(m.data, dtype=dt)
         phi = np.ascontiguousarray(phi, dtype=dt)
+        if phi.shape != (nq, n):
+            raise ValueError("phi has shape %s, expected (%d, %d) (sources x nodes)" % (phi.shape, nq, n))
         L.call("tmi_solver_project_all", self._h, m.shape[0], np.ascontiguousarray(m.indptr, np.int32),
                np.ascontiguousarray(m.indices, np.int32), vals.view(np.float64), phi.view(np.float64),
                L.DATA_LOG if scale == "log" else L.DATA_LIN, proj.view(np.float64))
