@@ -252,6 +252,19 @@ int tmi_solver_project_all(tmi_solver s, int nm, const int *mrowptr, const int *
  * used), node-major as tmi_solver_calc_fields_device writes them. */
 int tmi_solver_project_all_device(tmi_solver s, int nm, const int *mrowptr, const int *mcolidx,
                                   const double *mval, const void *d_phi, int ld, int dscale, double *proj);
+/* No reference counterpart: tmi_solver_project_all_device without any device
+ * allocation, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as tmi_solver_project_all_device (bit-identical
+ * projections: the same kernel).  The link table, the mvec CSR arrays and the
+ * projections are staged in a fixed device array of
+ * TMI_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call
+ * needing more (about 4 nq nm + 4 (nm + nqm) + (4 + 8 cw) mvec_nnz + 8 cw nqm
+ * bytes, cw = 2 for complex) returns TMI_ERR_INVALID_ARG before touching the
+ * device.  Calls on one GPU share that array and are serialised by a lock. */
+#define TMI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tmi_solver_project_all_device_noalloc(tmi_solver s, int nm, const int *mrowptr, const int *mcolidx,
+                                          const double *mval, const void *d_phi, int ld, int dscale, double *proj);
 /* Last CalcFields: GPU time (ms, HIP events) and mean Krylov iterations per column */
 int tmi_solver_last_timing(tmi_solver s, double *solve_ms, double *iters_per_col);
 /* Measurement hook (no reference counterpart): time `reps` launches of one

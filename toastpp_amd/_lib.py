@@ -105,6 +105,7 @@ _SIGS = {
     "tmi_solver_calc_fields_device": [P, C.c_int, P, P, P],
     "tmi_solver_project_all": [P, C.c_int, I, I, D, D, C.c_int, D],
     "tmi_solver_project_all_device": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],
+    "tmi_solver_project_all_device_noalloc": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],
     "tmi_solver_last_timing": [P, DP, DP],
     "tmi_solver_time_kernel": [P, C.c_int, C.c_int, DP, DP],
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],

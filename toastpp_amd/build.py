@@ -21,7 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-I" + os.path
 # files whose FP64 arithmetic must follow the reference operation by operation
 NO_CONTRACT = {"mesh_kernels.hip", "mesh_io.cpp", "jacobian_kernels.hip", "raster.cpp", "raster_kernels.hip"}
 SOURCES = ["mesh_kernels.hip", "krylov_kernels.hip", "jacobian_kernels.hip", "raster_kernels.hip", "gmres.hip", "mesh_io.cpp",
-           "order.cpp", "solver.cpp", "jacobian_api.cpp", "serve.cpp", "data_io.cpp", "api.cpp"]
+           "order.cpp", "solver.cpp", "jacobian_api.cpp", "serve.cpp", "data_io.cpp", "api.cpp", "noalloc.hip"]
 
 
 def _compile(src):
