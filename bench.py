@@ -3,16 +3,22 @@
 
 Workload (default): 3D ellipsoid 80x65x90 mm, linear tetrahedra (250,047
 nodes, 1,429,968 Tet4), 100 MHz complex system with seed-drawn mua/mus' blob
-maps, 64 Gaussian boundary sources, BiCGSTAB with diagonal preconditioning,
-relative residual tolerance 1e-10.
+maps, 64 Gaussian boundary sources, diagonal preconditioning, relative
+residual tolerance 1e-10 (the reference criterion ||r|| <= tol ||q||).
 
-A "step" is one CalcFields over all 64 sources: complex-FP32 block BiCGSTAB
-(64 columns, one block) inside FP64 iterative refinement, right-hand sides and
-fields resident in HBM.  value = field solves per second (columns / s).
+A "step" is one CalcFields over all 64 sources: complex-FP32 block COCG
+(conjugate orthogonal CG for the complex symmetric system; 64 columns, one
+block) inside FP64 reliable updates, right-hand sides and fields resident in
+HBM.  value = field solves per second (columns / s).  --solver BICGSTAB
+times the reference's method instead; either way the other method is timed
+beside it (other_solver).
 
 Also measured in the same run:
-  - the plain FP64 single-right-hand-side GPU path (same solver family,
-    same tolerance) on a subset of the columns -> fp64 solves / s, speedup;
+  - the plain FP64 single-right-hand-side GPU path (the reference's method,
+    BiCGSTAB, same tolerance) on a subset of the columns -> fp64 solves / s,
+    speedup;
+  - the FP64 block path (the headline's method, all columns, same
+    tolerance) -> the gain of precision alone;
   - field error of every mixed-precision column against an FP64 GPU block
     solve at relative residual 1e-13 (max relative L2);
   - roofline of the dominant kernel (block SpMM) timed with HIP events on
@@ -44,7 +50,19 @@ SPMM_KERNELS = {
                 "sharing their neighbour-row gathers)",
     "lds": "k_spmm_lds<float2,0,...> (inner block SpMM, neighbour rows staged in LDS through registers)",
     "dma": "k_spmm_dma<...> (inner block SpMM, neighbour rows staged in LDS by the LDS DMA, double-buffered)",
+    "rowblock_pipelined": "k_spmm_rbp<0,%s> (inner block SpMM + %s dots, complex FP32, 64 columns, row blocks of 8 "
+                          "with packed masks, double-buffered gather batches)",
 }
+
+
+def spmm_kernel_name(variant, solver):
+    """Name of the inner SpMM kernel the solve launches: BiCGSTAB dots with
+    the packed sign-pattern shadow residual, COCG with the direction
+    (unconjugated)."""
+    name = SPMM_KERNELS[variant]
+    if variant == "rowblock_pipelined":
+        name = name % (("true", "<rd,Y>") if solver == "BICGSTAB" else ("false", "p^T Y"))
+    return name
 
 
 def measured_traffic(workload, tile, kernel):
@@ -78,6 +96,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ellipsoid3d")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--solver", default="COCG", choices=["BICGSTAB", "COCG"],
+                    help="inner Krylov method of the timed solves; the other one is timed beside it")
     ap.add_argument("--tile", type=int, default=64)
     ap.add_argument("--fp64-cols", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -86,6 +106,9 @@ def main():
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--jac-workload", default="cyl3d_jac")
     ap.add_argument("--jac-reps", type=int, default=3)
+    ap.add_argument("--jacobian-only", action="store_true",
+                    help="run only the Jacobian leg (fields, projections, nodal and raster J) and print its "
+                         "record; for kernel traces and counter passes of the Jacobian kernels")
     ap.add_argument("--raster", type=int, default=64, help="raster basis grid size per axis")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -111,6 +134,9 @@ def main():
 
     dev, hbm = L.device_info()
     log("device", dev, "%.0f GiB" % hbm)
+    if args.jacobian_only:
+        print(json.dumps({"jacobian": jacobian_leg(args)}), flush=True)
+        return
     nodes, el, Q, M, mua, mus, ref, freq, desc = G.workload(args.workload)
     n = len(nodes)
     log("workload", desc, "nodes", n, "elements", len(el))
@@ -124,8 +150,8 @@ def main():
     dB = L.DeviceArray.from_host(B)
     dX = L.DeviceArray(B.shape, np.complex128)
 
-    def make_solver(mode, tol, tile=None):
-        f = T.FwdSolver(mesh, "BICGSTAB", tol, complex=True)
+    def make_solver(mode, tol, tile=None, solver=None):
+        f = T.FwdSolver(mesh, solver or args.solver, tol, complex=True)
         f.SetMode(mode)
         f.SetPrecon("DIAG")
         if tile:
@@ -175,6 +201,7 @@ def main():
     L.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
+    launches, graph_launches, host_syncs = fm.LastCounts()   # of the last timed solve
     value = nq * args.steps / elapsed
     iters = [s[0] for s in stats]
     sweeps = [s[1] for s in stats]
@@ -186,7 +213,7 @@ def main():
     log("field error vs FP64 reference: max %.2e median %.2e" % (err.max(), np.median(err)))
 
     # dominant kernel: the block SpMM of the inner solve
-    spmm_kernel = SPMM_KERNELS[tuned["spmm_variant"] if tuned else "rowblock"]
+    spmm_kernel = spmm_kernel_name(tuned["spmm_variant"] if tuned else "rowblock", args.solver)
     spmm_ms, spmm_bytes = fm.TimeKernel(0, reps=20)
     res_ms, res_bytes = fm.TimeKernel(1, reps=10)
     achieved = spmm_bytes / (spmm_ms * 1e-3) / 1e9
@@ -197,7 +224,7 @@ def main():
     # plain FP64 single-RHS path on a subset of columns
     fp64 = None
     if not args.no_fp64 and args.fp64_cols > 0:
-        ff, _ = make_solver("fp64", args.tol)
+        ff, _ = make_solver("fp64", args.tol, solver="BICGSTAB")   # the reference's method, plainly
         k = min(args.fp64_cols, nq)
         Bs = L.DeviceArray.from_host(np.ascontiguousarray(B[:, :k]))
         Xs = L.DeviceArray((n, k), np.complex128)
@@ -209,11 +236,62 @@ def main():
         st64 = ff.CalcFieldsDevice(k, Bs.ptr, Xs.ptr)
         L.synchronize()
         t64 = time.perf_counter() - t0
+        c64 = ff.LastCounts()
         e64 = np.linalg.norm(Xs.to_host() - Xref[:, :k], axis=0) / np.linalg.norm(Xref[:, :k], axis=0)
-        fp64 = dict(solves_per_s=k / t64, columns=k, iters_per_col=float(np.mean([s[0] for s in st64])),
+        fp64 = dict(solver="bicgstab", solves_per_s=k / t64, columns=k, iters_per_col=float(np.mean([s[0] for s in st64])),
                     max_rel_err=float(e64.max()))
-        log("fp64 single-RHS: %.2f solves/s over %d columns, iters/col %.0f, err %.2e" %
-            (k / t64, k, fp64["iters_per_col"], e64.max()))
+        ms_it = t64 * 1e3 / max(1.0, sum(s[0] for s in st64))
+        fp64["ms_per_iteration"] = ms_it
+        its64 = max(1.0, sum(s[0] for s in st64))
+        fp64["launches_per_iteration"] = round(c64[0] / its64, 2)
+        fp64["graph_launches_per_iteration"] = round(c64[1] / its64, 3)
+        fp64["host_syncs_per_iteration"] = round(c64[2] / its64, 3)
+        log("fp64 single-RHS: %.2f solves/s over %d columns, iters/col %.0f, %.3f ms per iteration, err %.2e" %
+            (k / t64, k, fp64["iters_per_col"], ms_it, e64.max()))
+
+    # FP64 block Krylov over all columns at the same tolerance: the gain of
+    # batching alone, beside which the mixed mode shows the gain of precision
+    fp64b = None
+    if not args.no_fp64:
+        fb, _ = make_solver("fp64_block", args.tol, args.tile)
+        dXb = L.DeviceArray(B.shape, np.complex128)
+        fb.CalcFieldsDevice(nq, dB.ptr, dXb.ptr)    # warm
+        L.synchronize()
+        t0 = time.perf_counter()
+        stb = fb.CalcFieldsDevice(nq, dB.ptr, dXb.ptr)
+        L.synchronize()
+        tb = time.perf_counter() - t0
+        eb = np.linalg.norm(dXb.to_host() - Xref, axis=0) / np.linalg.norm(Xref, axis=0)
+        fp64b = dict(solver=args.solver.lower(), solves_per_s=nq / tb, columns=nq, iters_per_col=float(np.mean([s[0] for s in stb])),
+                     max_rel_err=float(eb.max()), converged=bool(all(s[3] for s in stb)))
+        log("fp64 block: %.1f solves/s over %d columns, iters/col %.0f, err %.2e" %
+            (nq / tb, nq, fp64b["iters_per_col"], eb.max()))
+        del fb, dXb
+
+    # the other complex Krylov method (BiCGSTAB / COCG), mixed mode, same
+    # tolerance and SpMM variant: solves/s, SpMMs per column, field error
+    other = None
+    if not args.no_fp64:
+        oname = "COCG" if args.solver == "BICGSTAB" else "BICGSTAB"
+        fo, _ = make_solver("mixed", args.tol, tuned["tile"] if tuned else args.tile, solver=oname)
+        if tuned:
+            fo.SetSpmmVariant(tuned["spmm_variant"])
+        dXo = L.DeviceArray(B.shape, np.complex128)
+        fo.CalcFieldsDevice(nq, dB.ptr, dXo.ptr)    # warm (captures the graphs)
+        L.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            sto = fo.CalcFieldsDevice(nq, dB.ptr, dXo.ptr)
+        L.synchronize()
+        to = (time.perf_counter() - t0) / args.steps
+        eo = np.linalg.norm(dXo.to_host() - Xref, axis=0) / np.linalg.norm(Xref, axis=0)
+        ito = float(np.mean([s[0] for s in sto]))
+        other = dict(solver=oname.lower(), solves_per_s=nq / to, ms_per_step=to * 1e3, iters_per_col=ito,
+                     spmm_per_col=ito * (2 if oname == "BICGSTAB" else 1), max_rel_err=float(eo.max()),
+                     converged=bool(all(s[3] for s in sto)))
+        log("mixed %s: %.1f solves/s, iters/col %.0f (%.0f SpMMs), err %.2e" %
+            (oname, nq / to, ito, other["spmm_per_col"], eo.max()))
+        del fo, dXo
 
     del fm
     jac = None
@@ -225,10 +303,25 @@ def main():
         batch = multi_mesh_leg(args)
         serve = serving_leg(args)
 
-    # CPU baseline: FP64 oracle, one source per host thread
-    cpu = None
+    # CPU legs on the FP64 oracle: the error of the mixed fields against the
+    # oracle's own solve, and the CPU baseline (one source per host thread)
+    cpu, oerr = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(nodes, el, Q, mua, mus, ref, omega, args.tol, args.cpu_seconds,
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        om = O.Mesh(nodes=nodes, elems=el)
+        cm, ck, c2 = O.coeffs(mua, mus, ref)
+        system = (om,) + tuple(om.sysmat(cm, ck, c2, omega))
+        log("cpu oracle setup (mesh, FP64 assembly) %.1f s" % (time.perf_counter() - t0))
+        order = np.argsort(iters, kind="stable")
+        cols = sorted({int(order[0]), int(order[1]), int(order[-2]), int(order[-1])})
+        t0 = time.perf_counter()
+        errs, orr = oracle_errors(system, B, X, cols)
+        oerr = {"columns": cols, "iters": [int(iters[c]) for c in cols], "rel_l2": errs, "oracle_tol": 1e-12,
+                "oracle_max_relres": float(orr)}
+        log("field error vs FP64 oracle (columns %s: fastest and slowest, oracle tol 1e-12, %.1f s): %s"
+            % (cols, time.perf_counter() - t0, " ".join("%.2e" % e for e in errs)))
+        cpu = cpu_baseline(system, Q, args.tol, args.cpu_seconds,
                            int(fp64["iters_per_col"]) if fp64 else int(iters[0]))
 
     out = {
@@ -246,12 +339,20 @@ def main():
         "data": "synthetic (generated mesh, seed-drawn parameter maps)",
         "config": {"workload": "%s: %s" % (args.workload, desc), "nodes": n, "elements": int(len(el)),
                    "sources": nq, "tile": tuned["tile"] if tuned else args.tile, "tol": args.tol, "precon": "diag",
-                   "solver": "bicgstab", "autotuned": tuned},
+                   "solver": args.solver.lower(), "autotuned": tuned},
         "max_rel_err_vs_fp64": float(err.max()),
+        "max_rel_err_vs_oracle": (max(oerr["rel_l2"]) if oerr else None),
+        "oracle_check": oerr,
         "iters_per_col": float(np.mean(iters)),
+        "spmm_per_col": float(np.mean(iters)) * (2 if args.solver == "BICGSTAB" else 1),
+        "per_solve": {"kernel_launches": int(launches), "graph_launches": int(graph_launches),
+                      "host_syncs": int(host_syncs)},
+        "other_solver": other,
         "converged": bool(conv),
         "fp64_single": fp64,
         "speedup_vs_fp64_single": (round(value / fp64["solves_per_s"], 3) if fp64 else None),
+        "fp64_block": fp64b,
+        "speedup_vs_fp64_block": (round(value / fp64b["solves_per_s"], 3) if fp64b else None),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload, args.tile, spmm_kernel),
@@ -292,7 +393,7 @@ def jacobian_leg(args):
                                         mesh.Mvec(width=2.0, ref=ref).toarray()]).astype(np.complex128))
     dB = L.DeviceArray.from_host(B)
     dX = L.DeviceArray(B.shape, np.complex128)
-    f = T.FwdSolver(mesh, "BICGSTAB", args.tol, complex=True)
+    f = T.FwdSolver(mesh, args.solver, args.tol, complex=True)
     f.SetMode("mixed")
     f.SetPrecon("DIAG")
     f.SetTile(64)
@@ -410,9 +511,9 @@ def jacobian_leg(args):
     return out
 
 
-def _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, tol):
+def _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, tol, solver="COCG"):
     """one device mesh, its source block (complex and real) and a prepared
-    solver per frequency (BiCGSTAB complex, PCG real for CW)"""
+    solver per frequency (`solver` for complex systems, PCG for real CW ones)"""
     mesh = T.Mesh(data=(nodes, el))
     mesh.SetQM(Q, Q[:1])
     B = mesh.Qvec(width=2.0).toarray()
@@ -421,7 +522,7 @@ def _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, tol):
     solvers = {}
     for freq in freqs:
         cplx = freq != 0
-        f = T.FwdSolver(mesh, "BICGSTAB" if cplx else "CG", tol, complex=cplx)
+        f = T.FwdSolver(mesh, solver if cplx else "CG", tol, complex=cplx)
         f.SetPrecon("DIAG")
         f.Allocate()
         f.Reset(T.Solution(mua, mus, ref), freq * 2 * np.pi * 1e-6)
@@ -442,7 +543,7 @@ def multi_mesh_leg(args):
 
     meshes = []
     for name, nodes, el, Q, mua, mus, ref in G.ellipsoid_batch():
-        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, [100.0], args.tol)
+        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, [100.0], args.tol, args.solver)
         f, dB = solvers[100.0]
         meshes.append((mesh, f, dB, name))
     outs = [L.DeviceArray((dB.shape[0], 64), np.complex128) for _, _, dB, _ in meshes]
@@ -491,7 +592,7 @@ def serving_leg(args):
     prepared, keep = {}, []
     t0 = time.perf_counter()
     for mi, (name, nodes, el, Q, mua, mus, ref) in enumerate(pool):
-        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, args.tol)
+        mesh, solvers = _prepare(T, L, nodes, el, Q, mua, mus, ref, freqs, args.tol, args.solver)
         keep.append(mesh)
         for f in freqs:
             prepared[(mi, f)] = solvers[f]
@@ -539,25 +640,39 @@ def serving_leg(args):
     return out
 
 
-def cpu_baseline(nodes, el, Q, mua, mus, ref, omega, tol, budget_s, gpu_iters):
+def host_info():
+    """CPU model and core counts of the host the bench runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "affinity_cores": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
+# host threads of the CPU legs: the cores this process may use, at most the
+# 16-thread CPU share one GPU box gives a process
+CPU_THREAD_CAP = 16
+
+
+def cpu_baseline(system, Q, tol, budget_s, gpu_iters):
     """FP64 oracle (plain C restatement of the reference path): BiCGSTAB +
     diagonal preconditioner, one source per host thread, as the reference's
     THREAD_LEVEL 2 runs CalcFields (fwdsolver.cc:619-643); the oracle's C
-    solver runs outside the GIL.  Threads: the host cores this process may
-    use, at most 16.  If the solves would exceed the time budget, they are
-    cut at the budget and the wall time per iteration is scaled to the
-    iteration count the FP64 solve needs."""
+    solver runs outside the GIL.  If the solves would exceed the time
+    budget, they are cut at the budget and the wall time per iteration is
+    scaled to the iteration count the FP64 solve needs."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as O
-    O.lib()
-    nthr = max(1, min(16, len(os.sched_getaffinity(0)), len(Q)))
-    t0 = time.perf_counter()
-    om = O.Mesh(nodes=nodes, elems=el)
-    cm, ck, c2 = O.coeffs(mua, mus, ref)
-    rp, ci, val = om.sysmat(cm, ck, c2, omega)
+    om, rp, ci, val = system
+    info = host_info()
+    nthr = max(1, min(CPU_THREAD_CAP, info["affinity_cores"], len(Q)))
     qs = om.qvec_gaussian(Q[:nthr], 2.0).astype(np.complex128)
-    tsetup = time.perf_counter() - t0
     t1 = time.perf_counter()
     _, it0, _ = O.bicgstab(rp, ci, val, qs[0], tol, 5, True)
     per_it = (time.perf_counter() - t1) / max(1, it0 + 1)
@@ -576,9 +691,25 @@ def cpu_baseline(nodes, el, Q, mua, mus, ref, omega, tol, budget_s, gpu_iters):
         t_col = wall / max(1, max(its) + 1) * gpu_iters / nthr
         sample = ("%d columns, one per thread, %d BiCGSTAB iterations each timed (%.1f s wall), scaled to "
                   "the %d iterations the solve needs" % (nthr, max(its) + 1, wall, gpu_iters))
-    log("cpu oracle: setup %.1f s, %s -> %.3f s per field solve" % (tsetup, sample, t_col))
+    log("cpu oracle: %s -> %.3f s per field solve (%d threads; host %s, %d cores usable, %d online)"
+        % (sample, t_col, nthr, info["cpu_model"], info["affinity_cores"], info["os_cpu_count"]))
     return {"value": round(1.0 / t_col, 5), "unit": "field solves/s", "cores": nthr, "kind": "port",
-            "sample": sample}
+            "sample": sample, "threads": nthr, "host_cores": info["affinity_cores"],
+            "host_cpu_count": info["os_cpu_count"], "cpu_model": info["cpu_model"]}
+
+
+def oracle_errors(system, B, X, cols, tol=1e-12):
+    """Relative L2 error of GPU field columns against the FP64 oracle's
+    BiCGSTAB (diagonal preconditioner) at tol, one column per host thread."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+    om, rp, ci, val = system
+    nthr = max(1, min(CPU_THREAD_CAP, len(os.sched_getaffinity(0)), len(cols)))
+    with ThreadPoolExecutor(nthr) as ex:
+        res = list(ex.map(lambda c: O.bicgstab(rp, ci, val, np.ascontiguousarray(B[:, c]), tol, 50000, True), cols))
+    errs = [float(np.linalg.norm(X[:, c] - r[0]) / np.linalg.norm(r[0])) for c, r in zip(cols, res)]
+    return errs, max(r[2] for r in res)
 
 
 if __name__ == "__main__":

@@ -42,6 +42,11 @@ extern "C" {
 #define TMI_LSOLVER_CG        1
 #define TMI_LSOLVER_BICGSTAB  2
 #define TMI_LSOLVER_GMRES     3   /* GMRES(20), FP64 in every mode (gmres_imp.hpp:41-230) */
+/* COCG: conjugate orthogonal CG for the complex symmetric system K = A + i w B
+   (PCG with the unconjugated bilinear form; one SpMM per iteration against
+   BiCGSTAB's two).  No reference counterpart: an additional iterative method
+   for complex systems, selected like the reference's ITMETHOD names. */
+#define TMI_LSOLVER_COCG      4
 
 /* preconditioners (PreconType, precon.h) */
 #define TMI_PRECON_NONE  0
@@ -267,6 +272,17 @@ int tmi_solver_project_all_device_noalloc(tmi_solver s, int nm, const int *mrowp
                                           const double *mval, const void *d_phi, int ld, int dscale, double *proj);
 /* Last CalcFields: GPU time (ms, HIP events) and mean Krylov iterations per column */
 int tmi_solver_last_timing(tmi_solver s, double *solve_ms, double *iters_per_col);
+/* Last CalcFields' host-side work (no reference counterpart): kernel launch
+ * calls, HIP graph launches and host synchronisations (any may be NULL). */
+int tmi_solver_last_counts(tmi_solver s, long long *kernel_launches, long long *graph_launches,
+                           long long *host_syncs);
+/* HIP graphs for the block Krylov iteration (no reference counterpart): with
+ * enable = 1 (default) each check period of the BiCGSTAB / PCG iteration --
+ * 8 iterations and the reliable update when it falls due, 64 to 104 kernel
+ * launches -- is captured once per workspace and operator and replayed as
+ * one graph launch; 0 launches the kernels one by one.  Same kernels, same
+ * arguments, same order: results are identical either way. */
+int tmi_solver_set_graphs(tmi_solver s, int enable);
 /* Measurement hook (no reference counterpart): time `reps` launches of one
  * kernel of the last CalcFields tile with HIP events on the library stream.
  * which = 0: the inner block SpMM, 1: the FP64 residual SpMM.  Returns the

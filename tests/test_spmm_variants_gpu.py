@@ -17,7 +17,7 @@ TMI = pytest.importorskip("toastpp_amd")
 from toastpp_amd import _lib as L  # noqa: E402
 from toastpp_amd import meshgen as G  # noqa: E402
 
-VARIANTS = ["lds", "direct", "dma", "rowblock"]
+VARIANTS = ["lds", "direct", "dma", "rowblock", "rowblock_pipelined"]
 
 
 @pytest.fixture(scope="module")
@@ -77,9 +77,12 @@ def test_variants_agree_on_cylinder():
         stats = f.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
         assert all(s[3] for s in stats), variant
         out[variant] = dX.to_host()
-    for variant in ("lds", "dma", "rowblock"):
+    for variant in ("lds", "dma", "rowblock", "rowblock_pipelined"):
         rel = np.linalg.norm(out[variant] - out["direct"], axis=0) / np.linalg.norm(out["direct"], axis=0)
         assert rel.max() <= 1e-8, "%s vs direct: worst column rel L2 %.3g" % (variant, rel.max())
+    # the pipelined row-block kernel sums every row in the row-block kernel's
+    # order: identical products, identical fields
+    assert np.array_equal(out["rowblock_pipelined"], out["rowblock"])
 
 
 @pytest.mark.parametrize("mode", ["mixed", "fp64_block"])

@@ -21,7 +21,7 @@ TMI_ERR_STATE = 6
 TMI_ERR_UNSUPPORTED = 7
 TMI_ERR_TIMEOUT = 8
 
-LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB, LSOLVER_GMRES = 0, 1, 2, 3
+LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB, LSOLVER_GMRES, LSOLVER_COCG = 0, 1, 2, 3, 4
 PRECON_NONE, PRECON_DIAG = 0, 1
 MODE_FP64_SINGLE, MODE_MIXED_BLOCK, MODE_FP64_BLOCK = 0, 1, 2
 DATA_LIN, DATA_LOG = 0, 1
@@ -107,6 +107,8 @@ _SIGS = {
     "tmi_solver_project_all_device": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],
     "tmi_solver_project_all_device_noalloc": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],
     "tmi_solver_last_timing": [P, DP, DP],
+    "tmi_solver_last_counts": [P, P, P, P],
+    "tmi_solver_set_graphs": [P, C.c_int],
     "tmi_solver_time_kernel": [P, C.c_int, C.c_int, DP, DP],
     "tmi_jacobian_nodal": [P, C.c_int, C.c_int, D, D, D, C.c_int, D],
     "tmi_jacobian_nodal_device": [P, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P],

@@ -13,6 +13,7 @@
 namespace tmi {
 
 static thread_local std::string g_last_error;
+thread_local long long g_launch_calls = 0;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 Context &Context::get()
@@ -422,7 +423,7 @@ int tmi_solver_create(tmi_mesh mh, int complex_valued, int lsolver, double tol, 
     API_BEGIN
     Mesh &m = M(mh);
     NEED(out);
-    TMI_CHECK(lsolver >= TMI_LSOLVER_DIRECT && lsolver <= TMI_LSOLVER_GMRES, TMI_ERR_INVALID_ARG,
+    TMI_CHECK(lsolver >= TMI_LSOLVER_DIRECT && lsolver <= TMI_LSOLVER_COCG, TMI_ERR_INVALID_ARG,
               "unknown linear solver %d", lsolver);
     tmi_solver h = new tmi_solver_s;
     Solver &s = h->s;
@@ -440,6 +441,10 @@ int tmi_solver_create(tmi_mesh mh, int complex_valued, int lsolver, double tol, 
     if (s.cplx && s.lsolver == TMI_LSOLVER_CG) {
         delete h;
         fail(TMI_ERR_UNSUPPORTED, "CG on a complex system is not supported (the reference's complex CG is a stub)");
+    }
+    if (!s.cplx && s.lsolver == TMI_LSOLVER_COCG) {
+        delete h;
+        fail(TMI_ERR_UNSUPPORTED, "COCG solves complex symmetric systems; use CG for a real one");
     }
     *out = h;
     API_END
@@ -470,9 +475,12 @@ int tmi_solver_set_linsolver(tmi_solver h, const char *name, double tol)
     if (!strcasecmp(name, "CG")) ls = TMI_LSOLVER_CG;
     else if (!strcasecmp(name, "BICGSTAB")) ls = TMI_LSOLVER_BICGSTAB;
     else if (!strcasecmp(name, "GMRES")) ls = TMI_LSOLVER_GMRES;
-    else fail(TMI_ERR_UNSUPPORTED, "linear solver '%s' is not supported (DIRECT, CG, BICGSTAB, GMRES)", name);
+    else if (!strcasecmp(name, "COCG")) ls = TMI_LSOLVER_COCG;
+    else fail(TMI_ERR_UNSUPPORTED, "linear solver '%s' is not supported (DIRECT, CG, BICGSTAB, GMRES, COCG)", name);
     TMI_CHECK(!(s.cplx && ls == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG is a stub)");
+    TMI_CHECK(!(!s.cplx && ls == TMI_LSOLVER_COCG), TMI_ERR_UNSUPPORTED,
+              "COCG solves complex symmetric systems; use CG for a real one");
     TMI_CHECK(tol > 0.0, TMI_ERR_INVALID_ARG, "tolerance must be positive");
     s.lsolver = ls;
     s.tol = tol;
@@ -540,9 +548,9 @@ int tmi_solver_set_rows_per_wave(tmi_solver h, int rows)
 int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
 {
     API_BEGIN
-    TMI_CHECK(variant >= 0 && variant <= 3, TMI_ERR_INVALID_ARG,
-              "SpMM variant must be 0 (LDS, register staging), 1 (direct), 2 (LDS DMA, double-buffered) or 3 "
-              "(row blocks with shared gathers)");
+    TMI_CHECK(variant >= 0 && variant <= 4, TMI_ERR_INVALID_ARG,
+              "SpMM variant must be 0 (LDS, register staging), 1 (direct), 2 (LDS DMA, double-buffered), 3 "
+              "(row blocks with shared gathers) or 4 (row blocks, software-pipelined)");
     S(h).spmm_variant = variant;
     API_END
 }
@@ -748,16 +756,24 @@ static void project_all(Solver &s, int nm, const int *mrp, const int *mci, const
     std::vector<int> lk((size_t)nm * qm.nq, -1);
     for (int q = 0; q < qm.nq; q++)
         for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) lk[(size_t)qm.lidx[k] * qm.nq + q] = k;
+    TMI_CHECK(mrp[0] == 0, TMI_ERR_INVALID_ARG, "mvec row pointer must start at 0");
+    int maxrow = 0;
+    for (int i = 0; i < nm; i++) {
+        TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
+        maxrow = std::max(maxrow, mrp[i + 1] - mrp[i]);
+    }
     int mnnz = mrp[nm];
     for (int k = 0; k < mnnz; k++) TMI_CHECK(mci[k] >= 0 && mci[k] < n, TMI_ERR_INVALID_ARG, "mvec column out of range");
-    DBuf<int> dlk, drp, dci;
-    DBuf<double> dval, dproj((size_t)nqm * cw);
-    dlk.upload(lk.data(), lk.size(), s.st);
-    drp.upload(mrp, nm + 1, s.st);
-    dci.upload(mci, mnnz, s.st);
-    dval.upload(mval, (size_t)mnnz * cw, s.st);
-    launch_project(qm.nq, nm, dlk.p, drp.p, dci.p, dval.p, d_phi, sq, sj, s.cplx ? 1 : 0, dproj.p, s.st);
-    dproj.download(proj, (size_t)nqm * cw, s.st);
+    const ProjectPlan pl = project_plan(qm.nq, nm, maxrow, s.cplx ? 1 : 0);
+    s.pj_lk.upload(lk.data(), lk.size(), s.st);
+    s.pj_rp.upload(mrp, nm + 1, s.st);
+    s.pj_ci.upload(mci, mnnz, s.st);
+    s.pj_val.upload(mval, (size_t)mnnz * cw, s.st);
+    if (s.pj_part.n < pl.partial_doubles) s.pj_part.alloc(pl.partial_doubles);
+    if (s.pj_out.n < (size_t)nqm * cw) s.pj_out.alloc((size_t)nqm * cw);
+    launch_project(pl, qm.nq, nm, s.pj_lk.p, s.pj_rp.p, s.pj_ci.p, s.pj_val.p, d_phi, sq, sj, s.cplx ? 1 : 0,
+                   s.pj_part.p, s.pj_out.p, s.st);
+    s.pj_out.download(proj, (size_t)nqm * cw, s.st);
     TMI_HIP(hipStreamSynchronize(s.st));
     if (dscale == TMI_DATA_LOG) {   // complex log: Re = ln|y|, Im = arg y
         for (int k = 0; k < nqm; k++) {
@@ -811,6 +827,26 @@ int tmi_solver_last_timing(tmi_solver h, double *solve_ms, double *iters_per_col
     Solver &s = S(h);
     if (solve_ms) *solve_ms = s.ex.last_solve_ms;
     if (iters_per_col) *iters_per_col = s.ex.last_iters;
+    API_END
+}
+
+int tmi_solver_last_counts(tmi_solver h, long long *kernel_launches, long long *graph_launches,
+                           long long *host_syncs)
+{
+    API_BEGIN
+    Solver &s = S(h);
+    if (kernel_launches) *kernel_launches = s.ex.last_launch;
+    if (graph_launches) *graph_launches = s.ex.last_graph;
+    if (host_syncs) *host_syncs = s.ex.last_sync;
+    API_END
+}
+
+int tmi_solver_set_graphs(tmi_solver h, int enable)
+{
+    API_BEGIN
+    Solver &s = S(h);
+    s.use_graphs = enable != 0;
+    if (!s.use_graphs) s.ex.drop_graphs();
     API_END
 }
 

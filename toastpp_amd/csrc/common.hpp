@@ -48,8 +48,15 @@ struct Error : std::runtime_error {
         if (!(cond)) ::tmi::fail(code, __VA_ARGS__); \
     } while (0)
 
-// Check the launch status of the last kernel.
-#define TMI_LAUNCH_CHECK() TMI_HIP(hipGetLastError())
+// kernel launch calls made by this thread (the solver reports them per solve)
+extern thread_local long long g_launch_calls;
+
+// Check the launch status of the last kernel (and count the launch).
+#define TMI_LAUNCH_CHECK()                 \
+    do {                                   \
+        ++::tmi::g_launch_calls;           \
+        TMI_HIP(hipGetLastError());        \
+    } while (0)
 
 void set_last_error(const std::string &msg);
 

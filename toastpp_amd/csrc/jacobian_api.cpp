@@ -85,7 +85,8 @@ static void jacobian_nodal_dev(Mesh &m, int nq, int nm, const double *d_dphi, in
     TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
     const int dmax = mesh_dmax(m, st);
     ensure_links(m, st);
-    const size_t ws = (size_t)m.n * qm.nqm * 4;
+    // the node-major intermediate of one pass of the two-pass variant
+    const size_t ws = (size_t)jacobian_chunk_nodes(m.n, qm.nqm) * qm.nqm * 4;
     if (m.jac_ws.n < ws) m.jac_ws.alloc(ws);
     launch_jacobian_nodal(m, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lq.p, m.d_lm.p, qm.nqm, d_proj,
                           dscale == TMI_DATA_LOG ? 1 : 0, m.jac_ws.p, d_J, dmax, st, cw, d_Jkap);
@@ -509,7 +510,11 @@ static void project_from_mvec(Mesh &m, int nm, const int *mrp, const int *mci, c
     const QM &qm = m.qm;
     TMI_CHECK(nm == qm.nm, TMI_ERR_INVALID_ARG, "mvec has %d rows, the QM description %d detectors", nm, qm.nm);
     TMI_CHECK(mrp[0] == 0, TMI_ERR_INVALID_ARG, "mvec row pointer must start at 0");
-    for (int i = 0; i < nm; i++) TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
+    int maxrow = 0;
+    for (int i = 0; i < nm; i++) {
+        TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
+        maxrow = std::max(maxrow, mrp[i + 1] - mrp[i]);
+    }
     const int mnnz = mrp[nm];
     for (int k = 0; k < mnnz; k++)
         TMI_CHECK(mci[k] >= 0 && mci[k] < m.n, TMI_ERR_INVALID_ARG, "mvec column out of range");
@@ -517,14 +522,15 @@ static void project_from_mvec(Mesh &m, int nm, const int *mrp, const int *mci, c
     for (int q = 0; q < qm.nq; q++)
         for (int k = qm.lptr[q]; k < qm.lptr[q + 1]; k++) lk[(size_t)qm.lidx[k] * qm.nq + q] = k;
     const int cw = cplx ? 2 : 1;
+    const ProjectPlan pl = project_plan(qm.nq, nm, maxrow, cplx);
     DBuf<int> dlk, drp, dci;
-    DBuf<double> dval;
+    DBuf<double> dval, dpart(pl.partial_doubles);
     dlk.upload(lk.data(), lk.size(), st);
     drp.upload(mrp, nm + 1, st);
     dci.upload(mci, mnnz, st);
     dval.upload(mval, (size_t)mnnz * cw, st);
     d_proj.alloc((size_t)qm.nqm * cw);
-    launch_project(qm.nq, nm, dlk.p, drp.p, dci.p, dval.p, d_phi, 1, qm.nq, cplx, d_proj.p, st);
+    launch_project(pl, qm.nq, nm, dlk.p, drp.p, dci.p, dval.p, d_phi, 1, qm.nq, cplx, dpart.p, d_proj.p, st);
     TMI_HIP(hipStreamSynchronize(st));   // the uploads above go out of scope
 }
 

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "fe_integrals.hpp"
 #include "jacobian.hpp"
@@ -33,7 +35,16 @@ constexpr int JAC_NT = 256;
 #ifndef TMI_JAC_LDS_KIB
 #define TMI_JAC_LDS_KIB 40
 #endif
-constexpr int JAC_LDS_KIB = TMI_JAC_LDS_KIB;
+static int jac_lds_kib()
+{
+    static const int v = getenv("TMI_JAC_LDS_KIB") ? atoi(getenv("TMI_JAC_LDS_KIB")) : TMI_JAC_LDS_KIB;
+    return v;
+}
+static bool jac_block()
+{
+    static const bool v = getenv("TMI_JAC_BLOCK") ? atoi(getenv("TMI_JAC_BLOCK")) != 0 : true;
+    return v;
+}
 
 // J and the node-major intermediate are written once and read at most once:
 // streaming (nontemporal) accesses keep them out of the caches
@@ -50,6 +61,8 @@ __device__ inline double2 ld_stream(const double2 *p)
 
 struct JacDims {
     int n, nq, nm, nqm, dmax, chunk, ldd, lda;
+    int node0;    // two-pass: first node of this launch (the intermediate holds its nodes only)
+    int cached;   // two-pass: plain stores (the intermediate is read back from the caches), else streaming
 };
 
 __device__ inline int find_pos(const int *cols, int d, int v)
@@ -75,21 +88,68 @@ static size_t jac_lds_doubles(int dmax, int chunk)
     return (size_t)2 * dmax * dmax + jac_region_doubles(dmax, chunk) + (dmax + 1) / 2 + 1;
 }
 
-template <int DIM>
+// Output of the direct variant: J (complex: 2nqm x 2n, [Re; Im] x [mua |
+// kappa]) or the CW pair Jmua / Jkap (nqm x n each, either may be absent),
+// scaled by 1/proj (log data) and negated, as k_jac_transpose /
+// k_jac_transpose_cw write them from the intermediate.
+struct JacOut {
+    double *J, *Jkap;
+    const double *proj;
+    int logscale, cw;
+};
+
+__device__ inline void jac_emit(const JacOut &o, int n, int nqm, int i, int k, double mr, double mi, double pr,
+                                double pi)
+{
+    if (o.cw) {
+        double a = -mr, b = -pr;
+        if (o.logscale) {
+            const double ip = 1.0 / o.proj[k];
+            a *= ip;
+            b *= ip;
+        }
+        if (o.J) o.J[(size_t)k * n + i] = a;
+        if (o.Jkap) o.Jkap[(size_t)k * n + i] = b;
+        return;
+    }
+    double2 a = make_double2(mr, mi), b = make_double2(pr, pi);
+    if (o.logscale) {
+        const double2 p = ((const double2 *)o.proj)[k];
+        const double den = p.x * p.x + p.y * p.y;
+        a = make_double2((a.x * p.x + a.y * p.y) / den, (a.y * p.x - a.x * p.y) / den);
+        b = make_double2((b.x * p.x + b.y * p.y) / den, (b.y * p.x - b.x * p.y) / den);
+    }
+    const size_t ld = 2 * (size_t)n;
+    o.J[(size_t)k * ld + i] = -a.x;
+    o.J[(size_t)(k + nqm) * ld + i] = -a.y;
+    o.J[(size_t)k * ld + n + i] = -b.x;
+    o.J[(size_t)(k + nqm) * ld + n + i] = -b.y;
+}
+
+// node of workgroup b such that each XCD (workgroups b = x mod 8) walks a
+// contiguous run of nodes: the J entries of neighbouring nodes share cache
+// lines, which then fill in that XCD's L2 before they are written back
+__device__ inline int xcd_node(int b, int n)
+{
+    const int x = b & 7, s = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + s;
+}
+
+template <int DIM, bool DIRECT, bool BLK>
 __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__restrict__ rowptr,
                                                   const int *__restrict__ colidx, const int *__restrict__ n2e_ptr,
                                                   const int *__restrict__ n2e, const int *__restrict__ el,
                                                   const double *__restrict__ geo,
                                                   const double2 *__restrict__ dphi,   // [n][ldd]
                                                   const double2 *__restrict__ aphi,   // [n][lda]
-                                                  const int *__restrict__ lptr,       // [nq+1]
-                                                  const int *__restrict__ lq, const int *__restrict__ lm,
-                                                  double2 *__restrict__ Pt)           // [n][nqm][2]
+                                                  const int *__restrict__ lk,         // [nq][nm] link or -1
+                                                  double2 *__restrict__ Pt,           // [n][nqm][2]
+                                                  JacOut jo)
 {
     constexpr int NN = DIM + 1;
     constexpr int GS = DIM == 3 ? GEO3 : GEO2;
     extern __shared__ double lds[];
-    const int i = blockIdx.x;
+    const int i = DIRECT ? xcd_node(blockIdx.x, jd.n) : jd.node0 + blockIdx.x;
     const int tid = threadIdx.x;
     const int dmax = jd.dmax, nq = jd.nq, nm = jd.nm, C = jd.chunk;
     double *A = lds, *B = A + dmax * dmax;
@@ -140,7 +200,7 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
             __syncthreads();
         }
     }
-    double2 *out = Pt + (size_t)i * jd.nqm * 2;
+    double2 *out = Pt + (size_t)(i - jd.node0) * jd.nqm * 2;
     for (int m0 = 0; m0 < nm; m0 += C) {
         const int mc = min(C, nm - m0);
         // detector chunk G on the neighbour set
@@ -175,22 +235,76 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
             __syncthreads();
             {
 #pragma clang fp contract(fast)
-                // P(q,m) = F_q . H_m for the linked pairs of this chunk pair
-                for (int k = lptr[q0] + tid; k < lptr[q0 + qc]; k += JAC_NT) {
-                    const int mrel = lm[k] - m0;
-                    if (mrel < 0 || mrel >= mc) continue;
-                    const double2 *f = W + (lq[k] - q0) * dmax;
-                    const double2 *h = H + mrel * dmax, *kk = Kk + mrel * dmax;
-                    double mr = 0.0, mi = 0.0, pr = 0.0, pi = 0.0;
-                    for (int j = 0; j < d; j++) {
-                        const double2 fv = f[j], hv = h[j], kv = kk[j];
-                        mr += fv.x * hv.x - fv.y * hv.y;
-                        mi += fv.x * hv.y + fv.y * hv.x;
-                        pr += fv.x * kv.x - fv.y * kv.y;
-                        pi += fv.x * kv.y + fv.y * kv.x;
+                // P(q,m) = F_q . H_m over 2 x 2 blocks (q1, q2) x (m1, m2) of
+                // the chunk pair: each LDS read of F, H or K serves two
+                // links; a block with no link is skipped, unlinked pairs of a
+                // block are computed and not stored (lk[q nm + m] = -1), and
+                // a missing second row or column repeats the first
+                const int bq = (qc + 1) / 2, bm = (mc + 1) / 2;
+                if constexpr (!BLK) {
+                    // one thread per linked pair of the chunk pair
+                    for (int b = tid; b < qc * mc; b += JAC_NT) {
+                        const int r = b / mc, cc = b % mc;
+                        const int k = lk[(size_t)(q0 + r) * nm + m0 + cc];
+                        if (k < 0) continue;
+                        const double2 *f = W + r * dmax, *h = H + cc * dmax, *kk = Kk + cc * dmax;
+                        double mr = 0.0, mi = 0.0, pr = 0.0, pi = 0.0;
+                        for (int j = 0; j < d; j++) {
+                            const double2 fv = f[j], hv = h[j], kv = kk[j];
+                            mr += fv.x * hv.x - fv.y * hv.y;
+                            mi += fv.x * hv.y + fv.y * hv.x;
+                            pr += fv.x * kv.x - fv.y * kv.y;
+                            pi += fv.x * kv.y + fv.y * kv.x;
+                        }
+                        if constexpr (DIRECT) {
+                            jac_emit(jo, jd.n, jd.nqm, i, k, mr, mi, pr, pi);
+                        } else if (jd.cached) {
+                            out[2 * k] = make_double2(mr, mi);
+                            out[2 * k + 1] = make_double2(pr, pi);
+                        } else {
+                            st_stream(out + 2 * k, make_double2(mr, mi));
+                            st_stream(out + 2 * k + 1, make_double2(pr, pi));
+                        }
                     }
-                    st_stream(out + 2 * k, make_double2(mr, mi));
-                    st_stream(out + 2 * k + 1, make_double2(pr, pi));
+                } else
+                for (int b = tid; b < bq * bm; b += JAC_NT) {
+                    const int r1 = 2 * (b / bm), c1 = 2 * (b % bm);
+                    const int r2 = min(r1 + 1, qc - 1), c2 = min(c1 + 1, mc - 1);
+                    const int *lkq1 = lk + (size_t)(q0 + r1) * nm + m0, *lkq2 = lk + (size_t)(q0 + r2) * nm + m0;
+                    const int k11 = lkq1[c1], k12 = c2 > c1 ? lkq1[c2] : -1;
+                    const int k21 = r2 > r1 ? lkq2[c1] : -1, k22 = (r2 > r1 && c2 > c1) ? lkq2[c2] : -1;
+                    if ((k11 & k12 & k21 & k22) < 0) continue;   // all four negative: nothing linked
+                    const double2 *f1 = W + r1 * dmax, *f2 = W + r2 * dmax;
+                    const double2 *h1 = H + c1 * dmax, *h2 = H + c2 * dmax;
+                    const double2 *g1 = Kk + c1 * dmax, *g2 = Kk + c2 * dmax;
+                    double a[4][4] = {};   // [pair 11, 12, 21, 22][mr, mi, pr, pi]
+                    for (int j = 0; j < d; j++) {
+                        const double2 fa = f1[j], fb = f2[j], ha = h1[j], hb = h2[j], ka = g1[j], kb = g2[j];
+                        const double2 *fv[2] = {&fa, &fb}, *hv[2] = {&ha, &hb}, *kv[2] = {&ka, &kb};
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const double2 F = *fv[u >> 1], Hh = *hv[u & 1], Kv = *kv[u & 1];
+                            a[u][0] += F.x * Hh.x - F.y * Hh.y;
+                            a[u][1] += F.x * Hh.y + F.y * Hh.x;
+                            a[u][2] += F.x * Kv.x - F.y * Kv.y;
+                            a[u][3] += F.x * Kv.y + F.y * Kv.x;
+                        }
+                    }
+                    const int ks[4] = {k11, k12, k21, k22};
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int k = ks[u];
+                        if (k < 0) continue;
+                        if constexpr (DIRECT) {
+                            jac_emit(jo, jd.n, jd.nqm, i, k, a[u][0], a[u][1], a[u][2], a[u][3]);
+                        } else if (jd.cached) {
+                            out[2 * k] = make_double2(a[u][0], a[u][1]);
+                            out[2 * k + 1] = make_double2(a[u][2], a[u][3]);
+                        } else {
+                            st_stream(out + 2 * k, make_double2(a[u][0], a[u][1]));
+                            st_stream(out + 2 * k + 1, make_double2(a[u][2], a[u][3]));
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -205,18 +319,19 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
 constexpr int TJK = 16;
 __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const double2 *__restrict__ Pt,
                                                        const double2 *__restrict__ proj, int logscale,
-                                                       double *__restrict__ J)
+                                                       double *__restrict__ J, int node0, int nend, int cached)
 {
     __shared__ double2 tm[64][TJK + 1], tk[64][TJK + 1];
-    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int i0 = node0 + blockIdx.x * 64, k0 = blockIdx.y * TJK;
     const int t = threadIdx.x;
-    // read: node r, measurement c (2 values each)
+    // read: node r, measurement c (2 values each); nodes node0 .. nend-1
     for (int e = t; e < 64 * TJK; e += 256) {
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
-        if (i < n && k < nqm) {
-            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2);
-            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1);
+        if (i < nend && k < nqm) {
+            const double2 *src = Pt + ((size_t)(i - node0) * nqm + k) * 2;
+            tm[r][c] = cached ? src[0] : ld_stream(src);
+            tk[r][c] = cached ? src[1] : ld_stream(src + 1);
         }
     }
     __syncthreads();
@@ -224,7 +339,7 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
     for (int e = t; e < 64 * TJK; e += 256) {
         const int c = e / 64, r = e % 64;
         const int k = k0 + c, i = i0 + r;
-        if (k < nqm && i < n) {
+        if (k < nqm && i < nend) {
             double2 a = tm[r][c], b = tk[r][c];
             if (logscale) {
                 const double2 p = proj[k];
@@ -246,24 +361,26 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
 // factor is dropped); either output may be absent.  Same tiling as above.
 __global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const double2 *__restrict__ Pt,
                                                           const double *__restrict__ proj, int logscale,
-                                                          double *__restrict__ Jmua, double *__restrict__ Jkap)
+                                                          double *__restrict__ Jmua, double *__restrict__ Jkap,
+                                                          int node0, int nend, int cached)
 {
     __shared__ double tm[64][TJK + 1], tk[64][TJK + 1];
-    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int i0 = node0 + blockIdx.x * 64, k0 = blockIdx.y * TJK;
     const int t = threadIdx.x;
     for (int e = t; e < 64 * TJK; e += 256) {
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
-        if (i < n && k < nqm) {
-            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2).x;
-            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1).x;
+        if (i < nend && k < nqm) {
+            const double2 *src = Pt + ((size_t)(i - node0) * nqm + k) * 2;
+            tm[r][c] = (cached ? src[0] : ld_stream(src)).x;
+            tk[r][c] = (cached ? src[1] : ld_stream(src + 1)).x;
         }
     }
     __syncthreads();
     for (int e = t; e < 64 * TJK; e += 256) {
         const int c = e / 64, r = e % 64;
         const int k = k0 + c, i = i0 + r;
-        if (k < nqm && i < n) {
+        if (k < nqm && i < nend) {
             double a = -tm[r][c], b = -tk[r][c];
             if (logscale) {
                 const double ip = 1.0 / proj[k];
@@ -412,10 +529,30 @@ void launch_real_to_cplx(long rows, int cols, const double *src, int ld, double 
     TMI_LAUNCH_CHECK();
 }
 
+// How the nodal Jacobian kernel's output reaches J (TMI_JAC_MODE, read
+// once): "direct" -- the node kernel writes J itself (scattered 8-byte
+// stores); "two" -- a node-major intermediate of all nodes (n nqm 32 B,
+// streaming stores) and one tiled transpose; "chunk:<nodes>" or the default
+// -- the two passes alternate over node chunks whose intermediate stays in
+// the device caches (plain stores and loads), the default chunk holding at
+// most 128 MiB of intermediate.  Returns the nodes per pass (0: direct).
+int jacobian_chunk_nodes(int n, int nqm)
+{
+    static const char *mode = getenv("TMI_JAC_MODE");
+    if (mode && !strcmp(mode, "direct")) return 0;
+    if (mode && !strcmp(mode, "two")) return n;
+    long c = 0;
+    if (mode && !strncmp(mode, "chunk:", 6)) c = atol(mode + 6);
+    if (c <= 0) c = (128L << 20) / std::max<long>(1, (long)nqm * 32);
+    c = std::max<long>(64, c / 64 * 64);
+    return (int)std::min<long>(c, n);
+}
+
 void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
                            const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
                            int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw, double *d_Jkap)
 {
+    (void)d_lptr; (void)d_lq; (void)d_lm;   // the kernel reads the link table m.d_lk
     // LDS row stride: the neighbour count rounded up to odd, so that the
     // rows of A, B, W, H read by consecutive lanes start in different banks
     dmax |= 1;
@@ -424,30 +561,47 @@ void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, 
     // (at least 8 when the LDS of one CU allows)
     const int want = std::max(nq, nm);
     int chunk = want;
-    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > JAC_LDS_KIB * 1024) chunk--;
+    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > (size_t)jac_lds_kib() * 1024) chunk--;
     if (chunk < std::min(want, 8))
         while (chunk < want && chunk < 8 && jac_lds_doubles(dmax, chunk + 1) * sizeof(double) <= 160 * 1024) chunk++;
     const size_t lds = jac_lds_doubles(dmax, chunk) * sizeof(double);
     TMI_CHECK(lds <= 160 * 1024, TMI_ERR_UNSUPPORTED,
               "nodal Jacobian: %d neighbours per node exceed the LDS of one CU", dmax);
-    JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda};
-    if (m.dim == 3)
-        hipLaunchKernelGGL(k_jac_nodes<3>, dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
-                           m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
-                           (double2 *)d_Pt);
-    else
-        hipLaunchKernelGGL(k_jac_nodes<2>, dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p, m.n2e_ptr.p,
-                           m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi, d_lptr, d_lq, d_lm,
-                           (double2 *)d_Pt);
-    TMI_LAUNCH_CHECK();
-    dim3 g((m.n + 63) / 64, (nqm + TJK - 1) / TJK);
-    if (cw)
-        hipLaunchKernelGGL(k_jac_transpose_cw, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt, d_proj, logscale,
-                           d_J, d_Jkap);
-    else
-        hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
-                           (const double2 *)d_proj, logscale, d_J);
-    TMI_LAUNCH_CHECK();
+    const JacOut jo{d_J, d_Jkap, d_proj, logscale, cw ? 1 : 0};
+    const int cn = jacobian_chunk_nodes(m.n, nqm);
+    if (cn == 0) {
+        const JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda, 0, 0};
+#define JN(D, BL)                                                                                                    \
+    hipLaunchKernelGGL((k_jac_nodes<D, true, BL>), dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p,      \
+                       m.n2e_ptr.p, m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi,     \
+                       m.d_lk.p, (double2 *)nullptr, jo);
+        if (m.dim == 3) { if (jac_block()) JN(3, true) else JN(3, false) }
+        else { if (jac_block()) JN(2, true) else JN(2, false) }
+#undef JN
+        TMI_LAUNCH_CHECK();
+        return;
+    }
+    const int cached = cn < m.n ? 1 : 0;
+    for (int n0 = 0; n0 < m.n; n0 += cn) {
+        const int cnt = std::min(cn, m.n - n0);
+        const JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda, n0, cached};
+#define JN(D, BL)                                                                                                    \
+    hipLaunchKernelGGL((k_jac_nodes<D, false, BL>), dim3(cnt), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p,     \
+                       m.n2e_ptr.p, m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi,     \
+                       m.d_lk.p, (double2 *)d_Pt, jo);
+        if (m.dim == 3) { if (jac_block()) JN(3, true) else JN(3, false) }
+        else { if (jac_block()) JN(2, true) else JN(2, false) }
+#undef JN
+        TMI_LAUNCH_CHECK();
+        dim3 g((cnt + 63) / 64, (nqm + TJK - 1) / TJK);
+        if (cw)
+            hipLaunchKernelGGL(k_jac_transpose_cw, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt, d_proj,
+                               logscale, d_J, d_Jkap, n0, n0 + cnt, cached);
+        else
+            hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
+                               (const double2 *)d_proj, logscale, d_J, n0, n0 + cnt, cached);
+        TMI_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace tmi

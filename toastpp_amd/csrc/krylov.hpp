@@ -13,14 +13,17 @@
 namespace tmi {
 
 // offsets into the per-column double scalar array (each block of K doubles)
+// (the _I slots hold the imaginary parts of COCG's complex scalars)
 enum ScalarSlot {
-    S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BNORM2, S_RES2, S_TOL2, S_SCALE, S_AUX, S_RHOP, S_BEST, S_NSLOT
+    S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BNORM2, S_RES2, S_TOL2, S_SCALE, S_AUX, S_RHOP, S_BEST,
+    S_RHO_I, S_ALPHA_I, S_BETA_I, S_RHOP_I, S_NSLOT
 };
 // offsets into the per-column int array
 enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_NSLOT };
 
-// partial-sum slots written by the reduction epilogues
-constexpr int RED_SLOTS = 2;
+// partial-sum slots written by the reduction epilogues (COCG's update
+// kernel writes three: the complex <r, D^-1 r> and ||r||^2)
+constexpr int RED_SLOTS = 3;
 
 // Row blocks of the LDS-staged 64-column SpMM: consecutive rows whose
 // neighbour rows (the X rows their nonzeros read) plus the rows themselves
@@ -62,6 +65,7 @@ struct RowUnions {
     const rb_mask_t *mask = nullptr;     // rows of the block holding each column
     const void *val = nullptr;           // coefficients in (column, row) order per block
     const void *src = nullptr;           // the operator (CSR values) they were taken from
+    const unsigned *word = nullptr;      // packed mask | offset words: the pipelined kernel (variant 4)
 };
 
 struct BlockDesc {
@@ -81,12 +85,15 @@ inline size_t vbytes(VKind k) { return k == VK_C64 ? 16 : (k == VK_R32 ? 4 : 8);
 
 // ---- launchers (krylov_kernels.hip) ----
 // Y = A * D^-1 X (D^-1 = idiag, or identity if null); partials
-// red[0] = <W, Y>_R and, when want_yy, red[1] = <Y, Y>_R, per column.
+// red[0] = <W, Y>_R and, when want_yy = 1, red[1] = <Y, Y>_R, per column;
+// want_yy = DOT_BIL (complex vectors): red[0], red[1] = Re, Im of the
+// unconjugated W^T Y (COCG's p^T A p).
 // Wq (optional): W given as the packed shadow residual of a 64-column
 // complex FP32 tile (launch_pack_shadow); used by the row-block kernel,
 // the other kernels read W, which must hold the same values.
+constexpr int DOT_BIL = 2;
 void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
-                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st,
+                     const void *W, int want_yy, const int *iflags, double *partial, hipStream_t st,
                      const unsigned char *Wq = nullptr);
 // R = B - A X (FP64), red[0] = ||R||^2
 void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
@@ -103,14 +110,16 @@ void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, co
 // packed copy (4 bits per entry) of the sign-pattern shadow residual rd of
 // a 64-column complex FP32 tile: q is n x 32 bytes
 void launch_pack_shadow(int n, const void *rd, unsigned char *q, hipStream_t st);
-// PCG element-wise steps
+// PCG element-wise steps.  bil (complex vectors only): COCG -- complex
+// alpha / beta, and the partials are the unconjugated r^T D^-1 r (re, im)
+// and ||r||^2 in three slots
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
-                  const double *sc, const int *ic, double *partial, hipStream_t st);
+                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil = false);
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st);
+                 const int *ic, hipStream_t st, bool bil = false);
 // first PCG step: d = D^-1 r, partials [<r, D^-1 r>, <r, r>]
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                    const int *ic, double *partial, hipStream_t st);
+                    const int *ic, double *partial, hipStream_t st, bool bil = false);
 
 // scalar updates (one thread per column)
 enum FinKind {
@@ -125,6 +134,13 @@ enum FinKind {
     FIN_OUTER_RES,      // red: [||b - A x||^2] -> outer res2, converged / stalled columns leave
     FIN_BICG_RELIABLE,  // red: [<r,r>, <rd,r>] of the replaced residual -> res2, rho, beta
     FIN_CG_RELIABLE,    // red: [<r,r>, <r,D^-1 r>] of the replaced residual -> res2, rho
+    // COCG (complex symmetric systems: PCG with the unconjugated bilinear
+    // form r^T z and complex alpha, beta, rho); red holds complex values as
+    // (re, im) in slots 0-1 and ||r||^2 in slot 2
+    FIN_COCG_INIT,      // red: [r^T z, ||r||^2] -> rho, res2
+    FIN_COCG_ALPHA,     // red: [p^T A p] -> alpha = rho / p^T A p
+    FIN_COCG_BETA,      // red: [r^T z, ||r||^2] -> rho, beta, res2, convergence
+    FIN_COCG_RELIABLE,  // red: [r^T z, ||r||^2] of the replaced residual -> rho, beta, res2
 };
 // maxit: iteration cap of the Krylov loop (0 = none); tol2: squared outer
 // relative tolerance; inner_tol2: squared inner tolerance of the mixed mode
@@ -144,7 +160,8 @@ void launch_to_inner(VKind vk32, const BlockDesc &b, const void *r64, void *r32,
 // reliable update of the inner residual: r_in = r64 * scale, x_in = 0,
 // partials [<r_in, r_in>, <rd, r_in>] (rd given) or [<r_in,r_in>, <r_in, D^-1 r_in>]
 void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
-                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st);
+                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st,
+                     bool bil = false);
 // x64 += x32 / scale[c] for outer-active columns
 void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x32, const double *scale,
                        const int *oflags, hipStream_t st);

@@ -63,6 +63,22 @@ __device__ inline double dotR(float2 a, float2 b) { return (double)a.x * (double
 __device__ inline double dotR(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
 __device__ inline double dotR(float a, float b) { return (double)a * (double)b; }
 __device__ inline double dotR(double a, double b) { return a * b; }
+// SpMM epilogue dots of one entry: mode 0: d1 += <w, y>_R; mode 1: also
+// d2 += <y, y>_R; mode DOT_BIL (complex): d1, d2 += Re, Im of w y
+// (unconjugated, COCG's p^T A p)
+template <class V>
+__device__ inline void dot_epi(V w, V y, int mode, double &d1, double &d2)
+{
+    if constexpr (std::is_same<V, float2>::value || std::is_same<V, double2>::value) {
+        if (mode == DOT_BIL) {
+            d1 += (double)w.x * (double)y.x - (double)w.y * (double)y.y;
+            d2 += (double)w.x * (double)y.y + (double)w.y * (double)y.x;
+            return;
+        }
+    }
+    d1 += dotR(w, y);
+    if (mode) d2 += dotR(y, y);
+}
 
 // sign pattern of the initial residual (-1, 0, +1 per real component), the
 // shadow residual of the mixed-precision BiCGSTAB: any fixed shadow vector
@@ -185,6 +201,29 @@ __device__ inline void block_reduce2(double d1, double d2, double *partial, int 
     }
 }
 
+// Three sums per column (COCG's complex r^T z and ||r||^2), in the order of
+// block_reduce2, into partial slots 0, 1 and 2.
+template <int K>
+__device__ inline void block_reduce3(double d1, double d2, double d3, double *partial, int blk, int G)
+{
+    for (int off = K; off < 64; off <<= 1) {
+        d1 += __shfl_xor(d1, off);
+        d2 += __shfl_xor(d2, off);
+        d3 += __shfl_xor(d3, off);
+    }
+    __shared__ double s1[SPMM64_WAVES][K], s2[SPMM64_WAVES][K], s3[SPMM64_WAVES][K];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane < K) { s1[wave][lane] = d1; s2[wave][lane] = d2; s3[wave][lane] = d3; }
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        for (int w = 0; w < nw; w++) { t1 += s1[w][threadIdx.x]; t2 += s2[w][threadIdx.x]; t3 += s3[w][threadIdx.x]; }
+        partial[((size_t)0 * K + threadIdx.x) * G + blk] = t1;
+        partial[((size_t)1 * K + threadIdx.x) * G + blk] = t2;
+        partial[((size_t)2 * K + threadIdx.x) * G + blk] = t3;
+    }
+}
+
 // ----------------------------------------------------------- block SpMM
 // Gathers are issued in chunks of CH nonzeros (indices and values first,
 // then all CH neighbour loads, then the multiply-adds) so that a wave keeps
@@ -241,8 +280,7 @@ __global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__
         if (i < rs.end && act) {
             V acc = row_gather<V, K, PRE>(rowptr[i], rowptr[i + 1], c, colidx, val, idiag, X);
             Y[(size_t)i * K + c] = acc;
-            d1 += dotR(W[(size_t)i * K + c], acc);
-            if (want_yy) d2 += dotR(acc, acc);
+            dot_epi(W[(size_t)i * K + c], acc, want_yy, d1, d2);
         }
     }
     block_reduce2<K>(d1, d2, partial, blk, G, want_yy != 0);
@@ -318,8 +356,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *
             if (act) {
                 if (MODE == 0) {
                     Y[at] = acc;
-                    d1 += dotR(W[at], acc);
-                    if (want_yy) d2 += dotR(acc, acc);
+                    dot_epi(W[at], acc, want_yy, d1, d2);
                 } else {
                     V r = vsub(W[at], acc);
                     Y[at] = r;
@@ -436,8 +473,7 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
             if (act) {
                 if (MODE == 0) {
                     Y[at] = acc;
-                    d1 += dotR(W[at], acc);
-                    if (want_yy) d2 += dotR(acc, acc);
+                    dot_epi(W[at], acc, want_yy, d1, d2);
                 } else {
                     V r = vsub(W[at], acc);
                     Y[at] = r;
@@ -558,8 +594,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
                 if (MODE == 0) {
                     Y[at] = acc[r];
                     const V w = Wq ? sh_unpack(Wq, (size_t)(r0 + r), c) : W[at];   // Wq: packed shadow residual
-                    d1 += dotR(w, acc[r]);
-                    if (want_yy) d2 += dotR(acc[r], acc[r]);
+                    dot_epi(w, acc[r], want_yy, d1, d2);
                 } else {
                     const V y = vsub(W[at], acc[r]);
                     Y[at] = y;
@@ -567,6 +602,228 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
                 }
             }
         }
+    }
+    block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
+}
+
+// ------------------------- pipelined row-block SpMM, 64 columns (variant 4)
+// The row-block kernel's arithmetic (each row sums val * x over its columns
+// in ascending order, so Y is bit-identical to it) with its memory traffic
+// restructured for latency hiding inside each wave:
+//  * a batch is up to RBP_B distinct columns of one block; its loads are the
+//    RBP_B neighbour-row gathers plus ONE 64-lane load of the batch's
+//    coefficients (at most RBP_B x RB_ROWS = 64, contiguous: the block's
+//    coefficients are stored column then row), located by the packed word
+//    of its first column (row mask | offset of the column's first
+//    coefficient in the block << 16);
+//  * column indices and words come through the scalar cache (uniform
+//    addresses), so they never wait behind the vector gathers;
+//  * batches are double-buffered: the gathers of batch t+1 -- including the
+//    first batch of the wave's next block -- are in flight while batch t is
+//    summed, and every batch issues the same number of loads (indices past
+//    the block are clamped), so each wait covers only the batch it needs;
+//  * the packed shadow residual of a block's rows is loaded with the block's
+//    first batch, ahead of its use in the epilogue.
+#ifndef TMI_RBP_B
+#define TMI_RBP_B 8
+#endif
+constexpr int RBP_B = TMI_RBP_B;
+static_assert(RBP_B * RB_ROWS <= 64, "a batch's coefficients must fit one 64-lane window");
+static_assert(RBP_B + 1 < 64, "s_waitcnt counts at most 63 vector loads");
+// a wave-uniform pointer kept in scalar registers: both halves pass through
+// readfirstlane, so the optimizer cannot fold a per-lane offset into it and
+// loads p[lane] use the scalar-base + 32-bit lane-offset addressing
+// (global address space, so that the loads stay global loads).  Loads go
+// through integer types, bit-cast to the element type.
+template <class I>
+__device__ inline const __attribute__((address_space(1))) I *uniform_gptr(const void *p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return (const __attribute__((address_space(1))) I *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ inline float2 ld_row_f2(const float2 *row, unsigned i)
+{
+    return __builtin_bit_cast(float2, uniform_gptr<unsigned long long>(row)[i]);
+}
+__device__ inline unsigned ld_row_u8(const unsigned char *row, unsigned i)
+{
+    return uniform_gptr<unsigned char>(row)[i];
+}
+// s_waitcnt immediate (gfx9 family) waiting until at most n vector-memory
+// operations are outstanding; export and LGKM counts are left unconstrained
+constexpr unsigned waitcnt_vm(unsigned n) { return (n & 0xfu) | (0x7u << 4) | (0xfu << 8) | ((n >> 4) << 14); }
+
+struct RbpBatch {
+    int b;        // block (-1: none)
+    int p, cnt;   // first distinct column (index into the union arrays) and count
+    int u1;       // end of the block's union
+    int kb, ke;   // the block's coefficient range
+};
+
+// the wave's next batch after (blk, p): the rest of the block, else the
+// first batch of the wave's next block (b + step), else none
+__device__ inline RbpBatch rbp_batch(int b, int p, int xe, int step, int n, const int *__restrict__ bofs,
+                                     const int *__restrict__ rowptr, RbpBatch cur)
+{
+    RbpBatch nx;
+    if (b >= 0 && p < cur.u1) {
+        nx = cur;
+        nx.p = p;
+    } else {
+        const int nb = b < 0 ? b : b + step;
+        nx.b = (b >= 0 && nb < xe) ? nb : -1;
+        if (nx.b < 0) {
+            nx.p = nx.u1 = nx.kb = nx.ke = 0;
+            nx.cnt = 0;
+            return nx;
+        }
+        nx.b = __builtin_amdgcn_readfirstlane(nx.b);
+        nx.p = bofs[nx.b];
+        nx.u1 = bofs[nx.b + 1];
+        const int r0 = nx.b * RB_ROWS, nr = min(RB_ROWS, n - r0);
+        nx.kb = rowptr[r0];
+        nx.ke = rowptr[r0 + nr];
+    }
+    nx.cnt = min(RBP_B, nx.u1 - nx.p);
+    return nx;
+}
+
+template <int MODE, bool PACKED>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; PACKED: W given as Wq
+__global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rbp(int n, const int *__restrict__ rowptr,
+                                                            const int *__restrict__ bofs, const int *__restrict__ ucol,
+                                                            const unsigned *__restrict__ uword,
+                                                            const float2 *__restrict__ rbval,
+                                                            const float2 *__restrict__ X, float2 *__restrict__ Y,
+                                                            const float2 *__restrict__ W, int want_yy,
+                                                            const int *__restrict__ active,
+                                                            double *__restrict__ partial,
+                                                            const unsigned char *__restrict__ Wq)
+{
+    constexpr int B = RBP_B;
+    const int lane = threadIdx.x & 63, c = lane;
+    const bool act = active[c] != 0;
+    const int G = gridDim.x, nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
+    const int nb = (n + RB_ROWS - 1) / RB_ROWS;
+    const int per_x = (nb + NXCD - 1) / NXCD;
+    const int xs = min(nb, xcd * per_x), xe = min(nb, xs + per_x);
+    const int wpx = nbx * nw, wid = bx * nw + wave;
+    double d1 = 0.0, d2 = 0.0;
+
+    float2 xa[B], xn[B];   // gathered rows of the batch being summed / in flight
+    float2 ca, cn;         // their coefficient windows (lane l: coefficient l of the batch)
+    unsigned wa[B], wn[B]; // packed words
+    float2 wr[RB_ROWS];    // W of the rows of a finishing block
+    unsigned qr[RB_ROWS];  // ... or its packed bytes (decoded where used)
+    float2 acc[RB_ROWS];
+#pragma unroll
+    for (int r = 0; r < RB_ROWS; r++) acc[r] = vzero<float2>();
+#pragma unroll
+    for (int u = 0; u < B; u++) xa[u] = xn[u] = vzero<float2>();
+
+    // issue the loads of batch t into (x, cw, w).  Every batch issues the
+    // same B + 1 vector loads: the words and column indices of B consecutive
+    // union entries (one scalar load each; entries past the batch belong to
+    // the next block or to the arrays' 64-entry padding and are valid rows
+    // whose values are not used), the B gathered rows for all lanes, and the
+    // coefficient window.  Nothing is conditional and no register is reset,
+    // so each batch's wait covers exactly the loads it needs.
+    auto issue = [&](const RbpBatch &t, float2 *x, float2 &cw, unsigned *w) {
+        const int p = t.p;
+        int j[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            w[u] = uword[p + u];
+            j[u] = ucol[p + u];
+        }
+        // uniform row address + lane offset: scalar-base loads, no 64-bit
+        // per-lane address registers
+#pragma unroll
+        for (int u = 0; u < B; u++) x[u] = ld_row_f2(X + (size_t)j[u] * 64, c);
+        const int base = __builtin_amdgcn_readfirstlane(t.kb + (int)(w[0] >> 16));
+        cw = ld_row_f2(rbval + base, min(lane, max(t.ke - 1 - base, 0)));
+    };
+    // W (or its packed shadow form) of the rows of batch t's block, if t is
+    // the block's last batch: loaded before the next batch's gathers, used
+    // after them
+    auto load_w = [&](const RbpBatch &t) {
+        if (t.p + t.cnt >= t.u1) {
+            const int r0 = t.b * RB_ROWS;
+#pragma unroll
+            for (int r = 0; r < RB_ROWS; r++) {
+                const int row = min(r0 + r, n - 1);
+                if (PACKED) qr[r] = ld_row_u8(Wq + (size_t)row * 32, c >> 1);
+                else wr[r] = ld_row_f2(W + (size_t)row * 64, c);
+            }
+        }
+    };
+    // sum batch t (buffers x, cw, w); at the block's last batch write its rows
+    auto sum = [&](const RbpBatch &t, const float2 *x, const float2 &cw, const unsigned *w) {
+        // this batch's loads (and a finishing block's W) are all older than
+        // the next batch's B + 1 vector loads
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(B + 1));
+        const int off0 = (int)(w[0] >> 16);
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            if (u < t.cnt) {
+                const unsigned mk = w[u] & 0xffffu;
+                int k = (int)(w[u] >> 16) - off0;
+#pragma unroll
+                for (int r = 0; r < RB_ROWS; r++) {
+                    if (mk & (1u << r)) {
+                        acc[r] = vadd(acc[r], vmul(rdlane(cw, k), x[u]));
+                        k++;
+                    }
+                }
+            }
+        }
+        if (t.p + t.cnt >= t.u1) {   // the block is complete
+            const int r0 = t.b * RB_ROWS, nr = min(RB_ROWS, n - r0);
+#pragma unroll
+            for (int r = 0; r < RB_ROWS; r++) {
+                if (r < nr && act) {
+                    Y[(size_t)(r0 + r) * 64 + c] = acc[r];
+                    float2 wv = wr[r];
+                    if (PACKED) {
+                        const unsigned s = (qr[r] >> ((c & 1) * 4)) & 0xfu;
+                        wv = make_float2(sh_val(s & 3u), sh_val(s >> 2));
+                    }
+                    dot_epi(wv, acc[r], want_yy, d1, d2);
+                }
+                acc[r] = vzero<float2>();
+            }
+        }
+    };
+
+    RbpBatch ta;
+    ta.b = -1;
+    ta.p = ta.cnt = ta.u1 = ta.kb = ta.ke = 0;
+    {   // the wave's first block
+        const int b0 = xs + wid;
+        if (b0 < xe) {
+            ta.b = __builtin_amdgcn_readfirstlane(b0);
+            ta.p = bofs[ta.b];
+            ta.u1 = bofs[ta.b + 1];
+            const int r0 = ta.b * RB_ROWS, nr = min(RB_ROWS, n - r0);
+            ta.kb = rowptr[r0];
+            ta.ke = rowptr[r0 + nr];
+            ta.cnt = min(B, ta.u1 - ta.p);
+        }
+    }
+    issue(ta, xa, ca, wa);
+    while (ta.b >= 0) {
+        const RbpBatch tn = rbp_batch(ta.b, ta.p + ta.cnt, xe, wpx, n, bofs, rowptr, ta);
+        load_w(ta);
+        issue(tn, xn, cn, wn);
+        sum(ta, xa, ca, wa);
+        if (tn.b < 0) break;
+        ta = rbp_batch(tn.b, tn.p + tn.cnt, xe, wpx, n, bofs, rowptr, tn);
+        load_w(tn);
+        issue(ta, xa, ca, wa);
+        sum(tn, xn, cn, wn);
     }
     block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
 }
@@ -691,8 +948,7 @@ __global__ void __launch_bounds__(64 * NW) k_spmm_lds(int nb, const int *__restr
                 const size_t at = (size_t)(r0 + lr) * 64 + lane;
                 if (MODE == 0) {
                     Y[at] = acc;
-                    d1 += dotR(wv, acc);
-                    if (want_yy) d2 += dotR(acc, acc);
+                    dot_epi(wv, acc, want_yy, d1, d2);
                 } else {
                     const V r = vsub(wv, acc);
                     Y[at] = r;
@@ -941,8 +1197,7 @@ __global__ void __launch_bounds__(256) k_spmv_dot(int n, const int *__restrict__
         for (int off = LPR / 2; off > 0; off >>= 1) acc = shfl_add(acc, off);
         if (i < rs.end && act && sub == 0) {
             Y[i] = acc;
-            d1 += dotR(W[i], acc);
-            if (want_yy) d2 += dotR(acc, acc);
+            dot_epi(W[i], acc, want_yy, d1, d2);
         }
     }
     block_reduce2<1>(d1, d2, partial, blk, G, want_yy != 0);
@@ -1183,6 +1438,85 @@ __global__ void __launch_bounds__(256) k_cg_d(int n, const V *__restrict__ idiag
     if (partial) block_reduce2<K>(d1, d2, partial, blk, G, true);
 }
 
+// ---- COCG: PCG for complex symmetric systems with the unconjugated
+// bilinear form (van der Vorst & Melissen, IEEE Trans. Magn. 26 (1990)
+// 706-708): alpha = r^T z / p^T A p, beta = r_new^T z_new / r^T z, z = D^-1 r,
+// all complex; one SpMM per iteration.
+// complex value times a complex scalar (ar + i ai)
+__device__ inline float2 cscale(float2 v, double ar, double ai)
+{
+    const float a = (float)ar, b = (float)ai;
+    return make_float2(v.x * a - v.y * b, v.x * b + v.y * a);
+}
+__device__ inline double2 cscale(double2 v, double ar, double ai)
+{
+    return make_double2(v.x * ar - v.y * ai, v.x * ai + v.y * ar);
+}
+// unconjugated product terms: re and im of a b
+template <class V>
+__device__ inline void bil_add(V a, V b, double &re, double &im)
+{
+    if constexpr (std::is_same<V, float2>::value || std::is_same<V, double2>::value) {
+        re += (double)a.x * (double)b.x - (double)a.y * (double)b.y;
+        im += (double)a.x * (double)b.y + (double)a.y * (double)b.x;
+    } else {
+        re += (double)a * (double)b;
+    }
+}
+
+// x += alpha d; r -= alpha q; partials r^T D^-1 r (re, im) and ||r||^2
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ idiag, V *__restrict__ x,
+                                                 const V *__restrict__ d, const V *__restrict__ q,
+                                                 V *__restrict__ r, const double *__restrict__ sc,
+                                                 const int *__restrict__ ic, double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (ic[I_ACTIVE * K + c]) {
+        const double ar = sc[S_ALPHA * K + c], ai = sc[S_ALPHA_I * K + c];
+        struct L { V x, d, r, q, di; };
+        ew_for(
+            tot, [&](size_t t) { return L{ldnt(x + t), ldnt(d + t), ldnt(r + t), ldnt(q + t), idiag ? idiag[t / K] : V{}}; },
+            [&](size_t t, const L &a) {
+                stnt(x + t, vadd(a.x, cscale(a.d, ar, ai)));
+                const V rn = vsub(a.r, cscale(a.q, ar, ai));
+                stnt(r + t, rn);
+                const V z = idiag ? vmul(a.di, rn) : rn;
+                bil_add(rn, z, d1, d2);
+                d3 += dotR(rn, rn);
+            });
+    }
+    block_reduce3<K>(d1, d2, d3, partial, blk, G);
+}
+
+// d = beta d + D^-1 r (d = D^-1 r on the first call); with partials (start
+// of the iteration): r^T D^-1 r (re, im) and ||r||^2
+template <class V, int K>
+__global__ void __launch_bounds__(256) k_cocg_d(int n, const V *__restrict__ idiag, V *__restrict__ d,
+                                                const V *__restrict__ r, const double *__restrict__ sc,
+                                                const int *__restrict__ ic, double *__restrict__ partial)
+{
+    EW_PROLOGUE
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (ic[I_ACTIVE * K + c]) {
+        const bool first = ic[I_FIRST * K + c] != 0;
+        const double br = sc[S_BETA * K + c], bi = sc[S_BETA_I * K + c];
+        struct L { V r, d, di; };
+        ew_for(
+            tot, [&](size_t t) { return L{ldnt(r + t), first ? V{} : ldnt(d + t), idiag ? idiag[t / K] : V{}}; },
+            [&](size_t t, const L &a) {
+                const V z = idiag ? vmul(a.di, a.r) : a.r;
+                stnt(d + t, first ? z : vadd(cscale(a.d, br, bi), z));
+                if (partial) {
+                    bil_add(a.r, z, d1, d2);
+                    d3 += dotR(a.r, a.r);
+                }
+            });
+    }
+    if (partial) block_reduce3<K>(d1, d2, d3, partial, blk, G);
+}
+
 // r_in = r64 * scale ; rd_in = r_in (sign_shadow: its sign pattern) ;
 // x_in = 0 ; partials <r_in, r_in> and <rd_in, r_in>
 template <class V64, class V, int K>
@@ -1216,11 +1550,25 @@ template <class V64, class V, int K>
 __global__ void __launch_bounds__(256) k_reliable(int n, const V64 *__restrict__ r64, V *__restrict__ rin,
                                                   V *__restrict__ xin, const V *__restrict__ rd,
                                                   const V *__restrict__ idiag, const double *__restrict__ scale,
-                                                  const int *__restrict__ oflags, double *__restrict__ partial)
+                                                  const int *__restrict__ oflags, double *__restrict__ partial,
+                                                  int bil)
 {
     EW_PROLOGUE
-    double d1 = 0.0, d2 = 0.0;
-    if (oflags[c]) {
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (oflags[c] && bil) {
+        // COCG: r^T D^-1 r (re, im) and ||r||^2
+        const double s = scale[c];
+        struct L { V64 r; V di; };
+        ew_for(
+            tot, [&](size_t t) { return L{r64[t], idiag ? idiag[t / K] : V{}}; },
+            [&](size_t t, const L &a) {
+                V v = cvt<V>(vscale(a.r, s));
+                rin[t] = v;
+                xin[t] = vzero<V>();
+                bil_add(v, idiag ? vmul(a.di, v) : v, d1, d2);
+                d3 += dotR(v, v);
+            });
+    } else if (oflags[c]) {
         const double s = scale[c];
         struct L { V64 r; V rd, di; };
         ew_for(
@@ -1235,7 +1583,8 @@ __global__ void __launch_bounds__(256) k_reliable(int n, const V64 *__restrict__
                 else d2 += dotR(v, idiag ? vmul(a.di, v) : v);
             });
     }
-    block_reduce2<K>(d1, d2, partial, blk, G, true);
+    if (bil) block_reduce3<K>(d1, d2, d3, partial, blk, G);
+    else block_reduce2<K>(d1, d2, partial, blk, G, true);
 }
 
 template <class V, int K>
@@ -1266,10 +1615,19 @@ __global__ void __launch_bounds__(256) k_from_inner(int n, V64 *__restrict__ x64
     stnt(x64 + t, vadd(xv, vscale(to64(xi), 1.0 / cp.d[0][c])));
 }
 
-// The scalar recurrence of one column after a reduction (r0, r1: the two
+// (cr + i ci) = (ar + i ai) / (br + i bi); false when the quotient is not finite
+__device__ inline bool cdiv(double ar, double ai, double br, double bi, double &cr, double &ci)
+{
+    const double den = br * br + bi * bi;
+    cr = (ar * br + ai * bi) / den;
+    ci = (ai * br - ar * bi) / den;
+    return den != 0.0 && isfinite(cr) && isfinite(ci);
+}
+
+// The scalar recurrence of one column after a reduction (r0, r1, r2: the
 // summed partials); S and I hold the column's scalar and flag slots
 // (I[I_NSLOT]: the outer solve's active flag).
-__device__ inline void fin_column(int fk, double r0, double r1, double *S, int *I, int maxit, double tol2)
+__device__ inline void fin_column(int fk, double r0, double r1, double r2, double *S, int *I, int maxit, double tol2)
 {
 #define SC(slot) S[slot]
 #define IC(slot) I[slot]
@@ -1377,6 +1735,67 @@ __device__ inline void fin_column(int fk, double r0, double r1, double *S, int *
         if (maxit > 0 && IC(I_ITERS) >= maxit) { IC(I_ACTIVE) = 0; break; }
         break;
     }
+    // COCG: rho, alpha, beta are complex (real part in the slot, imaginary
+    // part in its _I slot); r0, r1 = re, im of a bilinear product, r2 = ||r||^2
+    case FIN_COCG_INIT:
+        if (!IC(I_ACTIVE)) break;
+        SC(S_BNORM2) = 1.0;   // the inner system is scaled by 1/||b||: absolute criterion
+        SC(S_RES2) = r2;
+        SC(S_RHO) = r0;
+        SC(S_RHO_I) = r1;
+        IC(I_FIRST) = 1;
+        IC(I_BREAK) = 0;
+        if (r2 <= SC(S_TOL2)) IC(I_ACTIVE) = 0;
+        break;
+    case FIN_COCG_ALPHA: {
+        if (!IC(I_ACTIVE)) break;
+        double ar, ai;
+        if (!cdiv(SC(S_RHO), SC(S_RHO_I), r0, r1, ar, ai) || (ar == 0.0 && ai == 0.0)) {
+            IC(I_ACTIVE) = 0;
+            IC(I_BREAK) = 1;
+            ar = ai = 0.0;
+        }
+        SC(S_ALPHA) = ar;
+        SC(S_ALPHA_I) = ai;
+        break;
+    }
+    case FIN_COCG_BETA: {
+        if (!IC(I_ACTIVE)) break;
+        IC(I_ITERS) += 1;
+        IC(I_FIRST) = 0;
+        double br, bi;
+        const bool ok = cdiv(r0, r1, SC(S_RHO), SC(S_RHO_I), br, bi);
+        SC(S_RHOP) = SC(S_RHO);
+        SC(S_RHOP_I) = SC(S_RHO_I);
+        SC(S_RHO) = r0;
+        SC(S_RHO_I) = r1;
+        SC(S_BETA) = br;
+        SC(S_BETA_I) = bi;
+        SC(S_RES2) = r2;
+        if (!ok || (br == 0.0 && bi == 0.0)) { IC(I_ACTIVE) = 0; IC(I_BREAK) = 1; break; }
+        if (r2 <= SC(S_TOL2)) { IC(I_ACTIVE) = 0; break; }
+        if (maxit > 0 && IC(I_ITERS) >= maxit) { IC(I_ACTIVE) = 0; break; }
+        break;
+    }
+    case FIN_COCG_RELIABLE: {
+        // the replacement sits between the beta step and the direction
+        // update, so beta is recomputed from the replaced residual
+        if (!I[I_NSLOT]) { IC(I_ACTIVE) = 0; break; }
+        SC(S_RES2) = r2;
+        SC(S_RHO) = r0;
+        SC(S_RHO_I) = r1;
+        double br, bi;
+        if (!IC(I_FIRST) && cdiv(r0, r1, SC(S_RHOP), SC(S_RHOP_I), br, bi)) {
+            SC(S_BETA) = br;
+            SC(S_BETA_I) = bi;
+        } else {
+            IC(I_FIRST) = 1;
+        }
+        if (IC(I_BREAK)) { IC(I_FIRST) = 1; IC(I_BREAK) = 0; }
+        IC(I_ACTIVE) = 1;
+        if (maxit > 0 && IC(I_ITERS) >= maxit) IC(I_ACTIVE) = 0;
+        break;
+    }
     }
 #undef SC
 #undef IC
@@ -1391,7 +1810,7 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
                                                   const int *__restrict__ oic)
 {
     const int c = blockIdx.x;
-    __shared__ double red[2][4];
+    __shared__ double red[RED_SLOTS][4];
     __shared__ double sS[S_NSLOT];
     __shared__ int sI[I_NSLOT + 1];
     // the column's scalars and flags come in with one parallel load, beside
@@ -1402,7 +1821,8 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
     // fixed order: per thread b = tid, tid + 256, ...; then a butterfly over
     // the wave's lanes; then the four waves in order
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int slot = 0; slot < 2; slot++) {
+    const int nslot = (fk == FIN_COCG_INIT || fk == FIN_COCG_BETA || fk == FIN_COCG_RELIABLE) ? 3 : 2;
+    for (int slot = 0; slot < nslot; slot++) {
         double s = 0.0;
         for (int b = threadIdx.x; b < G; b += 256) s += partial[((size_t)slot * K + c) * G + b];
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -1411,7 +1831,7 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
     __syncthreads();
     if (threadIdx.x == 0)
         fin_column(fk, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]), (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]),
-                   sS, sI, maxit, tol2);
+                   nslot == 3 ? (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]) : 0.0, sS, sI, maxit, tol2);
     __syncthreads();
     if ((int)threadIdx.x < S_NSLOT) sc[threadIdx.x * K + c] = sS[threadIdx.x];
     else if ((int)threadIdx.x >= 64 && (int)threadIdx.x < 64 + I_NSLOT) ic[(threadIdx.x - 64) * K + c] = sI[threadIdx.x - 64];
@@ -1574,7 +1994,7 @@ static inline void check_K(int K)
     }
 
 void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
-                     const void *W, bool want_yy, const int *iflags, double *partial, hipStream_t st,
+                     const void *W, int want_yy, const int *iflags, double *partial, hipStream_t st,
                      const unsigned char *Wq)
 {
     check_K(b.K);
@@ -1590,11 +2010,19 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
 #undef SPMV1
     } else {
 #define SPMMK(KK)                                                                                             \
-    if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val)                                \
+    if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val && b.rb.word && Wq)             \
+        hipLaunchKernelGGL((k_spmm_rbp<0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs, \
+                           b.rb.col, b.rb.word, (const float2 *)b.rb.val, (const float2 *)X, (float2 *)Y,      \
+                           (const float2 *)W, (int)want_yy, iflags, partial, Wq);                           \
+    else if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val && b.rb.word)              \
+        hipLaunchKernelGGL((k_spmm_rbp<0, false>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr,         \
+                           b.rb.ofs, b.rb.col, b.rb.word, (const float2 *)b.rb.val, (const float2 *)X,         \
+                           (float2 *)Y, (const float2 *)W, (int)want_yy, iflags, partial, Wq);              \
+    else if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val)                           \
         hipLaunchKernelGGL((k_spmm_rb<float2, 0>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs,   \
                            b.rb.col, b.rb.mask, (const float2 *)b.rb.val, (const float2 *)X, (float2 *)Y,      \
                            (const float2 *)W, (int)want_yy, iflags, partial, Wq);                           \
-    else if (KK == 64 && !idiag && vk == VK_C32 && b.lds.nb > 0 && b.lds.dma) {                            \
+    else if (KK == 64 && !idiag && vk == VK_C32 && b.lds.nb > 0 && b.lds.dma && want_yy != DOT_BIL) {      \
         if (want_yy)                                                                                          \
             hipLaunchKernelGGL((k_spmm_dma<true>), dim3(std::min(DMA_GRID, b.nblk)), dim3(DMA_NT), 0, st, b.lds.nb,       \
                                b.lds.brow, b.lds.bu, b.lds.ucols, b.rowptr, b.lds.lcol, (const float2 *)val, \
@@ -1689,37 +2117,55 @@ void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, co
     TMI_LAUNCH_CHECK();
 }
 
+// the COCG kernels exist for complex vectors only
+#define DISPATCH_C(vk, ...)                                                         \
+    switch (vk) {                                                                   \
+    case VK_C32: { using V = float2; __VA_ARGS__; } break;                          \
+    case VK_C64: { using V = double2; __VA_ARGS__; } break;                         \
+    default: throw Error(TMI_ERR_UNSUPPORTED, "COCG needs complex vectors");       \
+    }
+
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
-                  const double *sc, const int *ic, double *partial, hipStream_t st)
+                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil)
 {
     dim3 g(b.nblk), blk(256);
 #define CXK(KK)                                                                                               \
     hipLaunchKernelGGL((k_cg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,           \
                        (const V *)q, (V *)r, sc, ic, partial);
-    DISPATCH_V(vk, DISPATCH_K(b.K, CXK))
+#define BXK(KK)                                                                                               \
+    hipLaunchKernelGGL((k_cocg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,         \
+                       (const V *)q, (V *)r, sc, ic, partial);
+    if (bil) {
+        DISPATCH_C(vk, DISPATCH_K(b.K, BXK))
+    } else {
+        DISPATCH_V(vk, DISPATCH_K(b.K, CXK))
+    }
 #undef CXK
+#undef BXK
     TMI_LAUNCH_CHECK();
 }
 
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st)
+                 const int *ic, hipStream_t st, bool bil)
 {
-    dim3 g(b.nblk), blk(256);
-#define CDK(KK) \
-    hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, nullptr);
-    DISPATCH_V(vk, DISPATCH_K(b.K, CDK))
-#undef CDK
-    TMI_LAUNCH_CHECK();
+    launch_cg_init(vk, b, idiag, d, r, sc, ic, nullptr, st, bil);
 }
 
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                    const int *ic, double *partial, hipStream_t st)
+                    const int *ic, double *partial, hipStream_t st, bool bil)
 {
     dim3 g(b.nblk), blk(256);
 #define CIK(KK) \
     hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
-    DISPATCH_V(vk, DISPATCH_K(b.K, CIK))
+#define BIK(KK) \
+    hipLaunchKernelGGL((k_cocg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
+    if (bil) {
+        DISPATCH_C(vk, DISPATCH_K(b.K, BIK))
+    } else {
+        DISPATCH_V(vk, DISPATCH_K(b.K, CIK))
+    }
 #undef CIK
+#undef BIK
     TMI_LAUNCH_CHECK();
 }
 
@@ -1745,6 +2191,9 @@ __global__ void k_inner_reset(int K, const double *__restrict__ osc, const int *
     sc[S_BETA * K + c] = 0.0;
     sc[S_OMEGA * K + c] = 0.0;
     sc[S_ALPHA * K + c] = 0.0;
+    sc[S_RHOP_I * K + c] = 0.0;
+    sc[S_BETA_I * K + c] = 0.0;
+    sc[S_ALPHA_I * K + c] = 0.0;
 }
 
 void launch_inner_reset(const BlockDesc &b, const double *osc, const int *oic, double *sc, int *ic, hipStream_t st)
@@ -1795,12 +2244,14 @@ void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x3
 }
 
 void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
-                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st)
+                     const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st,
+                     bool bil)
 {
     dim3 g(b.nblk), blk(256);
+    if (bil && vk_in != VK_C32 && vk_in != VK_C64) throw Error(TMI_ERR_UNSUPPORTED, "COCG needs complex vectors");
 #define RLK(KK)                                                                                                 \
     hipLaunchKernelGGL((k_reliable<V64, V, KK>), g, blk, 0, st, b.n, (const V64 *)r64, (V *)rin, (V *)xin,      \
-                       (const V *)rd, (const V *)idiag, scale, oflags, partial);
+                       (const V *)rd, (const V *)idiag, scale, oflags, partial, bil ? 1 : 0);
     switch (vk_in) {
     case VK_C32: { using V = float2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }
     case VK_C64: { using V = double2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }

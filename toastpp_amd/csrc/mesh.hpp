@@ -62,6 +62,11 @@ struct Mesh {
         int rb_nb = 0;
         DBuf<int> rb_ofs, rb_col, rb_slot;
         DBuf<rb_mask_t> rb_mask;
+        // per distinct column: row mask | (offset of its first coefficient
+        // in the block << 16), for the pipelined row-block SpMM (valid when
+        // every block has fewer than 65536 coefficients)
+        DBuf<unsigned> rb_word;
+        bool rb_word_ok = false;
     } order[2];
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }
@@ -88,5 +93,19 @@ void launch_qvec_gaussian(const Mesh &m, int npts, const double *d_centres, doub
 // elbasis = 0: nodal coefficients (n each); 1: per-element coefficients (elen each)
 void launch_assemble(const Mesh &m, const double *d_cmua, const double *d_ckap, const double *d_c2a, double omega,
                      int complex_valued, double *d_val64, void *d_val32, hipStream_t st, int elbasis = 0);
+
+// projections (mesh_kernels.hip): proj[k] = sum_j phi[q][j] * mvec[m][j] for
+// the link k = lk[m * nq + q] of every linked pair (lk = -1: not linked).
+// The plan fixes the segment length S of the mvec rows, the sources per
+// wave and the partial-sum buffer (partial_doubles) the launch needs;
+// maxrow = the longest mvec row.
+struct ProjectPlan {
+    int qw = 1, S = 1, nseg = 1;
+    size_t partial_doubles = 0;
+};
+ProjectPlan project_plan(int nq, int nm, int maxrow, int cplx);
+void launch_project(const ProjectPlan &pl, int nq, int nm, const int *d_lk, const int *mrp, const int *mci,
+                    const double *mval, const double *phi, long sq, long sj, int cplx, double *partial, double *proj,
+                    hipStream_t st);
 
 }  // namespace tmi

@@ -460,81 +460,157 @@ void launch_qvec_gaussian(const Mesh &m, int npts, const double *d_centres, doub
 namespace tmi {
 
 // ----------------------------------------------------------- projections
-// ProjectSingle/ProjectAll (fwdsolver.cc:1415-1447): one thread per
-// measurement k = (q, m) sums phi_q[j] * mvec[m, j] over the nonzeros of
-// mvec row m in ascending j -- the order of the reference's dense dot
-// (vector_imp.hpp:1045-1052), whose other terms are exact zeros.
-// phi element (q, j) is at phi[q*sq + j*sj] (complex: interleaved).
-// Thread per (detector m, source q), the sources of one detector on
-// consecutive lanes: a wave walks one mvec row (entries broadcast to the
-// lanes) and, with node-major fields (sq = 1), reads consecutive phi
-// entries; lanes whose pair is not linked (lk = -1) stay idle.
+// ProjectSingle/ProjectAll (fwdsolver.cc:1415-1447): proj[k] = sum_j
+// phi_q[j] * mvec[m, j] for the link k = (q, m), over the nonzeros j of mvec
+// row m (the reference's dense dot, vector_imp.hpp:1045-1052, whose other
+// terms are exact zeros).  phi element (q, j) is at phi[q*sq + j*sj]
+// (complex: interleaved).
+//
+// A detector's row is split into segments of S nonzeros; one wave per
+// (segment, detector).  Its lanes are QW sources (consecutive q: with
+// node-major fields, sq = 1, one 16*QW-byte run of a field row per nonzero)
+// times EP = 64 / QW interleaved entry streams; each lane sums its entries in
+// ascending order, the EP streams are combined by a butterfly, and the
+// segment sums of each link are added in segment order by k_project_fin.
+// The summation order is fixed by (S, QW): results are reproducible run to
+// run.
 template <bool CPLX>
-__global__ void k_project(int nq, int nm, const int *__restrict__ lk, const int *__restrict__ mrp,
-                          const int *__restrict__ mci, const double *__restrict__ mval, const double *__restrict__ phi,
-                          long sq, long sj, double *__restrict__ proj)
+__global__ void __launch_bounds__(64) k_project_seg(int nq, int nm, int S, int qw, const int *__restrict__ lk,
+                                                    const int *__restrict__ mrp, const int *__restrict__ mci,
+                                                    const double *__restrict__ mval, const double *__restrict__ phi,
+                                                    long sq, long sj, double *__restrict__ partial)
 {
-    const int qt = (nq + 63) / 64 * 64;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    // one detector per wave (qt is a multiple of 64): its row index is uniform
-    const int m = __builtin_amdgcn_readfirstlane((int)(t / qt)), q = (int)(t % qt);
-    if (m >= nm || q >= nq) return;
-    const int k = lk[(long)m * nq + q];
-    if (k < 0) return;
-    // the row's terms are loaded PB at a time (the row entries first, then
-    // all field entries of the batch in flight together) and summed one
-    // after another in ascending order
-    constexpr int PB = 32;
     constexpr int CW = CPLX ? 2 : 1;
+    const int seg = blockIdx.x, m = blockIdx.y, nseg = gridDim.x;
+    const int r0 = mrp[m], r1 = mrp[m + 1];
+    const int e0 = r0 + seg * S;
+    if (e0 >= r1) return;   // the row ends before this segment
+    const int e1 = min(r1, e0 + S);
+    const int lane = threadIdx.x, ep = lane / qw, EP = 64 / qw;
+    for (int qb = 0; qb < nq; qb += qw) {
+        const int q = qb + lane % qw;
+        const bool qon = q < nq;
+        const long qo = (long)(qon ? q : 0) * sq;
+        double sr = 0.0, si = 0.0;
+        constexpr int U = 8;
+        for (int e = e0 + ep; e < e1; e += U * EP) {
+            long jj[U];
+            double c[U], d[U], a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int eu = min(e + u * EP, e1 - 1);
+                jj[u] = mci[eu];
+                c[u] = mval[CW * (long)eu];
+                d[u] = CPLX ? mval[CW * (long)eu + 1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const long at = qo + jj[u] * sj;
+                a[u] = phi[CW * at];
+                b[u] = CPLX ? phi[CW * at + 1] : 0.0;
+            }
+            // the batch's loads stay ahead of the ordered sums
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool on = e + u * EP < e1;   // entries past the segment end are not added
+                if (CPLX) {
+                    const double tr = a[u] * c[u] - b[u] * d[u], ti = a[u] * d[u] + b[u] * c[u];
+                    sr = on ? sr + tr : sr;
+                    si = on ? si + ti : si;
+                } else {
+                    const double tr = a[u] * c[u];
+                    sr = on ? sr + tr : sr;
+                }
+            }
+        }
+        // combine the EP entry streams of each source (same order in every lane)
+        for (int off = qw; off < 64; off <<= 1) {
+            sr += __shfl_xor(sr, off);
+            if (CPLX) si += __shfl_xor(si, off);
+        }
+        if (ep == 0 && qon && lk[(long)m * nq + q] >= 0) {
+            double *o = partial + CW * (((long)m * nseg + seg) * nq + q);
+            o[0] = sr;
+            if (CPLX) o[1] = si;
+        }
+    }
+}
+
+// proj[k] of every linked (m, q): the row's segment sums in segment order
+template <bool CPLX>
+__global__ void k_project_fin(int nq, int nm, int S, int nseg, const int *__restrict__ lk,
+                              const int *__restrict__ mrp, const double *__restrict__ partial,
+                              double *__restrict__ proj)
+{
+    constexpr int CW = CPLX ? 2 : 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nq * nm) return;
+    const int m = (int)(t / nq), q = (int)(t % nq);
+    const int k = lk[t];
+    if (k < 0) return;
+    const int len = mrp[m + 1] - mrp[m], ns = (len + S - 1) / S;
     double sr = 0.0, si = 0.0;
-    const int e0 = mrp[m], e1 = mrp[m + 1];
-    for (int e = e0; e < e1; e += PB) {
-        long jj[PB];
-        double c[PB], d[PB], a[PB], b[PB];
+    // loads in batches of B (independent), sums in segment order
+    constexpr int B = 16;
+    const double *o = partial + CW * ((long)m * nseg * nq + q);
+    const long step = (long)CW * nq;
+    for (int g0 = 0; g0 < ns; g0 += B) {
+        double a[B], b[B];
 #pragma unroll
-        for (int u = 0; u < PB; u++) {
-            const int eu = min(e + u, e1 - 1);
-            jj[u] = mci[eu];
-            c[u] = mval[CW * (long)eu];
-            d[u] = CPLX ? mval[CW * (long)eu + 1] : 0.0;
+        for (int u = 0; u < B; u++) {
+            const int g = min(g0 + u, ns - 1);
+            a[u] = o[g * step];
+            b[u] = CPLX ? o[g * step + 1] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < PB; u++) {
-            const long at = (long)q * sq + jj[u] * sj;
-            a[u] = phi[CW * at];
-            b[u] = CPLX ? phi[CW * at + 1] : 0.0;
-        }
-        // keep the batch's loads ahead of the sums (the scheduler would
-        // otherwise sink each load to its use in the serial chain)
-        __builtin_amdgcn_sched_barrier(0);
-        const int nu = min(PB, e1 - e);
-#pragma unroll
-        for (int u = 0; u < PB; u++) {
-            const bool on = u < nu;   // terms past the row end are not added
-            if (CPLX) {
-                const double tr = a[u] * c[u] - b[u] * d[u], ti = a[u] * d[u] + b[u] * c[u];
-                sr = on ? sr + tr : sr;
-                si = on ? si + ti : si;
-            } else {
-                const double tr = a[u] * c[u];
-                sr = on ? sr + tr : sr;
+        for (int u = 0; u < B; u++) {
+            if (g0 + u < ns) {
+                sr += a[u];
+                if (CPLX) si += b[u];
             }
         }
     }
-    if (CPLX) { proj[2 * (long)k] = sr; proj[2 * (long)k + 1] = si; }
-    else proj[k] = sr;
+    proj[CW * (long)k] = sr;
+    if (CPLX) proj[CW * (long)k + 1] = si;
 }
 
-void launch_project(int nq, int nm, const int *d_lk, const int *mrp, const int *mci, const double *mval,
-                    const double *phi, long sq, long sj, int cplx, double *proj, hipStream_t st)
+ProjectPlan project_plan(int nq, int nm, int maxrow, int cplx)
 {
-    const long threads = (long)nm * ((nq + 63) / 64 * 64);
+    ProjectPlan p;
+    p.qw = 1;
+    while (p.qw < nq && p.qw < 64) p.qw *= 2;
+    // segments: enough waves to fill the GPU (about 2048), at least 64
+    // entries per lane stream
+    const int ep = 64 / p.qw;
+    p.S = 4096;
+    while (p.S > 64 * ep && (long)nm * ((maxrow + p.S - 1) / p.S) < 2048) p.S /= 2;
+    p.nseg = std::max(1, (maxrow + p.S - 1) / p.S);
+    p.partial_doubles = (size_t)nm * p.nseg * nq * (cplx ? 2 : 1);
+    return p;
+}
+
+void launch_project(const ProjectPlan &pl, int nq, int nm, const int *d_lk, const int *mrp, const int *mci,
+                    const double *mval, const double *phi, long sq, long sj, int cplx, double *partial, double *proj,
+                    hipStream_t st)
+{
+    TMI_CHECK(pl.qw >= 1 && pl.qw <= 64 && (64 % pl.qw) == 0 && pl.S > 0 && pl.nseg > 0, TMI_ERR_INVALID_ARG,
+              "bad projection plan");
+    const dim3 g(pl.nseg, nm);
     if (cplx)
-        hipLaunchKernelGGL(k_project<true>, dim3(grid_for(threads, 256)), dim3(256), 0, st, nq, nm, d_lk, mrp, mci,
-                           mval, phi, sq, sj, proj);
+        hipLaunchKernelGGL(k_project_seg<true>, g, dim3(64), 0, st, nq, nm, pl.S, pl.qw, d_lk, mrp, mci, mval, phi,
+                           sq, sj, partial);
     else
-        hipLaunchKernelGGL(k_project<false>, dim3(grid_for(threads, 256)), dim3(256), 0, st, nq, nm, d_lk, mrp, mci,
-                           mval, phi, sq, sj, proj);
+        hipLaunchKernelGGL(k_project_seg<false>, g, dim3(64), 0, st, nq, nm, pl.S, pl.qw, d_lk, mrp, mci, mval, phi,
+                           sq, sj, partial);
+    TMI_LAUNCH_CHECK();
+    const long pairs = (long)nq * nm;
+    if (cplx)
+        hipLaunchKernelGGL(k_project_fin<true>, dim3(grid_for(pairs, 256)), dim3(256), 0, st, nq, nm, pl.S, pl.nseg,
+                           d_lk, mrp, partial, proj);
+    else
+        hipLaunchKernelGGL(k_project_fin<false>, dim3(grid_for(pairs, 256)), dim3(256), 0, st, nq, nm, pl.S,
+                           pl.nseg, d_lk, mrp, partial, proj);
     TMI_LAUNCH_CHECK();
 }
 

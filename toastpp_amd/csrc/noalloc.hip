@@ -90,22 +90,29 @@ extern "C" int tmi_solver_project_all_device_noalloc(tmi_solver h, int nm, const
         const int nq = qm.nq, nqm = qm.nqm, cw = s.cplx ? 2 : 1;
         if (nqm == 0) return TMI_OK;
         TMI_CHECK(mrp[0] == 0 && mrp[nm] >= 0, TMI_ERR_INVALID_ARG, "invalid mvec row pointer");
+        int maxrow = 0;
+        for (int i = 0; i < nm; i++) {
+            TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
+            maxrow = std::max(maxrow, mrp[i + 1] - mrp[i]);
+        }
         const long mnnz = mrp[nm];
+        const ProjectPlan pl = project_plan(nq, nm, maxrow, s.cplx ? 1 : 0);
         // scratch layout (each part 256-byte aligned): link list pointer and
-        // indices, link table, mvec row pointer, columns, values, projections.
+        // indices, link table, mvec row pointer, columns, values, segment
+        // partial sums, projections.
         // The size check comes before any read of the CSR arrays.
         const size_t o_lp = 0, o_li = o_lp + align256(sizeof(int) * (nq + 1)),
                      o_lk = o_li + align256(sizeof(int) * (size_t)nqm),
                      o_rp = o_lk + align256(sizeof(int) * (size_t)nq * nm),
                      o_ci = o_rp + align256(sizeof(int) * (size_t)(nm + 1)),
                      o_val = o_ci + align256(sizeof(int) * (size_t)mnnz),
-                     o_pr = o_val + align256(sizeof(double) * (size_t)mnnz * cw),
+                     o_part = o_val + align256(sizeof(double) * (size_t)mnnz * cw),
+                     o_pr = o_part + align256(sizeof(double) * pl.partial_doubles),
                      need = o_pr + align256(sizeof(double) * (size_t)nqm * cw);
         TMI_CHECK(need <= (size_t)TMI_NOALLOC_SCRATCH_BYTES, TMI_ERR_INVALID_ARG,
                   "ProjectAll without allocation needs %zu bytes of scratch for %d sources, %d detectors and %ld mvec "
                   "entries; the library's scratch holds %zu",
                   need, nq, nm, mnnz, (size_t)TMI_NOALLOC_SCRATCH_BYTES);
-        for (int i = 0; i < nm; i++) TMI_CHECK(mrp[i + 1] >= mrp[i], TMI_ERR_INVALID_ARG, "mvec row pointer not ascending");
         for (long k = 0; k < mnnz; k++)
             TMI_CHECK(mci[k] >= 0 && mci[k] < m.n, TMI_ERR_INVALID_ARG, "mvec column out of range");
         std::lock_guard<std::mutex> lk(g_scratch_mu);
@@ -113,7 +120,7 @@ extern "C" int tmi_solver_project_all_device_noalloc(tmi_solver h, int nm, const
         hipStream_t st = s.st;
         int *d_lp = (int *)(base + o_lp), *d_li = (int *)(base + o_li), *d_lk = (int *)(base + o_lk);
         int *d_rp = (int *)(base + o_rp), *d_ci = (int *)(base + o_ci);
-        double *d_val = (double *)(base + o_val), *d_pr = (double *)(base + o_pr);
+        double *d_val = (double *)(base + o_val), *d_part = (double *)(base + o_part), *d_pr = (double *)(base + o_pr);
         TMI_HIP(hipMemcpyAsync(d_lp, qm.lptr.data(), sizeof(int) * (nq + 1), hipMemcpyHostToDevice, st));
         TMI_HIP(hipMemcpyAsync(d_li, qm.lidx.data(), sizeof(int) * (size_t)nqm, hipMemcpyHostToDevice, st));
         TMI_HIP(hipMemcpyAsync(d_rp, mrp, sizeof(int) * (size_t)(nm + 1), hipMemcpyHostToDevice, st));
@@ -124,7 +131,8 @@ extern "C" int tmi_solver_project_all_device_noalloc(tmi_solver h, int nm, const
         const long pairs = (long)nq * nm;
         hipLaunchKernelGGL(k_link_table, dim3(grid_for(pairs, 256)), dim3(256), 0, st, nq, nm, d_lp, d_li, d_lk);
         TMI_LAUNCH_CHECK();
-        launch_project(nq, nm, d_lk, d_rp, d_ci, d_val, (const double *)d_phi, 1, ld, s.cplx ? 1 : 0, d_pr, st);
+        launch_project(pl, nq, nm, d_lk, d_rp, d_ci, d_val, (const double *)d_phi, 1, ld, s.cplx ? 1 : 0, d_part,
+                       d_pr, st);
         TMI_HIP(hipMemcpyAsync(proj, d_pr, sizeof(double) * (size_t)nqm * cw, hipMemcpyDeviceToHost, st));
         TMI_HIP(hipStreamSynchronize(st));
         if (dscale == TMI_DATA_LOG) {   // complex log: Re = ln|y|, Im = arg y

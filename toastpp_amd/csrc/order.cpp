@@ -201,10 +201,28 @@ const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
         std::vector<rb_mask_t> mask;
         row_block_unions(n, prp, pci, ofs, col, mask, rslot);
         order.rb_nb = (int)ofs.size() - 1;
+        // packed words of the pipelined kernel: row mask in the low 16 bits,
+        // offset of the column's first coefficient within its block (column,
+        // then row order) in the high 16 bits; 64 trailing entries so that a
+        // batch's loads never leave the arrays
+        std::vector<unsigned> word(col.size() + 64, 0u);
+        bool fits = true;
+        for (int b = 0; b < order.rb_nb && fits; b++) {
+            unsigned off = 0;
+            for (int u = ofs[b]; u < ofs[b + 1]; u++) {
+                if (off > 0xffffu) { fits = false; break; }
+                word[u] = (unsigned)mask[u] | (off << 16);
+                off += (unsigned)__builtin_popcount((unsigned)mask[u]);
+            }
+        }
+        std::vector<int> colp(col);
+        colp.insert(colp.end(), 64, col.empty() ? 0 : col.back());
         order.rb_ofs.upload(ofs.data(), ofs.size(), st);
-        order.rb_col.upload(col.data(), col.size(), st);
+        order.rb_col.upload(colp.data(), colp.size(), st);
         order.rb_mask.upload(mask.data(), mask.size(), st);
         order.rb_slot.upload(rslot.data(), rslot.size(), st);
+        order.rb_word_ok = fits;
+        if (fits) order.rb_word.upload(word.data(), word.size(), st);
         TMI_HIP(hipStreamSynchronize(st));
     }
     TMI_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope

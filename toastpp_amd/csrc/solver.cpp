@@ -31,9 +31,17 @@ Solver::Exec::Exec(bool new_stream)
     }
 }
 
+void Solver::Exec::drop_graphs()
+{
+    for (Graph &g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 Solver::Exec::~Exec()
 {
     if (st) (void)hipStreamSynchronize(st);
+    drop_graphs();
     if (h_flags) (void)hipHostFree(h_flags);
     if (h_dbl) (void)hipHostFree(h_dbl);
     free_gmres_work(gm);
@@ -80,7 +88,7 @@ size_t Solver::work_bytes(int nq) const
     const bool mixed = mode == TMI_MODE_MIXED_BLOCK;
     const size_t s64 = vbytes(vk64()), sin = vbytes(mixed ? vk32() : vk64());
     auto tiles = [&](size_t s_inner) {
-        return n * Kall * (3 * s64 + 7 * s_inner) + 2 * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
+        return n * Kall * (3 * s64 + 7 * s_inner) + RED_SLOTS * 2048 * Kall * 8 + (S_NSLOT * 2 * 8 + I_NSLOT * 2 * 4) * Kall;
     };
     size_t b = tiles(sin);
     if (mixed && K == 64 && cplx) b += 2 * n * 32;   // packed shadow residuals (both 64-column workspaces)
@@ -94,6 +102,7 @@ size_t Solver::work_bytes(int nq) const
 void Solver::Exec::release_work()
 {
     if (st) TMI_HIP(hipStreamSynchronize(st));
+    drop_graphs();   // their kernel arguments point into the workspaces
     for (Work *ws : {wk, wk_alt})
         for (int i = 0; i < 7; i++) {
             Work &w = ws[i];
@@ -130,8 +139,10 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
         b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
         b.lds.dma = spmm_variant == 2;
     }
-    if (spmm_variant == 3 && o.rb_nb > 0 && rb_src)
+    if ((spmm_variant == 3 || spmm_variant == 4) && o.rb_nb > 0 && rb_src) {
         b.rb = RowUnions{o.rb_nb, o.rb_ofs.p, o.rb_col.p, o.rb_mask.p, rbval32.p, rb_src};
+        if (spmm_variant == 4 && o.rb_word_ok) b.rb.word = o.rb_word.p;
+    }
     return b;
 }
 
@@ -205,7 +216,7 @@ Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md, bool alt)
     size_t nb64 = (size_t)n * K * vbytes(vk64()), nbi = (size_t)n * K * vbytes(vin);
     w.x64.alloc(nb64); w.b64.alloc(nb64); w.r64.alloc(nb64);
     w.xi.alloc(nbi); w.ri.alloc(nbi); w.rdi.alloc(nbi); w.p.alloc(nbi); w.v.alloc(nbi); w.s.alloc(nbi); w.t.alloc(nbi);
-    w.partial.alloc((size_t)2 * G * K);
+    w.partial.alloc((size_t)RED_SLOTS * G * K);
     if (K == 64 && vin == VK_C32) w.rdq.alloc((size_t)n * 32);
     w.sc.alloc((size_t)S_NSLOT * K); w.osc.alloc((size_t)S_NSLOT * K);
     w.ic.alloc((size_t)I_NSLOT * K); w.oic.alloc((size_t)I_NSLOT * K);
@@ -216,6 +227,7 @@ int Solver::count_active(Exec &e, const int *d_flags, int K)
 {
     TMI_HIP(hipMemcpyAsync(e.h_flags, d_flags, sizeof(int) * K, hipMemcpyDeviceToHost, e.st));
     TMI_HIP(hipStreamSynchronize(e.st));
+    e.n_sync++;
     int c = 0;
     for (int i = 0; i < K; i++) c += e.h_flags[i] != 0;
     return c;
@@ -263,7 +275,11 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     const bool mixed = (md == TMI_MODE_MIXED_BLOCK);
     const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
     const void *idiag_in = nullptr;
-    const bool use_cg = (lsolver == TMI_LSOLVER_CG);
+    // PCG (real) and COCG (complex symmetric: PCG with the unconjugated
+    // bilinear form) share the launch sequence; COCG's scalars are complex
+    const bool use_cg = (lsolver == TMI_LSOLVER_CG || lsolver == TMI_LSOLVER_COCG);
+    const bool bil = (lsolver == TMI_LSOLVER_COCG);
+    const int cg_dot = bil ? DOT_BIL : 0;
     // operator of the BiCGSTAB SpMMs: K D^-1 when diagonally preconditioned
     const void *op_in = val_in;
     if (precon == TMI_PRECON_DIAG) {
@@ -328,21 +344,26 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
                     mixed);
     if (rdq) launch_pack_shadow(n, wp->rdi.p, rdq, st);
     if (use_cg) {
-        launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st);
-        launch_finalize(FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
+        launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st, bil);
+        launch_finalize(bil ? FIN_COCG_INIT : FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
     } else {
         launch_finalize(FIN_BICG_INIT, bd, part, sc, ic, 0, 0, 0, st);
     }
     static const bool verbose = getenv("TMI_VERBOSE") != nullptr;
     int it = 0, nupd = 0, since = 0, notconv = 0, ncompact = 0;
-    auto reliable_update = [&]() {
+    // the launches of a reliable update (no host work, so that a captured
+    // period can hold them)
+    auto reliable_launches = [&]() {
         launch_from_inner(vk, bd, wp->x64.p, wp->xi.p, scale, oact, st);
         launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
         launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
         launch_reliable(vk, bd, wp->r64.p, wp->ri.p, wp->xi.p, use_cg ? nullptr : wp->rdi.p, idiag_in, scale, oact,
-                        part, st);
-        launch_finalize(use_cg ? FIN_CG_RELIABLE : FIN_BICG_RELIABLE, bd, part, sc, ic, maxit_total, 0, 0, st, osc,
-                        oic);
+                        part, st, bil);
+        launch_finalize(bil ? FIN_COCG_RELIABLE : use_cg ? FIN_CG_RELIABLE : FIN_BICG_RELIABLE, bd, part, sc, ic,
+                        maxit_total, 0, 0, st, osc, oic);
+    };
+    auto reliable_update = [&]() {
+        reliable_launches();
         nupd++;
         since = 0;
     };
@@ -355,6 +376,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         TMI_HIP(hipMemcpyAsync(e.cols.p, which.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
         launch_scatter_map(s64, n, K, perm, e.cols.p, wp->x64.p, d_x, ld, q0, st);
         TMI_HIP(hipStreamSynchronize(st));
+        e.n_sync++;
         for (int j = 0; j < K; j++) {
             const int c = which[j];
             if (c < 0) continue;
@@ -381,6 +403,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     auto compact = [&](bool repack) -> bool {
         TMI_HIP(hipMemcpyAsync(h_oflags, oact, sizeof(int) * K, hipMemcpyDeviceToHost, st));
         TMI_HIP(hipStreamSynchronize(st));
+        e.n_sync++;
         std::vector<int> done(K, -1), sel;
         for (int j = 0; j < K; j++) {
             if (cmap[j] < 0) continue;
@@ -422,30 +445,17 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         ncompact++;
         return true;
     };
-    for (;;) {
-        const int nact = count_active(e, oact, K);
-        if (nact == 0) break;
-        if (compact_min > 0 && nact <= K / 2 && K / 2 >= compact_min) compact(false);
-        else if (compact_min > 0 && K >= 32 && repack_pays(e.h_flags, K, nact)) compact(true);
-        if (count_active(e, ic + I_ACTIVE * K, K) == 0) {
-            // every column's recursive residual met the tolerance: FP64 modes
-            // stop here (as the reference does); the mixed mode verifies in FP64
-            if (!mixed) break;
-            reliable_update();
-            if (use_cg) launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st);
-            if (count_active(e, oact, K) == 0 || it >= maxit_total) break;
-            continue;
-        }
-        if (it >= maxit_total) break;
-        for (int k = 0; k < CHECK_EVERY; k++, it++, since++) {
+    // the launch sequence of one check period (no host work inside)
+    auto period_launches = [&](bool ru_due) {
+        for (int k = 0; k < CHECK_EVERY; k++) {
             if (use_cg) {
-                launch_spmm_dot(vk, bd, val_in, nullptr, wp->p.p, wp->v.p, wp->p.p, false, ic + I_ACTIVE * K, part, st);
-                launch_finalize(FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st);
-                launch_finalize(FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
-                // PCG replaces the residual before the direction update
-                if (k == CHECK_EVERY - 1 && since + 1 >= ru) reliable_update();
-                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st);
+                launch_spmm_dot(vk, bd, val_in, nullptr, wp->p.p, wp->v.p, wp->p.p, cg_dot, ic + I_ACTIVE * K, part,
+                                st);
+                launch_finalize(bil ? FIN_COCG_ALPHA : FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
+                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st, bil);
+                launch_finalize(bil ? FIN_COCG_BETA : FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
+                if (k == CHECK_EVERY - 1 && ru_due) reliable_launches();
+                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil);
             } else {
                 launch_bicg_p(vk, bd, wp->ri.p, wp->p.p, wp->v.p, sc, ic, st);
                 launch_spmm_dot(vk, bd, op_in, nullptr, wp->p.p, wp->v.p, wp->rdi.p, false, ic + I_ACTIVE * K, part,
@@ -459,7 +469,82 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
                 launch_finalize(FIN_BICG_RHO, bd, part, sc, ic, maxit_total, 0, 0, st);
             }
         }
-        if (!use_cg && since >= ru) reliable_update();
+        if (!use_cg && ru_due) reliable_launches();
+    };
+    // run a check period from an instantiated HIP graph: captured on first
+    // use for this key -- every value its kernels take as argument -- and
+    // replayed afterwards (one graph launch instead of 64-104 kernel launches)
+    auto run_period = [&](bool ru_due) {
+        if (!use_graphs) {
+            period_launches(ru_due);
+            return;
+        }
+        typedef unsigned long long u64;
+        auto P = [](const void *p) { return (u64)(uintptr_t)p; };
+        u64 tol_bits = 0;
+        std::memcpy(&tol_bits, &tol2, sizeof tol_bits);
+        const std::vector<u64> key = {
+            (u64)K, (u64)vk, (u64)lsolver, (u64)ru_due, (u64)maxit_total, tol_bits, (u64)bd.nblk, (u64)bd.rpw,
+            P(bd.rowptr), P(bd.colidx), (u64)bd.lds.nb, P(bd.lds.brow), P(bd.lds.bu), P(bd.lds.ucols),
+            P(bd.lds.lcol), (u64)bd.lds.dma, (u64)bd.rb.nb, P(bd.rb.ofs), P(bd.rb.col), P(bd.rb.mask),
+            P(bd.rb.val), P(bd.rb.src), P(bd.rb.word), P(op_in), P(val_in), P(idiag_in), P(rdq), P(sc), P(ic),
+            P(osc), P(oic), P(part), P(sval64.p), P(wp->x64.p), P(wp->b64.p), P(wp->r64.p), P(wp->xi.p),
+            P(wp->ri.p), P(wp->rdi.p), P(wp->p.p), P(wp->v.p), P(wp->s.p), P(wp->t.p), P(st)};
+        hipGraphExec_t exec = nullptr;
+        for (const Exec::Graph &g : e.graphs)
+            if (g.key == key) { exec = g.exec; break; }
+        if (!exec) {
+            const long long calls = g_launch_calls;
+            TMI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                period_launches(ru_due);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            TMI_HIP(hipStreamEndCapture(st, &graph));
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            TMI_HIP(ie);
+            g_launch_calls = calls;   // captured, not launched
+            if (e.graphs.size() >= 32) {   // keep the cache small: drop the oldest
+                (void)hipGraphExecDestroy(e.graphs.front().exec);
+                e.graphs.erase(e.graphs.begin());
+            }
+            e.graphs.push_back(Exec::Graph{key, exec});
+        }
+        TMI_HIP(hipGraphLaunch(exec, st));
+        e.n_graph++;
+    };
+    for (;;) {
+        const int nact = count_active(e, oact, K);
+        if (nact == 0) break;
+        if (compact_min > 0 && nact <= K / 2 && K / 2 >= compact_min) compact(false);
+        else if (compact_min > 0 && K >= 32 && repack_pays(e.h_flags, K, nact)) compact(true);
+        if (count_active(e, ic + I_ACTIVE * K, K) == 0) {
+            // every column's recursive residual met the tolerance: FP64 modes
+            // stop here (as the reference does); the mixed mode verifies in FP64
+            if (!mixed) break;
+            reliable_update();
+            if (use_cg) launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil);
+            if (count_active(e, oact, K) == 0 || it >= maxit_total) break;
+            continue;
+        }
+        if (it >= maxit_total) break;
+        // one check period: CHECK_EVERY iterations and, when it falls due,
+        // the reliable update (PCG replaces the residual before the last
+        // direction update, BiCGSTAB after the period)
+        const bool ru_due = since + CHECK_EVERY >= ru;
+        run_period(ru_due);
+        it += CHECK_EVERY;
+        if (ru_due) {
+            nupd++;
+            since = use_cg ? 1 : 0;   // PCG's last direction update follows its replacement
+        } else {
+            since += CHECK_EVERY;
+        }
     }
     // fold the last inner correction of columns still marked active (maxit
     // exhausted) and record the final true residuals
@@ -475,8 +560,9 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
 
 void Solver::prepare_row_blocks()
 {
-    // the FP32 operator of the mixed BiCGSTAB SpMMs
-    const float *op = (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) ? pval32.p : sval32.p;
+    // the FP32 operator of the mixed SpMMs: K D^-1 for a diagonally
+    // preconditioned BiCGSTAB, K itself for PCG / COCG
+    const float *op = (precon == TMI_PRECON_DIAG && !pcg()) ? pval32.p : sval32.p;
     if (rb_src == op) return;
     const Mesh::NodeOrder &o = ord();
     rbval32.alloc(nval() + 8);
@@ -490,6 +576,8 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
     TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
+    TMI_CHECK(!(!cplx && lsolver == TMI_LSOLVER_COCG), TMI_ERR_UNSUPPORTED,
+              "COCG solves complex symmetric systems; use CG for a real one");
     if (lsolver == TMI_LSOLVER_GMRES) return gmres_on(e, nq, d_b, d_x, ld, stats);
     if (md < 0) md = mode;
     if (precon == TMI_PRECON_DIAG) {
@@ -499,7 +587,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
             TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
         }
     }
-    if (spmm_variant == 3 && cplx) {
+    if ((spmm_variant == 3 || spmm_variant == 4) && cplx) {
         std::lock_guard<std::mutex> lk(prep_mu);
         prepare_row_blocks();
     }
@@ -517,6 +605,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     TMI_HIP(hipEventCreate(&e1));
     TMI_HIP(hipEventRecord(e0, es_));
     e.last_iters = 0.0;
+    const long long calls0 = g_launch_calls, graphs0 = e.n_graph, syncs0 = e.n_sync;
     int notconv = 0;
     for (int q0 = 0; q0 < nq; q0 += K) {
         int kc = std::min(K, nq - q0);
@@ -559,9 +648,14 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
             if (!c.converged) notconv++;
         }
         TMI_HIP(hipStreamSynchronize(es_));
+        e.n_sync++;
     }
     TMI_HIP(hipEventRecord(e1, es_));
     TMI_HIP(hipEventSynchronize(e1));
+    e.n_sync++;
+    e.last_launch = g_launch_calls - calls0;
+    e.last_graph = e.n_graph - graphs0;
+    e.last_sync = e.n_sync - syncs0;
     float ms = 0.f;
     TMI_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -592,14 +686,16 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
     act.upload(ones.data(), K, st);
     const void *idg = nullptr;
     const void *op = mixed ? (const void *)sval32.p : (const void *)sval64.p;
-    if (precon == TMI_PRECON_DIAG && lsolver != TMI_LSOLVER_CG) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
+    if (precon == TMI_PRECON_DIAG && !pcg()) op = mixed ? (const void *)pval32.p : (const void *)pval64.p;
     // the 64-column complex FP32 BiCGSTAB SpMM reads the packed shadow
-    // residual in the solve, and so here
+    // residual in the solve, and so here; PCG / COCG dot with the direction
     const unsigned char *wq =
-        (mixed && cplx && K == 64 && lsolver != TMI_LSOLVER_CG && w.vk_in == VK_C32 && w.rdq.p) ? w.rdq.p : nullptr;
+        (mixed && cplx && K == 64 && !pcg() && w.vk_in == VK_C32 && w.rdq.p) ? w.rdq.p : nullptr;
+    const void *wv = pcg() ? w.p.p : w.rdi.p;
+    const int dot = lsolver == TMI_LSOLVER_COCG ? DOT_BIL : 0;
     auto launch = [&]() {
         if (which == 0)
-            launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, w.rdi.p, false, act.p, w.partial.p, st, wq);
+            launch_spmm_dot(w.vk_in, bd, op, idg, w.p.p, w.v.p, wv, dot, act.p, w.partial.p, st, wq);
         else
             launch_resid(vk64(), bd, sval64.p, w.x64.p, w.b64.p, w.r64.p, act.p, w.partial.p, st);
     };
@@ -661,13 +757,13 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
         }
         fp64_fallback = fb0;
         struct Cand { int variant; int rpw; };
-        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}};
+        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}, {4, 0}};
         double best = 1e30;
         Cand pick{spmm_variant, rpw};
         for (const Cand &c : cands) {
             if ((c.variant == 0 || c.variant == 2) && ord().nb == 0) continue;
             spmm_variant = c.variant;
-            if (c.variant == 3) {
+            if (c.variant == 3 || c.variant == 4) {
                 std::lock_guard<std::mutex> lk(prep_mu);
                 prepare_row_blocks();
             }

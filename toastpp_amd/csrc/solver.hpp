@@ -17,6 +17,8 @@ struct Solver {
     Mesh *mesh = nullptr;
     bool cplx = true;
     int lsolver = TMI_LSOLVER_BICGSTAB;
+    // PCG and COCG share one launch sequence (one SpMM per iteration)
+    bool pcg() const { return lsolver == TMI_LSOLVER_CG || lsolver == TMI_LSOLVER_COCG; }
     double tol = 1e-10;
     int precon = TMI_PRECON_NONE;
     int maxit = 0;
@@ -71,6 +73,19 @@ struct Solver {
         double *h_dbl = nullptr;       // pinned host mirror of per-column doubles
         GmresWork *gm = nullptr;       // GMRES workspace (allocated by the first GMRES solve)
         double last_solve_ms = 0.0, last_iters = 0.0;
+        // instantiated graphs of the check period's launch sequence, keyed by
+        // every value the captured kernels take as argument
+        struct Graph {
+            std::vector<unsigned long long> key;
+            hipGraphExec_t exec = nullptr;
+        };
+        std::vector<Graph> graphs;
+        void drop_graphs();
+        // host-side work: graph launches and host synchronisations so far, and
+        // of the last solve the kernel launch calls (outside graph capture),
+        // graph launches and synchronisations
+        long long n_graph = 0, n_sync = 0;
+        long long last_launch = 0, last_graph = 0, last_sync = 0;
         Exec() = default;
         explicit Exec(bool new_stream);
         Exec(const Exec &) = delete;
@@ -100,8 +115,13 @@ struct Solver {
     // device bytes a solve of nq columns needs for its workspace
     size_t work_bytes(int nq) const;
     int tile_for(int nq, int md = -1) const;
+    bool use_graphs = true;          // replay the check period's launches from captured HIP graphs
     bool fp64_fallback = true;       // mixed mode: columns that do not converge are re-solved in FP64
     int mixed_maxit = 0;             // mixed mode: iteration cap of the FP32 iteration (0: maxit)
+    // device buffers of ProjectAll (link table, mvec CSR, partial sums,
+    // projections), kept between calls and grown when a call needs more
+    DBuf<int> pj_lk, pj_rp, pj_ci;
+    DBuf<double> pj_val, pj_part, pj_out;
     // Time `reps` launches of one kernel of the last tile (which: 0 = inner
     // SpMM, 1 = FP64 residual SpMM) with HIP events on the solver stream.
     // Returns the average ms and the algorithmic bytes of one launch.
@@ -121,9 +141,5 @@ struct Solver {
     int count_active(Exec &e, const int *d_flags, int K);
 };
 
-// projections (mesh_kernels.hip): proj[k] = sum_j phi[q][j] * mvec[m][j] for
-// the link k = lk[m * nq + q] of every linked pair (lk = -1: not linked)
-void launch_project(int nq, int nm, const int *d_lk, const int *mrp, const int *mci, const double *mval,
-                    const double *phi, long sq, long sj, int cplx, double *proj, hipStream_t st);
 
 }  // namespace tmi

@@ -11,7 +11,7 @@ import numpy as np
 from . import _lib as L
 
 # SpMM variant names, in the order of their C-ABI codes
-SPMM_VARIANTS = ("lds", "direct", "dma", "rowblock")
+SPMM_VARIANTS = ("lds", "direct", "dma", "rowblock", "rowblock_pipelined")
 
 
 def _f64(a):
@@ -39,10 +39,12 @@ class FwdSolver:
         self.complex = bool(complex)
         h = C.c_void_p()
         name = solver.upper()
+        # COCG (complex systems only) has no reference counterpart: the
+        # conjugate orthogonal CG for the complex symmetric diffusion matrix
         ls = {"DIRECT": L.LSOLVER_DIRECT, "CG": L.LSOLVER_CG, "BICGSTAB": L.LSOLVER_BICGSTAB,
-              "GMRES": L.LSOLVER_GMRES}.get(name)
+              "GMRES": L.LSOLVER_GMRES, "COCG": L.LSOLVER_COCG}.get(name)
         if ls is None:
-            raise ValueError("unsupported linear solver %r (DIRECT, CG, BICGSTAB, GMRES)" % solver)
+            raise ValueError("unsupported linear solver %r (DIRECT, CG, BICGSTAB, GMRES, COCG)" % solver)
         L.call("tmi_solver_create", mesh.handle_ptr(), 1 if self.complex else 0, ls, float(tol), C.byref(h))
         self._h = h
         self.stats = None
@@ -180,6 +182,18 @@ class FwdSolver:
         b = C.c_double()
         L.call("tmi_solver_work_bytes", self._h, int(nq), C.byref(b))
         return b.value
+
+    def SetGraphs(self, enable=True):
+        """replay the block iteration's check periods from captured HIP graphs
+        (default) or launch every kernel"""
+        L.call("tmi_solver_set_graphs", self._h, 1 if enable else 0)
+
+    def LastCounts(self):
+        """(kernel launch calls, graph launches, host synchronisations) of the
+        last CalcFields"""
+        k, g, s = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        L.call("tmi_solver_last_counts", self._h, C.byref(k), C.byref(g), C.byref(s))
+        return k.value, g.value, s.value
 
     def LastTiming(self):
         ms, it = C.c_double(), C.c_double()
