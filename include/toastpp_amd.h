@@ -248,6 +248,16 @@ int tmi_solver_calc_fields(tmi_solver s, int nq, const int *qrowptr, const int *
  * (element [j, q] at j*nq + q), complex128 or float64.  d_phi is overwritten
  * (zero initial guess).  Nothing is copied to the host except the stats. */
 int tmi_solver_calc_fields_device(tmi_solver s, int nq, const void *d_rhs, void *d_phi, tmi_colstat *stats);
+/* Several frequencies of one mesh in one block solve (no reference
+ * counterpart: the reference assembles and solves once per frequency,
+ * AssembleSystemMatrix + CalcFields, fwdsolver.cc:405-424, :653-810).
+ * Column q of d_rhs is solved at omega[q] (host array, nq values, the unit
+ * of tmi_solver_reset's omega) with the optical coefficients of the last
+ * reset / assemble: (A + i omega[q] B) phi_q = rhs_q, A and B the real and
+ * mass parts of the system.  Complex solvers with COCG, block modes (mixed,
+ * fp64_block); layouts as tmi_solver_calc_fields_device. */
+int tmi_solver_calc_fields_freqs_device(tmi_solver s, int nq, const void *d_rhs, void *d_phi, const double *omega,
+                                        tmi_colstat *stats);
 /* ProjectAll (fwdsolver.cc:1433-1447, :1415-1429): proj[Qofs[q]+i] =
  * sum_j phi_q[j] mvec[QMref[q][i]][j] (unconjugated), log if
  * dscale == TMI_DATA_LOG.  Uses the mesh's QM link list.  mvec: nm x nlen CSR. */

@@ -103,6 +103,7 @@ _SIGS = {
     "tmi_solver_get_sysmat": [P, P, P, P],
     "tmi_solver_calc_fields": [P, C.c_int, I, I, D, D, P],
     "tmi_solver_calc_fields_device": [P, C.c_int, P, P, P],
+    "tmi_solver_calc_fields_freqs_device": [P, C.c_int, P, P, P, P],
     "tmi_solver_project_all": [P, C.c_int, I, I, D, D, C.c_int, D],
     "tmi_solver_project_all_device": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],
     "tmi_solver_project_all_device_noalloc": [P, C.c_int, I, I, D, P, C.c_int, C.c_int, D],

@@ -741,6 +741,23 @@ int tmi_solver_calc_fields_device(tmi_solver h, int nq, const void *d_rhs, void 
     API_END
 }
 
+int tmi_solver_calc_fields_freqs_device(tmi_solver h, int nq, const void *d_rhs, void *d_phi, const double *omega,
+                                        tmi_colstat *stats)
+{
+    API_BEGIN
+    Solver &s = S(h);
+    NEED(d_rhs); NEED(d_phi); NEED(omega);
+    TMI_CHECK(nq > 0, TMI_ERR_INVALID_ARG, "source count must be positive");
+    size_t bytes = (size_t)s.mesh->n * nq * (s.cplx ? 16 : 8);
+    TMI_HIP(hipMemsetAsync(d_phi, 0, bytes, s.st));
+    int notconv = s.solve_on(s.ex, nq, d_rhs, d_phi, nq, stats, -1, omega);
+    if (notconv) {
+        set_last_error(std::to_string(notconv) + " right-hand side(s) did not reach the tolerance");
+        return TMI_ERR_NOT_CONVERGED;
+    }
+    API_END
+}
+
 // ProjectAll on fields held either on the host ([nq][n], source-major as the
 // reference's phi array) or on the device ([n][ld] node-major, block-solve
 // layout): sq / sj are the strides of source and node in complex entries

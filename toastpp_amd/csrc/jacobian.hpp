@@ -5,23 +5,20 @@
 namespace tmi {
 
 // d_dphi [n][ldd], d_aphi [n][lda] complex (node-major, the first nq / nm
-// entries of each row used); d_lptr: link-list
-// row pointer (nq+1); d_lq / d_lm: source and detector of every linked
-// measurement (link-list order, source-major); d_proj: nqm
-// complex linear projections; d_Pt: scratch of n*nqm*4 doubles (read only
+// entries of each row used); the links come from the mesh's link table
+// m.d_lk (link of source q and detector m at q nm + m, or -1); d_proj: nqm
+// complex linear projections; d_Pt: scratch of n*nqm*4 doubles (used only
 // by the two-pass variant); d_J: (2 nqm) x (2 n) row-major.  dmax: longest
 // CSR row of the mesh.
 // cw: CW Jacobian of real fields given with zero imaginary parts; d_proj is
 // then nqm real projections, d_J the nqm x n mua block and d_Jkap the
 // nqm x n kappa block (either may be null).
-// nodes per pass of the two-pass nodal Jacobian (node-major intermediate of
-// that many nodes, then a tiled transpose into J); 0 when the node kernel
-// writes J directly.  d_Pt must hold chunk x nqm x 4 doubles.
-int jacobian_chunk_nodes(int n, int nqm);
+// true when the environment sets TMI_JAC_MODE=two: node-major intermediate
+// (d_Pt) plus tiled transpose, instead of the node kernel writing J
+bool jacobian_two_pass();
 void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
-                           const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
-                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw = false,
-                           double *d_Jkap = nullptr);
+                           int nqm, const double *d_proj, int logscale, double *d_Pt, double *d_J, int dmax,
+                           hipStream_t st, bool cw = false, double *d_Jkap = nullptr);
 // Element-basis Jacobian: d_J (2 nqm) x (2 elen) row-major, or with cw the
 // nqm x elen blocks d_J (mua) and d_Jkap (kappa, either may be null) from real
 // fields with zero imaginary parts and nqm real projections.  mvec_n > 0: the

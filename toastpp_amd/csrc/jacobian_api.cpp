@@ -85,11 +85,11 @@ static void jacobian_nodal_dev(Mesh &m, int nq, int nm, const double *d_dphi, in
     TMI_CHECK(dscale == TMI_DATA_LIN || dscale == TMI_DATA_LOG, TMI_ERR_INVALID_ARG, "unknown data scale %d", dscale);
     const int dmax = mesh_dmax(m, st);
     ensure_links(m, st);
-    // the node-major intermediate of one pass of the two-pass variant
-    const size_t ws = (size_t)jacobian_chunk_nodes(m.n, qm.nqm) * qm.nqm * 4;
+    // the node-major intermediate of the two-pass variant
+    const size_t ws = jacobian_two_pass() ? (size_t)m.n * qm.nqm * 4 : 0;
     if (m.jac_ws.n < ws) m.jac_ws.alloc(ws);
-    launch_jacobian_nodal(m, nq, nm, d_dphi, ldd, d_aphi, lda, m.d_lptr.p, m.d_lq.p, m.d_lm.p, qm.nqm, d_proj,
-                          dscale == TMI_DATA_LOG ? 1 : 0, m.jac_ws.p, d_J, dmax, st, cw, d_Jkap);
+    launch_jacobian_nodal(m, nq, nm, d_dphi, ldd, d_aphi, lda, qm.nqm, d_proj, dscale == TMI_DATA_LOG ? 1 : 0,
+                          m.jac_ws.p, d_J, dmax, st, cw, d_Jkap);
 }
 
 // CW inputs on the host: real node-major fields and projections, uploaded and

@@ -35,16 +35,7 @@ constexpr int JAC_NT = 256;
 #ifndef TMI_JAC_LDS_KIB
 #define TMI_JAC_LDS_KIB 40
 #endif
-static int jac_lds_kib()
-{
-    static const int v = getenv("TMI_JAC_LDS_KIB") ? atoi(getenv("TMI_JAC_LDS_KIB")) : TMI_JAC_LDS_KIB;
-    return v;
-}
-static bool jac_block()
-{
-    static const bool v = getenv("TMI_JAC_BLOCK") ? atoi(getenv("TMI_JAC_BLOCK")) != 0 : true;
-    return v;
-}
+constexpr int JAC_LDS_KIB = TMI_JAC_LDS_KIB;
 
 // J and the node-major intermediate are written once and read at most once:
 // streaming (nontemporal) accesses keep them out of the caches
@@ -61,8 +52,6 @@ __device__ inline double2 ld_stream(const double2 *p)
 
 struct JacDims {
     int n, nq, nm, nqm, dmax, chunk, ldd, lda;
-    int node0;    // two-pass: first node of this launch (the intermediate holds its nodes only)
-    int cached;   // two-pass: plain stores (the intermediate is read back from the caches), else streaming
 };
 
 __device__ inline int find_pos(const int *cols, int d, int v)
@@ -135,7 +124,7 @@ __device__ inline int xcd_node(int b, int n)
     return x * q + min(x, r) + s;
 }
 
-template <int DIM, bool DIRECT, bool BLK>
+template <int DIM, bool DIRECT>
 __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__restrict__ rowptr,
                                                   const int *__restrict__ colidx, const int *__restrict__ n2e_ptr,
                                                   const int *__restrict__ n2e, const int *__restrict__ el,
@@ -149,7 +138,7 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
     constexpr int NN = DIM + 1;
     constexpr int GS = DIM == 3 ? GEO3 : GEO2;
     extern __shared__ double lds[];
-    const int i = DIRECT ? xcd_node(blockIdx.x, jd.n) : jd.node0 + blockIdx.x;
+    const int i = DIRECT ? xcd_node(blockIdx.x, jd.n) : blockIdx.x;
     const int tid = threadIdx.x;
     const int dmax = jd.dmax, nq = jd.nq, nm = jd.nm, C = jd.chunk;
     double *A = lds, *B = A + dmax * dmax;
@@ -200,7 +189,7 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
             __syncthreads();
         }
     }
-    double2 *out = Pt + (size_t)(i - jd.node0) * jd.nqm * 2;
+    double2 *out = Pt + (size_t)i * jd.nqm * 2;
     for (int m0 = 0; m0 < nm; m0 += C) {
         const int mc = min(C, nm - m0);
         // detector chunk G on the neighbour set
@@ -235,75 +224,26 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
             __syncthreads();
             {
 #pragma clang fp contract(fast)
-                // P(q,m) = F_q . H_m over 2 x 2 blocks (q1, q2) x (m1, m2) of
-                // the chunk pair: each LDS read of F, H or K serves two
-                // links; a block with no link is skipped, unlinked pairs of a
-                // block are computed and not stored (lk[q nm + m] = -1), and
-                // a missing second row or column repeats the first
-                const int bq = (qc + 1) / 2, bm = (mc + 1) / 2;
-                if constexpr (!BLK) {
-                    // one thread per linked pair of the chunk pair
-                    for (int b = tid; b < qc * mc; b += JAC_NT) {
-                        const int r = b / mc, cc = b % mc;
-                        const int k = lk[(size_t)(q0 + r) * nm + m0 + cc];
-                        if (k < 0) continue;
-                        const double2 *f = W + r * dmax, *h = H + cc * dmax, *kk = Kk + cc * dmax;
-                        double mr = 0.0, mi = 0.0, pr = 0.0, pi = 0.0;
-                        for (int j = 0; j < d; j++) {
-                            const double2 fv = f[j], hv = h[j], kv = kk[j];
-                            mr += fv.x * hv.x - fv.y * hv.y;
-                            mi += fv.x * hv.y + fv.y * hv.x;
-                            pr += fv.x * kv.x - fv.y * kv.y;
-                            pi += fv.x * kv.y + fv.y * kv.x;
-                        }
-                        if constexpr (DIRECT) {
-                            jac_emit(jo, jd.n, jd.nqm, i, k, mr, mi, pr, pi);
-                        } else if (jd.cached) {
-                            out[2 * k] = make_double2(mr, mi);
-                            out[2 * k + 1] = make_double2(pr, pi);
-                        } else {
-                            st_stream(out + 2 * k, make_double2(mr, mi));
-                            st_stream(out + 2 * k + 1, make_double2(pr, pi));
-                        }
-                    }
-                } else
-                for (int b = tid; b < bq * bm; b += JAC_NT) {
-                    const int r1 = 2 * (b / bm), c1 = 2 * (b % bm);
-                    const int r2 = min(r1 + 1, qc - 1), c2 = min(c1 + 1, mc - 1);
-                    const int *lkq1 = lk + (size_t)(q0 + r1) * nm + m0, *lkq2 = lk + (size_t)(q0 + r2) * nm + m0;
-                    const int k11 = lkq1[c1], k12 = c2 > c1 ? lkq1[c2] : -1;
-                    const int k21 = r2 > r1 ? lkq2[c1] : -1, k22 = (r2 > r1 && c2 > c1) ? lkq2[c2] : -1;
-                    if ((k11 & k12 & k21 & k22) < 0) continue;   // all four negative: nothing linked
-                    const double2 *f1 = W + r1 * dmax, *f2 = W + r2 * dmax;
-                    const double2 *h1 = H + c1 * dmax, *h2 = H + c2 * dmax;
-                    const double2 *g1 = Kk + c1 * dmax, *g2 = Kk + c2 * dmax;
-                    double a[4][4] = {};   // [pair 11, 12, 21, 22][mr, mi, pr, pi]
+                // P(q,m) = F_q . H_m: one thread per pair of the chunk pair,
+                // linked pairs only (lk[q nm + m] = link or -1)
+                for (int b = tid; b < qc * mc; b += JAC_NT) {
+                    const int r = b / mc, cc = b % mc;
+                    const int k = lk[(size_t)(q0 + r) * nm + m0 + cc];
+                    if (k < 0) continue;
+                    const double2 *f = W + r * dmax, *h = H + cc * dmax, *kk = Kk + cc * dmax;
+                    double mr = 0.0, mi = 0.0, pr = 0.0, pi = 0.0;
                     for (int j = 0; j < d; j++) {
-                        const double2 fa = f1[j], fb = f2[j], ha = h1[j], hb = h2[j], ka = g1[j], kb = g2[j];
-                        const double2 *fv[2] = {&fa, &fb}, *hv[2] = {&ha, &hb}, *kv[2] = {&ka, &kb};
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const double2 F = *fv[u >> 1], Hh = *hv[u & 1], Kv = *kv[u & 1];
-                            a[u][0] += F.x * Hh.x - F.y * Hh.y;
-                            a[u][1] += F.x * Hh.y + F.y * Hh.x;
-                            a[u][2] += F.x * Kv.x - F.y * Kv.y;
-                            a[u][3] += F.x * Kv.y + F.y * Kv.x;
-                        }
+                        const double2 fv = f[j], hv = h[j], kv = kk[j];
+                        mr += fv.x * hv.x - fv.y * hv.y;
+                        mi += fv.x * hv.y + fv.y * hv.x;
+                        pr += fv.x * kv.x - fv.y * kv.y;
+                        pi += fv.x * kv.y + fv.y * kv.x;
                     }
-                    const int ks[4] = {k11, k12, k21, k22};
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int k = ks[u];
-                        if (k < 0) continue;
-                        if constexpr (DIRECT) {
-                            jac_emit(jo, jd.n, jd.nqm, i, k, a[u][0], a[u][1], a[u][2], a[u][3]);
-                        } else if (jd.cached) {
-                            out[2 * k] = make_double2(a[u][0], a[u][1]);
-                            out[2 * k + 1] = make_double2(a[u][2], a[u][3]);
-                        } else {
-                            st_stream(out + 2 * k, make_double2(a[u][0], a[u][1]));
-                            st_stream(out + 2 * k + 1, make_double2(a[u][2], a[u][3]));
-                        }
+                    if constexpr (DIRECT) {
+                        jac_emit(jo, jd.n, jd.nqm, i, k, mr, mi, pr, pi);
+                    } else {
+                        st_stream(out + 2 * k, make_double2(mr, mi));
+                        st_stream(out + 2 * k + 1, make_double2(pr, pi));
                     }
                 }
             }
@@ -319,19 +259,18 @@ __global__ void __launch_bounds__(JAC_NT) k_jac_nodes(JacDims jd, const int *__r
 constexpr int TJK = 16;
 __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const double2 *__restrict__ Pt,
                                                        const double2 *__restrict__ proj, int logscale,
-                                                       double *__restrict__ J, int node0, int nend, int cached)
+                                                       double *__restrict__ J)
 {
     __shared__ double2 tm[64][TJK + 1], tk[64][TJK + 1];
-    const int i0 = node0 + blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
     const int t = threadIdx.x;
-    // read: node r, measurement c (2 values each); nodes node0 .. nend-1
+    // read: node r, measurement c (2 values each)
     for (int e = t; e < 64 * TJK; e += 256) {
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
-        if (i < nend && k < nqm) {
-            const double2 *src = Pt + ((size_t)(i - node0) * nqm + k) * 2;
-            tm[r][c] = cached ? src[0] : ld_stream(src);
-            tk[r][c] = cached ? src[1] : ld_stream(src + 1);
+        if (i < n && k < nqm) {
+            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2);
+            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1);
         }
     }
     __syncthreads();
@@ -339,7 +278,7 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
     for (int e = t; e < 64 * TJK; e += 256) {
         const int c = e / 64, r = e % 64;
         const int k = k0 + c, i = i0 + r;
-        if (k < nqm && i < nend) {
+        if (k < nqm && i < n) {
             double2 a = tm[r][c], b = tk[r][c];
             if (logscale) {
                 const double2 p = proj[k];
@@ -361,26 +300,24 @@ __global__ void __launch_bounds__(256) k_jac_transpose(int n, int nqm, const dou
 // factor is dropped); either output may be absent.  Same tiling as above.
 __global__ void __launch_bounds__(256) k_jac_transpose_cw(int n, int nqm, const double2 *__restrict__ Pt,
                                                           const double *__restrict__ proj, int logscale,
-                                                          double *__restrict__ Jmua, double *__restrict__ Jkap,
-                                                          int node0, int nend, int cached)
+                                                          double *__restrict__ Jmua, double *__restrict__ Jkap)
 {
     __shared__ double tm[64][TJK + 1], tk[64][TJK + 1];
-    const int i0 = node0 + blockIdx.x * 64, k0 = blockIdx.y * TJK;
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * TJK;
     const int t = threadIdx.x;
     for (int e = t; e < 64 * TJK; e += 256) {
         const int r = e / TJK, c = e % TJK;
         const int i = i0 + r, k = k0 + c;
-        if (i < nend && k < nqm) {
-            const double2 *src = Pt + ((size_t)(i - node0) * nqm + k) * 2;
-            tm[r][c] = (cached ? src[0] : ld_stream(src)).x;
-            tk[r][c] = (cached ? src[1] : ld_stream(src + 1)).x;
+        if (i < n && k < nqm) {
+            tm[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2).x;
+            tk[r][c] = ld_stream(Pt + ((size_t)i * nqm + k) * 2 + 1).x;
         }
     }
     __syncthreads();
     for (int e = t; e < 64 * TJK; e += 256) {
         const int c = e / 64, r = e % 64;
         const int k = k0 + c, i = i0 + r;
-        if (k < nqm && i < nend) {
+        if (k < nqm && i < n) {
             double a = -tm[r][c], b = -tk[r][c];
             if (logscale) {
                 const double ip = 1.0 / proj[k];
@@ -530,29 +467,20 @@ void launch_real_to_cplx(long rows, int cols, const double *src, int ld, double 
 }
 
 // How the nodal Jacobian kernel's output reaches J (TMI_JAC_MODE, read
-// once): "direct" -- the node kernel writes J itself (scattered 8-byte
-// stores); "two" -- a node-major intermediate of all nodes (n nqm 32 B,
-// streaming stores) and one tiled transpose; "chunk:<nodes>" or the default
-// -- the two passes alternate over node chunks whose intermediate stays in
-// the device caches (plain stores and loads), the default chunk holding at
-// most 128 MiB of intermediate.  Returns the nodes per pass (0: direct).
-int jacobian_chunk_nodes(int n, int nqm)
+// once): by default the node kernel writes J itself (nodes dealt to the XCDs
+// in contiguous runs, so that the 8-byte entries of neighbouring nodes fill
+// cache lines in one L2); "two" keeps the earlier layout, a node-major
+// intermediate of n nqm 32 B (streaming stores) and one tiled transpose.
+bool jacobian_two_pass()
 {
-    static const char *mode = getenv("TMI_JAC_MODE");
-    if (mode && !strcmp(mode, "direct")) return 0;
-    if (mode && !strcmp(mode, "two")) return n;
-    long c = 0;
-    if (mode && !strncmp(mode, "chunk:", 6)) c = atol(mode + 6);
-    if (c <= 0) c = (128L << 20) / std::max<long>(1, (long)nqm * 32);
-    c = std::max<long>(64, c / 64 * 64);
-    return (int)std::min<long>(c, n);
+    static const bool two = getenv("TMI_JAC_MODE") && !strcmp(getenv("TMI_JAC_MODE"), "two");
+    return two;
 }
 
 void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, int ldd, const double *d_aphi, int lda,
-                           const int *d_lptr, const int *d_lq, const int *d_lm, int nqm, const double *d_proj,
-                           int logscale, double *d_Pt, double *d_J, int dmax, hipStream_t st, bool cw, double *d_Jkap)
+                           int nqm, const double *d_proj, int logscale, double *d_Pt, double *d_J, int dmax,
+                           hipStream_t st, bool cw, double *d_Jkap)
 {
-    (void)d_lptr; (void)d_lq; (void)d_lm;   // the kernel reads the link table m.d_lk
     // LDS row stride: the neighbour count rounded up to odd, so that the
     // rows of A, B, W, H read by consecutive lanes start in different banks
     dmax |= 1;
@@ -561,47 +489,32 @@ void launch_jacobian_nodal(const Mesh &m, int nq, int nm, const double *d_dphi, 
     // (at least 8 when the LDS of one CU allows)
     const int want = std::max(nq, nm);
     int chunk = want;
-    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > (size_t)jac_lds_kib() * 1024) chunk--;
+    while (chunk > 1 && jac_lds_doubles(dmax, chunk) * sizeof(double) > JAC_LDS_KIB * 1024) chunk--;
     if (chunk < std::min(want, 8))
         while (chunk < want && chunk < 8 && jac_lds_doubles(dmax, chunk + 1) * sizeof(double) <= 160 * 1024) chunk++;
     const size_t lds = jac_lds_doubles(dmax, chunk) * sizeof(double);
     TMI_CHECK(lds <= 160 * 1024, TMI_ERR_UNSUPPORTED,
               "nodal Jacobian: %d neighbours per node exceed the LDS of one CU", dmax);
+    const JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda};
     const JacOut jo{d_J, d_Jkap, d_proj, logscale, cw ? 1 : 0};
-    const int cn = jacobian_chunk_nodes(m.n, nqm);
-    if (cn == 0) {
-        const JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda, 0, 0};
-#define JN(D, BL)                                                                                                    \
-    hipLaunchKernelGGL((k_jac_nodes<D, true, BL>), dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p,      \
-                       m.n2e_ptr.p, m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi,     \
-                       m.d_lk.p, (double2 *)nullptr, jo);
-        if (m.dim == 3) { if (jac_block()) JN(3, true) else JN(3, false) }
-        else { if (jac_block()) JN(2, true) else JN(2, false) }
-#undef JN
-        TMI_LAUNCH_CHECK();
-        return;
-    }
-    const int cached = cn < m.n ? 1 : 0;
-    for (int n0 = 0; n0 < m.n; n0 += cn) {
-        const int cnt = std::min(cn, m.n - n0);
-        const JacDims jd{m.n, nq, nm, nqm, dmax, chunk, ldd, lda, n0, cached};
-#define JN(D, BL)                                                                                                    \
-    hipLaunchKernelGGL((k_jac_nodes<D, false, BL>), dim3(cnt), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p,     \
+    const bool two = jacobian_two_pass();
+#define JN(D, DIR)                                                                                                  \
+    hipLaunchKernelGGL((k_jac_nodes<D, DIR>), dim3(m.n), dim3(JAC_NT), lds, st, jd, m.rowptr.p, m.colidx.p,          \
                        m.n2e_ptr.p, m.n2e.p, m.elist.p, m.geo.p, (const double2 *)d_dphi, (const double2 *)d_aphi,     \
                        m.d_lk.p, (double2 *)d_Pt, jo);
-        if (m.dim == 3) { if (jac_block()) JN(3, true) else JN(3, false) }
-        else { if (jac_block()) JN(2, true) else JN(2, false) }
+    if (m.dim == 3) { if (two) JN(3, false) else JN(3, true) }
+    else { if (two) JN(2, false) else JN(2, true) }
 #undef JN
-        TMI_LAUNCH_CHECK();
-        dim3 g((cnt + 63) / 64, (nqm + TJK - 1) / TJK);
-        if (cw)
-            hipLaunchKernelGGL(k_jac_transpose_cw, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt, d_proj,
-                               logscale, d_J, d_Jkap, n0, n0 + cnt, cached);
-        else
-            hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
-                               (const double2 *)d_proj, logscale, d_J, n0, n0 + cnt, cached);
-        TMI_LAUNCH_CHECK();
-    }
+    TMI_LAUNCH_CHECK();
+    if (!two) return;
+    dim3 g((m.n + 63) / 64, (nqm + TJK - 1) / TJK);
+    if (cw)
+        hipLaunchKernelGGL(k_jac_transpose_cw, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt, d_proj, logscale,
+                           d_J, d_Jkap);
+    else
+        hipLaunchKernelGGL(k_jac_transpose, g, dim3(256), 0, st, m.n, nqm, (const double2 *)d_Pt,
+                           (const double2 *)d_proj, logscale, d_J);
+    TMI_LAUNCH_CHECK();
 }
 
 }  // namespace tmi

@@ -16,7 +16,9 @@ namespace tmi {
 // (the _I slots hold the imaginary parts of COCG's complex scalars)
 enum ScalarSlot {
     S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BNORM2, S_RES2, S_TOL2, S_SCALE, S_AUX, S_RHOP, S_BEST,
-    S_RHO_I, S_ALPHA_I, S_BETA_I, S_RHOP_I, S_NSLOT
+    S_RHO_I, S_ALPHA_I, S_BETA_I, S_RHOP_I,
+    S_FREQ,   // outer: the column's omega in a multi-frequency solve
+    S_NSLOT
 };
 // offsets into the per-column int array
 enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_NSLOT };
@@ -94,10 +96,13 @@ inline size_t vbytes(VKind k) { return k == VK_C64 ? 16 : (k == VK_R32 ? 4 : 8);
 constexpr int DOT_BIL = 2;
 void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
                      const void *W, int want_yy, const int *iflags, double *partial, hipStream_t st,
-                     const unsigned char *Wq = nullptr);
+                     const unsigned char *Wq = nullptr, const double *omg = nullptr);
 // R = B - A X (FP64), red[0] = ||R||^2
+// omg (complex vectors, K >= 2): per-column frequencies -- val then holds
+// the operator with its imaginary part at omega = 1, and column c applies
+// re + i omg[c] im (a multi-frequency block solve of one mesh)
 void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
-                  const int *iflags, double *partial, hipStream_t st);
+                  const int *iflags, double *partial, hipStream_t st, const double *omg = nullptr);
 
 // BiCGSTAB element-wise steps (real inner products, real alpha/beta/omega)
 void launch_bicg_p(VKind vk, const BlockDesc &b, const void *r, void *p, const void *v, const double *sc,
@@ -113,13 +118,16 @@ void launch_pack_shadow(int n, const void *rd, unsigned char *q, hipStream_t st)
 // PCG element-wise steps.  bil (complex vectors only): COCG -- complex
 // alpha / beta, and the partials are the unconjugated r^T D^-1 r (re, im)
 // and ||r||^2 in three slots
+// omg (COCG only): per-column frequencies, idiag then holds the diagonal
+// itself with its imaginary part at omega = 1
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
-                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil = false);
+                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil = false,
+                  const double *omg = nullptr);
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st, bool bil = false);
+                 const int *ic, hipStream_t st, bool bil = false, const double *omg = nullptr);
 // first PCG step: d = D^-1 r, partials [<r, D^-1 r>, <r, r>]
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                    const int *ic, double *partial, hipStream_t st, bool bil = false);
+                    const int *ic, double *partial, hipStream_t st, bool bil = false, const double *omg = nullptr);
 
 // scalar updates (one thread per column)
 enum FinKind {
@@ -161,7 +169,7 @@ void launch_to_inner(VKind vk32, const BlockDesc &b, const void *r64, void *r32,
 // partials [<r_in, r_in>, <rd, r_in>] (rd given) or [<r_in,r_in>, <r_in, D^-1 r_in>]
 void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
                      const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st,
-                     bool bil = false);
+                     bool bil = false, const double *omg = nullptr);
 // x64 += x32 / scale[c] for outer-active columns
 void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x32, const double *scale,
                        const int *oflags, hipStream_t st);
@@ -169,6 +177,8 @@ void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x3
 void launch_norm2(VKind vk, const BlockDesc &b, const void *X, const int *flags, double *partial, hipStream_t st);
 // inverse diagonal (precon_imp.hpp:56-65): idiag[i] = 1/K[i,i] (1 where 0)
 void launch_inv_diag(VKind vk, int n, const int *diagpos, const double *val64, void *idiag, hipStream_t st);
+// the complex diagonal itself (vk complex)
+void launch_get_diag(VKind vk, int n, const int *diagpos, const double *val64, void *dg, hipStream_t st);
 // A D^-1: out64[k] = val64[k] * idiag[colidx[k]] (and an FP32 copy in out32)
 void launch_scale_cols(bool cplx, int nnz, const int *colidx, const double *val64, const void *idiag64, double *out64,
                        void *out32, hipStream_t st);

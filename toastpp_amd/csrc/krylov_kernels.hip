@@ -55,6 +55,33 @@ __device__ inline double2 vfma(double2 acc, double2 a, double2 b)
 __device__ inline float vfma(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
 __device__ inline double vfma(double acc, double a, double b) { return __builtin_fma(a, b, acc); }
 __device__ inline double vmul(double a, double b) { return a * b; }
+
+// scalar type of a vector element type
+template <class V> struct ScalarOf { using type = V; };
+template <> struct ScalarOf<float2> { using type = float; };
+template <> struct ScalarOf<double2> { using type = double; };
+// Multi-frequency operator (MF): the stored coefficient holds the real part
+// and the imaginary part at omega = 1 (the mass term); a column at omega w
+// uses re + i w im.  Real types and MF = false pass the coefficient through.
+template <bool MF, class V>
+__device__ inline V mfc(V c, typename ScalarOf<V>::type w)
+{
+    if constexpr (MF && (std::is_same<V, float2>::value || std::is_same<V, double2>::value)) {
+        c.y *= w;
+        return c;
+    } else {
+        (void)w;
+        return c;
+    }
+}
+// the column's omega (1 without MF)
+template <bool MF, class V>
+__device__ inline typename ScalarOf<V>::type mf_omega(const double *omg, int c)
+{
+    using S = typename ScalarOf<V>::type;
+    if constexpr (MF) return (S)omg[c];
+    else return (S)1;
+}
 __device__ inline float2 vscale(float2 a, double s) { float f = (float)s; return make_float2(a.x * f, a.y * f); }
 __device__ inline double2 vscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ inline float vscale(float a, double s) { return a * (float)s; }
@@ -233,9 +260,10 @@ __device__ inline void block_reduce3(double d1, double d2, double d3, double *pa
 #endif
 constexpr int CH = TMI_DOT_CH;
 
-template <class V, int K, bool PRE>
+template <class V, int K, bool PRE, bool MF = false>
 __device__ inline V row_gather(int kb, int ke, int c, const int *__restrict__ colidx, const V *__restrict__ val,
-                               const V *__restrict__ idiag, const V *__restrict__ X)
+                               const V *__restrict__ idiag, const V *__restrict__ X,
+                               typename ScalarOf<V>::type w = 1)
 {
     V acc = vzero<V>();
     for (int k0 = kb; k0 < ke; k0 += CH) {
@@ -255,17 +283,18 @@ __device__ inline V row_gather(int kb, int ke, int c, const int *__restrict__ co
             if (PRE) xx[u] = vmul(idiag[jj[u]], xx[u]);
         }
 #pragma unroll
-        for (int u = 0; u < CH; u++) acc = vadd(acc, vmul(aa[u], xx[u]));
+        for (int u = 0; u < CH; u++) acc = vadd(acc, vmul(mfc<MF>(aa[u], w), xx[u]));
     }
     return acc;
 }
 
-template <class V, int K, bool PRE>
+template <class V, int K, bool PRE, bool MF = false>
 __global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__ rowptr,
                                                   const int *__restrict__ colidx, const V *__restrict__ val,
                                                   const V *__restrict__ idiag, const V *__restrict__ X,
                                                   V *__restrict__ Y, const V *__restrict__ W, int want_yy,
-                                                  const int *__restrict__ active, double *__restrict__ partial)
+                                                  const int *__restrict__ active, double *__restrict__ partial,
+                                                  const double *__restrict__ omg = nullptr)
 {
     constexpr int R = 64 / K;
     const int G = gridDim.x;
@@ -278,7 +307,8 @@ __global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__
     for (int base = rs.first; base < rs.end; base += rs.stride) {
         const int i = base + rsub;
         if (i < rs.end && act) {
-            V acc = row_gather<V, K, PRE>(rowptr[i], rowptr[i + 1], c, colidx, val, idiag, X);
+            V acc = row_gather<V, K, PRE, MF>(rowptr[i], rowptr[i + 1], c, colidx, val, idiag, X,
+                                              mf_omega<MF, V>(omg, c));
             Y[(size_t)i * K + c] = acc;
             dot_epi(W[(size_t)i * K + c], acc, want_yy, d1, d2);
         }
@@ -315,16 +345,18 @@ __device__ inline float rdlane(float v, int l)
 }
 __device__ inline double rdlane(double v, int l) { return rdlane_d(v, l); }
 
-template <class V, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+template <class V, int MODE, bool MF = false>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
 __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
                                                 const V *__restrict__ val, const V *__restrict__ X,
                                                 V *__restrict__ Y, const V *__restrict__ W, int want_yy,
-                                                const int *__restrict__ active, double *__restrict__ partial, int R)
+                                                const int *__restrict__ active, double *__restrict__ partial, int R,
+                                                const double *__restrict__ omg = nullptr)
 {
     constexpr int B = TMI_SPMM64_BATCH;
     const int lane = threadIdx.x & 63;
     const int c = lane;
     const bool act = active[c] != 0;
+    const auto wmf = mf_omega<MF, V>(omg, c);
     const int G = gridDim.x, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
@@ -385,7 +417,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *
                     if (t0 + u < m) {
                         const int k = seg + t0 + u;
                         while (k >= rend && row < nr) flush();
-                        acc = vadd(acc, vmul(rdlane(vv, t0 + u), xb[u]));
+                        acc = vadd(acc, vmul(mfc<MF>(rdlane(vv, t0 + u), wmf), xb[u]));
                     }
                 }
             }
@@ -432,16 +464,18 @@ constexpr int NK_KMIN = 16;
 __host__ __device__ constexpr int nk_k(int K) { return K >= NK_KMIN && K <= 32 ? K : NK_KMIN; }
 __host__ __device__ constexpr int nk_rows(int K) { return (64 / K) * NK_RSMAX <= 63 ? NK_RSMAX : 63 / (64 / K); }
 
-template <class V, int K, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+template <class V, int K, int MODE, bool MF = false>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
 __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
                                                  const V *__restrict__ val, const V *__restrict__ X,
                                                  V *__restrict__ Y, const V *__restrict__ W, int want_yy,
-                                                 const int *__restrict__ active, double *__restrict__ partial)
+                                                 const int *__restrict__ active, double *__restrict__ partial,
+                                                 const double *__restrict__ omg = nullptr)
 {
     constexpr int S = 64 / K, RS = nk_rows(K), RW = S * RS, B = K < TMI_NK_BATCH ? K : TMI_NK_BATCH;
     static_assert(K >= 2 && K <= 32 && RW <= 63, "narrow SpMM: 2 <= K <= 32");
     const int lane = threadIdx.x & 63, c = lane % K, s = lane / K, sb = s * K;
     const bool act = active[c] != 0;
+    const auto wmf = mf_omega<MF, V>(omg, c);
     const int G = gridDim.x, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
@@ -503,7 +537,7 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
                     if (t0 + u < m) {
                         const int k = kb + o + t0 + u;
                         while (k >= rend && row < nr) flush();
-                        acc = vadd(acc, vmul(ab[u], xb[u]));
+                        acc = vadd(acc, vmul(mfc<MF>(ab[u], wmf), xb[u]));
                     }
                 }
             }
@@ -525,18 +559,20 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
 // refilled from a prefetched next window, all broadcast with readlane.  Each
 // row accumulates acc + val * x over its columns in ascending order, the
 // same operation sequence as the per-row kernels.
-template <class V, int MODE>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+template <class V, int MODE, bool MF = false>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
 __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int *__restrict__ rowptr,
                                                            const int *__restrict__ bofs, const int *__restrict__ ucol,
                                                            const rb_mask_t *__restrict__ umask,
                                                            const V *__restrict__ rbval, const V *__restrict__ X,
                                                            V *__restrict__ Y, const V *__restrict__ W, int want_yy,
                                                            const int *__restrict__ active, double *__restrict__ partial,
-                                                           const unsigned char *__restrict__ Wq)
+                                                           const unsigned char *__restrict__ Wq,
+                                                           const double *__restrict__ omg = nullptr)
 {
     constexpr int B = TMI_RB_BATCH;
     const int lane = threadIdx.x & 63, c = lane;
     const bool act = active[c] != 0;
+    const auto wmf = mf_omega<MF, V>(omg, c);
     const int G = gridDim.x, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
@@ -579,7 +615,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
                                     cw = cn;
                                     cn = rbval[min(cb + 64 + lane, ke - 1)];
                                 }
-                                acc[r] = vadd(acc[r], vmul(rdlane(cw, kc - cb), xb[u]));
+                                acc[r] = vadd(acc[r], vmul(mfc<MF>(rdlane(cw, kc - cb), wmf), xb[u]));
                                 kc++;
                             }
                         }
@@ -1204,11 +1240,12 @@ __global__ void __launch_bounds__(256) k_spmv_dot(int n, const int *__restrict__
 }
 
 // --------------------------------------------------------- FP64 residual
-template <class V, int K>
+template <class V, int K, bool MF = false>
 __global__ void __launch_bounds__(256) k_resid(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
                                                const V *__restrict__ val, const V *__restrict__ X,
                                                const V *__restrict__ B, V *__restrict__ Rr,
-                                               const int *__restrict__ active, double *__restrict__ partial)
+                                               const int *__restrict__ active, double *__restrict__ partial,
+                                               const double *__restrict__ omg = nullptr)
 {
     constexpr int R = 64 / K;
     const int G = gridDim.x;
@@ -1221,7 +1258,8 @@ __global__ void __launch_bounds__(256) k_resid(int n, const int *__restrict__ ro
     for (int base = rs.first; base < rs.end; base += rs.stride) {
         const int i = base + rsub;
         if (i < rs.end && act) {
-            V acc = row_gather<V, K, false>(rowptr[i], rowptr[i + 1], c, colidx, val, nullptr, X);
+            V acc = row_gather<V, K, false, MF>(rowptr[i], rowptr[i + 1], c, colidx, val, nullptr, X,
+                                                mf_omega<MF, V>(omg, c));
             V r = vsub(B[(size_t)i * K + c], acc);
             Rr[(size_t)i * K + c] = r;
             d1 += dotR(r, r);
@@ -1464,12 +1502,32 @@ __device__ inline void bil_add(V a, V b, double &re, double &im)
     }
 }
 
+// z = D^-1 r for column c.  Without omg, di is the row's inverse diagonal;
+// with omg (multi-frequency solve), di is the row's diagonal entry with its
+// imaginary part at omega = 1 and the column's diagonal is re + i omg[c] im.
+template <class V>
+__device__ inline V prec_apply(V di, V r, const double *__restrict__ omg, int c)
+{
+    if constexpr (std::is_same<V, float2>::value || std::is_same<V, double2>::value) {
+        if (omg) {
+            using S = typename ScalarOf<V>::type;
+            const S a = di.x, b = di.y * (S)omg[c], den = a * a + b * b;
+            V z;
+            z.x = (r.x * a + r.y * b) / den;
+            z.y = (r.y * a - r.x * b) / den;
+            return z;
+        }
+    }
+    return vmul(di, r);
+}
+
 // x += alpha d; r -= alpha q; partials r^T D^-1 r (re, im) and ||r||^2
 template <class V, int K>
 __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ idiag, V *__restrict__ x,
                                                  const V *__restrict__ d, const V *__restrict__ q,
                                                  V *__restrict__ r, const double *__restrict__ sc,
-                                                 const int *__restrict__ ic, double *__restrict__ partial)
+                                                 const int *__restrict__ ic, double *__restrict__ partial,
+                                                 const double *__restrict__ omg)
 {
     EW_PROLOGUE
     double d1 = 0.0, d2 = 0.0, d3 = 0.0;
@@ -1482,7 +1540,7 @@ __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ id
                 stnt(x + t, vadd(a.x, cscale(a.d, ar, ai)));
                 const V rn = vsub(a.r, cscale(a.q, ar, ai));
                 stnt(r + t, rn);
-                const V z = idiag ? vmul(a.di, rn) : rn;
+                const V z = idiag ? prec_apply(a.di, rn, omg, c) : rn;
                 bil_add(rn, z, d1, d2);
                 d3 += dotR(rn, rn);
             });
@@ -1495,7 +1553,8 @@ __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ id
 template <class V, int K>
 __global__ void __launch_bounds__(256) k_cocg_d(int n, const V *__restrict__ idiag, V *__restrict__ d,
                                                 const V *__restrict__ r, const double *__restrict__ sc,
-                                                const int *__restrict__ ic, double *__restrict__ partial)
+                                                const int *__restrict__ ic, double *__restrict__ partial,
+                                                const double *__restrict__ omg)
 {
     EW_PROLOGUE
     double d1 = 0.0, d2 = 0.0, d3 = 0.0;
@@ -1506,7 +1565,7 @@ __global__ void __launch_bounds__(256) k_cocg_d(int n, const V *__restrict__ idi
         ew_for(
             tot, [&](size_t t) { return L{ldnt(r + t), first ? V{} : ldnt(d + t), idiag ? idiag[t / K] : V{}}; },
             [&](size_t t, const L &a) {
-                const V z = idiag ? vmul(a.di, a.r) : a.r;
+                const V z = idiag ? prec_apply(a.di, a.r, omg, c) : a.r;
                 stnt(d + t, first ? z : vadd(cscale(a.d, br, bi), z));
                 if (partial) {
                     bil_add(a.r, z, d1, d2);
@@ -1551,7 +1610,7 @@ __global__ void __launch_bounds__(256) k_reliable(int n, const V64 *__restrict__
                                                   V *__restrict__ xin, const V *__restrict__ rd,
                                                   const V *__restrict__ idiag, const double *__restrict__ scale,
                                                   const int *__restrict__ oflags, double *__restrict__ partial,
-                                                  int bil)
+                                                  int bil, const double *__restrict__ omg)
 {
     EW_PROLOGUE
     double d1 = 0.0, d2 = 0.0, d3 = 0.0;
@@ -1565,7 +1624,7 @@ __global__ void __launch_bounds__(256) k_reliable(int n, const V64 *__restrict__
                 V v = cvt<V>(vscale(a.r, s));
                 rin[t] = v;
                 xin[t] = vzero<V>();
-                bil_add(v, idiag ? vmul(a.di, v) : v, d1, d2);
+                bil_add(v, idiag ? prec_apply(a.di, v, omg, c) : v, d1, d2);
                 d3 += dotR(v, v);
             });
     } else if (oflags[c]) {
@@ -1993,12 +2052,46 @@ static inline void check_K(int K)
     default: { using V = double; __VA_ARGS__; } break;         \
     }
 
+// kernels that exist for complex vectors only (COCG, per-column frequencies)
+#define DISPATCH_C(vk, ...)                                                         \
+    switch (vk) {                                                                   \
+    case VK_C32: { using V = float2; __VA_ARGS__; } break;                          \
+    case VK_C64: { using V = double2; __VA_ARGS__; } break;                         \
+    default: throw Error(TMI_ERR_UNSUPPORTED, "this path needs complex vectors");  \
+    }
+
 void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *idiag, const void *X, void *Y,
                      const void *W, int want_yy, const int *iflags, double *partial, hipStream_t st,
-                     const unsigned char *Wq)
+                     const unsigned char *Wq, const double *omg)
 {
     check_K(b.K);
     dim3 g(b.nblk), blk(256);
+    if (omg) {
+        // per-column frequencies: the row-block kernel (its coefficients
+        // gathered from this operator) or the per-row kernels, MF variants
+        TMI_CHECK(!idiag && !Wq && b.K >= 2, TMI_ERR_UNSUPPORTED,
+                  "per-column frequencies: unpreconditioned operator, tiles of 2 or more columns");
+#define MFK(KK)                                                                                                 \
+    if (KK == 64 && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val)                                            \
+        hipLaunchKernelGGL((k_spmm_rb<float2, 0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr,     \
+                           b.rb.ofs, b.rb.col, b.rb.mask, (const float2 *)b.rb.val, (const float2 *)X,         \
+                           (float2 *)Y, (const float2 *)W, want_yy, iflags, partial, nullptr, omg);          \
+    else if (KK == 64)                                                                                          \
+        hipLaunchKernelGGL((k_spmm64<V, 0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, \
+                           (const V *)val, (const V *)X, (V *)Y, (const V *)W, want_yy, iflags, partial, b.rpw,   \
+                           omg);                                                                                \
+    else if (KK >= NK_KMIN && KK <= 32)                                                                         \
+        hipLaunchKernelGGL((k_spmm_nk<V, nk_k(KK), 0, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx,           \
+                           (const V *)val, (const V *)X, (V *)Y, (const V *)W, want_yy, iflags, partial, omg);   \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_spmm_dot<V, KK, false, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx,            \
+                           (const V *)val, nullptr, (const V *)X, (V *)Y, (const V *)W, want_yy, iflags, partial, \
+                           omg);
+        DISPATCH_C(vk, DISPATCH_K(b.K, MFK))
+#undef MFK
+        TMI_LAUNCH_CHECK();
+        return;
+    }
     if (b.K == 1) {
         // lanes per row sized to the average row length
         int avg = b.n ? 0 : 1;
@@ -2056,10 +2149,27 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
 }
 
 void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, const void *B, void *R,
-                  const int *iflags, double *partial, hipStream_t st)
+                  const int *iflags, double *partial, hipStream_t st, const double *omg)
 {
     check_K(b.K);
     dim3 g(b.nblk), blk(256);
+    if (omg) {
+        TMI_CHECK(b.K >= 2, TMI_ERR_UNSUPPORTED, "per-column frequencies: tiles of 2 or more columns");
+#define MFR(KK)                                                                                                 \
+    if (KK == 64)                                                                                               \
+        hipLaunchKernelGGL((k_spmm64<V, 1, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, \
+                           (const V *)val, (const V *)X, (V *)R, (const V *)B, 0, iflags, partial, b.rpw, omg);   \
+    else if (KK >= NK_KMIN && KK <= 32)                                                                         \
+        hipLaunchKernelGGL((k_spmm_nk<V, nk_k(KK), 1, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx,           \
+                           (const V *)val, (const V *)X, (V *)R, (const V *)B, 0, iflags, partial, omg);         \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_resid<V, KK, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,      \
+                           (const V *)X, (const V *)B, (V *)R, iflags, partial, omg);
+        DISPATCH_C(vk, DISPATCH_K(b.K, MFR))
+#undef MFR
+        TMI_LAUNCH_CHECK();
+        return;
+    }
 #define RESK(KK)                                                                                              \
     if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                           \
         hipLaunchKernelGGL((k_spmm64<V, 1>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
@@ -2117,24 +2227,17 @@ void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, co
     TMI_LAUNCH_CHECK();
 }
 
-// the COCG kernels exist for complex vectors only
-#define DISPATCH_C(vk, ...)                                                         \
-    switch (vk) {                                                                   \
-    case VK_C32: { using V = float2; __VA_ARGS__; } break;                          \
-    case VK_C64: { using V = double2; __VA_ARGS__; } break;                         \
-    default: throw Error(TMI_ERR_UNSUPPORTED, "COCG needs complex vectors");       \
-    }
-
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
-                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil)
+                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil, const double *omg)
 {
+    TMI_CHECK(!omg || bil, TMI_ERR_UNSUPPORTED, "per-column frequencies need COCG");
     dim3 g(b.nblk), blk(256);
 #define CXK(KK)                                                                                               \
     hipLaunchKernelGGL((k_cg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,           \
                        (const V *)q, (V *)r, sc, ic, partial);
 #define BXK(KK)                                                                                               \
     hipLaunchKernelGGL((k_cocg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,         \
-                       (const V *)q, (V *)r, sc, ic, partial);
+                       (const V *)q, (V *)r, sc, ic, partial, omg);
     if (bil) {
         DISPATCH_C(vk, DISPATCH_K(b.K, BXK))
     } else {
@@ -2146,19 +2249,20 @@ void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, cons
 }
 
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st, bool bil)
+                 const int *ic, hipStream_t st, bool bil, const double *omg)
 {
-    launch_cg_init(vk, b, idiag, d, r, sc, ic, nullptr, st, bil);
+    launch_cg_init(vk, b, idiag, d, r, sc, ic, nullptr, st, bil, omg);
 }
 
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                    const int *ic, double *partial, hipStream_t st, bool bil)
+                    const int *ic, double *partial, hipStream_t st, bool bil, const double *omg)
 {
+    TMI_CHECK(!omg || bil, TMI_ERR_UNSUPPORTED, "per-column frequencies need COCG");
     dim3 g(b.nblk), blk(256);
 #define CIK(KK) \
     hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
 #define BIK(KK) \
-    hipLaunchKernelGGL((k_cocg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
+    hipLaunchKernelGGL((k_cocg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial, omg);
     if (bil) {
         DISPATCH_C(vk, DISPATCH_K(b.K, BIK))
     } else {
@@ -2245,13 +2349,14 @@ void launch_from_inner(VKind vk32, const BlockDesc &b, void *x64, const void *x3
 
 void launch_reliable(VKind vk_in, const BlockDesc &b, const void *r64, void *rin, void *xin, const void *rd,
                      const void *idiag, const double *scale, const int *oflags, double *partial, hipStream_t st,
-                     bool bil)
+                     bool bil, const double *omg)
 {
     dim3 g(b.nblk), blk(256);
+    TMI_CHECK(!omg || bil, TMI_ERR_UNSUPPORTED, "per-column frequencies need COCG");
     if (bil && vk_in != VK_C32 && vk_in != VK_C64) throw Error(TMI_ERR_UNSUPPORTED, "COCG needs complex vectors");
 #define RLK(KK)                                                                                                 \
     hipLaunchKernelGGL((k_reliable<V64, V, KK>), g, blk, 0, st, b.n, (const V64 *)r64, (V *)rin, (V *)xin,      \
-                       (const V *)rd, (const V *)idiag, scale, oflags, partial, bil ? 1 : 0);
+                       (const V *)rd, (const V *)idiag, scale, oflags, partial, bil ? 1 : 0, omg);
     switch (vk_in) {
     case VK_C32: { using V = float2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }
     case VK_C64: { using V = double2; using V64 = double2; DISPATCH_K(b.K, RLK) break; }
@@ -2268,6 +2373,25 @@ void launch_norm2(VKind vk, const BlockDesc &b, const void *X, const int *flags,
 #define NK(KK) hipLaunchKernelGGL((k_norm2<V, KK>), g, blk, 0, st, b.n, (const V *)X, flags, partial);
     DISPATCH_V(vk, DISPATCH_K(b.K, NK))
 #undef NK
+    TMI_LAUNCH_CHECK();
+}
+
+// the complex diagonal entries (not inverted), converted to the vector type
+template <class V>
+__global__ void k_get_diag(int n, const int *__restrict__ diagpos, const double *__restrict__ val64, V *__restrict__ dg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long k = diagpos[i];
+    dg[i] = cvt<V>(make_double2(val64[2 * k], val64[2 * k + 1]));
+}
+
+void launch_get_diag(VKind vk, int n, const int *diagpos, const double *val64, void *dg, hipStream_t st)
+{
+    dim3 g(grid_for(n, 256)), blk(256);
+#define GDV hipLaunchKernelGGL((k_get_diag<V>), g, blk, 0, st, n, diagpos, val64, (V *)dg);
+    DISPATCH_C(vk, GDV)
+#undef GDV
     TMI_LAUNCH_CHECK();
 }
 

@@ -164,6 +164,8 @@ void Solver::assemble(const double *cmua, const double *ckap, const double *c2a,
         TMI_HIP(hipMemcpyAsync(coef.p + 2 * n, c2a, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     }
     omega = om;
+    this->elbasis = elbasis;
+    mf_valid = false;       // so does the multi-frequency operator
     precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
     rb_src = nullptr;       // ... and so do the row-block coefficients
     launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st,
@@ -187,6 +189,26 @@ void Solver::reset_precon()
     pval32.alloc(nval() + 8);
     launch_scale_cols(cplx, mesh->nnz, ord().colidx.p, sval64.p, idiag64.p, pval64.p, pval32.p, st);
     precon_valid = true;
+}
+
+void Solver::prepare_mf()
+{
+    TMI_CHECK(assembled && cplx, TMI_ERR_STATE, "multi-frequency solve before assembly of a complex system");
+    if (mf_valid) return;
+    const int nc = elbasis ? mesh->elen : mesh->n, n = mesh->n;
+    // the system at omega = 1: real part A, imaginary part the mass term
+    DBuf<double> v64((size_t)nval());
+    launch_assemble(*mesh, coef.p, coef.p + nc, coef.p + 2 * (size_t)nc, 1.0, 1, v64.p, nullptr, st,
+                    elbasis ? 1 : 0);
+    mfval64.alloc(nval());
+    mfval32.alloc(nval() + 8);
+    launch_permute_vals(true, mesh->nnz, ord().slot.p, v64.p, mfval64.p, mfval32.p, st);
+    mfdiag64.alloc((size_t)n * vbytes(VK_C64));
+    mfdiag32.alloc((size_t)n * vbytes(VK_C32));
+    launch_get_diag(VK_C64, n, ord().diagpos.p, mfval64.p, mfdiag64.p, st);
+    launch_get_diag(VK_C32, n, ord().diagpos.p, mfval64.p, mfdiag32.p, st);
+    TMI_HIP(hipStreamSynchronize(st));   // v64 goes out of scope; other streams read the result next
+    mf_valid = true;
 }
 
 // workgroups of a tile's block kernels and reduction partials, at most
@@ -266,14 +288,20 @@ static bool repack_pays(const int *flags, int K, int nact)
 // restarting.  A column is done when its FP64 true residual meets the
 // tolerance (the reference criterion ||r|| <= tol ||q||), or when it stops
 // improving (no 4x drop of ||r||^2 over 8 true-residual evaluations).
-int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md)
+int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md,
+                       const double *h_omega)
 {
     hipStream_t st = e.st;
     int *h_flags = e.h_flags, *h_oflags = e.h_flags + I_NSLOT * 64;
     double *h_dbl = e.h_dbl;
     const int n = mesh->n;
     const bool mixed = (md == TMI_MODE_MIXED_BLOCK);
-    const void *val_in = mixed ? (const void *)sval32.p : (const void *)sval64.p;
+    // several frequencies: the operator at omega = 1 and each column's
+    // omega (outer scalar slot S_FREQ, moved with the column by compaction)
+    const bool mf = h_omega != nullptr;
+    const void *val_in = mf ? (mixed ? (const void *)mfval32.p : (const void *)mfval64.p)
+                            : (mixed ? (const void *)sval32.p : (const void *)sval64.p);
+    const double *res_op = mf ? mfval64.p : sval64.p;   // the FP64 residual's operator
     const void *idiag_in = nullptr;
     // PCG (real) and COCG (complex symmetric: PCG with the unconjugated
     // bilinear form) share the launch sequence; COCG's scalars are complex
@@ -283,7 +311,10 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     // operator of the BiCGSTAB SpMMs: K D^-1 when diagonally preconditioned
     const void *op_in = val_in;
     if (precon == TMI_PRECON_DIAG) {
-        idiag_in = mixed ? (const void *)idiag32.p : (const void *)idiag64.p;
+        if (mf)   // the diagonal itself: the kernels divide by re + i omega im per column
+            idiag_in = mixed ? (const void *)mfdiag32.p : (const void *)mfdiag64.p;
+        else
+            idiag_in = mixed ? (const void *)idiag32.p : (const void *)idiag64.p;
         if (!use_cg) op_in = mixed ? (const void *)pval32.p : (const void *)pval64.p;
     }
     const int maxit_total = (mixed && mixed_maxit > 0) ? mixed_maxit : (maxit > 0 ? maxit : n + 1);
@@ -303,6 +334,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     int K = K0;
     BlockDesc bd;
     double *sc, *osc, *part, *scale;
+    const double *omg = nullptr;
     int *ic, *oic;
     const int *oact;
     // packed copy of the (sign-pattern) shadow residual, read by the
@@ -319,6 +351,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         oic = wp->oic.p;
         oact = oic + I_ACTIVE * K;
         scale = osc + S_SCALE * K;
+        omg = mf ? osc + S_FREQ * K : nullptr;
     };
     bind();
 
@@ -326,6 +359,11 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     for (int c = 0; c < kc; c++) hic[I_ACTIVE * K + c] = 1;
     TMI_HIP(hipMemcpyAsync(oic, hic.data(), hic.size() * sizeof(int), hipMemcpyHostToDevice, st));
     TMI_HIP(hipMemsetAsync(osc, 0, wp->osc.bytes(), st));
+    if (mf) {
+        std::vector<double> hw(K, 0.0);
+        for (int c = 0; c < kc; c++) hw[c] = h_omega[c];
+        TMI_HIP(hipMemcpyAsync(osc + S_FREQ * K, hw.data(), K * sizeof(double), hipMemcpyHostToDevice, st));
+    }
     const double tol2 = tol * tol;
     // the recursive residual aims a little below the target so that the
     // verifying FP64 residual usually passes at the first check
@@ -333,7 +371,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
 
     launch_norm2(vk64(), bd, wp->b64.p, oact, part, st);
     launch_finalize(FIN_OUTER_BNORM, bd, part, osc, oic, 0, 0, 0, st);
-    launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
+    launch_resid(vk64(), bd, res_op, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st, omg);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
     // inner state
     launch_inner_reset(bd, osc, oic, sc, ic, st);
@@ -344,7 +382,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
                     mixed);
     if (rdq) launch_pack_shadow(n, wp->rdi.p, rdq, st);
     if (use_cg) {
-        launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st, bil);
+        launch_cg_init(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, part, st, bil, omg);
         launch_finalize(bil ? FIN_COCG_INIT : FIN_CG_INIT, bd, part, sc, ic, 0, 0, 0, st);
     } else {
         launch_finalize(FIN_BICG_INIT, bd, part, sc, ic, 0, 0, 0, st);
@@ -355,10 +393,10 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     // period can hold them)
     auto reliable_launches = [&]() {
         launch_from_inner(vk, bd, wp->x64.p, wp->xi.p, scale, oact, st);
-        launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
+        launch_resid(vk64(), bd, res_op, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st, omg);
         launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
         launch_reliable(vk, bd, wp->r64.p, wp->ri.p, wp->xi.p, use_cg ? nullptr : wp->rdi.p, idiag_in, scale, oact,
-                        part, st, bil);
+                        part, st, bil, omg);
         launch_finalize(bil ? FIN_COCG_RELIABLE : use_cg ? FIN_CG_RELIABLE : FIN_BICG_RELIABLE, bd, part, sc, ic,
                         maxit_total, 0, 0, st, osc, oic);
     };
@@ -450,12 +488,12 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         for (int k = 0; k < CHECK_EVERY; k++) {
             if (use_cg) {
                 launch_spmm_dot(vk, bd, val_in, nullptr, wp->p.p, wp->v.p, wp->p.p, cg_dot, ic + I_ACTIVE * K, part,
-                                st);
+                                st, nullptr, omg);
                 launch_finalize(bil ? FIN_COCG_ALPHA : FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st, bil);
+                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st, bil, omg);
                 launch_finalize(bil ? FIN_COCG_BETA : FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
                 if (k == CHECK_EVERY - 1 && ru_due) reliable_launches();
-                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil);
+                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil, omg);
             } else {
                 launch_bicg_p(vk, bd, wp->ri.p, wp->p.p, wp->v.p, sc, ic, st);
                 launch_spmm_dot(vk, bd, op_in, nullptr, wp->p.p, wp->v.p, wp->rdi.p, false, ic + I_ACTIVE * K, part,
@@ -484,11 +522,11 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         u64 tol_bits = 0;
         std::memcpy(&tol_bits, &tol2, sizeof tol_bits);
         const std::vector<u64> key = {
-            (u64)K, (u64)vk, (u64)lsolver, (u64)ru_due, (u64)maxit_total, tol_bits, (u64)bd.nblk, (u64)bd.rpw,
+            (u64)K, (u64)vk, (u64)lsolver, (u64)mf, (u64)ru_due, (u64)maxit_total, tol_bits, (u64)bd.nblk, (u64)bd.rpw,
             P(bd.rowptr), P(bd.colidx), (u64)bd.lds.nb, P(bd.lds.brow), P(bd.lds.bu), P(bd.lds.ucols),
             P(bd.lds.lcol), (u64)bd.lds.dma, (u64)bd.rb.nb, P(bd.rb.ofs), P(bd.rb.col), P(bd.rb.mask),
             P(bd.rb.val), P(bd.rb.src), P(bd.rb.word), P(op_in), P(val_in), P(idiag_in), P(rdq), P(sc), P(ic),
-            P(osc), P(oic), P(part), P(sval64.p), P(wp->x64.p), P(wp->b64.p), P(wp->r64.p), P(wp->xi.p),
+            P(osc), P(oic), P(part), P(res_op), P(wp->x64.p), P(wp->b64.p), P(wp->r64.p), P(wp->xi.p),
             P(wp->ri.p), P(wp->rdi.p), P(wp->p.p), P(wp->v.p), P(wp->s.p), P(wp->t.p), P(st)};
         hipGraphExec_t exec = nullptr;
         for (const Exec::Graph &g : e.graphs)
@@ -528,7 +566,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
             // stop here (as the reference does); the mixed mode verifies in FP64
             if (!mixed) break;
             reliable_update();
-            if (use_cg) launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil);
+            if (use_cg) launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil, omg);
             if (count_active(e, oact, K) == 0 || it >= maxit_total) break;
             continue;
         }
@@ -549,7 +587,7 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     // fold the last inner correction of columns still marked active (maxit
     // exhausted) and record the final true residuals
     launch_from_inner(vk, bd, wp->x64.p, wp->xi.p, scale, oact, st);
-    launch_resid(vk64(), bd, sval64.p, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st);
+    launch_resid(vk64(), bd, res_op, wp->x64.p, wp->b64.p, wp->r64.p, oact, part, st, omg);
     launch_finalize(FIN_OUTER_RES, bd, part, osc, oic, 0, tol2, 0, st);
     finish(cmap);
     if (verbose)
@@ -558,11 +596,8 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     return notconv;
 }
 
-void Solver::prepare_row_blocks()
+void Solver::prepare_row_blocks(const float *op)
 {
-    // the FP32 operator of the mixed SpMMs: K D^-1 for a diagonally
-    // preconditioned BiCGSTAB, K itself for PCG / COCG
-    const float *op = (precon == TMI_PRECON_DIAG && !pcg()) ? pval32.p : sval32.p;
     if (rb_src == op) return;
     const Mesh::NodeOrder &o = ord();
     rbval32.alloc(nval() + 8);
@@ -571,15 +606,27 @@ void Solver::prepare_row_blocks()
     rb_src = op;
 }
 
-int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md)
+int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md,
+                     const double *h_omega)
 {
     TMI_CHECK(assembled, TMI_ERR_STATE, "CalcFields called before Reset/AssembleSystemMatrix");
     TMI_CHECK(!(cplx && lsolver == TMI_LSOLVER_CG), TMI_ERR_UNSUPPORTED,
               "CG on a complex system is not supported (the reference's complex CG returns the initial guess)");
     TMI_CHECK(!(!cplx && lsolver == TMI_LSOLVER_COCG), TMI_ERR_UNSUPPORTED,
               "COCG solves complex symmetric systems; use CG for a real one");
-    if (lsolver == TMI_LSOLVER_GMRES) return gmres_on(e, nq, d_b, d_x, ld, stats);
     if (md < 0) md = mode;
+    const bool mf = h_omega != nullptr;
+    if (mf) {
+        TMI_CHECK(cplx && lsolver == TMI_LSOLVER_COCG, TMI_ERR_UNSUPPORTED,
+                  "a multi-frequency solve needs a complex solver with COCG");
+        TMI_CHECK(md == TMI_MODE_MIXED_BLOCK || md == TMI_MODE_FP64_BLOCK, TMI_ERR_UNSUPPORTED,
+                  "a multi-frequency solve runs in the block modes (mixed, fp64_block)");
+        for (int c = 0; c < nq; c++)
+            TMI_CHECK(std::isfinite(h_omega[c]), TMI_ERR_INVALID_ARG, "frequency of column %d is not finite", c);
+        std::lock_guard<std::mutex> lk(prep_mu);
+        prepare_mf();
+    }
+    if (lsolver == TMI_LSOLVER_GMRES) return gmres_on(e, nq, d_b, d_x, ld, stats);
     if (precon == TMI_PRECON_DIAG) {
         std::lock_guard<std::mutex> lk(prep_mu);
         if (!precon_valid) {
@@ -589,7 +636,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     }
     if ((spmm_variant == 3 || spmm_variant == 4) && cplx) {
         std::lock_guard<std::mutex> lk(prep_mu);
-        prepare_row_blocks();
+        prepare_row_blocks(mf ? mfval32.p : spmm_op32());
     }
     if (!e.st) e.st = st;
     std::vector<tmi_colstat> own;
@@ -597,7 +644,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
         own.resize(nq);
         stats = own.data();
     }
-    const int K = tile_for(nq, md);
+    const int K = mf ? std::max(2, tile_for(nq, md)) : tile_for(nq, md);
     hipStream_t es_ = e.st;
     size_t es = vbytes(vk64());
     hipEvent_t e0, e1;
@@ -615,7 +662,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
         const int *perm = ord().perm.p;
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_b, ld, q0, w.b64.p, es_);
         launch_gather_tile((int)es, mesh->n, K, kc, perm, d_x, ld, q0, w.x64.p, es_);
-        notconv += solve_tile(e, K, kc, stats + q0, d_x, ld, q0, md);
+        notconv += solve_tile(e, K, kc, stats + q0, d_x, ld, q0, md, mf ? h_omega + q0 : nullptr);
     }
     double iters = e.last_iters;
     // a column the mixed-precision iteration could not bring to the
@@ -634,7 +681,10 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
                                      mesh->n, hipMemcpyDeviceToDevice, es_));
         }
         std::vector<tmi_colstat> st2(nb);
-        solve_on(e, nb, tb.p, tx.p, nb, st2.data(), TMI_MODE_FP64_BLOCK);
+        std::vector<double> om2;
+        if (mf)
+            for (int j = 0; j < nb; j++) om2.push_back(h_omega[bad[j]]);
+        solve_on(e, nb, tb.p, tx.p, nb, st2.data(), TMI_MODE_FP64_BLOCK, mf ? om2.data() : nullptr);
         iters += e.last_iters * nb;
         notconv = 0;
         for (int j = 0; j < nb; j++) {
@@ -765,7 +815,7 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
             spmm_variant = c.variant;
             if (c.variant == 3 || c.variant == 4) {
                 std::lock_guard<std::mutex> lk(prep_mu);
-                prepare_row_blocks();
+                prepare_row_blocks(spmm_op32());
             }
             rpw = c.rpw;
             double ms = 0, bytes = 0;

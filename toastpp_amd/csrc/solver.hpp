@@ -19,6 +19,9 @@ struct Solver {
     int lsolver = TMI_LSOLVER_BICGSTAB;
     // PCG and COCG share one launch sequence (one SpMM per iteration)
     bool pcg() const { return lsolver == TMI_LSOLVER_CG || lsolver == TMI_LSOLVER_COCG; }
+    // the FP32 operator of the mixed SpMMs: K D^-1 for a diagonally
+    // preconditioned BiCGSTAB, K itself for PCG / COCG
+    const float *spmm_op32() const { return (precon == TMI_PRECON_DIAG && !pcg()) ? pval32.p : sval32.p; }
     double tol = 1e-10;
     int precon = TMI_PRECON_NONE;
     int maxit = 0;
@@ -34,7 +37,7 @@ struct Solver {
                                      // 3 row blocks with shared gathers (default)
     int compact_min = 2;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
-    void prepare_row_blocks();   // rbval32 for the current operator (caller holds prep_mu)
+    void prepare_row_blocks(const float *op);   // rbval32 for the FP32 operator op (caller holds prep_mu)
     double omega = 0.0;
     hipStream_t st = nullptr;
 
@@ -47,6 +50,15 @@ struct Solver {
     DBuf<float> rbval32;   // the FP32 operator in the row-block SpMM's coefficient order
     const void *rb_src = nullptr;   // the operator rbval32 holds (null: none)
     DBuf<double> coef;     // cmua | ckappa | c2a on the device
+    bool elbasis = false;  // coef holds per-element coefficients
+    // multi-frequency operator: the system assembled at omega = 1 (real part
+    // A, imaginary part the mass term B), in the solver's node order, and
+    // its diagonal; a column at omega w solves (A + i w B) x = b
+    DBuf<double> mfval64;
+    DBuf<float> mfval32;
+    DBuf<char> mfdiag64, mfdiag32;
+    bool mf_valid = false;
+    void prepare_mf();   // caller holds prep_mu
 
     // per-solve state: a stream, the workspace of one tile, pinned host
     // mirrors of the per-column flags and the last solve's timing.  The
@@ -111,7 +123,10 @@ struct Solver {
     // arrays (FP64, complex or real); d_x holds the initial guess.
     int solve(int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats) { return solve_on(ex, nq, d_b, d_x, ld, stats); }
     // md: precision / batching mode of this solve (-1: the solver's mode)
-    int solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md = -1);
+    // h_omega (host, nq values, complex COCG, block modes): column c solves
+    // at omega h_omega[c] -- several frequencies in one block solve
+    int solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats, int md = -1,
+                 const double *h_omega = nullptr);
     // device bytes a solve of nq columns needs for its workspace
     size_t work_bytes(int nq) const;
     int tile_for(int nq, int md = -1) const;
@@ -135,7 +150,8 @@ struct Solver {
    private:
     BlockDesc block(const Exec::Work &w, int K) const;
     Exec::Work &ensure_work(Exec &e, int K, int md, bool alt = false);
-    int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md);
+    int solve_tile(Exec &e, int K, int kc, tmi_colstat *stats, void *d_x, int ld, int q0, int md,
+                   const double *h_omega);
     // GMRES(20) in FP64 on tiles of the columns (gmres.hip); every mode
     int gmres_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_colstat *stats);
     int count_active(Exec &e, const int *d_flags, int K);

@@ -158,6 +158,37 @@ class FwdSolver:
         self.converged = st == L.TMI_OK
         return self.stats
 
+    def CalcFieldsFreqsDevice(self, nq, d_rhs, d_phi, omegas):
+        """Several frequencies of this mesh in one block solve: column q of
+        the [n][nq] device arrays is solved at omegas[q] (the unit of Reset's
+        omega) with the optical coefficients of the last Reset; COCG, block
+        modes.  No reference counterpart (the reference solves each frequency
+        after its own AssembleSystemMatrix).  Returns the per-column records."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        if om.shape != (nq,):
+            raise ValueError("omegas has shape %s, expected (%d,)" % (om.shape, nq))
+        stats = (L.ColStat * nq)()
+        st = L.call("tmi_solver_calc_fields_freqs_device", self._h, nq, C.c_void_p(d_rhs), C.c_void_p(d_phi),
+                    om.ctypes.data_as(C.c_void_p), C.cast(stats, C.c_void_p), allow_notconv=True)
+        self.stats = [(s.iters, s.sweeps, s.relres, bool(s.converged)) for s in stats]
+        self.converged = st == L.TMI_OK
+        return self.stats
+
+    def CalcFieldsFreqs(self, qvec, omegas):
+        """CalcFields with column q solved at omegas[q] (see
+        CalcFieldsFreqsDevice); qvec (nQ x n), returns phi (nQ x n)."""
+        from scipy import sparse
+
+        q = sparse.csr_matrix(qvec)
+        nq, n = q.shape
+        if n != self.mesh.NodeCount():
+            raise ValueError("qvec has %d columns, the mesh %d nodes" % (n, self.mesh.NodeCount()))
+        B = np.ascontiguousarray(q.T.toarray().astype(np.complex128))   # [n][nq]
+        dB = L.DeviceArray.from_host(B)
+        dX = L.DeviceArray(B.shape, np.complex128)
+        self.CalcFieldsFreqsDevice(nq, dB.ptr, dX.ptr, omegas)
+        return np.ascontiguousarray(dX.to_host().T)
+
     def Autotune(self, nq, d_rhs, d_phi, ld=None):
         """Pick tile width, row schedule and SpMM variant for nq device
         right-hand sides ([n][ld] node-major); returns them as a dict."""
