@@ -39,8 +39,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the inner SpMM, from two rocprofv3 --pmc passes
-# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy; the file gives the commands
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmm_pmc_traffic_ellipsoid250k_64src.md")
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy, one file per inner
+# solver (the SpMM's fused dot reads a different W); the files give the
+# commands
+TRAFFIC_FILES = {
+    "COCG": os.path.join(ROOT, "profiles", "spmm_pmc_traffic_ellipsoid250k_64src_cocg.md"),
+    "BICGSTAB": os.path.join(ROOT, "profiles", "spmm_pmc_traffic_ellipsoid250k_64src.md"),
+}
 
 
 # the 64-column SpMM variants: kernel name and what it does
@@ -65,13 +70,15 @@ def spmm_kernel_name(variant, solver):
     return name
 
 
-def measured_traffic(workload, tile, kernel):
+def measured_traffic(workload, tile, kernel, solver):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (None unless it was measured for this workload, tile and kernel)."""
-    if workload != "ellipsoid3d" or tile != 64 or not os.path.exists(TRAFFIC_FILE):
+    (None unless it was measured for this workload, tile, kernel and inner
+    solver)."""
+    path = TRAFFIC_FILES.get(solver)
+    if workload != "ellipsoid3d" or tile != 64 or not path or not os.path.exists(path):
         return None
     found, traffic = None, None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             if line.startswith("traffic_kernel"):
                 found = line.split("=")[1].strip()
@@ -359,7 +366,7 @@ def main():
         "speedup_vs_fp64_block": (round(value / fp64b["solves_per_s"], 3) if fp64b else None),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.workload, args.tile, spmm_kernel),
+                     "traffic": measured_traffic(args.workload, args.tile, spmm_kernel, args.solver),
                      "kernel": spmm_kernel,
                      "avg_launch_ms": round(spmm_ms, 4), "algorithmic_bytes": int(spmm_bytes)},
         "jacobian": jac,

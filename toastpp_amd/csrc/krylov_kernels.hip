@@ -56,6 +56,27 @@ __device__ inline float vfma(float acc, float a, float b) { return __builtin_fma
 __device__ inline double vfma(double acc, double a, double b) { return __builtin_fma(a, b, acc); }
 __device__ inline double vmul(double a, double b) { return a * b; }
 
+// acc + c x as four fused multiply-adds (complex) or one (real): the
+// accumulation of the SpMM kernels
+__device__ inline float2 vmac(float2 acc, float2 c, float2 x)
+{
+    acc.x = fmaf(c.x, x.x, acc.x);
+    acc.x = fmaf(-c.y, x.y, acc.x);
+    acc.y = fmaf(c.x, x.y, acc.y);
+    acc.y = fmaf(c.y, x.x, acc.y);
+    return acc;
+}
+__device__ inline double2 vmac(double2 acc, double2 c, double2 x)
+{
+    acc.x = fma(c.x, x.x, acc.x);
+    acc.x = fma(-c.y, x.y, acc.x);
+    acc.y = fma(c.x, x.y, acc.y);
+    acc.y = fma(c.y, x.x, acc.y);
+    return acc;
+}
+__device__ inline float vmac(float acc, float c, float x) { return fmaf(c, x, acc); }
+__device__ inline double vmac(double acc, double c, double x) { return fma(c, x, acc); }
+
 // scalar type of a vector element type
 template <class V> struct ScalarOf { using type = V; };
 template <> struct ScalarOf<float2> { using type = float; };
@@ -283,7 +304,7 @@ __device__ inline V row_gather(int kb, int ke, int c, const int *__restrict__ co
             if (PRE) xx[u] = vmul(idiag[jj[u]], xx[u]);
         }
 #pragma unroll
-        for (int u = 0; u < CH; u++) acc = vadd(acc, vmul(mfc<MF>(aa[u], w), xx[u]));
+        for (int u = 0; u < CH; u++) acc = vmac(acc, mfc<MF>(aa[u], w), xx[u]);
     }
     return acc;
 }
@@ -326,6 +347,25 @@ __global__ void __launch_bounds__(256) k_spmm_dot(int n, const int *__restrict__
 // waves (the waves of an XCD then sweep the region together, so their
 // neighbour rows overlap in that XCD's L2), R <= 0 gives every wave one
 // contiguous share of the region, processed in runs of up to 63 rows.
+// a[i] with a wave-uniform row pointer a kept in scalar registers (the
+// load then takes the scalar-base + lane-offset form, no per-lane 64-bit
+// address arithmetic)
+// (loads go through integer types, bit-cast to the element type)
+typedef unsigned long long ldrow_u2 __attribute__((ext_vector_type(2)));
+template <class V>
+__device__ inline V ld_row(const V *a, unsigned i)
+{
+    static_assert(sizeof(V) == 8 || sizeof(V) == 16, "ld_row: 8- or 16-byte elements");
+    const unsigned long long p = (unsigned long long)a;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
+    const unsigned long long u = ((unsigned long long)hi << 32) | lo;
+    if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, ((const __attribute__((address_space(1))) unsigned long long *)u)[i]);
+    else
+        return __builtin_bit_cast(V, ((const __attribute__((address_space(1))) ldrow_u2 *)u)[i]);
+}
+
 __device__ inline float2 rdlane(float2 v, int l)
 {
     return make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.x), l)),
@@ -417,7 +457,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm64(int n, const int *
                     if (t0 + u < m) {
                         const int k = seg + t0 + u;
                         while (k >= rend && row < nr) flush();
-                        acc = vadd(acc, vmul(mfc<MF>(rdlane(vv, t0 + u), wmf), xb[u]));
+                        acc = vmac(acc, mfc<MF>(rdlane(vv, t0 + u), wmf), xb[u]);
                     }
                 }
             }
@@ -537,7 +577,7 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
                     if (t0 + u < m) {
                         const int k = kb + o + t0 + u;
                         while (k >= rend && row < nr) flush();
-                        acc = vadd(acc, vmul(mfc<MF>(ab[u], wmf), xb[u]));
+                        acc = vmac(acc, mfc<MF>(ab[u], wmf), xb[u]);
                     }
                 }
             }
@@ -570,6 +610,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
                                                            const double *__restrict__ omg = nullptr)
 {
     constexpr int B = TMI_RB_BATCH;
+    static_assert(B * RB_ROWS <= 64, "a batch's coefficients must fit one 64-lane window");
     const int lane = threadIdx.x & 63, c = lane;
     const bool act = active[c] != 0;
     const auto wmf = mf_omega<MF, V>(omg, c);
@@ -585,9 +626,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
         const int u0 = bofs[b], u1 = bofs[b + 1];
         const int r0 = b * RB_ROWS, nr = min(RB_ROWS, n - r0);
         const int kb = rowptr[r0], ke = rowptr[r0 + nr];
-        int kc = kb, cb = kb;
-        V cw = rbval[min(cb + lane, ke - 1)];        // coefficients cb .. cb+63
-        V cn = rbval[min(cb + 64 + lane, ke - 1)];   // the next window
+        int kc = kb;   // the block's next coefficient (they follow the union entries)
         V acc[RB_ROWS];
 #pragma unroll
         for (int r = 0; r < RB_ROWS; r++) acc[r] = vzero<V>();
@@ -597,11 +636,17 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
             const int jv = ucol[uu];
             const int mv = umask[uu];
             for (int t0 = 0; t0 < m; t0 += B) {
+                // a batch of B <= 8 entries holds at most 8 B <= 64
+                // coefficients: one window load, issued with the gathers
+                const V cw = ld_row(rbval + kc, (unsigned)min(lane, ke - 1 - kc));
+                int kk = 0;
                 V xb[B];
 #pragma unroll
                 for (int u = 0; u < B; u++) {
+                    // unconditional: j is a column of the block (in bounds);
+                    // an inactive column's sums are never stored
                     const int j = __builtin_amdgcn_readlane(jv, min(t0 + u, m - 1));
-                    xb[u] = act ? X[(size_t)j * 64 + c] : vzero<V>();
+                    xb[u] = ld_row(X + (size_t)j * 64, (unsigned)c);
                 }
 #pragma unroll
                 for (int u = 0; u < B; u++) {
@@ -610,17 +655,13 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
 #pragma unroll
                         for (int r = 0; r < RB_ROWS; r++) {
                             if (mk & (1 << r)) {
-                                if (kc - cb >= 64) {
-                                    cb += 64;
-                                    cw = cn;
-                                    cn = rbval[min(cb + 64 + lane, ke - 1)];
-                                }
-                                acc[r] = vadd(acc[r], vmul(mfc<MF>(rdlane(cw, kc - cb), wmf), xb[u]));
-                                kc++;
+                                acc[r] = vmac(acc[r], mfc<MF>(rdlane(cw, kk), wmf), xb[u]);
+                                kk++;
                             }
                         }
                     }
                 }
+                kc += kk;
             }
         }
 #pragma unroll
@@ -810,7 +851,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rbp(int n, const int
 #pragma unroll
                 for (int r = 0; r < RB_ROWS; r++) {
                     if (mk & (1u << r)) {
-                        acc[r] = vadd(acc[r], vmul(rdlane(cw, k), x[u]));
+                        acc[r] = vmac(acc[r], rdlane(cw, k), x[u]);
                         k++;
                     }
                 }
@@ -1226,7 +1267,7 @@ __global__ void __launch_bounds__(256) k_spmv_dot(int n, const int *__restrict__
                 const int j = colidx[k];
                 V xv = X[j];
                 if (idiag) xv = vmul(idiag[j], xv);
-                acc = vadd(acc, vmul(val[k], xv));
+                acc = vmac(acc, val[k], xv);
             }
         }
         // sum the LPR partial rows (all lanes take part in the shuffles)

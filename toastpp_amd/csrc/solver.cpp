@@ -3,9 +3,11 @@
 // CalcFields runs every mode on tiles of K right-hand-side columns:
 //   FP64:   r = q - K x (FP64 CSR SpMM) and the per-column convergence test
 //           ||r|| <= tol ||q|| (the reference criterion, matrix_imp.hpp:350-413)
-//   Krylov: block BiCGSTAB (complex) or PCG (real) -- complex/real FP32 on the
-//           system scaled by 1/||q|| in TMI_MODE_MIXED_BLOCK, FP64 in the FP64
-//           modes.  The mixed mode makes reliable updates: every few
+//   Krylov: block BiCGSTAB or COCG (complex) or PCG (real) -- complex/real
+//           FP32 on the system scaled by 1/||q|| in TMI_MODE_MIXED_BLOCK, FP64
+//           in the FP64 modes; COCG columns may each carry their own
+//           frequency (the operator then assembled at omega = 1 and scaled per
+//           column).  The mixed mode makes reliable updates: every few
 //           iterations, and whenever every column's recursive residual meets
 //           the tolerance, the FP32 correction is folded into the FP64
 //           solution and the FP64 true residual replaces the FP32 recursive
@@ -548,6 +550,9 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
             TMI_HIP(ie);
             g_launch_calls = calls;   // captured, not launched
             if (e.graphs.size() >= 32) {   // keep the cache small: drop the oldest
+                // it may have been launched moments ago: let the stream drain first
+                TMI_HIP(hipStreamSynchronize(st));
+                e.n_sync++;
                 (void)hipGraphExecDestroy(e.graphs.front().exec);
                 e.graphs.erase(e.graphs.begin());
             }
@@ -763,10 +768,15 @@ void Solver::time_kernel(int which, int reps, double *avg_ms, double *bytes)
     (void)hipEventDestroy(e1);
     *avg_ms = ms / reps;
     // algorithmic bytes: matrix values + column indices once, row pointers,
-    // each X row read once, each Y row written once (+ the inverse diagonal)
+    // each X row read once, each Y row written once, and BiCGSTAB's shadow
+    // residual for the fused dot (4 bits per entry where the packed copy is
+    // read).  For PCG / COCG the dot's W is X itself, already counted (the
+    // kernel reads a row's W right after gathering that row as a neighbour,
+    // from the cache).
     double sv = (double)vbytes(which == 0 ? w.vk_in : vk64());
     double nnz = mesh->nnz;
     double b = nnz * (sv + 4.0) + 4.0 * (n + 1) + 2.0 * n * K * sv;
+    if (which == 0 && !pcg()) b += wq ? (double)n * K / 2.0 : (double)n * K * sv;
 
     if (which == 1) b += (double)n * K * sv;   // the right-hand side block is read as well
     *bytes = b;

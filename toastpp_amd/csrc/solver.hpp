@@ -1,5 +1,6 @@
 // Forward solver: the GPU counterpart of TFwdSolver<T> (fwdsolver.h:77-508).
 #pragma once
+#include <cstdlib>
 #include <mutex>
 
 #include "krylov.hpp"
@@ -130,7 +131,15 @@ struct Solver {
     // device bytes a solve of nq columns needs for its workspace
     size_t work_bytes(int nq) const;
     int tile_for(int nq, int md = -1) const;
-    bool use_graphs = true;          // replay the check period's launches from captured HIP graphs
+    // replay the check period's launches from captured HIP graphs (default
+    // on; the environment variable TMI_GRAPHS=0 turns it off for new solvers,
+    // e.g. for kernel traces under a profiler)
+    bool use_graphs = graphs_default();
+    static bool graphs_default()
+    {
+        const char *v = getenv("TMI_GRAPHS");
+        return !(v && v[0] == '0');
+    }
     bool fp64_fallback = true;       // mixed mode: columns that do not converge are re-solved in FP64
     int mixed_maxit = 0;             // mixed mode: iteration cap of the FP32 iteration (0: maxit)
     // device buffers of ProjectAll (link table, mvec CSR, partial sums,
