@@ -70,9 +70,12 @@ def test_nodal_jacobian_optical_2d(mode):
     Jacobian divides by the projections, which span ~20 decades between near
     and far source/detector pairs, so the far rows carry the fields' error
     relative to their own size: the solves run near the FP64 floor (relative
-    residual 1e-13), and the bound is on each block's relative Frobenius
-    norm, with the worst single entry reported and held to 1e-5."""
-    tol = 1e-13
+    residual 3e-13; at 1e-13 a column of the single-column FP64 path can end
+    with its true residual just above the tolerance, depending on the
+    summation order, and is reported unconverged), and the bound is on each
+    block's relative Frobenius norm, with the worst single entry reported and
+    held to 1e-5."""
+    tol = 3e-13
     case = CASES["2d_cplx_bicgstab"]
     om = O.Mesh(case["mesh"])
     Q, Mp, lp, li = om.read_qm(case["qm"])

@@ -12,6 +12,10 @@ from oracle import oracle as O
 from tests.cases import CASES
 
 pytestmark = pytest.mark.gpu
+# fields near the FP64 floor: at 1e-13 a column of the single-column FP64
+# path can end with its true residual just above the tolerance, depending on
+# the summation order, and is reported unconverged
+FP64_FLOOR_TOL = 3e-13
 
 TMI = pytest.importorskip("toastpp_amd")
 
@@ -156,7 +160,7 @@ def test_basis_jacobian_optical_2d():
     freq = 100.0
     gr = TMI.Raster(gm, np.array([32, 32]), np.array([64, 64]))
     Jg = gm.JacobianOptical(gr, gm.Qvec(width=2.0), gm.Mvec(width=2.0, ref=1.4), mua, mus, ref, freq,
-                            solver="bicgstab", tol=1e-13, mode="fp64")
+                            solver="bicgstab", tol=FP64_FLOOR_TOL, mode="fp64")
     cm, ck, c2 = O.coeffs(mua, mus, ref)
     rp, ci, val = om.sysmat(cm, ck, c2, freq * 2 * np.pi * 1e-6)
     qv = om.qvec_gaussian(Q, 2.0)

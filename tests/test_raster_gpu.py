@@ -8,6 +8,10 @@ from oracle import oracle as O
 from tests.cases import CASES
 
 pytestmark = pytest.mark.gpu
+# fields near the FP64 floor: at 1e-13 a column of the single-column FP64
+# path can end with its true residual just above the tolerance, depending on
+# the summation order, and is reported unconverged
+FP64_FLOOR_TOL = 3e-13
 
 TMI = pytest.importorskip("toastpp_amd")
 
@@ -95,7 +99,7 @@ def test_raster_jacobian_optical_2d():
     freq = 100.0
     gr = TMI.Raster(gm, np.array([48, 48]))
     Jg = gm.JacobianOptical(gr, gm.Qvec(width=2.0), gm.Mvec(width=2.0, ref=1.4), mua, mus, ref, freq,
-                            solver="bicgstab", tol=1e-13, mode="fp64")
+                            solver="bicgstab", tol=FP64_FLOOR_TOL, mode="fp64")
     cm, ck, c2 = O.coeffs(mua, mus, ref)
     rp, ci, val = om.sysmat(cm, ck, c2, freq * 2 * np.pi * 1e-6)
     qv = om.qvec_gaussian(Q, 2.0)
@@ -157,7 +161,7 @@ def test_jacobian_cw_end_to_end_2d():
     gr = TMI.Raster(gm, np.array([40, 40]))
     qv = gm.Qvec(width=2.0).real
     mv = gm.Mvec(width=2.0, ref=1.4).real
-    Jg = gm.JacobianCW(gr, qv, mv, mua, mus, ref, solver="bicgstab", tol=1e-13)
+    Jg = gm.JacobianCW(gr, qv, mv, mua, mus, ref, solver="bicgstab", tol=FP64_FLOOR_TOL)
     cm, ck, c2 = O.coeffs(mua, mus, ref)
     rp, ci, val = om.sysmat(cm, ck, c2, 0.0)
     oq = om.qvec_gaussian(Q, 2.0)

@@ -15,7 +15,11 @@
 namespace tmi {
 
 // row mask of a distinct column in a block of the row-block SpMM (one bit per row)
+#if defined(TMI_RB_ROWS) && TMI_RB_ROWS > 8
+typedef unsigned short rb_mask_t;   // row blocks of up to 16 rows
+#else
 typedef unsigned char rb_mask_t;
+#endif
 
 // A failure carried as a C++ exception inside the library and converted to a
 // status code at the C-ABI boundary (never escapes extern "C").

@@ -232,8 +232,13 @@ Solver::Exec::Work &Solver::ensure_work(Exec &e, int K, int md, bool alt)
     w.n = n;
     w.vk_in = vin;
     // reduction workgroups: a multiple of 8 (one slice per XCD), enough to
-    // fill the GPU, few enough that the per-block partials stay small
-    long rows_work = ((long)n * K + 1023) / 1024;
+    // fill the GPU, few enough that the per-block partials stay small.  The
+    // narrow tiles' SpMMs walk rows (a chain of dependent row-pointer,
+    // column and gather loads per row), so the count follows the rows as
+    // well as the elements: one workgroup per 128 rows at least (with one
+    // per 1024 elements only, a single-column SpMV ran 248 workgroups and
+    // took 86 us on the 250k-node mesh)
+    long rows_work = std::max(((long)n * K + 1023) / 1024, ((long)n + 127) / 128);
     int G = (int)std::min<long>(TMI_WORK_G, std::max<long>(8, rows_work));
     G = (G + 7) / 8 * 8;
     w.G = G;
