@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="run only the plain FP64 single-column leg (for a kernel trace of that path) and print "
+                         "its record as JSON")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="short run for hardware-counter passes: one mixed solve with the default settings "
                          "(no autotuner: its timings are distorted under the profiler), 5 timed launches of the "
@@ -172,10 +175,13 @@ def main():
     fm, asm_ms = make_solver("mixed", args.tol, args.tile)
     log("assembly + preconditioner (host call incl. upload) %.1f ms" % asm_ms)
     tuned = None
-    if not args.no_autotune and not args.pmc_pass:
+    if not args.no_autotune and not args.pmc_pass and not args.fp64_only:
         t0 = time.perf_counter()
         tuned = fm.Autotune(nq, dB.ptr, dX.ptr, nq)
         log("autotune (%.1f s): %s" % (time.perf_counter() - t0, tuned))
+    if args.fp64_only:
+        print(json.dumps({"fp64_single": fp64_single_leg(args, make_solver, L, B, n, nq, None)}), flush=True)
+        return
     if args.pmc_pass:
         fm.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
         fm.TimeKernel(0, reps=5)
@@ -232,30 +238,7 @@ def main():
     # plain FP64 single-RHS path on a subset of columns
     fp64 = None
     if not args.no_fp64 and args.fp64_cols > 0:
-        ff, _ = make_solver("fp64", args.tol, solver="BICGSTAB")   # the reference's method, plainly
-        k = min(args.fp64_cols, nq)
-        Bs = L.DeviceArray.from_host(np.ascontiguousarray(B[:, :k]))
-        Xs = L.DeviceArray((n, k), np.complex128)
-        B1 = L.DeviceArray.from_host(np.ascontiguousarray(B[:, :1]))
-        X1 = L.DeviceArray((n, 1), np.complex128)
-        ff.CalcFieldsDevice(1, B1.ptr, X1.ptr)      # warm the single-column kernels
-        L.synchronize()
-        t0 = time.perf_counter()
-        st64 = ff.CalcFieldsDevice(k, Bs.ptr, Xs.ptr)
-        L.synchronize()
-        t64 = time.perf_counter() - t0
-        c64 = ff.LastCounts()
-        e64 = np.linalg.norm(Xs.to_host() - Xref[:, :k], axis=0) / np.linalg.norm(Xref[:, :k], axis=0)
-        fp64 = dict(solver="bicgstab", solves_per_s=k / t64, columns=k, iters_per_col=float(np.mean([s[0] for s in st64])),
-                    max_rel_err=float(e64.max()))
-        ms_it = t64 * 1e3 / max(1.0, sum(s[0] for s in st64))
-        fp64["ms_per_iteration"] = ms_it
-        its64 = max(1.0, sum(s[0] for s in st64))
-        fp64["launches_per_iteration"] = round(c64[0] / its64, 2)
-        fp64["graph_launches_per_iteration"] = round(c64[1] / its64, 3)
-        fp64["host_syncs_per_iteration"] = round(c64[2] / its64, 3)
-        log("fp64 single-RHS: %.2f solves/s over %d columns, iters/col %.0f, %.3f ms per iteration, err %.2e" %
-            (k / t64, k, fp64["iters_per_col"], ms_it, e64.max()))
+        fp64 = fp64_single_leg(args, make_solver, L, B, n, nq, Xref)
 
     # FP64 block Krylov over all columns at the same tolerance: the gain of
     # batching alone, beside which the mixed mode shows the gain of precision
@@ -376,6 +359,39 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def fp64_single_leg(args, make_solver, L, B, n, nq, Xref):
+    """The plain FP64 single-right-hand-side GPU path (the reference's
+    method, BiCGSTAB, one column at a time) on the first columns: solves/s,
+    iterations, time per iteration, launches / graph launches / host
+    synchronisations per iteration; the field error against Xref when given."""
+    ff, _ = make_solver("fp64", args.tol, solver="BICGSTAB")   # the reference's method, plainly
+    k = min(args.fp64_cols, nq)
+    Bs = L.DeviceArray.from_host(np.ascontiguousarray(B[:, :k]))
+    Xs = L.DeviceArray((n, k), np.complex128)
+    B1 = L.DeviceArray.from_host(np.ascontiguousarray(B[:, :1]))
+    X1 = L.DeviceArray((n, 1), np.complex128)
+    ff.CalcFieldsDevice(1, B1.ptr, X1.ptr)      # warm the single-column kernels
+    L.synchronize()
+    t0 = time.perf_counter()
+    st64 = ff.CalcFieldsDevice(k, Bs.ptr, Xs.ptr)
+    L.synchronize()
+    t64 = time.perf_counter() - t0
+    c64 = ff.LastCounts()
+    e64 = (np.linalg.norm(Xs.to_host() - Xref[:, :k], axis=0) / np.linalg.norm(Xref[:, :k], axis=0)
+           if Xref is not None else np.full(k, np.nan))
+    fp64 = dict(solver="bicgstab", solves_per_s=k / t64, columns=k, iters_per_col=float(np.mean([s[0] for s in st64])),
+                max_rel_err=(float(e64.max()) if Xref is not None else None))
+    ms_it = t64 * 1e3 / max(1.0, sum(s[0] for s in st64))
+    fp64["ms_per_iteration"] = ms_it
+    its64 = max(1.0, sum(s[0] for s in st64))
+    fp64["launches_per_iteration"] = round(c64[0] / its64, 2)
+    fp64["graph_launches_per_iteration"] = round(c64[1] / its64, 3)
+    fp64["host_syncs_per_iteration"] = round(c64[2] / its64, 3)
+    log("fp64 single-RHS: %.2f solves/s over %d columns, iters/col %.0f, %.3f ms per iteration, err %.2e" %
+        (k / t64, k, fp64["iters_per_col"], ms_it, e64.max()))
+    return fp64
 
 
 def jacobian_leg(args):
