@@ -168,14 +168,23 @@ int tmi_solver_set_mode(tmi_solver s, int mode);                /* TMI_MODE_* */
 /* Column tile of the block modes (RHS columns per Krylov block; 0 = auto) */
 int tmi_solver_set_tile(tmi_solver s, int tile);
 /* Node order of the solver's private copy of the system (no reference
- * counterpart): TMI_ORDER_MORTON (default) relabels the nodes along a
- * Z-order curve of their coordinates so that the sparse kernels' neighbour
- * gathers stay in cache; TMI_ORDER_NATURAL keeps the mesh order.  Inputs,
- * outputs and tmi_solver_get_sysmat always use the mesh order.  Changing it
- * requires tmi_solver_allocate again. */
+ * counterpart): TMI_ORDER_MORTON relabels the nodes along a Z-order curve of
+ * their coordinates so that the sparse kernels' neighbour gathers stay in
+ * cache; TMI_ORDER_CLUSTER (default) starts from the Morton order and groups
+ * the rows, 8 at a time, so that each group's rows share most of their
+ * columns (heavy-edge matching on the shared-column counts), which the
+ * row-block SpMM turns into fewer gathers; TMI_ORDER_NATURAL keeps the mesh
+ * order.  Inputs, outputs and tmi_solver_get_sysmat always use the mesh
+ * order.  Changing it requires tmi_solver_allocate again. */
 #define TMI_ORDER_NATURAL 0
 #define TMI_ORDER_MORTON  1
+#define TMI_ORDER_CLUSTER 2
 int tmi_solver_set_order(tmi_solver s, int order);
+/* The node order of a kind for a mesh given by its node coordinates (n x dim,
+ * row-major) and the CSR pattern of its system (rowptr n+1, colidx, mesh
+ * order): perm[r] = mesh node at row r of the relabelled system.  Host code
+ * only (no device work); the order the solver builds in tmi_solver_allocate. */
+int tmi_node_order(int kind, int dim, int n, const double *nodes, const int *rowptr, const int *colidx, int *perm);
 /* Rows per run of the 64-column block SpMM (default 4): runs of this many
  * consecutive rows are dealt round-robin to the waves of each XCD; 0 gives
  * every wave one contiguous share instead.  Tuning parameter of the row

@@ -25,7 +25,7 @@ LSOLVER_DIRECT, LSOLVER_CG, LSOLVER_BICGSTAB, LSOLVER_GMRES, LSOLVER_COCG = 0, 1
 PRECON_NONE, PRECON_DIAG = 0, 1
 MODE_FP64_SINGLE, MODE_MIXED_BLOCK, MODE_FP64_BLOCK = 0, 1, 2
 DATA_LIN, DATA_LOG = 0, 1
-ORDER_NATURAL, ORDER_MORTON = 0, 1
+ORDER_NATURAL, ORDER_MORTON, ORDER_CLUSTER = 0, 1, 2
 ELID_TET4, ELID_TRI3 = 3, 15
 BASIS_NODAL, BASIS_ELEMENT, BASIS_RASTER = 0, 1, 2
 
@@ -88,6 +88,7 @@ _SIGS = {
     "tmi_solver_set_mode": [P, C.c_int],
     "tmi_solver_set_tile": [P, C.c_int],
     "tmi_solver_set_order": [P, C.c_int],
+    "tmi_node_order": [C.c_int, C.c_int, C.c_int, D, I, I, I],
     "tmi_solver_set_rows_per_wave": [P, C.c_int],
     "tmi_solver_set_spmm_variant": [P, C.c_int],
     "tmi_solver_work_bytes": [P, C.c_int, DP],

@@ -527,13 +527,37 @@ int tmi_solver_set_tile(tmi_solver h, int tile)
 int tmi_solver_set_order(tmi_solver h, int kind)
 {
     API_BEGIN
-    TMI_CHECK(kind == TMI_ORDER_NATURAL || kind == TMI_ORDER_MORTON, TMI_ERR_INVALID_ARG, "unknown node order %d",
-              kind);
+    TMI_CHECK(kind == TMI_ORDER_NATURAL || kind == TMI_ORDER_MORTON || kind == TMI_ORDER_CLUSTER, TMI_ERR_INVALID_ARG,
+              "unknown node order %d", kind);
     Solver &s = S(h);
     if (s.order_kind != kind) {
         s.order_kind = kind;
         s.allocated = s.assembled = s.precon_valid = false;   // Allocate builds the solver's copy anew
     }
+    API_END
+}
+
+int tmi_node_order(int kind, int dim, int n, const double *nodes, const int *rowptr, const int *colidx, int *perm)
+{
+    API_BEGIN
+    TMI_CHECK(kind == TMI_ORDER_NATURAL || kind == TMI_ORDER_MORTON || kind == TMI_ORDER_CLUSTER, TMI_ERR_INVALID_ARG,
+              "unknown node order %d", kind);
+    TMI_CHECK(n > 0 && (dim == 2 || dim == 3) && perm && rowptr && colidx && (nodes || kind == TMI_ORDER_NATURAL),
+              TMI_ERR_INVALID_ARG, "bad node order request");
+    TMI_CHECK(rowptr[0] == 0, TMI_ERR_INVALID_ARG, "row pointers must start at 0");
+    for (int i = 0; i < n; i++)
+        TMI_CHECK(rowptr[i + 1] >= rowptr[i], TMI_ERR_INVALID_ARG, "row pointers must not decrease");
+    const std::vector<int> rp(rowptr, rowptr + n + 1), ci(colidx, colidx + rowptr[n]);
+    for (int c : ci) TMI_CHECK(c >= 0 && c < n, TMI_ERR_INVALID_ARG, "column index %d outside 0..%d", c, n - 1);
+    std::vector<int> p;
+    if (kind == TMI_ORDER_NATURAL) {
+        p.resize(n);
+        for (int i = 0; i < n; i++) p[i] = i;
+    } else {
+        p = morton_order(dim, n, nodes);
+        if (kind == TMI_ORDER_CLUSTER) p = cluster_order(n, rp, ci, p, RB_ROWS);
+    }
+    std::copy(p.begin(), p.end(), perm);
     API_END
 }
 

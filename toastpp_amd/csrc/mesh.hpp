@@ -44,7 +44,7 @@ struct Mesh {
     DBuf<int> d_lptr, d_lq, d_lm, d_lk;   // link list; d_lk[q * nm + m] = link of (q, m) or -1
     DBuf<double> jac_ws;
     // node orders of the solvers' private copies of the system (order.cpp),
-    // one per kind (ORDER_NATURAL, ORDER_MORTON): row r of the relabelled
+    // one per kind (ORDER_NATURAL, ORDER_MORTON, ORDER_CLUSTER): row r of the relabelled
     // system is node perm[r]; its CSR pattern with slot[k] = nonzero of the
     // reference-order pattern held by entry k
     struct NodeOrder {
@@ -67,7 +67,7 @@ struct Mesh {
         // every block has fewer than 65536 coefficients)
         DBuf<unsigned> rb_word;
         bool rb_word_ok = false;
-    } order[2];
+    } order[3];
 
     int geo_stride() const { return dim == 3 ? GEO3 : GEO2; }
     void setup(hipStream_t st);        // Mesh::Setup on the GPU
@@ -75,7 +75,13 @@ struct Mesh {
     const NodeOrder &build_order(int kind, hipStream_t st);
 };
 
-enum { ORDER_NATURAL = 0, ORDER_MORTON = 1 };
+enum { ORDER_NATURAL = 0, ORDER_MORTON = 1, ORDER_CLUSTER = 2 };
+
+// node orders of the solver's copy of the system (order.cpp, host code):
+// perm[r] = mesh node at row r
+std::vector<int> morton_order(int dim, int n, const double *x);
+std::vector<int> cluster_order(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                               const std::vector<int> &seed, int group);
 
 // host parser helpers
 void read_mesh_file(const char *path, int &dim, std::vector<double> &nodes, std::vector<int> &elist,

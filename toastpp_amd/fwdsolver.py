@@ -76,9 +76,10 @@ class FwdSolver:
         L.call("tmi_solver_set_tile", self._h, int(tile))
 
     def SetOrder(self, order):
-        """'morton' (default) or 'natural' node order of the solver's copy of
-        the system; call Allocate afterwards."""
-        L.call("tmi_solver_set_order", self._h, {"natural": L.ORDER_NATURAL, "morton": L.ORDER_MORTON}[order])
+        """'cluster' (default), 'morton' or 'natural' node order of the
+        solver's copy of the system; call Allocate afterwards."""
+        L.call("tmi_solver_set_order", self._h, {"natural": L.ORDER_NATURAL, "morton": L.ORDER_MORTON,
+                                                 "cluster": L.ORDER_CLUSTER}[order])
 
     def SetRowsPerWave(self, rows):
         L.call("tmi_solver_set_rows_per_wave", self._h, int(rows))
