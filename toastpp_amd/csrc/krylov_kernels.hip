@@ -598,8 +598,10 @@ __global__ void __launch_bounds__(256) k_spmm_nk(int n, const int *__restrict__ 
 // in the same column-then-row order) through a 64-entry window that is
 // refilled from a prefetched next window, all broadcast with readlane.  Each
 // row accumulates acc + val * x over its columns in ascending order, the
-// same operation sequence as the per-row kernels.
-template <class V, int MODE, bool MF = false>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
+// same operation sequence as the per-row kernels.  KC = 32: tiles of 32
+// columns, lanes 32-63 idle (the walk costs the same per union entry, so
+// this beats the per-row 32-column kernel once the blocks share gathers).
+template <class V, int MODE, bool MF = false, int KC = 64>   // MODE 0: Y = A X, dots <W,Y> [, <Y,Y>]; MODE 1: Y = W - A X, dot <Y,Y>
 __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int *__restrict__ rowptr,
                                                            const int *__restrict__ bofs, const int *__restrict__ ucol,
                                                            const rb_mask_t *__restrict__ umask,
@@ -611,8 +613,11 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
 {
     constexpr int B = TMI_RB_BATCH;
     static_assert(B * RB_ROWS <= 64, "a batch's coefficients must fit one 64-lane window");
-    const int lane = threadIdx.x & 63, c = lane;
-    const bool act = active[c] != 0;
+    static_assert(KC == 64 || KC == 32, "row-block tiles of 64 or 32 columns");
+    const int lane = threadIdx.x & 63;
+    const bool live = KC == 64 || lane < KC;
+    const int c = live ? lane : 0;   // idle lanes load column 0 (in bounds) and store nothing
+    const bool act = live && active[c] != 0;
     const auto wmf = mf_omega<MF, V>(omg, c);
     const int G = gridDim.x, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
                     // unconditional: j is a column of the block (in bounds);
                     // an inactive column's sums are never stored
                     const int j = __builtin_amdgcn_readlane(jv, min(t0 + u, m - 1));
-                    xb[u] = ld_row(X + (size_t)j * 64, (unsigned)c);
+                    xb[u] = ld_row(X + (size_t)j * KC, (unsigned)c);
                 }
 #pragma unroll
                 for (int u = 0; u < B; u++) {
@@ -667,10 +672,10 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
 #pragma unroll
         for (int r = 0; r < RB_ROWS; r++) {
             if (r < nr && act) {
-                const size_t at = (size_t)(r0 + r) * 64 + c;
+                const size_t at = (size_t)(r0 + r) * KC + c;
                 if (MODE == 0) {
                     Y[at] = acc[r];
-                    const V w = Wq ? sh_unpack(Wq, (size_t)(r0 + r), c) : W[at];   // Wq: packed shadow residual
+                    const V w = (KC == 64 && Wq) ? sh_unpack(Wq, (size_t)(r0 + r), c) : W[at];   // Wq: packed shadow residual
                     dot_epi(w, acc[r], want_yy, d1, d2);
                 } else {
                     const V y = vsub(W[at], acc[r]);
@@ -680,7 +685,7 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
             }
         }
     }
-    block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
+    block_reduce2<KC>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
 }
 
 // ------------------------- pipelined row-block SpMM, 64 columns (variant 4)
@@ -2117,6 +2122,10 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
         hipLaunchKernelGGL((k_spmm_rb<float2, 0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr,     \
                            b.rb.ofs, b.rb.col, b.rb.mask, (const float2 *)b.rb.val, (const float2 *)X,         \
                            (float2 *)Y, (const float2 *)W, want_yy, iflags, partial, nullptr, omg);          \
+    else if (KK == 32 && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val)                                       \
+        hipLaunchKernelGGL((k_spmm_rb<float2, 0, true, 32>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, \
+                           b.rb.ofs, b.rb.col, b.rb.mask, (const float2 *)b.rb.val, (const float2 *)X,         \
+                           (float2 *)Y, (const float2 *)W, want_yy, iflags, partial, nullptr, omg);          \
     else if (KK == 64)                                                                                          \
         hipLaunchKernelGGL((k_spmm64<V, 0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, \
                            (const V *)val, (const V *)X, (V *)Y, (const V *)W, want_yy, iflags, partial, b.rpw,   \
@@ -2174,6 +2183,10 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
     else if (KK == 64 && !idiag && (vk == VK_C32 || vk == VK_C64))                                           \
         hipLaunchKernelGGL((k_spmm64<V, 0>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                            (V *)Y, (const V *)W, (int)want_yy, iflags, partial, b.rpw);                      \
+    else if (KK == 32 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val)                           \
+        hipLaunchKernelGGL((k_spmm_rb<float2, 0, false, 32>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, \
+                           b.rb.ofs, b.rb.col, b.rb.mask, (const float2 *)b.rb.val, (const float2 *)X,          \
+                           (float2 *)Y, (const float2 *)W, (int)want_yy, iflags, partial, nullptr);         \
     else if (idiag)                                                                                                \
         hipLaunchKernelGGL((k_spmm_dot<V, KK, true>), g, blk, 0, st, b.n, b.rowptr, b.colidx, (const V *)val,  \
                            (const V *)idiag, (const V *)X, (V *)Y, (const V *)W, (int)want_yy, iflags, partial); \
