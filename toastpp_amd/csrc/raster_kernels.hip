@@ -613,73 +613,85 @@ void launch_raster_map(const Raster &r, int ncols, const double *d_f, int ld, do
 // ImageGradient (jacobian.cc:944-1060) at the solution voxels: for column
 // c and solution voxel s, fgs[c][s] = (value, d/dx, d/dy, d/dz) with the
 // reference's spacing gsize/n and mask rules (neighbours tested first).
-__device__ inline double2 grad1(const double2 *__restrict__ v, long ldv, const int *__restrict__ mask, bool nomask,
-                                long g, int pos, int len, long s, double ih, double i2h)
-{
-    // without a mask every grid point counts as inside (ImageGradient with
-    // mask == 0, jacobian.cc:953-958)
-    const bool bcnt = nomask || mask[g] >= 0;
-    const bool blo = pos > 0 && (nomask || mask[g - s] >= 0);
-    const bool bhi = pos < len - 1 && (nomask || mask[g + s] >= 0);
-    if (blo && bhi) {
-        const double2 a = v[(g + s) * ldv], b = v[(g - s) * ldv];
-        return make_double2((a.x - b.x) * i2h, (a.y - b.y) * i2h);
-    }
-    if (!bcnt) return make_double2(0.0, 0.0);
-    if (blo) {
-        const double2 a = v[g * ldv], b = v[(g - s) * ldv];
-        return make_double2((a.x - b.x) * ih, (a.y - b.y) * ih);
-    }
-    if (bhi) {
-        const double2 a = v[(g + s) * ldv], b = v[g * ldv];
-        return make_double2((a.x - b.x) * ih, (a.y - b.y) * ih);
-    }
-    return make_double2(0.0, 0.0);
-}
-
 struct GradDesc {
     int nx, ny, nz, dim;
     double ih[3], i2h[3];
 };
 
 // columns c >= unmasked_from take their gradients without the mask.  Block
-// of 64 column lanes x 4 voxel lanes over 16 consecutive voxels: the reads
-// of a grid point's columns are contiguous across the lanes, and the 16
-// voxels of a column are written as one contiguous run.
+// of GG_V consecutive solution voxels x 64 columns: each wave takes every
+// 4th voxel of the block, lanes over the columns, so that the reads of a grid
+// point's columns are contiguous (one 1 KB row of fgv per value or
+// neighbour); which neighbours count (mask, grid edges) is decided once per
+// voxel (wave-uniform).  The records go to an LDS tile [column][voxel][4] and
+// leave it column by column: the GG_V voxels of a column are one contiguous
+// 1 KB run of fgs.
 constexpr int GG_V = 16;
 
 __global__ void __launch_bounds__(256) k_grid_grad(int slen, int ncols, const int *__restrict__ s2b,
                                                    const int *__restrict__ mask, const double2 *__restrict__ fgv,
                                                    GradDesc gd, double2 *__restrict__ fgs, int unmasked_from)
 {
-    const int c0 = threadIdx.x & 63, vl = threadIdx.x >> 6;
+    __shared__ double2 tile[64][GG_V][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long ldv = ncols;
     const int len[3] = {gd.nx, gd.ny, gd.nz};
     const long str[3] = {1, gd.nx, (long)gd.nx * gd.ny};
-    for (int i = vl; i < GG_V; i += 4) {
-        const int s = blockIdx.x * GG_V + i;
-        if (s >= slen) break;
-        const long g = s2b[s];
-        const int pos[3] = {(int)(g % gd.nx), (int)((g / gd.nx) % gd.ny), (int)(g / ((long)gd.nx * gd.ny))};
-        for (int c = c0; c < ncols; c += 64) {
-            const double2 *v = fgv + c;    // column c of [g][ncols]
-            double2 *o = fgs + ((long)c * slen + s) * 4;
-            o[0] = v[g * ldv];
-            for (int d = 0; d < 3; d++)
-                o[1 + d] = d < gd.dim ? grad1(v, ldv, mask, c >= unmasked_from, g, pos[d], len[d], str[d], gd.ih[d],
-                                              gd.i2h[d])
-                                      : make_double2(0.0, 0.0);
+    const int s0 = blockIdx.x * GG_V, nv = min(GG_V, slen - s0);
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        const int c = c0 + lane;
+        const int um = c >= unmasked_from ? 1 : 0;
+        for (int i = w; i < nv; i += 4) {
+            const long g = s2b[s0 + i];
+            const int pos[3] = {(int)(g % gd.nx), (int)((g / gd.nx) % gd.ny), (int)(g / ((long)gd.nx * gd.ny))};
+            const bool ctm = mask[g] >= 0;
+            double2 rec[4];
+            const double2 *v = fgv + (c < ncols ? c : 0);   // column c of [g][ncols]
+            const double2 vc = v[g * ldv];
+            rec[0] = vc;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double2 r = make_double2(0.0, 0.0);
+                if (d < gd.dim) {
+                    const bool in_lo = pos[d] > 0, in_hi = pos[d] < len[d] - 1;
+                    const bool mlo = in_lo && mask[g - str[d]] >= 0, mhi = in_hi && mask[g + str[d]] >= 0;
+                    const bool bl = um ? in_lo : mlo, bh = um ? in_hi : mhi, ct = um ? true : ctm;
+                    if (bl && bh) {
+                        const double2 a = v[(g + str[d]) * ldv], b = v[(g - str[d]) * ldv];
+                        r = make_double2((a.x - b.x) * gd.i2h[d], (a.y - b.y) * gd.i2h[d]);
+                    } else if (ct && bl) {
+                        const double2 b = v[(g - str[d]) * ldv];
+                        r = make_double2((vc.x - b.x) * gd.ih[d], (vc.y - b.y) * gd.ih[d]);
+                    } else if (ct && bh) {
+                        const double2 a = v[(g + str[d]) * ldv];
+                        r = make_double2((a.x - vc.x) * gd.ih[d], (a.y - vc.y) * gd.ih[d]);
+                    }
+                }
+                rec[1 + d] = r;
+            }
+#pragma unroll
+            for (int d = 0; d < 4; d++) tile[lane][i][d] = rec[d];
         }
+        __syncthreads();
+        // column runs: thread t writes record part e = t + 256 k of the
+        // chunk, (column, voxel, part) with the part fastest
+        const int nc = min(64, ncols - c0);
+        for (int e = threadIdx.x; e < nc * GG_V * 4; e += 256) {
+            const int cc = e / (GG_V * 4), r = e % (GG_V * 4), i = r >> 2, d = r & 3;
+            if (i < nv) fgs[((long)(c0 + cc) * slen + s0 + i) * 4 + d] = tile[cc][i][d];
+        }
+        __syncthreads();
     }
 }
 
-// J rows through the detectors: block (voxel tile of 64, group of JG_M
-// detectors), thread (voxel, detector m) holds the adjoint values and
-// gradients of m at its voxel and walks the sources q linked to m, reading
-// the direct ones (shared by the block's detectors, so each adjoint value
-// is read once and each direct one once per detector group).  PMDF_mua =
-// -d a, PMDF_kap = -sum_d grad_d d grad_d a (jacobian.cc:57-79), divided by
-// the projection for log data, times the voxel size sz.
+// J rows through the detectors: block (voxel tile of 64, group of 2 JG_M
+// detectors), thread (voxel, detectors m and m + JG_M) holds the adjoint
+// values and gradients of its two detectors at its voxel and walks the
+// sources q, reading the direct ones once for both (shared by the block's
+// detectors through the cache, so each adjoint value is read once and each
+// direct one once per detector group).  PMDF_mua = -d a, PMDF_kap = -sum_d
+// grad_d d grad_d a (jacobian.cc:57-79), divided by the projection for log
+// data (one reciprocal of |proj|^2 per link), times the voxel size sz.
 constexpr int JG_M = 8;
 
 __global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm, int dim,
@@ -688,38 +700,51 @@ __global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm
                                                         int nqm, double *__restrict__ J)
 {
     const int s = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int m = blockIdx.y * JG_M + (threadIdx.x >> 6);
-    if (s >= slen || m >= nm) return;
-    const double2 *A = fgs + ((long)(nq + m) * slen + s) * 4;
-    const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3];
+    const int mb = blockIdx.y * 2 * JG_M + (threadIdx.x >> 6);
+    if (s >= slen) return;
+    double2 a[2][4];
+    bool mv[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int m = mb + t * JG_M;
+        mv[t] = m < nm;
+        const double2 *A = fgs + ((long)(nq + min(m, nm - 1)) * slen + s) * 4;
+#pragma unroll
+        for (int d = 0; d < 4; d++) a[t][d] = A[d];
+    }
     const size_t ld = 2 * (size_t)slen;
     for (int q = 0; q < nq; q++) {
-        const int k = lk[(long)q * nm + m];
-        if (k < 0) continue;
         const double2 *D = fgs + ((long)q * slen + s) * 4;
         const double2 d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
-        double pm_r = -(d0.x * a0.x - d0.y * a0.y), pm_i = -(d0.x * a0.y + d0.y * a0.x);
-        double pk_r = 0.0, pk_i = 0.0;
-        pk_r -= d1.x * a1.x - d1.y * a1.y;
-        pk_i -= d1.x * a1.y + d1.y * a1.x;
-        pk_r -= d2.x * a2.x - d2.y * a2.y;
-        pk_i -= d2.x * a2.y + d2.y * a2.x;
-        if (dim > 2) {
-            pk_r -= d3.x * a3.x - d3.y * a3.y;
-            pk_i -= d3.x * a3.y + d3.y * a3.x;
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            if (!mv[t]) continue;
+            const int k = lk[(long)q * nm + mb + t * JG_M];
+            if (k < 0) continue;
+            const double2 a0 = a[t][0], a1 = a[t][1], a2 = a[t][2], a3 = a[t][3];
+            double pm_r = -(d0.x * a0.x - d0.y * a0.y), pm_i = -(d0.x * a0.y + d0.y * a0.x);
+            double pk_r = 0.0, pk_i = 0.0;
+            pk_r -= d1.x * a1.x - d1.y * a1.y;
+            pk_i -= d1.x * a1.y + d1.y * a1.x;
+            pk_r -= d2.x * a2.x - d2.y * a2.y;
+            pk_i -= d2.x * a2.y + d2.y * a2.x;
+            if (dim > 2) {
+                pk_r -= d3.x * a3.x - d3.y * a3.y;
+                pk_i -= d3.x * a3.y + d3.y * a3.x;
+            }
+            if (logscale) {
+                const double2 p = proj[k];
+                const double id = 1.0 / (p.x * p.x + p.y * p.y);
+                const double ar = (pm_r * p.x + pm_i * p.y) * id, ai = (pm_i * p.x - pm_r * p.y) * id;
+                const double br = (pk_r * p.x + pk_i * p.y) * id, bi = (pk_i * p.x - pk_r * p.y) * id;
+                pm_r = ar; pm_i = ai; pk_r = br; pk_i = bi;
+            }
+            // J is written once: streaming stores keep it out of the caches
+            __builtin_nontemporal_store(pm_r * sz, J + (size_t)k * ld + s);
+            __builtin_nontemporal_store(pm_i * sz, J + (size_t)(k + nqm) * ld + s);
+            __builtin_nontemporal_store(pk_r * sz, J + (size_t)k * ld + slen + s);
+            __builtin_nontemporal_store(pk_i * sz, J + (size_t)(k + nqm) * ld + slen + s);
         }
-        if (logscale) {
-            const double2 p = proj[k];
-            const double den = p.x * p.x + p.y * p.y;
-            const double ar = (pm_r * p.x + pm_i * p.y) / den, ai = (pm_i * p.x - pm_r * p.y) / den;
-            const double br = (pk_r * p.x + pk_i * p.y) / den, bi = (pk_i * p.x - pk_r * p.y) / den;
-            pm_r = ar; pm_i = ai; pk_r = br; pk_i = bi;
-        }
-        // J is written once: streaming stores keep it out of the caches
-        __builtin_nontemporal_store(pm_r * sz, J + (size_t)k * ld + s);
-        __builtin_nontemporal_store(pm_i * sz, J + (size_t)(k + nqm) * ld + s);
-        __builtin_nontemporal_store(pk_r * sz, J + (size_t)k * ld + slen + s);
-        __builtin_nontemporal_store(pk_i * sz, J + (size_t)(k + nqm) * ld + slen + s);
     }
 }
 
@@ -895,14 +920,15 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
                           bool cw, double *d_Jkap)
 {
     const double sz = grid_fields(r, nq, nm, d_dphi, ldd, d_aphi, lda, cw ? nq : nq + nm, st);
-    const dim3 grid((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + JG_M - 1) / JG_M));
+    const dim3 grid((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + JG_M - 1) / JG_M));   // CW kernel
+    const dim3 grid2((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + 2 * JG_M - 1) / (2 * JG_M)));
     const dim3 blk(64 * JG_M);
     if (!r.has_g) {   // solution voxels = grid points inside the mesh
         if (cw)
             hipLaunchKernelGGL(k_jac_grid_cw, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
                                d_lk, d_proj, logscale, sz, d_J, d_Jkap);
         else
-            hipLaunchKernelGGL(k_jac_grid, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
+            hipLaunchKernelGGL(k_jac_grid, grid2, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p,
                                d_lk, (const double2 *)d_proj, logscale, sz, nqm, d_J);
         TMI_LAUNCH_CHECK();
         return;
@@ -923,7 +949,7 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
         return;
     }
     DBuf<double> jg((size_t)4 * nqm * gs);
-    hipLaunchKernelGGL(k_jac_grid, grid, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p, d_lk,
+    hipLaunchKernelGGL(k_jac_grid, grid2, blk, 0, st, r.gslen, nq, nm, r.dim, (const double2 *)r.fgs.p, d_lk,
                        (const double2 *)d_proj, logscale, 1.0, nqm, jg.p);
     TMI_LAUNCH_CHECK();
     const long ldo = 2 * (long)r.slen;
