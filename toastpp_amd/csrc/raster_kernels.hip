@@ -684,15 +684,19 @@ __global__ void __launch_bounds__(256) k_grid_grad(int slen, int ncols, const in
     }
 }
 
-// J rows through the detectors: block (voxel tile of 64, group of 2 JG_M
-// detectors), thread (voxel, detectors m and m + JG_M) holds the adjoint
-// values and gradients of its two detectors at its voxel and walks the
-// sources q, reading the direct ones once for both (shared by the block's
+// J rows through the detectors: block (voxel tile of 64, group of JG_T JG_M
+// detectors), thread (voxel, detectors m + t JG_M, t < JG_T) holds the
+// adjoint values and gradients of its detectors at its voxel and walks the
+// sources q, reading the direct ones once for all of them (shared by the block's
 // detectors through the cache, so each adjoint value is read once and each
 // direct one once per detector group).  PMDF_mua = -d a, PMDF_kap = -sum_d
 // grad_d d grad_d a (jacobian.cc:57-79), divided by the projection for log
 // data (one reciprocal of |proj|^2 per link), times the voxel size sz.
 constexpr int JG_M = 8;
+#ifndef TMI_JG_T
+#define TMI_JG_T 4
+#endif
+constexpr int JG_T = TMI_JG_T;
 
 __global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm, int dim,
                                                         const double2 *__restrict__ fgs, const int *__restrict__ lk,
@@ -700,12 +704,12 @@ __global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm
                                                         int nqm, double *__restrict__ J)
 {
     const int s = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int mb = blockIdx.y * 2 * JG_M + (threadIdx.x >> 6);
+    const int mb = blockIdx.y * JG_T * JG_M + (threadIdx.x >> 6);
     if (s >= slen) return;
-    double2 a[2][4];
-    bool mv[2];
+    double2 a[JG_T][4];
+    bool mv[JG_T];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < JG_T; t++) {
         const int m = mb + t * JG_M;
         mv[t] = m < nm;
         const double2 *A = fgs + ((long)(nq + min(m, nm - 1)) * slen + s) * 4;
@@ -717,7 +721,7 @@ __global__ void __launch_bounds__(64 * JG_M) k_jac_grid(int slen, int nq, int nm
         const double2 *D = fgs + ((long)q * slen + s) * 4;
         const double2 d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
+        for (int t = 0; t < JG_T; t++) {
             if (!mv[t]) continue;
             const int k = lk[(long)q * nm + mb + t * JG_M];
             if (k < 0) continue;
@@ -921,7 +925,7 @@ void launch_jacobian_grid(Raster &r, int nq, int nm, const double *d_dphi, int l
 {
     const double sz = grid_fields(r, nq, nm, d_dphi, ldd, d_aphi, lda, cw ? nq : nq + nm, st);
     const dim3 grid((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + JG_M - 1) / JG_M));   // CW kernel
-    const dim3 grid2((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + 2 * JG_M - 1) / (2 * JG_M)));
+    const dim3 grid2((unsigned)((r.gslen + 63) / 64), (unsigned)((nm + JG_T * JG_M - 1) / (JG_T * JG_M)));
     const dim3 blk(64 * JG_M);
     if (!r.has_g) {   // solution voxels = grid points inside the mesh
         if (cw)
