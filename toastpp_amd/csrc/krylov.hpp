@@ -21,7 +21,7 @@ enum ScalarSlot {
     S_NSLOT
 };
 // offsets into the per-column int array
-enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_NSLOT };
+enum IntSlot { I_ACTIVE = 0, I_ITERS, I_FIRST, I_BREAK, I_STALL, I_XPEND, I_NSLOT };
 
 // partial-sum slots written by the reduction epilogues (COCG's update
 // kernel writes three: the complex <r, D^-1 r> and ||r||^2)
@@ -120,11 +120,15 @@ void launch_pack_shadow(int n, const void *rd, unsigned char *q, hipStream_t st)
 // and ||r||^2 in three slots
 // omg (COCG only): per-column frequencies, idiag then holds the diagonal
 // itself with its imaginary part at omega = 1
+// xdefer (COCG only): leave x += alpha d to the direction update, which
+// then takes x (launch_cg_d with x given) and applies it for the columns
+// whose alpha step ran this iteration (I_XPEND, set by FIN_COCG_ALPHA):
+// x and d are then streamed once per iteration instead of twice
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
                   const double *sc, const int *ic, double *partial, hipStream_t st, bool bil = false,
-                  const double *omg = nullptr);
+                  const double *omg = nullptr, bool xdefer = false);
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st, bool bil = false, const double *omg = nullptr);
+                 const int *ic, hipStream_t st, bool bil = false, const double *omg = nullptr, void *x = nullptr);
 // first PCG step: d = D^-1 r, partials [<r, D^-1 r>, <r, r>]
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
                     const int *ic, double *partial, hipStream_t st, bool bil = false, const double *omg = nullptr);

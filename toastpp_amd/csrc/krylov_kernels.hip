@@ -1568,7 +1568,9 @@ __device__ inline V prec_apply(V di, V r, const double *__restrict__ omg, int c)
 }
 
 // x += alpha d; r -= alpha q; partials r^T D^-1 r (re, im) and ||r||^2
-template <class V, int K>
+// XD: the x update x += alpha d is left to the direction update (k_cocg_d<.., true>),
+// which reads d anyway; this kernel then streams only r and q
+template <class V, int K, bool XD = false>
 __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ idiag, V *__restrict__ x,
                                                  const V *__restrict__ d, const V *__restrict__ q,
                                                  V *__restrict__ r, const double *__restrict__ sc,
@@ -1581,9 +1583,13 @@ __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ id
         const double ar = sc[S_ALPHA * K + c], ai = sc[S_ALPHA_I * K + c];
         struct L { V x, d, r, q, di; };
         ew_for(
-            tot, [&](size_t t) { return L{ldnt(x + t), ldnt(d + t), ldnt(r + t), ldnt(q + t), idiag ? idiag[t / K] : V{}}; },
+            tot,
+            [&](size_t t) {
+                return L{XD ? V{} : ldnt(x + t), XD ? V{} : ldnt(d + t), ldnt(r + t), ldnt(q + t),
+                         idiag ? idiag[t / K] : V{}};
+            },
             [&](size_t t, const L &a) {
-                stnt(x + t, vadd(a.x, cscale(a.d, ar, ai)));
+                if (!XD) stnt(x + t, vadd(a.x, cscale(a.d, ar, ai)));
                 const V rn = vsub(a.r, cscale(a.q, ar, ai));
                 stnt(r + t, rn);
                 const V z = idiag ? prec_apply(a.di, rn, omg, c) : rn;
@@ -1596,26 +1602,38 @@ __global__ void __launch_bounds__(256) k_cocg_xr(int n, const V *__restrict__ id
 
 // d = beta d + D^-1 r (d = D^-1 r on the first call); with partials (start
 // of the iteration): r^T D^-1 r (re, im) and ||r||^2
-template <class V, int K>
+template <class V, int K, bool XD = false>
 __global__ void __launch_bounds__(256) k_cocg_d(int n, const V *__restrict__ idiag, V *__restrict__ d,
                                                 const V *__restrict__ r, const double *__restrict__ sc,
                                                 const int *__restrict__ ic, double *__restrict__ partial,
-                                                const double *__restrict__ omg)
+                                                const double *__restrict__ omg, V *__restrict__ x)
 {
     EW_PROLOGUE
     double d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    if (ic[I_ACTIVE * K + c]) {
+    const bool act = ic[I_ACTIVE * K + c] != 0;
+    // XD: the x update of the iteration, for the columns whose alpha step ran
+    // (I_XPEND; a column the beta step just stopped still takes it)
+    const bool xp = XD && ic[I_XPEND * K + c] != 0;
+    if (act || xp) {
         const bool first = ic[I_FIRST * K + c] != 0;
         const double br = sc[S_BETA * K + c], bi = sc[S_BETA_I * K + c];
-        struct L { V r, d, di; };
+        const double ar = XD ? sc[S_ALPHA * K + c] : 0.0, ai = XD ? sc[S_ALPHA_I * K + c] : 0.0;
+        struct L { V r, d, di, x; };
         ew_for(
-            tot, [&](size_t t) { return L{ldnt(r + t), first ? V{} : ldnt(d + t), idiag ? idiag[t / K] : V{}}; },
+            tot,
+            [&](size_t t) {
+                return L{act ? ldnt(r + t) : V{}, (first && !xp) ? V{} : ldnt(d + t),
+                         (act && idiag) ? idiag[t / K] : V{}, xp ? ldnt(x + t) : V{}};
+            },
             [&](size_t t, const L &a) {
-                const V z = idiag ? prec_apply(a.di, a.r, omg, c) : a.r;
-                stnt(d + t, first ? z : vadd(cscale(a.d, br, bi), z));
-                if (partial) {
-                    bil_add(a.r, z, d1, d2);
-                    d3 += dotR(a.r, a.r);
+                if (xp) stnt(x + t, vadd(a.x, cscale(a.d, ar, ai)));
+                if (act) {
+                    const V z = idiag ? prec_apply(a.di, a.r, omg, c) : a.r;
+                    stnt(d + t, first ? z : vadd(cscale(a.d, br, bi), z));
+                    if (partial) {
+                        bil_add(a.r, z, d1, d2);
+                        d3 += dotR(a.r, a.r);
+                    }
                 }
             });
     }
@@ -1853,6 +1871,10 @@ __device__ inline void fin_column(int fk, double r0, double r1, double r2, doubl
         if (r2 <= SC(S_TOL2)) IC(I_ACTIVE) = 0;
         break;
     case FIN_COCG_ALPHA: {
+        // I_XPEND: the columns whose x update this iteration owes, the ones
+        // the update kernel runs for (the direction update applies it when
+        // the update kernel defers it)
+        IC(I_XPEND) = 0;
         if (!IC(I_ACTIVE)) break;
         double ar, ai;
         if (!cdiv(SC(S_RHO), SC(S_RHO_I), r0, r1, ar, ai) || (ar == 0.0 && ai == 0.0)) {
@@ -1862,6 +1884,7 @@ __device__ inline void fin_column(int fk, double r0, double r1, double r2, doubl
         }
         SC(S_ALPHA) = ar;
         SC(S_ALPHA_I) = ai;
+        IC(I_XPEND) = IC(I_ACTIVE);
         break;
     }
     case FIN_COCG_BETA: {
@@ -2282,16 +2305,22 @@ void launch_bicg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, co
 }
 
 void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, const void *d, const void *q, void *r,
-                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil, const double *omg)
+                  const double *sc, const int *ic, double *partial, hipStream_t st, bool bil, const double *omg,
+                  bool xdefer)
 {
+    TMI_CHECK(!xdefer || bil, TMI_ERR_UNSUPPORTED, "the deferred x update is COCG's");
     TMI_CHECK(!omg || bil, TMI_ERR_UNSUPPORTED, "per-column frequencies need COCG");
     dim3 g(b.nblk), blk(256);
 #define CXK(KK)                                                                                               \
     hipLaunchKernelGGL((k_cg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,           \
                        (const V *)q, (V *)r, sc, ic, partial);
 #define BXK(KK)                                                                                               \
-    hipLaunchKernelGGL((k_cocg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,         \
-                       (const V *)q, (V *)r, sc, ic, partial, omg);
+    if (xdefer)                                                                                               \
+        hipLaunchKernelGGL((k_cocg_xr<V, KK, true>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d, \
+                           (const V *)q, (V *)r, sc, ic, partial, omg);                                       \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_cocg_xr<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)x, (const V *)d,     \
+                           (const V *)q, (V *)r, sc, ic, partial, omg);
     if (bil) {
         DISPATCH_C(vk, DISPATCH_K(b.K, BXK))
     } else {
@@ -2303,9 +2332,20 @@ void launch_cg_xr(VKind vk, const BlockDesc &b, const void *idiag, void *x, cons
 }
 
 void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
-                 const int *ic, hipStream_t st, bool bil, const double *omg)
+                 const int *ic, hipStream_t st, bool bil, const double *omg, void *x)
 {
-    launch_cg_init(vk, b, idiag, d, r, sc, ic, nullptr, st, bil, omg);
+    if (!x) {
+        launch_cg_init(vk, b, idiag, d, r, sc, ic, nullptr, st, bil, omg);
+        return;
+    }
+    TMI_CHECK(bil, TMI_ERR_UNSUPPORTED, "the deferred x update is COCG's");
+    dim3 g(b.nblk), blk(256);
+#define BDK(KK)                                                                                               \
+    hipLaunchKernelGGL((k_cocg_d<V, KK, true>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, \
+                       nullptr, omg, (V *)x);
+    DISPATCH_C(vk, DISPATCH_K(b.K, BDK))
+#undef BDK
+    TMI_LAUNCH_CHECK();
 }
 
 void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, const void *r, const double *sc,
@@ -2316,7 +2356,8 @@ void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, co
 #define CIK(KK) \
     hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
 #define BIK(KK) \
-    hipLaunchKernelGGL((k_cocg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial, omg);
+    hipLaunchKernelGGL((k_cocg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial, omg, \
+                       (V *)nullptr);
     if (bil) {
         DISPATCH_C(vk, DISPATCH_K(b.K, BIK))
     } else {

@@ -494,13 +494,16 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     auto period_launches = [&](bool ru_due) {
         for (int k = 0; k < CHECK_EVERY; k++) {
             if (use_cg) {
+                // COCG defers x += alpha p to the direction update, except in
+                // the iteration whose reliable update folds x between the two
+                const bool rel = k == CHECK_EVERY - 1 && ru_due, xd = bil && !rel;
                 launch_spmm_dot(vk, bd, val_in, nullptr, wp->p.p, wp->v.p, wp->p.p, cg_dot, ic + I_ACTIVE * K, part,
                                 st, nullptr, omg);
                 launch_finalize(bil ? FIN_COCG_ALPHA : FIN_CG_ALPHA, bd, part, sc, ic, 0, 0, 0, st);
-                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st, bil, omg);
+                launch_cg_xr(vk, bd, idiag_in, wp->xi.p, wp->p.p, wp->v.p, wp->ri.p, sc, ic, part, st, bil, omg, xd);
                 launch_finalize(bil ? FIN_COCG_BETA : FIN_CG_BETA, bd, part, sc, ic, maxit_total, 0, 0, st);
-                if (k == CHECK_EVERY - 1 && ru_due) reliable_launches();
-                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil, omg);
+                if (rel) reliable_launches();
+                launch_cg_d(vk, bd, idiag_in, wp->p.p, wp->ri.p, sc, ic, st, bil, omg, xd ? wp->xi.p : nullptr);
             } else {
                 launch_bicg_p(vk, bd, wp->ri.p, wp->p.p, wp->v.p, sc, ic, st);
                 launch_spmm_dot(vk, bd, op_in, nullptr, wp->p.p, wp->v.p, wp->rdi.p, false, ic + I_ACTIVE * K, part,
