@@ -1950,9 +1950,29 @@ __global__ void __launch_bounds__(256) k_finalize(int fk, int G, int K, const do
     // the wave's lanes; then the four waves in order
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nslot = (fk == FIN_COCG_INIT || fk == FIN_COCG_BETA || fk == FIN_COCG_RELIABLE) ? 3 : 2;
-    for (int slot = 0; slot < nslot; slot++) {
+    // every slot's first FIN_PER partials per thread are loaded before any
+    // is summed (one memory round trip instead of one per slot and chunk);
+    // the sum order per thread stays b = tid, tid + 256, ...
+    // (indices past G are clamped to G - 1, in bounds, and not summed)
+    constexpr int FIN_PER = 8;
+    double v[RED_SLOTS][FIN_PER];
+#pragma unroll
+    for (int slot = 0; slot < RED_SLOTS; slot++)
+#pragma unroll
+        for (int i = 0; i < FIN_PER; i++) {
+            const int b = min((int)threadIdx.x + i * 256, G - 1);
+            v[slot][i] = slot < nslot ? partial[((size_t)slot * K + c) * G + b] : 0.0;
+        }
+#pragma unroll
+    for (int slot = 0; slot < RED_SLOTS; slot++) {
+        if (slot >= nslot) break;
         double s = 0.0;
-        for (int b = threadIdx.x; b < G; b += 256) s += partial[((size_t)slot * K + c) * G + b];
+#pragma unroll
+        for (int i = 0; i < FIN_PER; i++) {
+            const double t = s + v[slot][i];
+            s = (int)threadIdx.x + i * 256 < G ? t : s;
+        }
+        for (int b = threadIdx.x + FIN_PER * 256; b < G; b += 256) s += partial[((size_t)slot * K + c) * G + b];
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) red[slot][wave] = s;
     }

@@ -469,14 +469,18 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
         auto mv = [&](int bytes, long rows, const void *src, void *dst) {
             launch_select_cols(bytes, rows, K, K2, ns, d_sel, src, dst, st);
         };
+        // r64 is recomputed (b - A x) before every read; CG/COCG read no
+        // shadow residual, and their v (= A p) is rewritten by the next
+        // SpMM; BiCGSTAB's s and t are rewritten before they are read
         mv(s64, n, wp->x64.p, w2.x64.p);
         mv(s64, n, wp->b64.p, w2.b64.p);
-        mv(s64, n, wp->r64.p, w2.r64.p);
         mv(sin, n, wp->xi.p, w2.xi.p);
         mv(sin, n, wp->ri.p, w2.ri.p);
-        mv(sin, n, wp->rdi.p, w2.rdi.p);
         mv(sin, n, wp->p.p, w2.p.p);
-        mv(sin, n, wp->v.p, w2.v.p);   // s and t are rewritten before they are read
+        if (!use_cg) {
+            mv(sin, n, wp->rdi.p, w2.rdi.p);
+            mv(sin, n, wp->v.p, w2.v.p);
+        }
         mv(8, S_NSLOT, sc, w2.sc.p);
         mv(8, S_NSLOT, osc, w2.osc.p);
         mv(4, I_NSLOT, ic, w2.ic.p);
