@@ -17,7 +17,7 @@ TMI = pytest.importorskip("toastpp_amd")
 from toastpp_amd import _lib as L  # noqa: E402
 from toastpp_amd import meshgen as G  # noqa: E402
 
-VARIANTS = ["lds", "direct", "dma", "rowblock", "rowblock_pipelined"]
+VARIANTS = ["lds", "direct", "dma", "rowblock", "rowblock_pipelined", "rowblock_mfma"]
 
 
 @pytest.fixture(scope="module")
@@ -54,8 +54,9 @@ def test_variant_fields_vs_oracle_3d(golden_3d, variant):
 
 
 def test_variants_agree_on_cylinder():
-    """31.5k-node cylinder, 64 sources at 100 MHz, tolerance 1e-10: the three
-    variants' fields agree to 1e-8 relative L2 per source."""
+    """31.5k-node cylinder, 64 sources at 100 MHz, tolerance 1e-10: every
+    variant's fields agree with the direct-gather variant's to 1e-8
+    relative L2 per source."""
     nodes, el = G.cylinder(25.0, 50.0, 20, 24)
     Q = np.vstack([G.ring(32, 25.0, z) for z in (-12.5, 12.5)])
     mesh = TMI.Mesh(data=(nodes, el))
@@ -77,7 +78,7 @@ def test_variants_agree_on_cylinder():
         stats = f.CalcFieldsDevice(nq, dB.ptr, dX.ptr)
         assert all(s[3] for s in stats), variant
         out[variant] = dX.to_host()
-    for variant in ("lds", "dma", "rowblock", "rowblock_pipelined"):
+    for variant in ("lds", "dma", "rowblock", "rowblock_pipelined", "rowblock_mfma"):
         rel = np.linalg.norm(out[variant] - out["direct"], axis=0) / np.linalg.norm(out["direct"], axis=0)
         assert rel.max() <= 1e-8, "%s vs direct: worst column rel L2 %.3g" % (variant, rel.max())
     # the pipelined row-block kernel sums every row in the row-block kernel's
@@ -107,3 +108,29 @@ def test_narrow_tiles_vs_oracle_3d(golden_3d, mode, tile):
     for i in range(4):
         rel = np.linalg.norm(phi[i] - ref[i]) / np.linalg.norm(ref[i])
         assert rel <= 1e-8, "%s tile %d column %d rel L2 %.3g" % (mode, tile, i, rel)
+
+
+@pytest.mark.parametrize("solver", ["BICGSTAB", "COCG"])
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_matrix_core_rowblock_tiles_vs_oracle_3d(golden_3d, solver, tile):
+    """The matrix-core row-block kernel on its three tile widths (16, 32 and
+    64 columns; 64 with the packed shadow residual under BiCGSTAB), both
+    complex solvers, against the FP64 oracle."""
+    case, ref = golden_3d
+    gm = TMI.Mesh(case["mesh"])
+    gm.ReadQM(case["qm"])
+    n = gm.NodeCount()
+    fws = TMI.FwdSolver(gm, solver, 1e-12, complex=True)
+    fws.SetMode("mixed")
+    fws.SetPrecon("DIAG")
+    fws.SetSpmmVariant("rowblock_mfma")
+    fws.SetTile(tile)
+    fws.Allocate()
+    fws.Reset(TMI.Solution(np.full(n, case["mua"]), np.full(n, case["mus"]), np.full(n, case["n"])),
+              case["freq"] * 2 * np.pi * 1e-6)
+    qv = sparse.csr_matrix(gm.Qvec(width=2.0).T)
+    phi = fws.CalcFields(qv)
+    assert all(s[3] for s in fws.stats)
+    for i in range(4):
+        rel = np.linalg.norm(phi[i] - ref[i]) / np.linalg.norm(ref[i])
+        assert rel <= 1e-8, "%s tile %d column %d rel L2 %.3g" % (solver, tile, i, rel)

@@ -572,9 +572,10 @@ int tmi_solver_set_rows_per_wave(tmi_solver h, int rows)
 int tmi_solver_set_spmm_variant(tmi_solver h, int variant)
 {
     API_BEGIN
-    TMI_CHECK(variant >= 0 && variant <= 4, TMI_ERR_INVALID_ARG,
+    TMI_CHECK(variant >= 0 && variant <= 5, TMI_ERR_INVALID_ARG,
               "SpMM variant must be 0 (LDS, register staging), 1 (direct), 2 (LDS DMA, double-buffered), 3 "
-              "(row blocks with shared gathers) or 4 (row blocks, software-pipelined)");
+              "(row blocks with shared gathers), 4 (row blocks, software-pipelined) or 5 (row blocks on the "
+              "matrix cores)");
     S(h).spmm_variant = variant;
     API_END
 }

@@ -68,6 +68,7 @@ struct RowUnions {
     const void *val = nullptr;           // coefficients in (column, row) order per block
     const void *src = nullptr;           // the operator (CSR values) they were taken from
     const unsigned *word = nullptr;      // packed mask | offset words: the pipelined kernel (variant 4)
+    bool mfma = false;                   // matrix-core kernel (variant 5; needs word)
 };
 
 struct BlockDesc {

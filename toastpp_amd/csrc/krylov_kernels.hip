@@ -910,6 +910,288 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rbp(int n, const int
     block_reduce2<64>(d1, d2, partial, blockIdx.x, G, MODE == 0 && want_yy != 0);
 }
 
+// --------------------- row-block SpMM on the matrix cores (variant 5)
+// The block of RB_ROWS = 8 rows times the neighbour rows of its column
+// union is a small dense product: Y_b (8 x KC, complex) = C_b X_b, with C_b
+// the 8 x U coefficients (zero where a row lacks the column) and X_b the U
+// gathered rows of X.  In real form, with [Y_re; Y_im] as one 16-row
+// matrix,
+//     [Y_re; Y_im] += [C_re; C_im] X_re + [-C_im; C_re] X_im,
+// which is two v_mfma_f32_16x16x4_f32 per 4 union entries and 16-column
+// tile (KC = 16 NT).  Operand layout (lane l):
+//  * A: row m = l % 16 of the 16 (m < 8: real part of block row m, else the
+//    imaginary part of row m - 8), entry k = l / 16 of the group; the
+//    second product's A is the first's rotated by 8 lanes inside each
+//    16-lane row (DPP), the real-part rows negated;
+//  * B: entry l / 16, tile column l % 16; tile t's column n is the tile's
+//    column NT n + t, so lane l needs columns NT (l % 16) .. + NT - 1 of the
+//    neighbour row, NT x 8 contiguous bytes (16-byte loads; 16 lanes read
+//    the whole 512-byte row of a 64-column tile);
+//  * D: register v holds row 4 (l / 16) + v of the 16, tile column l % 16.
+// Vector-memory instructions are what bound the row-block kernels, so the
+// loads are wide: per group of 4 entries NT / 2 16-byte gathers (one
+// 8-byte one when NT = 1), per batch of 8 entries one 64-lane window of the
+// block's coefficients handed to the lanes with ds_bpermute, and in the
+// epilogue the real- and imaginary-part lanes (l, l ^ 32) swap halves so
+// that each stores and loads whole complex numbers, 16 bytes a lane.
+// The coefficients come from the block's (column, row) ordered list at
+// offset (word >> 16) + popcount of the column's mask bits below the row;
+// the words are the pipelined kernel's.  The sums run in the matrix cores'
+// order, not the per-row kernels': results agree to rounding, not bit for
+// bit.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+template <int NT, bool PACKED>   // NT 16-column tiles; W given as Wq when PACKED
+__global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rbm(int n, const int *__restrict__ rowptr,
+                                                            const int *__restrict__ bofs, const int *__restrict__ ucol,
+                                                            const unsigned *__restrict__ uword,
+                                                            const float2 *__restrict__ rbv, const float2 *__restrict__ X,
+                                                            float2 *__restrict__ Y, const float2 *__restrict__ W,
+                                                            int want_yy, const int *__restrict__ active,
+                                                            double *__restrict__ partial,
+                                                            const unsigned char *__restrict__ Wq)
+{
+    static_assert(RB_ROWS == 8, "the 16-row operand holds 8 block rows, real and imaginary parts");
+    static_assert(NT == 1 || NT == 2 || NT == 4, "tiles of 16, 32 or 64 columns");
+    constexpr int KC = 16 * NT;
+    constexpr int MB = 2;                  // groups per batch: 8 entries, at most 64 coefficients
+    constexpr int NC = NT > 1 ? NT / 2 : 1;   // complex columns per lane in the epilogue
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, q = lane >> 4;
+    const int rr = m & 7, part = m >> 3;            // this lane's A row: block row, real/imaginary part
+    const unsigned below = (1u << rr) - 1u;
+    const float sgn = part ? 1.f : -1.f;            // second A operand: sgn * (first A of lane l ^ 8)
+    const int dpart = q >> 1, drow0 = 4 * (q & 1);  // D rows 4q..4q+3: part, first block row
+    // epilogue columns of this lane: NC consecutive from cb (NT = 1: the
+    // real-part lane holds the column, its partner none)
+    const int cb = NT > 1 ? NT * m + dpart * NC : m;
+    const bool owner = NT > 1 || dpart == 0;
+    bool act[NC];
+#pragma unroll
+    for (int e = 0; e < NC; e++) act[e] = owner && active[cb + e] != 0;
+    const int G = gridDim.x, nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = blockIdx.x % NXCD, bx = blockIdx.x / NXCD, nbx = G / NXCD;
+    const int nb = (n + RB_ROWS - 1) / RB_ROWS;
+    const int per_x = (nb + NXCD - 1) / NXCD;
+    const int xs = min(nb, xcd * per_x), xe = min(nb, xs + per_x);
+    const int wpx = nbx * nw, wid = bx * nw + wave;
+    double d1[NC], d2[NC];
+#pragma unroll
+    for (int e = 0; e < NC; e++) d1[e] = d2[e] = 0.0;
+    // column indices and words of entries s0 .. s0 + 63 (past s1: the last
+    // entry's, every load in bounds and unconditional; the words of entries
+    // past the union are dropped where they are used, not here, so that
+    // nothing waits on these loads until then)
+    auto meta = [&](int s0, int s1, int &jv, unsigned &wv) {
+        const int u = s0 + min(lane, s1 - s0 - 1);
+        jv = ucol[u];
+        wv = uword[u];
+    };
+    struct Batch {
+        float2 cw;
+        float2 xv[MB][NT];
+        int widx[MB];
+        unsigned hasb;
+    };
+    // the loads of the batch at entry g of a segment of cnt entries (column
+    // indices and words jv, wv; the block's coefficients [kb, ke))
+    auto issue = [&](int g, int cnt, int jv, unsigned wv, int kb, int ke, Batch &B) {
+        // the batch's coefficients: at most 8 x 8 from the first entry's
+        const int w0 = kb + (int)(__builtin_amdgcn_readlane(wv, g) >> 16);
+        B.cw = rbv[min(w0 + lane, ke - 1)];
+        B.hasb = 0;
+#pragma unroll
+        for (int gi = 0; gi < MB; gi++) {
+            // entries past the union: its last column, no rows (in bounds)
+            const int idx = g + 4 * gi + q;   // <= 63: g <= 64 - 4 MB
+            // (both permutes with every lane active: a lane left out of a
+            // ds_bpermute provides no data to the lanes reading from it)
+            const int j = __builtin_amdgcn_ds_bpermute(4 * idx, jv);
+            const unsigned wraw = (unsigned)__builtin_amdgcn_ds_bpermute(4 * idx, (int)wv);
+            const unsigned w = wraw & (0u - (unsigned)(idx < cnt));   // no rows past the union
+            const unsigned mk = w & 0xffu;
+            const unsigned has = (mk >> rr) & 1u;
+            B.hasb |= has << gi;
+            // window lane of this lane's coefficient (0 when absent)
+            B.widx[gi] = has ? kb + (int)(w >> 16) + __popc(mk & below) - w0 : 0;
+            const float2 *xr = X + (size_t)j * KC + NT * m;
+            if constexpr (NT == 4) {
+                const float4 v0 = *reinterpret_cast<const float4 *>(xr);
+                const float4 v1 = *reinterpret_cast<const float4 *>(xr + 2);
+                B.xv[gi][0] = make_float2(v0.x, v0.y);
+                B.xv[gi][1] = make_float2(v0.z, v0.w);
+                B.xv[gi][2] = make_float2(v1.x, v1.y);
+                B.xv[gi][3] = make_float2(v1.z, v1.w);
+            } else if constexpr (NT == 2) {
+                const float4 v0 = *reinterpret_cast<const float4 *>(xr);
+                B.xv[gi][0] = make_float2(v0.x, v0.y);
+                B.xv[gi][1] = make_float2(v0.z, v0.w);
+            } else {
+                B.xv[gi][0] = xr[0];
+            }
+        }
+    };
+    mfma_f4 acc[NT];
+    // (a group wholly past the union has zero coefficients: its products add
+    // zeros; not branching over it keeps its loads ahead of the barrier)
+    auto compute = [&](const Batch &B) {
+#pragma unroll
+        for (int gi = 0; gi < MB; gi++) {
+            const float cre = __builtin_bit_cast(
+                float, __builtin_amdgcn_ds_bpermute(4 * B.widx[gi], __builtin_bit_cast(int, B.cw.x)));
+            const float cim = __builtin_bit_cast(
+                float, __builtin_amdgcn_ds_bpermute(4 * B.widx[gi], __builtin_bit_cast(int, B.cw.y)));
+            const float a = ((B.hasb >> gi) & 1u) ? (part ? cim : cre) : 0.f;
+            // row_ror:8 inside each 16-lane row: lane l takes lane l ^ 8
+            const float ab = sgn * __builtin_bit_cast(
+                                       float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, a), 0x128, 0xf, 0xf,
+                                                                       false));
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, B.xv[gi][t].x, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab, B.xv[gi][t].y, acc[t], 0, 0, 0);
+        }
+    };
+    // The wave walks its blocks as one stream of batches, double-buffered:
+    // the loads of the next batch are issued before the products of the
+    // current one, the two buffers in turn over pairs of batches (no copies:
+    // a copy would wait on the loads; no exit between a batch's loads and its
+    // products: the compiler would sink the loads past the other buffer's
+    // products).  A segment with an odd batch count ends with a padding
+    // batch (the last one's loads again, no coefficients); the spare load
+    // slot of a segment's last pair takes the next segment's first batch --
+    // for a block's last segment the first batch of the wave's next block,
+    // whose column indices and words are loaded one block ahead (past the
+    // wave's last block: a stand-in, in bounds, never used).
+    int b = xs + wid;
+    if (b < xe) {
+        int jn;
+        unsigned wn;
+        meta(bofs[b], bofs[b + 1], jn, wn);
+        Batch A, Bn;
+        issue(0, min(64, bofs[b + 1] - bofs[b]), jn, wn, rowptr[b * RB_ROWS], rowptr[min(b * RB_ROWS + RB_ROWS, n)], A);
+        for (; b < xe; b += wpx) {
+            const int u0 = bofs[b], u1 = bofs[b + 1];
+            const int r0 = b * RB_ROWS, nr = min(RB_ROWS, n - r0);
+            const int kb = rowptr[r0], ke = rowptr[r0 + nr];
+            // W of the block's rows, for the epilogue (rows past n: row n - 1, unused)
+            float2 wpre[4][NC];
+            if constexpr (!PACKED) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float2 *wr = W + (size_t)min(r0 + drow0 + v, n - 1) * KC + cb;
+                    if constexpr (NC == 2) {
+                        const float4 w4 = *reinterpret_cast<const float4 *>(wr);
+                        wpre[v][0] = make_float2(w4.x, w4.y);
+                        wpre[v][1] = make_float2(w4.z, w4.w);
+                    } else {
+                        wpre[v][0] = wr[0];
+                    }
+                }
+            }
+            int jv = jn;
+            unsigned wv = wn;
+            const int bn = min(b + wpx, xe - 1);
+            const int kbn = rowptr[bn * RB_ROWS], ken = rowptr[min(bn * RB_ROWS + RB_ROWS, n)];
+            const int cntn = min(64, bofs[bn + 1] - bofs[bn]);
+            meta(bofs[bn], bofs[bn + 1], jn, wn);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+            // segments of up to 64 entries (one in almost every block); A holds
+            // the segment's first batch
+            for (int seg = u0; seg < u1; seg += 64) {
+                const int cnt = min(64, u1 - seg);
+                if (seg > u0) {
+                    meta(seg, u1, jv, wv);
+                    issue(0, cnt, jv, wv, kb, ke, A);
+                }
+                const int glast = (cnt - 1) / (4 * MB) * (4 * MB);
+                const int npair = (glast / (4 * MB) + 2) / 2;
+                for (int pi = 0; pi + 1 < npair; pi++) {
+                    const int g1 = (2 * pi + 1) * 4 * MB;
+                    issue(min(g1, glast), cnt, jv, wv, kb, ke, Bn);
+                    __builtin_amdgcn_sched_barrier(0);
+                    compute(A);
+                    issue(g1 + 4 * MB, cnt, jv, wv, kb, ke, A);
+                    __builtin_amdgcn_sched_barrier(0);
+                    compute(Bn);
+                }
+                {
+                    // the segment's last pair
+                    const int g1 = (2 * npair - 1) * 4 * MB;
+                    issue(min(g1, glast), cnt, jv, wv, kb, ke, Bn);
+                    if (g1 > glast) Bn.hasb = 0;
+                    __builtin_amdgcn_sched_barrier(0);
+                    compute(A);
+                    if (seg + 64 >= u1) issue(0, cntn, jn, wn, kbn, ken, A);   // the next block's first batch
+                    else issue(glast, cnt, jv, wv, kb, ke, A);                 // stand-in: reloaded above
+                    __builtin_amdgcn_sched_barrier(0);
+                    compute(Bn);
+                }
+            }
+            // lane l holds part dpart of rows drow0 .. drow0 + 3, tile column m;
+            // with its partner l ^ 32 (the other part of the same rows) it trades
+            // halves: columns cb .. cb + NC - 1 whole
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = drow0 + v;
+                float2 y[NC];
+#pragma unroll
+                for (int e2 = 0; e2 < NC; e2++) {
+                    const float mine = NT > 1 ? acc[dpart ? NC + e2 : e2][v] : acc[0][v];
+                    const float send = NT > 1 ? acc[dpart ? e2 : NC + e2][v] : acc[0][v];
+                    const float recv = __shfl_xor(send, 32);
+                    y[e2] = dpart ? make_float2(recv, mine) : make_float2(mine, recv);
+                }
+                if (row < nr) {
+                    const size_t at = (size_t)(r0 + row) * KC + cb;
+                    float2 w[NC];
+                    if constexpr (NC == 2) {
+                        if (act[0] && act[1]) {
+                            *reinterpret_cast<float4 *>(Y + at) = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+                        } else {
+                            if (act[0]) Y[at] = y[0];
+                            if (act[1]) Y[at + 1] = y[1];
+                        }
+                    } else {
+                        if (act[0]) Y[at] = y[0];
+                    }
+#pragma unroll
+                    for (int e2 = 0; e2 < NC; e2++)
+                        w[e2] = PACKED ? sh_unpack(Wq, (size_t)(r0 + row), cb + e2) : wpre[v][e2];
+#pragma unroll
+                    for (int e2 = 0; e2 < NC; e2++)
+                        if (act[e2]) dot_epi(w[e2], y[e2], want_yy, d1[e2], d2[e2]);
+                }
+            }
+        }
+    }
+    // per column: the lanes of rows 0-3 and 4-7 (l, l ^ 16), then the waves
+    const bool want2 = want_yy != 0;
+#pragma unroll
+    for (int e = 0; e < NC; e++) {
+        d1[e] += __shfl_xor(d1[e], 16);
+        if (want2) d2[e] += __shfl_xor(d2[e], 16);
+    }
+    __shared__ double s1[SPMM64_WAVES][KC], s2[SPMM64_WAVES][KC];
+    if ((q & 1) == 0 && owner) {
+#pragma unroll
+        for (int e = 0; e < NC; e++) {
+            s1[wave][cb + e] = d1[e];
+            s2[wave][cb + e] = d2[e];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < KC) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int w = 0; w < nw; w++) { t1 += s1[w][threadIdx.x]; t2 += s2[w][threadIdx.x]; }
+        partial[((size_t)0 * KC + threadIdx.x) * G + blockIdx.x] = t1;
+        if (want2) partial[((size_t)1 * KC + threadIdx.x) * G + blockIdx.x] = t2;
+    }
+}
+
 // ------------------------------- LDS-staged block SpMM, 64 columns
 // One workgroup per block of consecutive rows (blocks of one XCD's region
 // dealt round-robin to that XCD's workgroups).  The workgroup first copies
@@ -2196,7 +2478,18 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
 #undef SPMV1
     } else {
 #define SPMMK(KK)                                                                                             \
-    if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val && b.rb.word && Wq)             \
+    if ((KK == 64 || KK == 32 || KK == 16) && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val &&     \
+        b.rb.word && b.rb.mfma) {                                                                             \
+        if (KK == 64 && Wq)                                                                                   \
+            hipLaunchKernelGGL((k_spmm_rbm<4, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs, \
+                               b.rb.col, b.rb.word, (const float2 *)b.rb.val, (const float2 *)X, (float2 *)Y,  \
+                               (const float2 *)W, (int)want_yy, iflags, partial, Wq);                         \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_spmm_rbm<(KK >= 16 ? KK / 16 : 1), false>), g, dim3(64 * SPMM64_WAVES), 0, st, \
+                               b.n, b.rowptr, b.rb.ofs, b.rb.col, b.rb.word, (const float2 *)b.rb.val,         \
+                               (const float2 *)X, (float2 *)Y, (const float2 *)W, (int)want_yy, iflags, partial, \
+                               nullptr);                                                                      \
+    } else if (KK == 64 && !idiag && vk == VK_C32 && b.rb.nb > 0 && b.rb.src == val && b.rb.word && Wq)      \
         hipLaunchKernelGGL((k_spmm_rbp<0, true>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs, \
                            b.rb.col, b.rb.word, (const float2 *)b.rb.val, (const float2 *)X, (float2 *)Y,      \
                            (const float2 *)W, (int)want_yy, iflags, partial, Wq);                           \

@@ -141,9 +141,10 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
         b.lds = RowBlocks{o.nb, o.brow.p, o.bu.p, o.ucols.p, o.lcol.p};
         b.lds.dma = spmm_variant == 2;
     }
-    if ((spmm_variant == 3 || spmm_variant == 4) && o.rb_nb > 0 && rb_src) {
+    if (spmm_variant >= 3 && o.rb_nb > 0 && rb_src) {
         b.rb = RowUnions{o.rb_nb, o.rb_ofs.p, o.rb_col.p, o.rb_mask.p, rbval32.p, rb_src};
-        if (spmm_variant == 4 && o.rb_word_ok) b.rb.word = o.rb_word.p;
+        if (spmm_variant >= 4 && o.rb_word_ok) b.rb.word = o.rb_word.p;
+        b.rb.mfma = spmm_variant == 5 && o.rb_word_ok;
     }
     return b;
 }
@@ -651,7 +652,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
             TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
         }
     }
-    if ((spmm_variant == 3 || spmm_variant == 4) && cplx) {
+    if (spmm_variant >= 3 && cplx) {
         std::lock_guard<std::mutex> lk(prep_mu);
         prepare_row_blocks(mf ? mfval32.p : spmm_op32());
     }
@@ -829,13 +830,13 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
         }
         fp64_fallback = fb0;
         struct Cand { int variant; int rpw; };
-        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}, {4, 0}};
+        const Cand cands[] = {{1, 0}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {0, 0}, {2, 0}, {3, 0}, {4, 0}, {5, 0}};
         double best = 1e30;
         Cand pick{spmm_variant, rpw};
         for (const Cand &c : cands) {
             if ((c.variant == 0 || c.variant == 2) && ord().nb == 0) continue;
             spmm_variant = c.variant;
-            if (c.variant == 3 || c.variant == 4) {
+            if (c.variant >= 3) {
                 std::lock_guard<std::mutex> lk(prep_mu);
                 prepare_row_blocks(spmm_op32());
             }

@@ -11,7 +11,7 @@ import numpy as np
 from . import _lib as L
 
 # SpMM variant names, in the order of their C-ABI codes
-SPMM_VARIANTS = ("lds", "direct", "dma", "rowblock", "rowblock_pipelined")
+SPMM_VARIANTS = ("lds", "direct", "dma", "rowblock", "rowblock_pipelined", "rowblock_mfma")
 
 
 def _f64(a):
