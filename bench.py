@@ -57,6 +57,8 @@ SPMM_KERNELS = {
     "dma": "k_spmm_dma<...> (inner block SpMM, neighbour rows staged in LDS by the LDS DMA, double-buffered)",
     "rowblock_pipelined": "k_spmm_rbp<0,%s> (inner block SpMM + %s dots, complex FP32, 64 columns, row blocks of 8 "
                           "with packed masks, double-buffered gather batches)",
+    "rowblock_mfma": "k_spmm_rbm<4,%s> (inner block SpMM + %s dots, complex FP32, 64 columns, blocks of 8 rows as "
+                     "16x16x4 FP32 matrix-core products over their shared neighbour rows)",
 }
 
 
@@ -65,7 +67,7 @@ def spmm_kernel_name(variant, solver):
     the packed sign-pattern shadow residual, COCG with the direction
     (unconjugated)."""
     name = SPMM_KERNELS[variant]
-    if variant == "rowblock_pipelined":
+    if variant in ("rowblock_pipelined", "rowblock_mfma"):
         name = name % (("true", "<rd,Y>") if solver == "BICGSTAB" else ("false", "p^T Y"))
     return name
 

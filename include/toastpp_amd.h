@@ -194,9 +194,11 @@ int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
  * global memory per nonzero; 0 stages each block of rows' neighbour
  * rows in LDS first (through registers); 2 stages them with the LDS DMA
  * into two alternating buffers, the next block's copy overlapping the
- * current block's compute; 3 (default) walks blocks of 8 consecutive rows
- * over the union of their columns and gathers each shared neighbour row
- * once.
+ * current block's compute; 3 walks blocks of 8 consecutive rows over the
+ * union of their columns and gathers each shared neighbour row once; 4 is
+ * 3 with double-buffered gathers; 5 (default) computes each block of 8 rows
+ * as a dense product on the matrix cores (16x16x4 FP32 MFMA) over the same
+ * shared neighbour rows, and also runs the 32- and 16-column tiles.
  * Tuning parameter, no effect on results beyond rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
 /* Autotuner (no reference counterpart): times the 64-column SpMM for the

@@ -89,7 +89,7 @@ def test_multifreq_matches_oracle_and_single_frequency_solves(cyl2, mode, precon
             assert r <= 1e-8, (FREQS[fi], c, r)
 
 
-@pytest.mark.parametrize("variant", ["rowblock", "direct", "lds"])
+@pytest.mark.parametrize("variant", ["rowblock", "rowblock_mfma", "direct", "lds"])
 def test_multifreq_64_column_tiles(cyl2, variant):
     """64 columns in one tile (16 sources x 4 frequencies): the row-block
     SpMM or the per-row 64-column kernel with per-column frequencies, then

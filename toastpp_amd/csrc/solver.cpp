@@ -845,6 +845,33 @@ void Solver::autotune(int nq, const void *d_b, void *d_x, int ld, int *tile_out,
             time_kernel(0, 10, &ms, &bytes);
             if (ms < best) { best = ms; pick = c; }
         }
+        // the row-block variants differ most on the narrow tiles column
+        // compaction leaves (the matrix-core kernel also runs the 32- and
+        // 16-column ones): among them, whole 64-column solves decide
+        if (pick.variant >= 3 && ord().rb_word_ok) {
+            double bs = 1e30;
+            Cand sp = pick;
+            fp64_fallback = false;   // as for the trial solve above
+            try {
+                for (const int v : {3, 5}) {
+                    spmm_variant = v;
+                    rpw = 0;
+                    {
+                        std::lock_guard<std::mutex> lk(prep_mu);
+                        prepare_row_blocks(spmm_op32());
+                    }
+                    zero_x();
+                    solve(std::min(nq, 64), d_b, d_x, ld, nullptr);
+                    if (ex.last_solve_ms < bs) { bs = ex.last_solve_ms; sp = Cand{v, 0}; }
+                }
+            } catch (...) {
+                fp64_fallback = fb0;
+                tile = tile0;
+                throw;
+            }
+            fp64_fallback = fb0;
+            pick = sp;
+        }
         spmm_variant = pick.variant;
         rpw = pick.rpw;
         tile = tile0;

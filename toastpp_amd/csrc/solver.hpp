@@ -34,9 +34,9 @@ struct Solver {
     int reliable_every = TMI_RELIABLE_EVERY;   // inner iterations between FP64 residual replacements (mixed mode)
     int order_kind = ORDER_CLUSTER;  // node order of the solver's copy of the system
     int rpw = 4;                     // rows per run of the 64-column SpMM (0: one contiguous share per wave)
-    int spmm_variant = 3;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
-                                     // 3 row blocks with shared gathers (default), 4 the same pipelined,
-                                     // 5 row blocks on the matrix cores (also the 32- and 16-column tiles)
+    int spmm_variant = 5;            // 64-column SpMM: 0 LDS (register staging), 1 direct gathers, 2 LDS DMA,
+                                     // 3 row blocks with shared gathers, 4 the same pipelined, 5 (default)
+                                     // row blocks on the matrix cores (also the 32- and 16-column tiles)
     int compact_min = 2;             // narrowest tile the column compaction goes down to (0: off)
     bool allocated = false, assembled = false, precon_valid = false;
     void prepare_row_blocks(const float *op);   // rbval32 for the FP32 operator op (caller holds prep_mu)
