@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multifreq", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--spmm-variant", default=None, choices=list(SPMM_KERNELS),
+                    help="64-column SpMM variant of the solvers (default: the library's; the autotuner may change it)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -169,6 +171,8 @@ def main():
         f.SetPrecon("DIAG")
         if tile:
             f.SetTile(tile)
+        if args.spmm_variant:
+            f.SetSpmmVariant(args.spmm_variant)
         f.Allocate()
         t0 = time.perf_counter()
         f.Reset(sol, omega)
@@ -229,7 +233,7 @@ def main():
     log("field error vs FP64 reference: max %.2e median %.2e" % (err.max(), np.median(err)))
 
     # dominant kernel: the block SpMM of the inner solve
-    spmm_kernel = spmm_kernel_name(tuned["spmm_variant"] if tuned else "rowblock", args.solver)
+    spmm_kernel = spmm_kernel_name(tuned["spmm_variant"] if tuned else (args.spmm_variant or "rowblock_mfma"), args.solver)
     spmm_ms, spmm_bytes = fm.TimeKernel(0, reps=20)
     res_ms, res_bytes = fm.TimeKernel(1, reps=10)
     achieved = spmm_bytes / (spmm_ms * 1e-3) / 1e9
