@@ -69,6 +69,8 @@ struct RowUnions {
     const void *src = nullptr;           // the operator (CSR values) they were taken from
     const unsigned *word = nullptr;      // packed mask | offset words: the pipelined kernel (variant 4)
     bool mfma = false;                   // matrix-core kernel (variant 5; needs word)
+    const void *val64 = nullptr;         // complex FP64 coefficients in the same order (the FP64 residual)
+    const void *src64 = nullptr;         // the FP64 operator they were taken from
 };
 
 struct BlockDesc {
@@ -195,6 +197,7 @@ void launch_permute_vals(bool cplx, int nnz, const int *slot, const double *val6
                          hipStream_t st);
 // FP32 values into the row-block SpMM's order: out[k] = in[slot[k]] (w floats each)
 void launch_gather_vals32(int nnz, int w, const int *slot, const float *in, float *out, hipStream_t st);
+void launch_gather_vals64(int nnz, int w, const int *slot, const double *in, double *out, hipStream_t st);
 // tile of right-hand sides / solutions between the caller's [n][ld] layout
 // (reference node order) and the solver's [n][K] tile (solver node order):
 // gather dst[r][c] = src[perm[r]][q0 + c] for c < kc, 0 for kc <= c < K;

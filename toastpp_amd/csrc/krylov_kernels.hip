@@ -673,9 +673,13 @@ __global__ void __launch_bounds__(64 * SPMM64_WAVES) k_spmm_rb(int n, const int 
         for (int r = 0; r < RB_ROWS; r++) {
             if (r < nr && act) {
                 const size_t at = (size_t)(r0 + r) * KC + c;
-                if (MODE == 0) {
+                if constexpr (MODE == 0) {
                     Y[at] = acc[r];
-                    const V w = (KC == 64 && Wq) ? sh_unpack(Wq, (size_t)(r0 + r), c) : W[at];   // Wq: packed shadow residual
+                    V w;
+                    if constexpr (std::is_same<V, float2>::value)
+                        w = (KC == 64 && Wq) ? sh_unpack(Wq, (size_t)(r0 + r), c) : W[at];   // Wq: packed shadow residual
+                    else
+                        w = W[at];
                     dot_epi(w, acc[r], want_yy, d1, d2);
                 } else {
                     const V y = vsub(W[at], acc[r]);
@@ -2561,7 +2565,11 @@ void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, 
         return;
     }
 #define RESK(KK)                                                                                              \
-    if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                           \
+    if (KK == 64 && vk == VK_C64 && b.rb.nb > 0 && b.rb.val64 && b.rb.src64 == val)                            \
+        hipLaunchKernelGGL((k_spmm_rb<double2, 1>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs, \
+                           b.rb.col, b.rb.mask, (const double2 *)b.rb.val64, (const double2 *)X, (double2 *)R,  \
+                           (const double2 *)B, 0, iflags, partial, nullptr);                                   \
+    else if (KK == 64 && (vk == VK_C32 || vk == VK_C64))                                                      \
         hipLaunchKernelGGL((k_spmm64<V, 1>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                            (V *)R, (const V *)B, 0, iflags, partial, b.rpw);                                 \
     else if (KK >= NK_KMIN && KK <= 32)                                                                       \
@@ -2849,6 +2857,20 @@ __global__ void k_gather_vals32(int cnt, int w, const int *__restrict__ slot, co
 void launch_gather_vals32(int nnz, int w, const int *slot, const float *in, float *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gather_vals32, dim3(grid_for(nnz, 256)), dim3(256), 0, st, nnz, w, slot, in, out);
+    TMI_LAUNCH_CHECK();
+}
+
+__global__ void k_gather_vals64(int cnt, int w, const int *__restrict__ slot, const double *__restrict__ in,
+                                double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    for (int t = 0; t < w; t++) out[(size_t)k * w + t] = in[(size_t)slot[k] * w + t];
+}
+
+void launch_gather_vals64(int nnz, int w, const int *slot, const double *in, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gather_vals64, dim3(grid_for(nnz, 256)), dim3(256), 0, st, nnz, w, slot, in, out);
     TMI_LAUNCH_CHECK();
 }
 

@@ -145,6 +145,10 @@ BlockDesc Solver::block(const Exec::Work &w, int K) const
         b.rb = RowUnions{o.rb_nb, o.rb_ofs.p, o.rb_col.p, o.rb_mask.p, rbval32.p, rb_src};
         if (spmm_variant >= 4 && o.rb_word_ok) b.rb.word = o.rb_word.p;
         b.rb.mfma = spmm_variant == 5 && o.rb_word_ok;
+        if (rb64_src) {
+            b.rb.val64 = rbval64.p;
+            b.rb.src64 = rb64_src;
+        }
     }
     return b;
 }
@@ -171,6 +175,7 @@ void Solver::assemble(const double *cmua, const double *ckap, const double *c2a,
     mf_valid = false;       // so does the multi-frequency operator
     precon_valid = false;   // the preconditioner and K D^-1 follow the matrix
     rb_src = nullptr;       // ... and so do the row-block coefficients
+    rb64_src = nullptr;
     launch_assemble(*mesh, coef.p, coef.p + n, coef.p + 2 * n, om, cplx ? 1 : 0, val64.p, nullptr, st,
                     elbasis ? 1 : 0);
     // the solver's copy in its own node order
@@ -614,6 +619,16 @@ int Solver::solve_tile(Exec &e, int K0, int kc, tmi_colstat *stats, void *d_x, i
     return notconv;
 }
 
+void Solver::prepare_row_blocks64(const double *op)
+{
+    if (rb64_src == op) return;
+    const Mesh::NodeOrder &o = ord();
+    rbval64.alloc((size_t)2 * mesh->nnz);
+    launch_gather_vals64(mesh->nnz, 2, o.rb_slot.p, op, rbval64.p, st);
+    TMI_HIP(hipStreamSynchronize(st));   // other streams read it next
+    rb64_src = op;
+}
+
 void Solver::prepare_row_blocks(const float *op)
 {
     if (rb_src == op) return;
@@ -655,6 +670,7 @@ int Solver::solve_on(Exec &e, int nq, const void *d_b, void *d_x, int ld, tmi_co
     if (spmm_variant >= 3 && cplx) {
         std::lock_guard<std::mutex> lk(prep_mu);
         prepare_row_blocks(mf ? mfval32.p : spmm_op32());
+        if (!mf) prepare_row_blocks64(sval64.p);   // the FP64 residual's operator
     }
     if (!e.st) e.st = st;
     std::vector<tmi_colstat> own;

@@ -51,6 +51,9 @@ struct Solver {
     DBuf<float> pval32;    // A D^-1 (FP32, solver order)
     DBuf<float> rbval32;   // the FP32 operator in the row-block SpMM's coefficient order
     const void *rb_src = nullptr;   // the operator rbval32 holds (null: none)
+    DBuf<double> rbval64;  // the complex FP64 operator in that order (the 64-column FP64 residual)
+    const void *rb64_src = nullptr;   // the operator rbval64 holds (null: none)
+    void prepare_row_blocks64(const double *op);   // rbval64 for op (caller holds prep_mu)
     DBuf<double> coef;     // cmua | ckappa | c2a on the device
     bool elbasis = false;  // coef holds per-element coefficients
     // multi-frequency operator: the system assembled at omega = 1 (real part
