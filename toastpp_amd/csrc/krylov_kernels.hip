@@ -2520,6 +2520,10 @@ void launch_spmm_dot(VKind vk, const BlockDesc &b, const void *val, const void *
                            b.lds.nb, b.lds.brow, b.lds.bu,                                                    \
                            b.lds.ucols, b.rowptr, b.lds.lcol, (const float2 *)val, (const float2 *)X,          \
                            (float2 *)Y, (const float2 *)W, (int)want_yy, iflags, partial);                    \
+    else if (KK == 64 && !idiag && vk == VK_C64 && b.rb.nb > 0 && b.rb.val64 && b.rb.src64 == val)          \
+        hipLaunchKernelGGL((k_spmm_rb<double2, 0>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.rb.ofs, \
+                           b.rb.col, b.rb.mask, (const double2 *)b.rb.val64, (const double2 *)X, (double2 *)Y, \
+                           (const double2 *)W, (int)want_yy, iflags, partial, nullptr);                       \
     else if (KK == 64 && !idiag && (vk == VK_C32 || vk == VK_C64))                                           \
         hipLaunchKernelGGL((k_spmm64<V, 0>), g, dim3(64 * SPMM64_WAVES), 0, st, b.n, b.rowptr, b.colidx, (const V *)val, (const V *)X, \
                            (V *)Y, (const V *)W, (int)want_yy, iflags, partial, b.rpw);                      \
