@@ -202,7 +202,8 @@ int tmi_solver_set_rows_per_wave(tmi_solver s, int rows);
  * Tuning parameter, no effect on results beyond rounding. */
 int tmi_solver_set_spmm_variant(tmi_solver s, int variant);
 /* Autotuner (no reference counterpart): times the 64-column SpMM for the
- * row-schedule / variant candidates and whole CalcFields of the caller's nq
+ * row-schedule / variant candidates (among the row-block variants 3 and 5,
+ * whole 64-column solves decide), then whole CalcFields of the caller's nq
  * right-hand sides (device, node-major [nlen][ld]) for tile widths 8..64,
  * keeps the fastest settings on the solver and reports them (any output
  * pointer may be NULL).  d_phi is scratch and ends holding a solution. */

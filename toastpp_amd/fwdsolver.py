@@ -87,7 +87,11 @@ class FwdSolver:
     def SetSpmmVariant(self, variant):
         """'direct': per-nonzero gathers; 'lds': LDS-staged 64-column
         SpMM; 'dma': LDS-staged with the LDS DMA, double-buffered; 'rowblock':
-        blocks of 8 rows gathering each shared neighbour row once (default)."""
+        blocks of 8 rows gathering each shared neighbour row once;
+        'rowblock_pipelined': the same with double-buffered gathers;
+        'rowblock_mfma' (default): each block of 8 rows a dense product on
+        the matrix cores over the same shared rows (also the 32- and
+        16-column tiles)."""
         L.call("tmi_solver_set_spmm_variant", self._h, SPMM_VARIANTS.index(variant))
 
     # --- lifecycle ---
