@@ -2587,6 +2587,15 @@ void launch_resid(VKind vk, const BlockDesc &b, const void *val, const void *X, 
     TMI_LAUNCH_CHECK();
 }
 
+// grid of the streaming kernels without partial sums that take EW_U
+// elements per thread (ew_for): one chunk per thread, no grid-stride loop
+static dim3 ew_chunk_grid(const BlockDesc &b)
+{
+    const size_t tot = (size_t)b.n * b.K, per = (size_t)256 * EW_U;
+    TMI_CHECK(tot / per < (size_t)INT32_MAX, TMI_ERR_INVALID_ARG, "block vector of %zu elements is too large", tot);
+    return dim3((unsigned)std::max<size_t>(1, (tot + per - 1) / per));
+}
+
 // grid of the one-element-per-thread kernels
 static dim3 ew1_grid(const BlockDesc &b)
 {
@@ -2664,7 +2673,7 @@ void launch_cg_d(VKind vk, const BlockDesc &b, const void *idiag, void *d, const
         return;
     }
     TMI_CHECK(bil, TMI_ERR_UNSUPPORTED, "the deferred x update is COCG's");
-    dim3 g(b.nblk), blk(256);
+    dim3 g = ew_chunk_grid(b), blk(256);
 #define BDK(KK)                                                                                               \
     hipLaunchKernelGGL((k_cocg_d<V, KK, true>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, \
                        nullptr, omg, (V *)x);
@@ -2677,7 +2686,9 @@ void launch_cg_init(VKind vk, const BlockDesc &b, const void *idiag, void *d, co
                     const int *ic, double *partial, hipStream_t st, bool bil, const double *omg)
 {
     TMI_CHECK(!omg || bil, TMI_ERR_UNSUPPORTED, "per-column frequencies need COCG");
-    dim3 g(b.nblk), blk(256);
+    // without partial sums (COCG's direction update) the grid is free: one
+    // chunk per thread; with them, the reduction's grid
+    dim3 g = (bil && !partial) ? ew_chunk_grid(b) : dim3(b.nblk), blk(256);
 #define CIK(KK) \
     hipLaunchKernelGGL((k_cg_d<V, KK>), g, blk, 0, st, b.n, (const V *)idiag, (V *)d, (const V *)r, sc, ic, partial);
 #define BIK(KK) \
